@@ -1,0 +1,172 @@
+"""Throughput of the INS2D explicit-RK time stepping on one MI355X.
+
+Workload (default): Taylor-Green vortex on the periodic box [0,2pi]^2,
+1024 x 1024 cells, u = sin x cos y, v = -cos x sin y plus seeded noise of
+amplitude 1e-3, Re = 2000, second-order energy-conserving discretization,
+classical RK4 with a fixed time step from CFL 0.5, FFT pressure solve, FP64.
+A "step" is one RK4 time step (4 stages) of the whole field plus the
+additional pressure solve; the timed region excludes setup and diagnostics.
+
+metric: Mcell-RK-stages/s = cells x RK stages x steps / seconds / 1e6.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: the path runs on one
+GPU (a multi-GPU layout is described in DESIGN.md).
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--case', default='taylor_green')
+    ap.add_argument('--N', type=int, default=1024)
+    ap.add_argument('--order4', type=int, default=0)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-steps', type=int, default=2)
+    return ap.parse_args()
+
+
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for line in fh:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(case_args, steps):
+    """The CPU oracle (numpy/scipy restatement of the reference algorithm)
+    timed on a bounded sample of the same workload."""
+    from ins2d_amd.cases import load
+    from oracle import ins2d_oracle as O
+    case = load(**case_args)
+    t0 = time.perf_counter()
+    opts = O.build(case)
+    V, p, _ = O.input_check(opts)
+    t_setup = time.perf_counter() - t0
+    dt = opts.time.dt
+    t0 = time.perf_counter()
+    for n in range(steps):
+        V, p = O.time_ERK(V, p, n * dt, dt, opts)
+    el = time.perf_counter() - t0
+    stages = len(O.get_rk_method(opts.time.RK)[1])
+    cells = opts.grid.Np
+    return {'value': cells * stages * steps / el / 1e6, 'unit': 'Mcell-RK-stages/s', 'cores': 1, 'kind': 'port',
+            'sample': f'{steps} RK steps of the same {opts.grid.Nx}x{opts.grid.Ny} case '
+                      f'({el:.1f} s, setup {t_setup:.1f} s excluded), numpy/scipy single thread, CPU {cpu_model()}, '
+                      f'nproc {os.cpu_count()}'}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({'error': 'this path runs on one GPU; multi-GPU sharding is described in DESIGN.md, '
+                                   'not built', 'n_gpus': a.gpus}))
+        sys.exit(2)
+    try:
+        import torch
+        have_torch = torch.cuda.is_available()
+    except Exception:
+        have_torch = False
+    from ins2d_amd import host
+
+    case_args = dict(name=a.case, Nx=a.N, Ny=a.N, order4=a.order4)
+    if a.case == 'decaying_turbulence':
+        case_args.update(fft_symbol=1)
+    solver, opts, V0, p0 = host.setup(case_args.pop('name'), **case_args)
+    case_args['name'] = a.case
+    solver.set_state(V0, p0)
+    solver.input_check(0.0, 1)
+    dt = opts.dt
+    stages = len(host.get_rk_method(opts.prm.get('RK', 'RK44'))[1])
+    cells = solver.Np
+    t = 0.0
+    solver.step(t, dt, a.warmup)
+    t += a.warmup * dt
+    solver.sync()
+    if have_torch:
+        torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.step(t, dt, a.steps)
+    solver.sync()
+    if have_torch:
+        torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    t += a.steps * dt
+    value = cells * stages * a.steps / el / 1e6
+
+    # per-kernel timing with HIP events on the library's stream (separate pass)
+    nprof = max(5, min(a.steps, 50))
+    solver.profile(True)
+    solver.step(t, dt, nprof)
+    solver.sync()
+    kinds = {0: 'momentum_rhs', 1: 'divergence', 2: 'poisson', 3: 'projection'}
+    prof = {}
+    for k, nm in kinds.items():
+        ms, n = solver.profile_read(k)
+        prof[nm] = {'ms_total': ms, 'launches': n, 'us_per_launch': (1e3 * ms / n) if n else None}
+    solver.profile(False)
+    diag = solver.diagnostics(t)
+
+    # algorithmic bytes of the momentum RHS kernel (fused stage combination):
+    # reads V (2 fields), writes k_i when a later stage needs it and Vtemp
+    # (2 fields each), reads the earlier k_j of the stage's tableau row.
+    A, b, c = host.get_rk_method(opts.prm.get('RK', 'RK44'))
+    s = len(b)
+    At = np.vstack([A[1:], b[None, :]])
+    keep = [any(At[i, j] != 0 for i in range(j + 1, s)) for j in range(s)]
+    fields = 0
+    for i in range(s):
+        fields += 2 + 2 + (2 if keep[i] else 0) + 2 * sum(1 for j in range(i) if At[i, j] != 0)
+    bytes_per_stage_cell = 8.0 * fields / s
+    rhs = prof['momentum_rhs']
+    achieved = bytes_per_stage_cell * cells / (rhs['us_per_launch'] * 1e-6) / 1e9 if rhs['us_per_launch'] else None
+
+    out = {
+        'metric': 'Mcell-RK-stages/s per GPU',
+        'value': value,
+        'unit': 'Mcell-RK-stages/s',
+        'n_gpus': 1,
+        'steps': a.steps,
+        'warmup': a.warmup,
+        'ms_per_step': 1e3 * el / a.steps,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic',
+        'config': {'workload': f'{a.case} {opts.grid.Nx}x{opts.grid.Ny} periodic, order {4 if a.order4 else 2}, '
+                               f'{opts.prm.get("RK", "RK44")}, FFT pressure, Re={opts.prm["Re"]:g}, dt={dt:.4e}',
+                   'cells': cells, 'rk_stages': stages, 'kernel_path': solver.path},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': (achieved / HBM_PEAK_GBS) if achieved else None, 'traffic': None,
+                     'kernel': 'momentum RHS + stage combination',
+                     'bytes_per_cell': bytes_per_stage_cell},
+        'kernels': prof,
+        'diagnostics': {'maxres': diag[0], 'maxdiv': diag[1], 'k': diag[4]},
+    }
+    solver.close()
+    if not a.no_cpu_baseline:
+        out['cpu_baseline'] = cpu_baseline(case_args, a.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,131 @@
+/*
+ * libins2d_hip: MI355X (gfx950) implementation of the unsteady explicit
+ * Runge-Kutta time stepping of INS2D (energy-conserving staggered finite
+ * volumes for 2-D incompressible Navier-Stokes), FP64.
+ *
+ * C ABI: plain pointers and sizes, host arrays in the reference's layout
+ *   V = [u(:); v(:)], u stored Nux_in x Nuy_in and v Nvx_in x Nvy_in,
+ *   p Npx x Npy, x index fastest (column-major, operator_mesh.m:417-419).
+ * Host arrays are borrowed for the duration of a call and copied; the state,
+ * the operators, FFT plans and solver buffers live on the GPU inside the
+ * handle.  One HIP stream per handle; a handle is not thread-safe.  Every call
+ * returns with its results on the host, except ins2d_step, which leaves the
+ * advanced state resident on the device.
+ *
+ * Each entry point names the reference function it replaces (paths relative
+ * to the reference root).
+ */
+#ifndef INS2D_H
+#define INS2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define INS2D_OK 0
+#define INS2D_E_ARG 1          /* invalid argument / unsupported option   */
+#define INS2D_E_HIP 2          /* HIP runtime error                       */
+#define INS2D_E_NOCONV 3       /* PCG did not converge within maxit       */
+#define INS2D_E_UNSUPPORTED 4  /* combination not implemented            */
+
+/* boundary kinds (boundary_conditions.m:17-40) */
+#define INS2D_BC_DIR 0
+#define INS2D_BC_PER 1
+#define INS2D_BC_PRES 2
+#define INS2D_BC_SYM 3
+
+/* pressure solvers (pressure_poisson.m:42-104 numbering) */
+#define INS2D_POISSON_DIRECT 1 /* emulated: PCG to the requested tolerance */
+#define INS2D_POISSON_CG 3     /* preconditioned CG (cg.F slot)             */
+#define INS2D_POISSON_CG_IC 4  /* preconditioned CG (cg_matlab.m slot)      */
+#define INS2D_POISSON_FFT 6    /* periodic uniform grids                    */
+
+typedef struct ins2d_handle ins2d_handle;
+
+/* Case description (the data main.m hands to solver_unsteady through
+ * options: grid lines from <case>_mesh.m, BC types from <case>_BCtype.m, the
+ * parameters of <case>_parameters.m and the Butcher tableau of
+ * getRKmethod.m). */
+typedef struct {
+  int Nx, Ny;                /* pressure cells                                  */
+  const double* x;           /* Nx+1 grid lines in x                            */
+  const double* y;           /* Ny+1 grid lines in y                            */
+  int bc_u[4];               /* u: left, right, low, up                         */
+  int bc_v[4];               /* v: left, right, low, up                         */
+  double Re;                 /* Reynolds number (1e100 = inviscid)              */
+  int order4;                /* 0: second order, 1: fourth order (periodic)     */
+  double alfa, beta;         /* fourth-order parameters (81, 9/8)               */
+  int rk_stages;             /* s                                               */
+  const double* rk_A;        /* s*s, row-major                                  */
+  const double* rk_b;        /* s                                               */
+  const double* rk_c;        /* s                                               */
+  int poisson;               /* INS2D_POISSON_*                                 */
+  int fft_symbol;            /* 0: operator_divergence.m:283 closed form
+                                (second order, also used by the reference for
+                                fourth order); 1: exact symbol of the
+                                assembled A = M Om^-1 G                         */
+  double cg_tol;             /* relative residual ||r||/||f|| (CG_acc)          */
+  int cg_maxit;              /* CG_maxit                                        */
+  int p_add_solve;           /* 1: additional pressure solve per step           */
+  int kernels;               /* 0: auto, 1: force generic kron-factored kernels */
+} ins2d_case_desc;
+
+/* operator construction: replaces operators.m -> operator_mesh/averaging/
+ * interpolation/divergence/convection_diffusion (spatial/operators/operator_*.m) and
+ * the solver preparation of operator_divergence.m:219-291 */
+int ins2d_create(const ins2d_case_desc* desc, ins2d_handle** out);
+void ins2d_destroy(ins2d_handle* h);
+const char* ins2d_last_error(const ins2d_handle* h);
+
+/* sizes: out[0..6] = Nux_in, Nuy_in, Nvx_in, Nvy_in, Npx, Npy, kernel path
+ * (0 generic, 2 periodic second order, 4 periodic fourth order); xin (Nux_in),
+ * yin (Nvy_in) are the interior velocity grid lines (operator_mesh.m:185-320) */
+int ins2d_grid_info(const ins2d_handle* h, int* out, double* xin, double* yin);
+
+/* boundary samples at time t, in the order set_bc_vectors.m:53-71 takes them:
+ * uLo, uUp (x, len Nx+1), uLo_i, uUp_i (xin), uLe_i, uRi_i (yp, len Ny),
+ * vLe, vRi (y, len Ny+1), vLo_i, vUp_i (xp, len Nx), vLe_i, vRi_i (yin).
+ * Replaces set_bc_vectors.m (steady boundary conditions). */
+int ins2d_set_bc(ins2d_handle* h, double t, const double* const* samples);
+
+/* state transfer (host arrays, reference layout) */
+int ins2d_set_state(ins2d_handle* h, const double* V, const double* p);
+int ins2d_get_state(ins2d_handle* h, double* V, double* p);
+
+/* F.m:1-117: F = -conv + diff - (G p + y_p); maxres = max|F| */
+int ins2d_rhs(ins2d_handle* h, const double* V, const double* p, double t, double* F, double* maxres);
+
+/* pressure_poisson.m: solve A dp = f (x0 = 0); iters/relres for CG modes */
+int ins2d_poisson(ins2d_handle* h, const double* f, double* dp, int* iters, double* relres);
+
+/* input_check.m:92-128 on the resident state: initial projection when
+ * max|M V + yM| > 1e-12, then the initial pressure (pressure_additional_solve)
+ * when p_initial = 1 */
+int ins2d_input_check(ins2d_handle* h, double t, int p_initial);
+
+/* nsteps of time_ERK.m:1-173 (with pressure_additional_solve.m at the end of
+ * each step) on the resident state, t = tn, tn + dt, ... */
+int ins2d_step(ins2d_handle* h, double tn, double dt, int nsteps);
+
+/* check_conservation.m:1-76 + the extra F of process_iteration.m:22 on the
+ * resident state: out = {maxres, maxdiv, umom, vmom, k} */
+int ins2d_diagnostics(ins2d_handle* h, double t, double* out);
+
+/* last Poisson solve: iterations and relative residual (CG modes) */
+int ins2d_solver_stats(const ins2d_handle* h, int* iters, double* relres, long long* total_iters);
+
+/* kernel timing with HIP events on the handle's stream:
+ * enable != 0 starts accumulating; read returns the accumulated milliseconds
+ * and launch count of one kernel class (0 momentum RHS, 1 divergence,
+ * 2 Poisson solve, 3 projection) */
+int ins2d_profile(ins2d_handle* h, int enable);
+int ins2d_profile_read(const ins2d_handle* h, int kind, double* ms, long long* count);
+
+/* wait for all work queued on the handle's stream */
+int ins2d_sync(ins2d_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,59 @@
+"""ctypes binding of libins2d_hip.so (include/ins2d.h).
+
+The library must have been built in-tree (``python -m ins2d_amd.build``);
+loading fails loudly otherwise -- there is no CPU fallback."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libins2d_hip.so")
+
+c_int, c_double, c_void_p, c_char_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_char_p
+P_double = ctypes.POINTER(ctypes.c_double)
+P_int = ctypes.POINTER(ctypes.c_int)
+
+
+class CaseDesc(ctypes.Structure):
+    _fields_ = [("Nx", c_int), ("Ny", c_int), ("x", P_double), ("y", P_double),
+                ("bc_u", c_int * 4), ("bc_v", c_int * 4), ("Re", c_double), ("order4", c_int),
+                ("alfa", c_double), ("beta", c_double), ("rk_stages", c_int), ("rk_A", P_double),
+                ("rk_b", P_double), ("rk_c", P_double), ("poisson", c_int), ("fft_symbol", c_int),
+                ("cg_tol", c_double), ("cg_maxit", c_int), ("p_add_solve", c_int), ("kernels", c_int)]
+
+
+# every entry point declared in include/ins2d.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "ins2d_create": (c_int, [ctypes.POINTER(CaseDesc), ctypes.POINTER(c_void_p)]),
+    "ins2d_destroy": (None, [c_void_p]),
+    "ins2d_last_error": (c_char_p, [c_void_p]),
+    "ins2d_grid_info": (c_int, [c_void_p, P_int, P_double, P_double]),
+    "ins2d_set_bc": (c_int, [c_void_p, c_double, ctypes.POINTER(P_double)]),
+    "ins2d_set_state": (c_int, [c_void_p, P_double, P_double]),
+    "ins2d_get_state": (c_int, [c_void_p, P_double, P_double]),
+    "ins2d_rhs": (c_int, [c_void_p, P_double, P_double, c_double, P_double, P_double]),
+    "ins2d_poisson": (c_int, [c_void_p, P_double, P_double, P_int, P_double]),
+    "ins2d_input_check": (c_int, [c_void_p, c_double, c_int]),
+    "ins2d_step": (c_int, [c_void_p, c_double, c_double, c_int]),
+    "ins2d_diagnostics": (c_int, [c_void_p, c_double, P_double]),
+    "ins2d_solver_stats": (c_int, [c_void_p, P_int, P_double, ctypes.POINTER(ctypes.c_longlong)]),
+    "ins2d_profile": (c_int, [c_void_p, c_int]),
+    "ins2d_profile_read": (c_int, [c_void_p, c_int, P_double, ctypes.POINTER(ctypes.c_longlong)]),
+    "ins2d_sync": (c_int, [c_void_p]),
+}
+
+_LIB = None
+
+
+def load(path=LIB_PATH):
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"libins2d_hip is not built ({path} missing): run `python -m ins2d_amd.build`")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib

@@ -1,0 +1,814 @@
+// C-ABI of libins2d_hip: handle management, operator upload, the explicit RK
+// step (time_ERK.m), the Poisson solvers and the diagnostics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ins2d.h"
+#include "device.hpp"
+#include "launch.hpp"
+#include "ops1d.hpp"
+
+using namespace ins2d;
+
+namespace {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct NoConv : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct Unsupported : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) throw HipError(std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+struct ins2d_handle {
+  std::string err;
+  hipStream_t st = nullptr;
+  int path = 0;              // 0 generic, 2/4 periodic uniform order
+  int Nx = 0, Ny = 0, Nu = 0, Nv = 0, NV = 0, Np = 0;
+  int nux = 0, nuy = 0, nvx = 0, nvy = 0;
+  double hx = 0, hy = 0, Re = 1, invRe = 0;
+  std::unique_ptr<Plan> plan;
+  std::vector<void*> allocs;
+  // operators (generic)
+  DComp hCu{}, hCv{};
+  DComp *dCu = nullptr, *dCv = nullptr;
+  DLap lap{};
+  std::vector<void*> yv_allocs;   // boundary vectors (re-uploaded by set_bc)
+  // FFT
+  FFTPlan fft{};
+  bool has_fft = false;
+  // state and work buffers
+  double *V = nullptr, *Vb = nullptr, *p = nullptr, *Vt = nullptr, *f = nullptr, *dp = nullptr, *tV = nullptr;
+  std::vector<double*> kv;
+  double *cx = nullptr, *cr = nullptr, *cz = nullptr, *cp = nullptr, *cq = nullptr;
+  CGStateH cg{};
+  int* d_it = nullptr;
+  double* d_rel = nullptr;
+  double* part = nullptr;
+  int nred = 0;
+  double* ox = nullptr;   // volume factors (all paths)
+  double* oy = nullptr;
+  double* ovx = nullptr;
+  double* ovy = nullptr;
+  // time integration
+  int s = 0;
+  std::vector<double> At, ct;   // shifted tableau (time_ERK.m:29,32)
+  std::vector<bool> keep;        // kV_j needed by a later stage
+  int poisson = 6;
+  int fft_symbol = 0;
+  double tol = 1e-8;
+  int maxit = 1000;
+  int p_add = 1;
+  int order4 = 0;
+  // boundary samples (for the momentum/energy boundary terms)
+  BCSamples samp;
+  bool have_bc = false;
+  // stats
+  int last_it = 0;
+  double last_rel = 0.0;
+  long long total_it = 0;
+  // profiling
+  bool prof = false;
+  double prof_ms[4] = {0, 0, 0, 0};
+  long long prof_n[4] = {0, 0, 0, 0};
+  std::vector<hipEvent_t> ev_free;
+  struct Pending {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pend;
+};
+
+namespace {
+
+template <typename T>
+T* dalloc(ins2d_handle* h, size_t n, std::vector<void*>* list = nullptr) {
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  (list ? *list : h->allocs).push_back(p);
+  return (T*)p;
+}
+
+template <typename T>
+T* dupload(ins2d_handle* h, const std::vector<T>& v, std::vector<void*>* list = nullptr) {
+  T* p = dalloc<T>(h, v.size(), list);
+  if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+DOp1 up_op(ins2d_handle* h, const HOp1& o) {
+  DOp1 d{};
+  d.nrows = o.nrows;
+  d.ncols = o.ncols;
+  d.K = o.K;
+  d.col = dupload(h, o.col);
+  d.val = dupload(h, o.val);
+  return d;
+}
+DKron up_kron(ins2d_handle* h, const HKron& k) {
+  DKron d;
+  d.X = up_op(h, k.X);
+  d.Y = up_op(h, k.Y);
+  return d;
+}
+
+void up_yv(ins2d_handle* h, const HYVec& Y, DYV& d) {
+  std::vector<std::vector<double>> ax, ay;
+  eval_yvec(Y, h->samp, ax, ay);
+  if (ax.size() > 4) throw std::runtime_error("too many boundary terms");
+  d.nt = (int)ax.size();
+  for (int t = 0; t < d.nt; ++t) {
+    d.ax[t] = dupload(h, ax[t], &h->yv_allocs);
+    d.ay[t] = dupload(h, ay[t], &h->yv_allocs);
+  }
+  for (int t = d.nt; t < 4; ++t) d.ax[t] = d.ay[t] = nullptr;
+}
+
+void upload_comp_ops(ins2d_handle* h, const HComp& C, DComp& D) {
+  D.nx = C.nx;
+  D.ny = C.ny;
+  D.nconv = (int)C.conv.size();
+  D.ndiff = (int)C.diff.size();
+  if (D.nconv > 2 || D.ndiff > 2) throw std::runtime_error("operator term count");
+  for (int t = 0; t < D.nconv; ++t) {
+    D.conv[t].D = up_kron(h, C.conv[t].D);
+    D.conv[t].I = up_kron(h, C.conv[t].I);
+    D.conv[t].A = up_kron(h, C.conv[t].A);
+    D.conv[t].icomp = C.conv[t].icomp;
+    D.conv[t].scale = C.conv[t].scale;
+    D.conv[t].yI.nt = D.conv[t].yA.nt = 0;
+  }
+  for (int t = 0; t < D.ndiff; ++t) {
+    D.diff[t].D = up_kron(h, C.diff[t].D[0]);
+    D.diff[t].S = up_kron(h, C.diff[t].S[0]);
+    D.diff[t].sscale = C.diff[t].sscale[0];
+    D.diff[t].yS.nt = 0;
+  }
+  D.ox = dupload(h, C.ox);
+  D.oy = dupload(h, C.oy);
+  D.G = up_kron(h, C.G);
+  D.M = up_kron(h, C.M);
+  D.yM.nt = 0;
+}
+
+void upload_comp_bc(ins2d_handle* h, const HComp& C, DComp& D) {
+  for (int t = 0; t < D.nconv; ++t) {
+    up_yv(h, C.conv[t].yI, D.conv[t].yI);
+    up_yv(h, C.conv[t].yA, D.conv[t].yA);
+  }
+  for (int t = 0; t < D.ndiff; ++t) up_yv(h, C.diff[t].yS, D.diff[t].yS);
+  up_yv(h, C.yM, D.yM);
+}
+
+void push_comps(ins2d_handle* h) {
+  if (!h->dCu) {
+    h->dCu = dalloc<DComp>(h, 1);
+    h->dCv = dalloc<DComp>(h, 1);
+  }
+  HIPCHK(hipMemcpy(h->dCu, &h->hCu, sizeof(DComp), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->dCv, &h->hCv, sizeof(DComp), hipMemcpyHostToDevice));
+}
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// timing with HIP events on the handle's stream
+struct ProfScope {
+  ins2d_handle* h;
+  int kind;
+  hipEvent_t a{}, b{};
+  ProfScope(ins2d_handle* h_, int k) : h(h_), kind(k) {
+    if (!h->prof) return;
+    a = get_ev();
+    b = get_ev();
+    HIPCHK(hipEventRecord(a, h->st));
+  }
+  hipEvent_t get_ev() {
+    if (!h->ev_free.empty()) {
+      hipEvent_t e = h->ev_free.back();
+      h->ev_free.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+  ~ProfScope() {
+    if (!h->prof) return;
+    if (hipEventRecord(b, h->st) == hipSuccess) h->pend.push_back({kind, a, b});
+  }
+};
+
+void prof_collect(ins2d_handle* h) {
+  for (auto& pe : h->pend) {
+    float ms = 0.f;
+    if (hipEventSynchronize(pe.b) == hipSuccess && hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) {
+      h->prof_ms[pe.kind] += ms;
+      h->prof_n[pe.kind] += 1;
+    }
+    h->ev_free.push_back(pe.a);
+    h->ev_free.push_back(pe.b);
+  }
+  h->pend.clear();
+}
+
+// ------------------------------------------------------------------ kernels
+const StageCombine kNoCombine{0, 0.0, {0}, {nullptr}};
+
+void rhs(ins2d_handle* h, const double* V, const double* p, int mode, double* out, const StageCombine& sc, double* vt) {
+  ProfScope ps(h, 0);
+  if (h->path) {
+    launch_rhs_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, h->invRe, V, V + h->Nu, p, mode, out, sc, vt);
+  } else {
+    StageCombine scv = sc;
+    for (int q = 0; q < sc.n; ++q) scv.k[q] = sc.k[q] + h->Nu;
+    launch_rhs_generic(h->st, h->dCu, h->dCv, h->nux, h->nuy, h->nvx, h->nvy, V, V + h->Nu, p, mode, out,
+                       out ? out + h->Nu : nullptr, sc, scv, vt, vt ? vt + h->Nu : nullptr);
+  }
+}
+
+// f = oscale * (M (ca A + cb B) + ycoef yM)
+void div(ins2d_handle* h, const double* A, const double* B, double ca, double cb, double ycoef, double oscale,
+         double* f) {
+  ProfScope ps(h, 1);
+  if (h->path)
+    launch_div_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, A, B, ca, cb, oscale, f);
+  else
+    launch_div_generic(h->st, h->dCu, h->dCv, h->Nx, h->Ny, A, B, ca, cb, ycoef, oscale, h->Nu, f);
+}
+
+void proj(ins2d_handle* h, const double* Vn, const double* Vt, const double* dp, double dt, double c, double* V) {
+  ProfScope ps(h, 3);
+  if (h->path)
+    launch_proj_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, Vn, Vt, dp, dt, c, V);
+  else
+    launch_proj_generic(h->st, h->dCu, h->dCv, h->nux, h->nuy, h->nvx, h->nvy, h->Nu, Vn, Vt, dp, dt, c, V);
+}
+
+// A dp = f, x0 = 0 (pressure_poisson.m:12)
+void poisson(ins2d_handle* h, const double* f, double* dp) {
+  ProfScope ps(h, 2);
+  if (h->poisson == INS2D_POISSON_FFT) {
+    launch_fft_poisson(h->st, h->fft, f, dp);
+    return;
+  }
+  const double tol2 = h->tol * h->tol;
+  if (h->Np <= 32768) {
+    launch_cg_single(h->st, h->lap, f, dp, h->cr, h->cz, h->cp, h->cq, tol2, h->maxit, 1, h->d_it, h->d_rel);
+    int it = 0;
+    double rel = 0;
+    HIPCHK(hipMemcpyAsync(&it, h->d_it, sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(&rel, h->d_rel, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->last_it = it;
+    h->last_rel = std::sqrt(rel);
+    h->total_it += it;
+    if (it >= h->maxit && rel > tol2) throw NoConv("CG did not converge in maxit iterations");
+    return;
+  }
+  HIPCHK(hipMemsetAsync(dp, 0, sizeof(double) * h->Np, h->st));
+  launch_cg_init(h->st, h->lap, h->cg, f, dp, h->cr, h->cz, h->cp, 1, tol2);
+  int flag[3] = {0, 0, 0};
+  while (true) {
+    for (int k = 0; k < 16; ++k) launch_cg_iter(h->st, h->lap, h->cg, dp, h->cr, h->cz, h->cp, h->cq);
+    HIPCHK(hipMemcpyAsync(flag, h->cg.flag, 3 * sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (flag[0]) break;
+  }
+  double rel = 0;
+  HIPCHK(hipMemcpy(&rel, h->cg.s + 4, sizeof(double), hipMemcpyDeviceToHost));
+  h->last_it = flag[1];
+  h->last_rel = std::sqrt(rel);
+  h->total_it += flag[1];
+  if (flag[2]) throw NoConv("CG did not converge in maxit iterations");
+}
+
+// p <- p + A^-1 (M Om^-1 F(V, p) + ydM)   (pressure_additional_solve.m:19-25)
+void additional_solve(ins2d_handle* h, const double* V) {
+  rhs(h, V, h->p, RHS_WITH_P | RHS_SCALE_OM, h->tV, kNoCombine, nullptr);
+  div(h, h->tV, nullptr, 1.0, 0.0, 0.0, 1.0, h->f);
+  poisson(h, h->f, h->dp);
+  launch_axpy(h->st, (size_t)h->Np, 1.0, h->dp, h->p);
+}
+
+// one step of time_ERK.m (steady boundary conditions)
+void erk_step(ins2d_handle* h, double tn, double dt) {
+  (void)tn;
+  double* Vn = h->V;
+  double* Vs = h->Vb;   // stage velocity, V_{n+1} at the end
+  for (int i = 0; i < h->s; ++i) {
+    const double* Vsrc = (i == 0) ? Vn : Vs;
+    StageCombine sc{};
+    sc.aself = h->At[i * h->s + i];
+    sc.n = 0;
+    for (int j = 0; j < i; ++j) {
+      const double a = h->At[i * h->s + j];
+      if (a != 0.0) {
+        sc.a[sc.n] = a;
+        sc.k[sc.n] = h->kv[j];
+        ++sc.n;
+      }
+    }
+    // kV_i = Om^-1 (F_i + G p) = Om^-1 (-conv + diff - y_p)   (time_ERK.m:90-95)
+    // Vtemp = sum_j At(i,j) kV_j                              (time_ERK.m:101)
+    rhs(h, Vsrc, h->p, RHS_SCALE_OM, h->keep[i] ? h->kv[i] : nullptr, sc, h->Vt);
+    const double c = h->ct[i];
+    const double* dpp;
+    if (i > 0) {
+      // f = (M (Vn/dt + Vtemp) + yM/dt) / c_i                 (time_ERK.m:117)
+      div(h, Vn, h->Vt, 1.0 / dt, 1.0, 1.0 / dt, 1.0 / c, h->f);
+      poisson(h, h->f, h->dp);
+      dpp = h->dp;
+    } else {
+      dpp = h->p;   // steady BC, first stage: dp = pn (time_ERK.m:125-126)
+    }
+    // V = Vn + dt (Vtemp - c_i Om^-1 G dp)                    (time_ERK.m:132)
+    proj(h, Vn, h->Vt, dpp, dt, c, Vs);
+  }
+  std::swap(h->V, h->Vb);
+  if (h->p_add) additional_solve(h, h->V);
+}
+
+void fill_samples_default(ins2d_handle* h) {
+  const Plan* P = h->plan.get();
+  for (int s = 0; s < S_NSRC; ++s) {
+    int n = 0;
+    switch (s) {
+      case S_uLo: case S_uUp: n = h->Nx + 1; break;
+      case S_uLo_i: case S_uUp_i: n = h->nux; break;
+      case S_uLe_i: case S_uRi_i: n = h->Ny; break;
+      case S_vLe: case S_vRi: n = h->Ny + 1; break;
+      case S_vLo_i: case S_vUp_i: n = h->Nx; break;
+      default: n = h->nvy; break;
+    }
+    (void)P;
+    h->samp.s[s].assign(n, 0.0);
+  }
+}
+
+void build_fft(ins2d_handle* h, int order) {
+  FFTPlan& F = h->fft;
+  F.Nx = h->Nx;
+  F.Ny = h->Ny;
+  F.Nh = h->Nx / 2 + 1;
+  F.pow2x = is_pow2(h->Nx);
+  F.pow2y = is_pow2(h->Ny);
+  if ((F.pow2x && h->Nx > 4096) || (F.pow2y && h->Ny > 4096) || (!F.pow2x && h->Nx > 2048) ||
+      (!F.pow2y && h->Ny > 2048))
+    throw Unsupported("FFT Poisson: grid dimension too large (power-of-two up to 4096, other sizes up to 2048)");
+  auto twid = [](int N) {
+    std::vector<double> t(2 * (size_t)N);
+    for (int k = 0; k < N; ++k) {
+      const double a = -2.0 * M_PI * (double)k / (double)N;
+      t[2 * k] = std::cos(a);
+      t[2 * k + 1] = std::sin(a);
+    }
+    return t;
+  };
+  F.twx = dupload(h, twid(h->Nx));
+  F.twy = dupload(h, twid(h->Ny));
+  std::vector<double> ax(F.Nh), ay(h->Ny);
+  const double dx = h->hx, dy = h->hy;
+  if (order == 2 || h->fft_symbol == 0) {
+    // operator_divergence.m:283: Ahat = (dx*dy)*4*((1/dx^2) sin^2(pi I/Npx) + (1/dy^2) sin^2(pi J/Npy))
+    F.c = (dx * dy) * 4.0;
+    for (int k = 0; k < F.Nh; ++k) {
+      const double sn = std::sin((double)k * M_PI / (double)h->Nx);
+      ax[k] = (1.0 / (dx * dx)) * (sn * sn);
+    }
+    for (int k = 0; k < h->Ny; ++k) {
+      const double sn = std::sin((double)k * M_PI / (double)h->Ny);
+      ay[k] = (1.0 / (dy * dy)) * (sn * sn);
+    }
+  } else {
+    // exact symbol of the fourth-order A = (alfa M1 - M3) Om^-1 G with
+    // Om = 72 hx hy: -A hat = 4 hy/(72 hx) (81 s1 - 3 s3)^2 + (same in y)
+    F.c = 1.0;
+    for (int k = 0; k < F.Nh; ++k) {
+      const double t = (double)k * M_PI / (double)h->Nx;
+      const double m = 81.0 * std::sin(t) - 3.0 * std::sin(3.0 * t);
+      ax[k] = 4.0 * dy / (72.0 * dx) * m * m;
+    }
+    for (int k = 0; k < h->Ny; ++k) {
+      const double t = (double)k * M_PI / (double)h->Ny;
+      const double m = 81.0 * std::sin(t) - 3.0 * std::sin(3.0 * t);
+      ay[k] = 4.0 * dx / (72.0 * dy) * m * m;
+    }
+  }
+  F.ax = dupload(h, ax);
+  F.ay = dupload(h, ay);
+  F.H = dalloc<double>(h, 2 * (size_t)F.Nh * h->Ny);
+  F.HT = dalloc<double>(h, 2 * (size_t)F.Nh * h->Ny);
+  h->has_fft = true;
+}
+
+void stats_reduce(ins2d_handle* h, const double* x, int nx, int ny, const double* wx, const double* wy, double out[3]) {
+  launch_reduce_stats(h->st, x, nx, ny, wx, wy, h->part, h->nred);
+  std::vector<double> part(4 * (size_t)h->nred);
+  HIPCHK(hipMemcpyAsync(part.data(), h->part, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  out[0] = out[1] = out[2] = 0.0;
+  for (int b = 0; b < h->nred; ++b) {
+    out[0] = std::max(out[0], part[4 * b]);
+    out[1] += part[4 * b + 1];
+    out[2] += part[4 * b + 2];
+  }
+}
+
+int fail(ins2d_handle* h, const std::exception& e, int code) {
+  if (h) h->err = e.what();
+  return code;
+}
+
+#define API_TRY try {
+#define API_CATCH(h)                                    \
+  }                                                     \
+  catch (const NoConv& e) { return fail(h, e, INS2D_E_NOCONV); } \
+  catch (const Unsupported& e) { return fail(h, e, INS2D_E_UNSUPPORTED); } \
+  catch (const ArgError& e) { return fail(h, e, INS2D_E_ARG); } \
+  catch (const HipError& e) { return fail(h, e, INS2D_E_HIP); } \
+  catch (const std::exception& e) { return fail(h, e, INS2D_E_ARG); }
+
+}  // namespace
+
+extern "C" {
+
+static std::string g_create_err;
+
+const char* ins2d_last_error(const ins2d_handle* h) { return h ? h->err.c_str() : g_create_err.c_str(); }
+
+int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
+  if (!d || !out) return INS2D_E_ARG;
+  *out = nullptr;
+  auto h = std::make_unique<ins2d_handle>();
+  try {
+    if (d->Nx < 2 || d->Ny < 2 || !d->x || !d->y) throw ArgError("bad grid");
+    if (d->rk_stages < 1 || d->rk_stages > 8 || !d->rk_A || !d->rk_b || !d->rk_c) throw ArgError("bad tableau");
+    for (int k = 0; k < 4; ++k)
+      if (d->bc_u[k] < 0 || d->bc_u[k] > 3 || d->bc_v[k] < 0 || d->bc_v[k] > 3) throw ArgError("bad boundary kind");
+    h->Nx = d->Nx;
+    h->Ny = d->Ny;
+    h->Re = d->Re;
+    h->invRe = 1.0 / d->Re;
+    h->poisson = d->poisson;
+    h->fft_symbol = d->fft_symbol;
+    h->tol = d->cg_tol;
+    h->maxit = d->cg_maxit;
+    h->p_add = d->p_add_solve;
+    h->order4 = d->order4;
+    HIPCHK(hipStreamCreate(&h->st));
+    bool allper = true;
+    for (int k = 0; k < 4; ++k) allper = allper && d->bc_u[k] == INS2D_BC_PER && d->bc_v[k] == INS2D_BC_PER;
+    double hxmin = 1e300, hxmax = 0, hymin = 1e300, hymax = 0;
+    for (int i = 0; i < d->Nx; ++i) { double t = d->x[i + 1] - d->x[i]; hxmin = std::min(hxmin, t); hxmax = std::max(hxmax, t); }
+    for (int j = 0; j < d->Ny; ++j) { double t = d->y[j + 1] - d->y[j]; hymin = std::min(hymin, t); hymax = std::max(hymax, t); }
+    const bool uniform = (hxmax - hxmin) <= 1e-14 && (hymax - hymin) <= 1e-14;
+    if (allper && (d->Nx < 8 || d->Ny < 8)) throw Unsupported("periodic grids need at least 8 cells per direction");
+    if (d->order4) {
+      if (!allper || !uniform) throw Unsupported("fourth order is implemented for periodic uniform grids");
+      if (d->alfa != 81.0 || d->beta != 9.0 / 8.0) throw Unsupported("fourth order needs alfa = 81, beta = 9/8");
+      if (d->kernels == 1) throw Unsupported("fourth order has no generic kernel path");
+      h->path = 4;
+    } else if (allper && uniform && d->kernels != 1) {
+      h->path = 2;
+    }
+    if (h->poisson == INS2D_POISSON_FFT && !(allper && uniform))
+      throw Unsupported("FFT Poisson needs periodic boundaries and a uniform grid (operator_divergence.m:268-290)");
+    if (h->poisson != INS2D_POISSON_FFT && d->order4)
+      throw Unsupported("fourth order pressure is solved by FFT");
+    if (h->poisson != INS2D_POISSON_FFT && h->poisson != INS2D_POISSON_CG && h->poisson != INS2D_POISSON_CG_IC &&
+        h->poisson != INS2D_POISSON_DIRECT)
+      throw ArgError("unknown poisson mode");
+    if (!d->order4) {
+      CaseDesc cd;
+      cd.Nx = d->Nx;
+      cd.Ny = d->Ny;
+      cd.x.assign(d->x, d->x + d->Nx + 1);
+      cd.y.assign(d->y, d->y + d->Ny + 1);
+      cd.u_left = d->bc_u[0]; cd.u_right = d->bc_u[1]; cd.u_low = d->bc_u[2]; cd.u_up = d->bc_u[3];
+      cd.v_left = d->bc_v[0]; cd.v_right = d->bc_v[1]; cd.v_low = d->bc_v[2]; cd.v_up = d->bc_v[3];
+      cd.Re = d->Re;
+      cd.order4 = 0;
+      h->plan.reset(new Plan());
+      build_plan(cd, *h->plan);
+      const Plan& P = *h->plan;
+      h->nux = P.Nux_in; h->nuy = P.Nuy_in; h->nvx = P.Nvx_in; h->nvy = P.Nvy_in;
+      upload_comp_ops(h.get(), P.cu, h->hCu);
+      upload_comp_ops(h.get(), P.cv, h->hCv);
+      fill_samples_default(h.get());
+      upload_comp_bc(h.get(), P.cu, h->hCu);
+      upload_comp_bc(h.get(), P.cv, h->hCv);
+      push_comps(h.get());
+      // Laplacian and Jacobi preconditioner
+      h->lap.L1 = up_kron(h.get(), P.L1);
+      h->lap.L2 = up_kron(h.get(), P.L2);
+      std::vector<double> dinv(P.Adiag.size());
+      for (size_t k = 0; k < dinv.size(); ++k) dinv[k] = 1.0 / P.Adiag[k];
+      h->lap.dinv = dupload(h.get(), dinv);
+      h->lap.nx = P.Npx;
+      h->lap.ny = P.Npy;
+      h->ox = dupload(h.get(), P.cu.ox);
+      h->oy = dupload(h.get(), P.cu.oy);
+      h->ovx = dupload(h.get(), P.cv.ox);
+      h->ovy = dupload(h.get(), P.cv.oy);
+      h->hx = P.hx[0];
+      h->hy = P.hy[0];
+    } else {
+      h->nux = h->Nx; h->nuy = h->Ny; h->nvx = h->Nx; h->nvy = h->Ny;
+    }
+    if (h->path) {
+      h->nux = h->Nx; h->nuy = h->Ny; h->nvx = h->Nx; h->nvy = h->Ny;
+      h->hx = (d->x[d->Nx] - d->x[0]) / d->Nx;
+      h->hy = (d->y[d->Ny] - d->y[0]) / d->Ny;
+      h->hx = d->x[1] - d->x[0];
+      h->hy = d->y[1] - d->y[0];
+      const double f = (h->path == 4) ? 72.0 : 1.0;
+      std::vector<double> ox(h->Nx, f * h->hx), oy(h->Ny, h->hy);
+      h->ox = dupload(h.get(), ox);
+      h->oy = dupload(h.get(), oy);
+      h->ovx = h->ox;
+      h->ovy = h->oy;
+      if (!h->plan) fill_samples_default(h.get());
+    }
+    h->Nu = h->nux * h->nuy;
+    h->Nv = h->nvx * h->nvy;
+    h->NV = h->Nu + h->Nv;
+    h->Np = h->Nx * h->Ny;
+    if (h->poisson == INS2D_POISSON_FFT) build_fft(h.get(), h->path == 4 ? 4 : 2);
+    // tableau (time_ERK.m:21-32)
+    const int s = d->rk_stages;
+    h->s = s;
+    h->At.assign((size_t)s * s, 0.0);
+    h->ct.assign(s, 0.0);
+    for (int i = 0; i < s; ++i) {
+      for (int j = 0; j < s; ++j) h->At[i * s + j] = (i < s - 1) ? d->rk_A[(i + 1) * s + j] : d->rk_b[j];
+      h->ct[i] = (i < s - 1) ? d->rk_c[i + 1] : 1.0;
+    }
+    for (int i = 0; i < s; ++i)
+      for (int j = i + 1; j < s; ++j)
+        if (h->At[i * s + j] != 0.0) throw ArgError("tableau is not explicit");
+    h->keep.assign(s, false);
+    for (int j = 0; j < s; ++j)
+      for (int i = j + 1; i < s; ++i)
+        if (h->At[i * s + j] != 0.0) h->keep[j] = true;
+    for (int i = 1; i < s; ++i)
+      if (h->ct[i] == 0.0) throw Unsupported("zero stage time increment");
+    // buffers
+    h->V = dalloc<double>(h.get(), h->NV);
+    h->Vb = dalloc<double>(h.get(), h->NV);
+    h->Vt = dalloc<double>(h.get(), h->NV);
+    h->tV = dalloc<double>(h.get(), h->NV);
+    h->kv.assign(s, nullptr);
+    for (int j = 0; j < s; ++j)
+      if (h->keep[j]) h->kv[j] = dalloc<double>(h.get(), h->NV);
+    h->p = dalloc<double>(h.get(), h->Np);
+    h->f = dalloc<double>(h.get(), h->Np);
+    h->dp = dalloc<double>(h.get(), h->Np);
+    HIPCHK(hipMemset(h->V, 0, sizeof(double) * h->NV));
+    HIPCHK(hipMemset(h->p, 0, sizeof(double) * h->Np));
+    if (h->poisson != INS2D_POISSON_FFT) {
+      h->cr = dalloc<double>(h.get(), h->Np);
+      h->cz = dalloc<double>(h.get(), h->Np);
+      h->cp = dalloc<double>(h.get(), h->Np);
+      h->cq = dalloc<double>(h.get(), h->Np);
+      h->d_it = dalloc<int>(h.get(), 1);
+      h->d_rel = dalloc<double>(h.get(), 1);
+      h->cg.nblk = (int)std::min<long>(2048, std::max<long>(1, (h->Np + 255) / 256));
+      h->cg.s = dalloc<double>(h.get(), 8);
+      h->cg.flag = dalloc<int>(h.get(), 4);
+      h->cg.part = dalloc<double>(h.get(), 2 * (size_t)h->cg.nblk);
+      h->cg.maxit = h->maxit;
+    }
+    h->nred = (int)std::min<long>(1024, std::max<long>(1, ((long)std::max(h->NV, h->Np) + 255) / 256));
+    h->part = dalloc<double>(h.get(), 4 * (size_t)h->nred);
+    HIPCHK(hipDeviceSynchronize());
+  } catch (const std::exception& e) {
+    g_create_err = e.what();
+    int code = INS2D_E_ARG;
+    if (dynamic_cast<const HipError*>(&e)) code = INS2D_E_HIP;
+    if (dynamic_cast<const Unsupported*>(&e)) code = INS2D_E_UNSUPPORTED;
+    for (void* p : h->allocs) (void)hipFree(p);
+    for (void* p : h->yv_allocs) (void)hipFree(p);
+    h->allocs.clear();
+    h->yv_allocs.clear();
+    if (h->st) (void)hipStreamDestroy(h->st);
+    h->st = nullptr;
+    return code;
+  }
+  *out = h.release();
+  return INS2D_OK;
+}
+
+void ins2d_destroy(ins2d_handle* h) {
+  if (!h) return;
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  for (auto& pe : h->pend) { h->ev_free.push_back(pe.a); h->ev_free.push_back(pe.b); }
+  for (auto e : h->ev_free) (void)hipEventDestroy(e);
+  for (void* p : h->allocs) (void)hipFree(p);
+  for (void* p : h->yv_allocs) (void)hipFree(p);
+  if (h->st) (void)hipStreamDestroy(h->st);
+  delete h;
+}
+
+int ins2d_grid_info(const ins2d_handle* h, int* out, double* xin, double* yin) {
+  if (!h || !out) return INS2D_E_ARG;
+  out[0] = h->nux; out[1] = h->nuy; out[2] = h->nvx; out[3] = h->nvy;
+  out[4] = h->Nx; out[5] = h->Ny; out[6] = h->path;
+  if (h->plan) {
+    if (xin) std::copy(h->plan->xin.begin(), h->plan->xin.end(), xin);
+    if (yin) std::copy(h->plan->yin.begin(), h->plan->yin.end(), yin);
+  }
+  return INS2D_OK;
+}
+
+int ins2d_set_bc(ins2d_handle* h, double t, const double* const* samples) {
+  if (!h || !samples) return INS2D_E_ARG;
+  (void)t;
+  API_TRY
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (int s = 0; s < S_NSRC; ++s) {
+    if (!samples[s]) throw ArgError("missing boundary sample");
+    std::copy(samples[s], samples[s] + h->samp.s[s].size(), h->samp.s[s].begin());
+  }
+  h->have_bc = true;
+  if (h->plan) {
+    for (void* p : h->yv_allocs) (void)hipFree(p);
+    h->yv_allocs.clear();
+    upload_comp_bc(h, h->plan->cu, h->hCu);
+    upload_comp_bc(h, h->plan->cv, h->hCv);
+    push_comps(h);
+  }
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_set_state(ins2d_handle* h, const double* V, const double* p) {
+  if (!h) return INS2D_E_ARG;
+  API_TRY
+  if (V) HIPCHK(hipMemcpyAsync(h->V, V, sizeof(double) * h->NV, hipMemcpyHostToDevice, h->st));
+  if (p) HIPCHK(hipMemcpyAsync(h->p, p, sizeof(double) * h->Np, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_get_state(ins2d_handle* h, double* V, double* p) {
+  if (!h) return INS2D_E_ARG;
+  API_TRY
+  if (V) HIPCHK(hipMemcpyAsync(V, h->V, sizeof(double) * h->NV, hipMemcpyDeviceToHost, h->st));
+  if (p) HIPCHK(hipMemcpyAsync(p, h->p, sizeof(double) * h->Np, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_rhs(ins2d_handle* h, const double* V, const double* p, double t, double* F, double* maxres) {
+  if (!h || !V || !p) return INS2D_E_ARG;
+  (void)t;
+  API_TRY
+  HIPCHK(hipMemcpyAsync(h->Vb, V, sizeof(double) * h->NV, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->dp, p, sizeof(double) * h->Np, hipMemcpyHostToDevice, h->st));
+  rhs(h, h->Vb, h->dp, RHS_WITH_P, h->tV, kNoCombine, nullptr);
+  if (F) HIPCHK(hipMemcpyAsync(F, h->tV, sizeof(double) * h->NV, hipMemcpyDeviceToHost, h->st));
+  if (maxres) {
+    double a[3], b[3];
+    stats_reduce(h, h->tV, h->nux, h->nuy, nullptr, nullptr, a);
+    stats_reduce(h, h->tV + h->Nu, h->nvx, h->nvy, nullptr, nullptr, b);
+    *maxres = std::max(a[0], b[0]);
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_poisson(ins2d_handle* h, const double* f, double* dp, int* iters, double* relres) {
+  if (!h || !f || !dp) return INS2D_E_ARG;
+  API_TRY
+  HIPCHK(hipMemcpyAsync(h->f, f, sizeof(double) * h->Np, hipMemcpyHostToDevice, h->st));
+  poisson(h, h->f, h->dp);
+  HIPCHK(hipMemcpyAsync(dp, h->dp, sizeof(double) * h->Np, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (iters) *iters = h->poisson == INS2D_POISSON_FFT ? 0 : h->last_it;
+  if (relres) *relres = h->poisson == INS2D_POISSON_FFT ? 0.0 : h->last_rel;
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_input_check(ins2d_handle* h, double t, int p_initial) {
+  if (!h) return INS2D_E_ARG;
+  (void)t;
+  API_TRY
+  // maxdiv = max|M V + yM| (check_conservation.m:37)
+  div(h, h->V, nullptr, 1.0, 0.0, 1.0, 1.0, h->f);
+  double st[3];
+  stats_reduce(h, h->f, h->Nx, h->Ny, nullptr, nullptr, st);
+  if (st[0] > 1e-12) {
+    // input_check.m:104-107: V = V - Om^-1 G A^-1 (M V + yM)
+    poisson(h, h->f, h->dp);
+    HIPCHK(hipMemcpyAsync(h->Vb, h->V, sizeof(double) * h->NV, hipMemcpyDeviceToDevice, h->st));
+    proj(h, h->Vb, nullptr, h->dp, 1.0, 1.0, h->V);
+  }
+  if (p_initial) additional_solve(h, h->V);
+  HIPCHK(hipStreamSynchronize(h->st));
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_step(ins2d_handle* h, double tn, double dt, int nsteps) {
+  if (!h || nsteps < 0 || !(dt > 0)) return INS2D_E_ARG;
+  API_TRY
+  for (int n = 0; n < nsteps; ++n) erk_step(h, tn + n * dt, dt);
+  if (h->prof) prof_collect(h);
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_diagnostics(ins2d_handle* h, double t, double* out) {
+  if (!h || !out) return INS2D_E_ARG;
+  (void)t;
+  API_TRY
+  double a[3], b[3];
+  rhs(h, h->V, h->p, RHS_WITH_P, h->tV, kNoCombine, nullptr);
+  stats_reduce(h, h->tV, h->nux, h->nuy, nullptr, nullptr, a);
+  stats_reduce(h, h->tV + h->Nu, h->nvx, h->nvy, nullptr, nullptr, b);
+  out[0] = std::max(a[0], b[0]);
+  div(h, h->V, nullptr, 1.0, 0.0, 1.0, 1.0, h->f);
+  stats_reduce(h, h->f, h->Nx, h->Ny, nullptr, nullptr, a);
+  out[1] = a[0];
+  stats_reduce(h, h->V, h->nux, h->nuy, h->ox, h->oy, a);
+  stats_reduce(h, h->V + h->Nu, h->nvx, h->nvy, h->ovx, h->ovy, b);
+  double umom = a[1], vmom = b[1], k = 0.5 * a[2] + 0.5 * b[2];
+  // Dirichlet boundary contributions (check_conservation.m:45-76)
+  if (h->plan) {
+    const Plan& P = *h->plan;
+    const auto& S = h->samp.s;
+    if (P.cd.u_left == BC_DIR)
+      for (int j = 0; j < h->Ny; ++j) { umom += S[S_uLe_i][j] * P.hy[j] * P.gx[0]; k += 0.5 * S[S_uLe_i][j] * S[S_uLe_i][j] * P.hy[j] * P.gx[0]; }
+    if (P.cd.u_right == BC_DIR)
+      for (int j = 0; j < h->Ny; ++j) { umom += S[S_uRi_i][j] * P.hy[j] * P.gx[h->Nx]; k += 0.5 * S[S_uRi_i][j] * S[S_uRi_i][j] * P.hy[j] * P.gx[h->Nx]; }
+    if (P.cd.v_low == BC_DIR)
+      for (int i = 0; i < h->Nx; ++i) { vmom += S[S_vLo_i][i] * P.hx[i] * P.gy[0]; k += 0.5 * S[S_vLo_i][i] * S[S_vLo_i][i] * P.hx[i] * P.gy[0]; }
+    if (P.cd.v_up == BC_DIR)
+      for (int i = 0; i < h->Nx; ++i) { vmom += S[S_vUp_i][i] * P.hx[i] * P.gy[h->Ny]; k += 0.5 * S[S_vUp_i][i] * S[S_vUp_i][i] * P.hx[i] * P.gy[h->Ny]; }
+  }
+  out[2] = umom;
+  out[3] = vmom;
+  out[4] = k;
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_solver_stats(const ins2d_handle* h, int* iters, double* relres, long long* total) {
+  if (!h) return INS2D_E_ARG;
+  if (iters) *iters = h->last_it;
+  if (relres) *relres = h->last_rel;
+  if (total) *total = h->total_it;
+  return INS2D_OK;
+}
+
+int ins2d_profile(ins2d_handle* h, int enable) {
+  if (!h) return INS2D_E_ARG;
+  API_TRY
+  prof_collect(h);
+  h->prof = enable != 0;
+  for (int k = 0; k < 4; ++k) { h->prof_ms[k] = 0.0; h->prof_n[k] = 0; }
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_profile_read(const ins2d_handle* h, int kind, double* ms, long long* count) {
+  if (!h || kind < 0 || kind > 3) return INS2D_E_ARG;
+  if (ms) *ms = h->prof_ms[kind];
+  if (count) *count = h->prof_n[kind];
+  return INS2D_OK;
+}
+
+int ins2d_sync(ins2d_handle* h) {
+  if (!h) return INS2D_E_ARG;
+  API_TRY
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (h->prof) prof_collect(h);
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+}  // extern "C"

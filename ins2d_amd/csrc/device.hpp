@@ -1,0 +1,77 @@
+// Device-side descriptors of the kron-factored operators and the kernel entry
+// points shared by the translation units of libins2d_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace ins2d {
+
+// 1-D operator, ELL layout: row r has K (col, val) pairs
+struct DOp1 {
+  int nrows, ncols, K, pad;
+  const int* col;
+  const double* val;
+};
+// kron(Y, X): input field (X.ncols x Y.ncols, x fastest) -> (X.nrows x Y.nrows)
+struct DKron {
+  DOp1 X, Y;
+};
+// y(f, g) = sum_t ax[t][f] * ay[t][g]
+struct DYV {
+  int nt, pad;
+  const double* ax[4];
+  const double* ay[4];
+};
+// D * [(I c + yI) .* (A own + yA)] * scale
+struct DConv {
+  DKron D, I, A;
+  DYV yI, yA;
+  int icomp, pad;
+  double scale;
+};
+// D * (sscale * S own + yS)
+struct DDiff {
+  DKron D, S;
+  DYV yS;
+  double sscale;
+};
+struct DComp {
+  int nx, ny, nconv, ndiff;
+  DConv conv[2];
+  DDiff diff[2];
+  const double* ox;   // volume = ox[i] * oy[j]
+  const double* oy;
+  DKron G;            // gradient of p onto this component
+  DKron M;            // divergence of this component onto p
+  DYV yM;
+};
+struct DLap {        // A = kron(L1.Y, L1.X) + kron(L2.Y, L2.X)
+  DKron L1, L2;
+  const double* dinv;   // 1 / diag(A)
+  int nx, ny;
+};
+
+// ---- stage combination: Vtemp = sum_j a[j] k[j] (+ aself * k_i)
+struct StageCombine {
+  int n;
+  double aself;
+  double a[8];
+  const double* k[8];
+};
+
+// ---- FFT Poisson plan (periodic, uniform)
+struct FFTPlan {
+  int Nx, Ny, Nh;            // Nh = Nx/2 + 1
+  bool pow2x, pow2y;
+  double* twx;               // exp(-2 pi i k / Nx), k < Nx (interleaved re, im)
+  double* twy;
+  double* ax;                // symbol: Ahat(kx, ky) = c * (ax[kx] + ay[ky]), Ahat(0,0) = 1
+  double* ay;
+  double c;
+  double* H;                 // Ny x Nh complex (row transforms)
+  double* HT;                // Nh x Ny complex (transposed)
+};
+
+}  // namespace ins2d

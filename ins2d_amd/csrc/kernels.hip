@@ -1,0 +1,945 @@
+// HIP kernels of libins2d_hip (gfx950, FP64).
+//
+//  * generic kron-factored stencils (any boundary combination, non-uniform
+//    grids, second order): momentum right-hand side with the stage
+//    combination fused in its epilogue, divergence, projection, Laplacian;
+//  * periodic uniform stencils (second and fourth order) with the boundary
+//    closures resolved at compile time;
+//  * hand-written FFT Poisson solve (radix-4/2 Stockham in LDS, direct DFT for
+//    lengths that are not powers of two);
+//  * Jacobi-preconditioned CG (multi-kernel, device-resident scalars, and a
+//    one-workgroup-per-system form for small systems);
+//  * reductions for the per-step diagnostics.
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace ins2d {
+
+// ===========================================================================
+// kron-factored helpers
+// ===========================================================================
+__device__ __forceinline__ double kron_at(const DKron& K, const double* __restrict__ in, int f, int g) {
+  const int KY = K.Y.K, KX = K.X.K, ld = K.X.ncols;
+  const int* cy = K.Y.col + (size_t)g * KY;
+  const double* wy = K.Y.val + (size_t)g * KY;
+  const int* cx = K.X.col + (size_t)f * KX;
+  const double* wx = K.X.val + (size_t)f * KX;
+  double acc = 0.0;
+  for (int a = 0; a < KY; ++a) {
+    const double* row = in + (size_t)cy[a] * ld;
+    double s = 0.0;
+    for (int b = 0; b < KX; ++b) s += wx[b] * row[cx[b]];
+    acc += wy[a] * s;
+  }
+  return acc;
+}
+
+// kron applied to (ca * A + cb * B) without forming the sum
+__device__ __forceinline__ double kron_at2(const DKron& K, const double* __restrict__ A, double ca,
+                                           const double* __restrict__ B, double cb, int f, int g) {
+  const int KY = K.Y.K, KX = K.X.K, ld = K.X.ncols;
+  const int* cy = K.Y.col + (size_t)g * KY;
+  const double* wy = K.Y.val + (size_t)g * KY;
+  const int* cx = K.X.col + (size_t)f * KX;
+  const double* wx = K.X.val + (size_t)f * KX;
+  double acc = 0.0;
+  for (int a = 0; a < KY; ++a) {
+    const size_t ro = (size_t)cy[a] * ld;
+    double s = 0.0;
+    if (B) {
+      for (int b = 0; b < KX; ++b) s += wx[b] * (ca * A[ro + cx[b]] + cb * B[ro + cx[b]]);
+    } else {
+      for (int b = 0; b < KX; ++b) s += wx[b] * (ca * A[ro + cx[b]]);
+    }
+    acc += wy[a] * s;
+  }
+  return acc;
+}
+
+__device__ __forceinline__ double yv_at(const DYV& Y, int f, int g) {
+  double s = 0.0;
+  for (int t = 0; t < Y.nt; ++t) s += Y.ax[t][f] * Y.ay[t][g];
+  return s;
+}
+
+// ===========================================================================
+// generic momentum right-hand side
+//   out = [ -sum conv + sum diff  (- G p if WITH_P) ] (/ volume if SCALE_OM)
+// and, for an RK stage, vtemp = aself * out + sum_j a_j k_j (same point)
+// ===========================================================================
+template <int COMP>
+__global__ void __launch_bounds__(256) k_rhs_generic(const DComp* __restrict__ Cp, const double* __restrict__ u,
+                                                    const double* __restrict__ v, const double* __restrict__ p,
+                                                    int mode, double* __restrict__ out, StageCombine sc,
+                                                    double* __restrict__ vtemp) {
+  const DComp& C = *Cp;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= C.nx || j >= C.ny) return;
+  const double* own = COMP == 0 ? u : v;
+  double acc = 0.0;
+  for (int t = 0; t < C.nconv; ++t) {
+    const DConv& T = C.conv[t];
+    const double* cf = T.icomp == 0 ? u : v;
+    const int KDY = T.D.Y.K, KDX = T.D.X.K;
+    double s = 0.0;
+    for (int a = 0; a < KDY; ++a) {
+      const int g = T.D.Y.col[(size_t)j * KDY + a];
+      const double dy = T.D.Y.val[(size_t)j * KDY + a];
+      if (dy == 0.0) continue;
+      double sx = 0.0;
+      for (int b = 0; b < KDX; ++b) {
+        const int f = T.D.X.col[(size_t)i * KDX + b];
+        const double dx = T.D.X.val[(size_t)i * KDX + b];
+        if (dx == 0.0) continue;
+        const double fi = kron_at(T.I, cf, f, g) + yv_at(T.yI, f, g);
+        const double fa = kron_at(T.A, own, f, g) + yv_at(T.yA, f, g);
+        sx += dx * (fi * fa);
+      }
+      s += dy * sx;
+    }
+    acc -= T.scale * s;
+  }
+  for (int t = 0; t < C.ndiff; ++t) {
+    const DDiff& T = C.diff[t];
+    const int KDY = T.D.Y.K, KDX = T.D.X.K;
+    double s = 0.0;
+    for (int a = 0; a < KDY; ++a) {
+      const int g = T.D.Y.col[(size_t)j * KDY + a];
+      const double dy = T.D.Y.val[(size_t)j * KDY + a];
+      if (dy == 0.0) continue;
+      double sx = 0.0;
+      for (int b = 0; b < KDX; ++b) {
+        const int f = T.D.X.col[(size_t)i * KDX + b];
+        const double dx = T.D.X.val[(size_t)i * KDX + b];
+        if (dx == 0.0) continue;
+        sx += dx * (T.sscale * kron_at(T.S, own, f, g) + yv_at(T.yS, f, g));
+      }
+      s += dy * sx;
+    }
+    acc += s;
+  }
+  if (mode & RHS_WITH_P) acc -= kron_at(C.G, p, i, j);
+  if (mode & RHS_SCALE_OM) acc = acc / (C.ox[i] * C.oy[j]);
+  const size_t idx = (size_t)i + (size_t)C.nx * j;
+  if (out) out[idx] = acc;
+  if (vtemp) {
+    double vt = sc.aself * acc;
+    for (int q = 0; q < sc.n; ++q) vt += sc.a[q] * sc.k[q][idx];
+    vtemp[idx] = vt;
+  }
+}
+
+// f = oscale * ( Mu (ca A_u + cb B_u) + Mv (ca A_v + cb B_v) + ycoef yM )
+__global__ void __launch_bounds__(256) k_div_generic(const DComp* __restrict__ Cu, const DComp* __restrict__ Cv,
+                                                    int npx, int npy, const double* __restrict__ A,
+                                                    const double* __restrict__ B, double ca, double cb,
+                                                    double ycoef, double oscale, int nu, double* __restrict__ f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= npx || j >= npy) return;
+  double s = kron_at2(Cu->M, A, ca, B, cb, i, j);
+  s += kron_at2(Cv->M, A + nu, ca, B ? B + nu : nullptr, cb, i, j);
+  if (ycoef != 0.0) s += ycoef * (yv_at(Cu->yM, i, j) + yv_at(Cv->yM, i, j));
+  f[(size_t)i + (size_t)npx * j] = oscale * s;
+}
+
+// V = Vn + dt * (Vtemp - c * Om^-1 G dp)      (time_ERK.m:132)
+template <int COMP>
+__global__ void __launch_bounds__(256) k_proj_generic(const DComp* __restrict__ Cp, const double* __restrict__ Vn,
+                                                     const double* __restrict__ Vt, const double* __restrict__ dp,
+                                                     double dt, double c, double* __restrict__ V) {
+  const DComp& C = *Cp;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= C.nx || j >= C.ny) return;
+  const size_t idx = (size_t)i + (size_t)C.nx * j;
+  const double g = kron_at(C.G, dp, i, j) / (C.ox[i] * C.oy[j]);
+  const double vt = Vt ? Vt[idx] : 0.0;
+  V[idx] = Vn[idx] + dt * (vt - c * g);
+}
+
+// ===========================================================================
+// periodic uniform stencils (second and fourth order)
+// ===========================================================================
+struct PerGrid {
+  int Nx, Ny;
+  double hx, hy, invRe;
+};
+
+__device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// second order, one thread computes both components at (i, j)
+// (interior formulas of the operators in operator_averaging.m,
+// operator_interpolation.m and operator_convection_diffusion.m specialised to
+// all-periodic uniform grids)
+__device__ __forceinline__ void rhs_per2(const PerGrid& G, const double* __restrict__ u, const double* __restrict__ v,
+                                         int i, int j, double& fu, double& fv) {
+  const int Nx = G.Nx;
+  const int im = wrap(i - 1, Nx), ip = wrap(i + 1, Nx);
+  const int jm = wrap(j - 1, G.Ny), jp = wrap(j + 1, G.Ny);
+  const size_t r0 = (size_t)Nx * j, rm = (size_t)Nx * jm, rp = (size_t)Nx * jp;
+  const double hx = G.hx, hy = G.hy;
+  const double u0 = u[i + r0], ue = u[ip + r0], uw = u[im + r0], un = u[i + rp], us = u[i + rm];
+  const double v0 = v[i + r0], ve = v[ip + r0], vw = v[im + r0], vn = v[i + rp], vs = v[i + rm];
+  const double v_wn = v[im + rp];   // v(i-1, j+1)
+  const double u_se = u[ip + rm];   // u(i+1, j-1)
+  // u-momentum
+  const double Ue = 0.5 * (u0 + ue), Uw = 0.5 * (uw + u0);
+  const double du2dx = (hy * Ue) * Ue - (hy * Uw) * Uw;
+  const double Vn = 0.5 * hx * (v_wn + vn), Vs = 0.5 * hx * (vw + v0);
+  const double duvdy = Vn * (0.5 * (u0 + un)) - Vs * (0.5 * (us + u0));
+  const double d2u = hy * ((ue - u0) / hx - (u0 - uw) / hx) + hx * ((un - u0) / hy - (u0 - us) / hy);
+  fu = -(du2dx + duvdy) + G.invRe * d2u;
+  // v-momentum
+  const double Ue2 = 0.5 * hy * (u_se + ue), Uw2 = 0.5 * hy * (us + u0);
+  const double duvdx = Ue2 * (0.5 * (v0 + ve)) - Uw2 * (0.5 * (vw + v0));
+  const double Vn2 = 0.5 * (v0 + vn), Vs2 = 0.5 * (vs + v0);
+  const double dv2dy = (hx * Vn2) * Vn2 - (hx * Vs2) * Vs2;
+  const double d2v = hy * ((ve - v0) / hx - (v0 - vw) / hx) + hx * ((vn - v0) / hy - (v0 - vs) / hy);
+  fv = -(duvdx + dv2dy) + G.invRe * d2v;
+}
+
+// fourth order (alfa = 81, beta = 9/8): alfa * (1h operators) - (3h operators)
+// (convection.m:38-43, operator_convection_diffusion.m:463-470)
+__device__ __forceinline__ void rhs_per4(const PerGrid& G, const double* __restrict__ u, const double* __restrict__ v,
+                                         int i, int j, double& fu, double& fv) {
+  const int Nx = G.Nx, Ny = G.Ny;
+  const double hx = G.hx, hy = G.hy;
+  const double w1 = 9.0 / 16.0, w2 = -1.0 / 16.0;   // beta/2, (1-beta)/2
+  int ix[8], jy[8];   // offsets -3..+4
+#pragma unroll
+  for (int k = 0; k < 8; ++k) { ix[k] = wrap(i + k - 3, Nx); jy[k] = wrap(j + k - 3, Ny); }
+#define U(a, b) u[ix[(a) + 3] + (size_t)Nx * jy[(b) + 3]]
+#define V(a, b) v[ix[(a) + 3] + (size_t)Nx * jy[(b) + 3]]
+  // ---------------- u-momentum ----------------
+  // x: 1h faces f = i (between u_{i-1}, u_i) and i+1; 3h faces at i-3/2, i+3/2
+  auto Ix1 = [&](int f) { return hy * (w2 * U(f - 2, 0) + w1 * U(f - 1, 0) + w1 * U(f, 0) + w2 * U(f + 1, 0)); };
+  auto Ax1 = [&](int f) { return 0.5 * (U(f - 1, 0) + U(f, 0)); };
+  const double du2dx1 = Ix1(1) * Ax1(1) - Ix1(0) * Ax1(0);
+  auto Ix3 = [&](int r) { return 3.0 * hy * (w2 * U(r - 3, 0) + w1 * U(r - 2, 0) + w1 * U(r - 1, 0) + w2 * U(r, 0)); };
+  auto Ax3 = [&](int r) { return 0.5 * (U(r - 3, 0) + U(r, 0)); };
+  const double du2dx3 = Ix3(3) * Ax3(3) - Ix3(0) * Ax3(0);
+  // y: 1h faces g = j, j+1 (grid lines y_j, y_{j+1}); v interpolated in x
+  auto Vf1 = [&](int g) { return hx * (w2 * V(-2, g) + w1 * V(-1, g) + w1 * V(0, g) + w2 * V(1, g)); };
+  auto Ay1 = [&](int g) { return 0.5 * (U(0, g - 1) + U(0, g)); };
+  const double duvdy1 = Vf1(1) * Ay1(1) - Vf1(0) * Ay1(0);
+  // 3h faces g = j, j+3 on grid lines y_{g-1}
+  auto Vf3 = [&](int g) { return 3.0 * hx * (w2 * V(-2, g - 1) + w1 * V(-1, g - 1) + w1 * V(0, g - 1) + w2 * V(1, g - 1)); };
+  auto Ay3 = [&](int g) { return 0.5 * (U(0, g - 3) + U(0, g)); };
+  const double duvdy3 = Vf3(3) * Ay3(3) - Vf3(0) * Ay3(0);
+  const double convu = 81.0 * (du2dx1 + duvdy1) - (du2dx3 + duvdy3);
+  // diffusion: Phi_{k+1/2} = [81 (u_{k+1}-u_k) - 3 (u_{k+2}-u_{k-1})] / (72 h)
+  auto Px = [&](int k) { return (81.0 * (U(k + 1, 0) - U(k, 0)) - 3.0 * (U(k + 2, 0) - U(k - 1, 0))) / (72.0 * hx); };
+  auto Py = [&](int k) { return (81.0 * (U(0, k + 1) - U(0, k)) - 3.0 * (U(0, k + 2) - U(0, k - 1))) / (72.0 * hy); };
+  const double d2u = hy * (81.0 * (Px(0) - Px(-1)) - 3.0 * (Px(1) - Px(-2))) +
+                     hx * (81.0 * (Py(0) - Py(-1)) - 3.0 * (Py(1) - Py(-2)));
+  fu = -convu + G.invRe * d2u;
+  // ---------------- v-momentum ----------------
+  auto Uf1 = [&](int f) { return hy * (w2 * U(f, -2) + w1 * U(f, -1) + w1 * U(f, 0) + w2 * U(f, 1)); };
+  auto Bx1 = [&](int f) { return 0.5 * (V(f - 1, 0) + V(f, 0)); };
+  const double duvdx1 = Uf1(1) * Bx1(1) - Uf1(0) * Bx1(0);
+  auto Uf3 = [&](int f) { return 3.0 * hy * (w2 * U(f - 1, -2) + w1 * U(f - 1, -1) + w1 * U(f - 1, 0) + w2 * U(f - 1, 1)); };
+  auto Bx3 = [&](int f) { return 0.5 * (V(f - 3, 0) + V(f, 0)); };
+  const double duvdx3 = Uf3(3) * Bx3(3) - Uf3(0) * Bx3(0);
+  auto Iy1 = [&](int g) { return hx * (w2 * V(0, g - 2) + w1 * V(0, g - 1) + w1 * V(0, g) + w2 * V(0, g + 1)); };
+  auto By1 = [&](int g) { return 0.5 * (V(0, g - 1) + V(0, g)); };
+  const double dv2dy1 = Iy1(1) * By1(1) - Iy1(0) * By1(0);
+  auto Iy3 = [&](int g) { return 3.0 * hx * (w2 * V(0, g - 3) + w1 * V(0, g - 2) + w1 * V(0, g - 1) + w2 * V(0, g)); };
+  auto By3 = [&](int g) { return 0.5 * (V(0, g - 3) + V(0, g)); };
+  const double dv2dy3 = Iy3(3) * By3(3) - Iy3(0) * By3(0);
+  const double convv = 81.0 * (duvdx1 + dv2dy1) - (duvdx3 + dv2dy3);
+  auto Qx = [&](int k) { return (81.0 * (V(k + 1, 0) - V(k, 0)) - 3.0 * (V(k + 2, 0) - V(k - 1, 0))) / (72.0 * hx); };
+  auto Qy = [&](int k) { return (81.0 * (V(0, k + 1) - V(0, k)) - 3.0 * (V(0, k + 2) - V(0, k - 1))) / (72.0 * hy); };
+  const double d2v = hy * (81.0 * (Qx(0) - Qx(-1)) - 3.0 * (Qx(1) - Qx(-2))) +
+                     hx * (81.0 * (Qy(0) - Qy(-1)) - 3.0 * (Qy(1) - Qy(-2)));
+  fv = -convv + G.invRe * d2v;
+#undef U
+#undef V
+}
+
+template <int ORDER>
+__device__ __forceinline__ void grad_per(const PerGrid& G, const double* __restrict__ p, int i, int j, double& gx,
+                                         double& gy) {
+  const int Nx = G.Nx, Ny = G.Ny;
+  const size_t r0 = (size_t)Nx * j;
+  const int im = wrap(i - 1, Nx);
+  const int jm = wrap(j - 1, Ny);
+  const double p0 = p[i + r0];
+  if (ORDER == 2) {
+    gx = G.hy * (p0 - p[im + r0]);
+    gy = G.hx * (p0 - p[i + (size_t)Nx * jm]);
+  } else {
+    const int ip = wrap(i + 1, Nx), imm = wrap(i - 2, Nx);
+    const int jp = wrap(j + 1, Ny), jmm = wrap(j - 2, Ny);
+    gx = 81.0 * G.hy * (p0 - p[im + r0]) - 3.0 * G.hy * (p[ip + r0] - p[imm + r0]);
+    gy = 81.0 * G.hx * (p0 - p[i + (size_t)Nx * jm]) - 3.0 * G.hx * (p[i + (size_t)Nx * jp] - p[i + (size_t)Nx * jmm]);
+  }
+}
+
+template <int ORDER>
+__global__ void __launch_bounds__(256) k_rhs_per(PerGrid G, const double* __restrict__ u, const double* __restrict__ v,
+                                                const double* __restrict__ p, int mode, double* __restrict__ out,
+                                                StageCombine sc, double* __restrict__ vtemp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= G.Nx || j >= G.Ny) return;
+  double fu, fv;
+  if (ORDER == 2) rhs_per2(G, u, v, i, j, fu, fv);
+  else rhs_per4(G, u, v, i, j, fu, fv);
+  if (mode & RHS_WITH_P) {
+    double gx, gy;
+    grad_per<ORDER>(G, p, i, j, gx, gy);
+    fu -= gx;
+    fv -= gy;
+  }
+  if (mode & RHS_SCALE_OM) {
+    const double om = (ORDER == 2 ? 1.0 : 72.0) * G.hx * G.hy;
+    fu /= om;
+    fv /= om;
+  }
+  const size_t N = (size_t)G.Nx * G.Ny;
+  const size_t idx = (size_t)i + (size_t)G.Nx * j;
+  if (out) { out[idx] = fu; out[N + idx] = fv; }
+  if (vtemp) {
+    double a = sc.aself * fu, b = sc.aself * fv;
+    for (int q = 0; q < sc.n; ++q) { a += sc.a[q] * sc.k[q][idx]; b += sc.a[q] * sc.k[q][N + idx]; }
+    vtemp[idx] = a;
+    vtemp[N + idx] = b;
+  }
+}
+
+template <int ORDER>
+__global__ void __launch_bounds__(256) k_div_per(PerGrid G, const double* __restrict__ A, const double* __restrict__ B,
+                                                double ca, double cb, double oscale, double* __restrict__ f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= G.Nx || j >= G.Ny) return;
+  const int Nx = G.Nx, Ny = G.Ny;
+  const size_t N = (size_t)Nx * Ny;
+  auto W = [&](size_t k) { return B ? ca * A[k] + cb * B[k] : ca * A[k]; };
+  const int ip = wrap(i + 1, Nx), jp = wrap(j + 1, Ny);
+  const size_t r0 = (size_t)Nx * j;
+  double s;
+  if (ORDER == 2) {
+    s = G.hy * (W(ip + r0) - W(i + r0)) + G.hx * (W(N + i + (size_t)Nx * jp) - W(N + i + r0));
+  } else {
+    const int ipp = wrap(i + 2, Nx), im = wrap(i - 1, Nx), jpp = wrap(j + 2, Ny), jm = wrap(j - 1, Ny);
+    s = 81.0 * G.hy * (W(ip + r0) - W(i + r0)) - 3.0 * G.hy * (W(ipp + r0) - W(im + r0)) +
+        81.0 * G.hx * (W(N + i + (size_t)Nx * jp) - W(N + i + r0)) -
+        3.0 * G.hx * (W(N + i + (size_t)Nx * jpp) - W(N + i + (size_t)Nx * jm));
+  }
+  f[(size_t)i + r0] = oscale * s;
+}
+
+template <int ORDER>
+__global__ void __launch_bounds__(256) k_proj_per(PerGrid G, const double* __restrict__ Vn, const double* __restrict__ Vt,
+                                                 const double* __restrict__ dp, double dt, double c,
+                                                 double* __restrict__ V) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= G.Nx || j >= G.Ny) return;
+  const size_t N = (size_t)G.Nx * G.Ny;
+  const size_t idx = (size_t)i + (size_t)G.Nx * j;
+  double gx, gy;
+  grad_per<ORDER>(G, dp, i, j, gx, gy);
+  const double om = (ORDER == 2 ? 1.0 : 72.0) * G.hx * G.hy;
+  const double tu = Vt ? Vt[idx] : 0.0, tv = Vt ? Vt[N + idx] : 0.0;
+  V[idx] = Vn[idx] + dt * (tu - c * (gx / om));
+  V[N + idx] = Vn[N + idx] + dt * (tv - c * (gy / om));
+}
+
+// ===========================================================================
+// FFT Poisson solve (periodic, uniform):  A dp = f  <=>  -Ahat dphat = fhat
+// (solvers/pressure/pressure_poisson.m:92-104)
+// ===========================================================================
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 conjd(double2 a) { return make_double2(a.x, -a.y); }
+
+// in-place FFT of z[0..N) in LDS; INV selects exp(+2 pi i k n / N) (unscaled)
+template <bool INV>
+__device__ void lds_fft(double2* z, int N, int log2N, const double2* __restrict__ tw, double2* scratch) {
+  const int T = blockDim.x;
+  if (N == 1) return;
+  // radix-4 Stockham passes, then one radix-2 pass if log2N is odd
+  int Ns = 1;
+  int passes4 = log2N / 2;
+  for (int pass = 0; pass < passes4; ++pass) {
+    const int Q = N / 4;
+    double2 vals[16];
+    int cnt = 0;
+    for (int jq = threadIdx.x; jq < Q; jq += T) {
+      const int k = jq % Ns;
+      double2 a0 = z[jq], a1 = z[jq + Q], a2 = z[jq + 2 * Q], a3 = z[jq + 3 * Q];
+      const int step = N / (Ns * 4);
+      int m1 = (k * step) % N, m2 = (2 * k * step) % N, m3 = (3 * k * step) % N;
+      double2 t1 = tw[m1], t2 = tw[m2], t3 = tw[m3];
+      if (INV) { t1 = conjd(t1); t2 = conjd(t2); t3 = conjd(t3); }
+      a1 = cmul(a1, t1); a2 = cmul(a2, t2); a3 = cmul(a3, t3);
+      const double2 s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = csub(a1, a3);
+      // -i * d13 (forward) or +i * d13 (inverse)
+      const double2 jd13 = INV ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
+      vals[cnt * 4 + 0] = cadd(s02, s13);
+      vals[cnt * 4 + 1] = cadd(d02, jd13);
+      vals[cnt * 4 + 2] = csub(s02, s13);
+      vals[cnt * 4 + 3] = csub(d02, jd13);
+      ++cnt;
+      if (cnt == 4) break;   // at most 4 butterflies per thread per pass (T >= N/16)
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int jq = threadIdx.x; jq < Q; jq += T) {
+      const int k = jq % Ns;
+      const int o = (jq / Ns) * Ns * 4 + k;
+      z[o] = vals[cnt * 4 + 0];
+      z[o + Ns] = vals[cnt * 4 + 1];
+      z[o + 2 * Ns] = vals[cnt * 4 + 2];
+      z[o + 3 * Ns] = vals[cnt * 4 + 3];
+      ++cnt;
+      if (cnt == 4) break;
+    }
+    __syncthreads();
+    Ns *= 4;
+  }
+  if (log2N & 1) {
+    const int Q = N / 2;
+    double2 vals[16];
+    int cnt = 0;
+    for (int jq = threadIdx.x; jq < Q; jq += T) {
+      const int k = jq % Ns;
+      double2 a0 = z[jq], a1 = z[jq + Q];
+      double2 t1 = tw[(k * (N / (Ns * 2))) % N];
+      if (INV) t1 = conjd(t1);
+      a1 = cmul(a1, t1);
+      vals[cnt * 2 + 0] = cadd(a0, a1);
+      vals[cnt * 2 + 1] = csub(a0, a1);
+      ++cnt;
+      if (cnt == 8) break;
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int jq = threadIdx.x; jq < Q; jq += T) {
+      const int k = jq % Ns;
+      const int o = (jq / Ns) * Ns * 2 + k;
+      z[o] = vals[cnt * 2 + 0];
+      z[o + Ns] = vals[cnt * 2 + 1];
+      ++cnt;
+      if (cnt == 8) break;
+    }
+    __syncthreads();
+  }
+  (void)scratch;
+}
+
+// direct DFT for lengths that are not powers of two (O(N^2), N <= 4096)
+template <bool INV>
+__device__ void lds_dft(double2* z, double2* out, int N, const double2* __restrict__ tw) {
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    double2 acc = make_double2(0.0, 0.0);
+    int m = 0;
+    for (int n = 0; n < N; ++n) {
+      double2 t = tw[m];
+      if (INV) t = conjd(t);
+      acc = cadd(acc, cmul(z[n], t));
+      m += k;
+      if (m >= N) m -= N;
+    }
+    out[k] = acc;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < N; k += blockDim.x) z[k] = out[k];
+  __syncthreads();
+}
+
+template <bool INV>
+__device__ __forceinline__ void lds_transform(double2* z, double2* scratch, int N, bool pow2, int log2N,
+                                              const double2* __restrict__ tw) {
+  if (pow2) lds_fft<INV>(z, N, log2N, tw, scratch);
+  else lds_dft<INV>(z, scratch, N, tw);
+}
+
+// rows y0 = 2b and y1 = 2b+1 packed as one complex row; half spectra out
+__global__ void __launch_bounds__(256) k_fft_r2c_rows(int Nx, int Ny, int Nh, bool pow2, int log2N,
+                                                     const double2* __restrict__ tw, const double* __restrict__ f,
+                                                     double2* __restrict__ H) {
+  extern __shared__ double2 sm[];
+  double2* z = sm;
+  double2* scratch = sm + Nx;
+  const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
+  const bool has1 = y1 < Ny;
+  for (int n = threadIdx.x; n < Nx; n += blockDim.x)
+    z[n] = make_double2(f[(size_t)y0 * Nx + n], has1 ? f[(size_t)y1 * Nx + n] : 0.0);
+  __syncthreads();
+  lds_transform<false>(z, scratch, Nx, pow2, log2N, tw);
+  for (int k = threadIdx.x; k < Nh; k += blockDim.x) {
+    const double2 a = z[k], b = conjd(z[(Nx - k) % Nx]);
+    H[(size_t)y0 * Nh + k] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+    // (a - b) / (2i) = -i/2 (a - b)
+    if (has1) H[(size_t)y1 * Nh + k] = make_double2(0.5 * (a.y - b.y), -0.5 * (a.x - b.x));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fft_c2r_rows(int Nx, int Ny, int Nh, bool pow2, int log2N,
+                                                     const double2* __restrict__ tw, const double2* __restrict__ H,
+                                                     double* __restrict__ out) {
+  extern __shared__ double2 sm[];
+  double2* z = sm;
+  double2* scratch = sm + Nx;
+  const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
+  const bool has1 = y1 < Ny;
+  for (int k = threadIdx.x; k < Nx; k += blockDim.x) {
+    double2 a, b;
+    if (k < Nh) {
+      a = H[(size_t)y0 * Nh + k];
+      b = has1 ? H[(size_t)y1 * Nh + k] : make_double2(0.0, 0.0);
+    } else {
+      a = conjd(H[(size_t)y0 * Nh + (Nx - k)]);
+      b = has1 ? conjd(H[(size_t)y1 * Nh + (Nx - k)]) : make_double2(0.0, 0.0);
+    }
+    if (k == 0 || (2 * k == Nx)) { a.y = 0.0; b.y = 0.0; }
+    // Z = a + i b
+    z[k] = make_double2(a.x - b.y, a.y + b.x);
+  }
+  __syncthreads();
+  lds_transform<true>(z, scratch, Nx, pow2, log2N, tw);
+  for (int n = threadIdx.x; n < Nx; n += blockDim.x) {
+    out[(size_t)y0 * Nx + n] = z[n].x;
+    if (has1) out[(size_t)y1 * Nx + n] = z[n].y;
+  }
+}
+
+// complex transpose (rows x cols) -> (cols x rows), 32x32 tiles
+__global__ void k_transpose_c(const double2* __restrict__ in, double2* __restrict__ out, int rows, int cols) {
+  __shared__ double2 tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  for (int dy = threadIdx.y; dy < 32; dy += blockDim.y) {
+    const int r = by + dy, c = bx + threadIdx.x;
+    if (r < rows && c < cols) tile[dy][threadIdx.x] = in[(size_t)r * cols + c];
+  }
+  __syncthreads();
+  for (int dy = threadIdx.y; dy < 32; dy += blockDim.y) {
+    const int c = bx + dy, r = by + threadIdx.x;
+    if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[threadIdx.x][dy];
+  }
+}
+
+// one block per kx: forward FFT along y, divide by -Ahat, inverse FFT along y
+__global__ void __launch_bounds__(256) k_fft_cols_solve(int Nx, int Ny, bool pow2, int log2N,
+                                                       const double2* __restrict__ tw, const double* __restrict__ ax,
+                                                       const double* __restrict__ ay, double c,
+                                                       double2* __restrict__ HT) {
+  extern __shared__ double2 sm[];
+  double2* z = sm;
+  double2* scratch = sm + Ny;
+  const int kx = blockIdx.x;
+  double2* row = HT + (size_t)kx * Ny;
+  for (int n = threadIdx.x; n < Ny; n += blockDim.x) z[n] = row[n];
+  __syncthreads();
+  lds_transform<false>(z, scratch, Ny, pow2, log2N, tw);
+  const double scale = 1.0 / ((double)Nx * (double)Ny);
+  for (int ky = threadIdx.x; ky < Ny; ky += blockDim.x) {
+    double Ah = c * (ax[kx] + ay[ky]);
+    if (kx == 0 && ky == 0) Ah = 1.0;
+    const double s = -scale / Ah;
+    z[ky] = make_double2(z[ky].x * s, z[ky].y * s);
+  }
+  __syncthreads();
+  lds_transform<true>(z, scratch, Ny, pow2, log2N, tw);
+  for (int n = threadIdx.x; n < Ny; n += blockDim.x) row[n] = z[n];
+}
+
+// ===========================================================================
+// Laplacian and Jacobi-PCG (cg.F:158-280 stopping rule ||r||^2/||f||^2 <= tol^2)
+// ===========================================================================
+__device__ __forceinline__ double lap_at(const DLap& L, const double* __restrict__ p, int i, int j) {
+  return kron_at(L.L1, p, i, j) + kron_at(L.L2, p, i, j);
+}
+
+__global__ void __launch_bounds__(256) k_lap(DLap L, const double* __restrict__ p, double* __restrict__ q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= L.nx || j >= L.ny) return;
+  q[(size_t)i + (size_t)L.nx * j] = lap_at(L, p, i, j);
+}
+
+// block reduction of a double (sum) -> returns on thread 0
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+  return s;
+}
+__device__ __forceinline__ double block_max(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s = fmax(s, red[w]);
+  return s;
+}
+
+// CG state on the device: s[0]=rz, s[1]=nf2 (|f|^2), s[2]=alpha, s[3]=beta,
+// s[4]=relres^2, s[5]=tol^2 ; flag[0]=done, flag[1]=iterations, flag[2]=failed
+struct CGState {
+  double* s;
+  int* flag;
+  double* part;   // per-block partial sums (2 per block)
+  int nblk;
+  int maxit;
+};
+
+// 1-D grid over n points, 256 threads: q = A p on (i, j) = (idx % nx, idx / nx)
+__global__ void __launch_bounds__(256) k_cg_apply(DLap L, CGState S, const double* __restrict__ p,
+                                                 double* __restrict__ q) {
+  __shared__ double red[4];
+  if (S.flag[0]) return;
+  const size_t n = (size_t)L.nx * L.ny;
+  double acc = 0.0;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % L.nx), j = (int)(idx / L.nx);
+    const double a = lap_at(L, p, i, j);
+    q[idx] = a;
+    acc += p[idx] * a;
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) S.part[blockIdx.x] = s;
+}
+
+__global__ void k_cg_alpha(CGState S) {
+  __shared__ double red[4];
+  if (S.flag[0]) return;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < S.nblk; b += blockDim.x) acc += S.part[b];
+  const double pq = block_sum(acc, red);
+  if (threadIdx.x == 0) S.s[2] = S.s[0] / pq;
+}
+
+__global__ void __launch_bounds__(256) k_cg_update(DLap L, CGState S, const double* __restrict__ p,
+                                                  const double* __restrict__ q, double* __restrict__ x,
+                                                  double* __restrict__ r, double* __restrict__ z) {
+  __shared__ double red[4];
+  if (S.flag[0]) return;
+  const size_t n = (size_t)L.nx * L.ny;
+  const double alpha = S.s[2];
+  double rr = 0.0, rz = 0.0;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    x[idx] += alpha * p[idx];
+    const double rn = r[idx] - alpha * q[idx];
+    r[idx] = rn;
+    const double zn = L.dinv[idx] * rn;
+    z[idx] = zn;
+    rr += rn * rn;
+    rz += rn * zn;
+  }
+  const double a = block_sum(rr, red);
+  const double b = block_sum(rz, red);
+  if (threadIdx.x == 0) { S.part[2 * blockIdx.x] = a; S.part[2 * blockIdx.x + 1] = b; }
+}
+
+__global__ void k_cg_beta(CGState S) {
+  __shared__ double red[4];
+  if (S.flag[0]) return;
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < S.nblk; k += blockDim.x) { a += S.part[2 * k]; b += S.part[2 * k + 1]; }
+  const double rr = block_sum(a, red);
+  const double rz = block_sum(b, red);
+  if (threadIdx.x == 0) {
+    const int it = S.flag[1] + 1;
+    S.flag[1] = it;
+    const double rel = rr / S.s[1];
+    S.s[4] = rel;
+    if (rel <= S.s[5]) {
+      S.flag[0] = 1;
+    } else if (it >= S.maxit) {
+      S.flag[0] = 1;
+      S.flag[2] = 1;
+    }
+    S.s[3] = rz / S.s[0];
+    S.s[0] = rz;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_cg_dir(CGState S, size_t n, const double* __restrict__ z,
+                                               double* __restrict__ p) {
+  if (S.flag[0]) return;
+  const double beta = S.s[3];
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x)
+    p[idx] = z[idx] + beta * p[idx];
+}
+
+// start from x0: r = f - A x0, z = dinv r, p = z; partials of f.f and r.z
+__global__ void __launch_bounds__(256) k_cg_init(DLap L, CGState S, const double* __restrict__ f,
+                                                const double* __restrict__ x, double* __restrict__ r,
+                                                double* __restrict__ z, double* __restrict__ p, int x0_zero) {
+  __shared__ double red[4];
+  const size_t n = (size_t)L.nx * L.ny;
+  double ff = 0.0, rz = 0.0;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % L.nx), j = (int)(idx / L.nx);
+    const double rn = x0_zero ? f[idx] : f[idx] - lap_at(L, x, i, j);
+    r[idx] = rn;
+    const double zn = L.dinv[idx] * rn;
+    z[idx] = zn;
+    p[idx] = zn;
+    ff += f[idx] * f[idx];
+    rz += rn * zn;
+  }
+  const double a = block_sum(ff, red);
+  const double b = block_sum(rz, red);
+  if (threadIdx.x == 0) { S.part[2 * blockIdx.x] = a; S.part[2 * blockIdx.x + 1] = b; }
+}
+
+__global__ void k_cg_init_fin(CGState S, double tol2) {
+  __shared__ double red[4];
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < S.nblk; k += blockDim.x) { a += S.part[2 * k]; b += S.part[2 * k + 1]; }
+  const double ff = block_sum(a, red);
+  const double rz = block_sum(b, red);
+  if (threadIdx.x == 0) {
+    S.s[0] = rz;
+    S.s[1] = ff;
+    S.s[4] = 1.0;
+    S.s[5] = tol2;
+    S.flag[1] = 0;
+    S.flag[2] = 0;
+    // cg.F:222: an already tiny right-hand side returns x0 (absolute test)
+    S.flag[0] = (ff <= tol2) ? 1 : 0;
+  }
+}
+
+// one workgroup per system: the whole PCG loop in one launch (small systems;
+// vectors in global memory, reductions in LDS)
+__global__ void __launch_bounds__(1024) k_cg_single(DLap L, const double* __restrict__ f, double* __restrict__ x,
+                                                   double* __restrict__ r, double* __restrict__ z,
+                                                   double* __restrict__ p, double* __restrict__ q, double tol2,
+                                                   int maxit, int x0_zero, int* __restrict__ out_it,
+                                                   double* __restrict__ out_rel) {
+  __shared__ double red[16];
+  __shared__ double bc[2];
+  const int n = L.nx * L.ny;
+  const int T = blockDim.x;
+  double ff = 0.0, rz = 0.0;
+  for (int idx = threadIdx.x; idx < n; idx += T) {
+    const int i = idx % L.nx, j = idx / L.nx;
+    const double rn = x0_zero ? f[idx] : f[idx] - lap_at(L, x, i, j);
+    if (x0_zero) x[idx] = 0.0;
+    r[idx] = rn;
+    const double zn = L.dinv[idx] * rn;
+    z[idx] = zn;
+    p[idx] = zn;
+    ff += f[idx] * f[idx];
+    rz += rn * zn;
+  }
+  ff = block_sum(ff, red);
+  rz = block_sum(rz, red);
+  if (threadIdx.x == 0) { bc[0] = ff; bc[1] = rz; }
+  __syncthreads();
+  ff = bc[0];
+  rz = bc[1];
+  if (ff <= tol2) {
+    if (threadIdx.x == 0) { *out_it = 0; *out_rel = 0.0; }
+    return;
+  }
+  int it = 0;
+  double rel = 1.0;
+  while (true) {
+    double pq = 0.0;
+    for (int idx = threadIdx.x; idx < n; idx += T) {
+      const int i = idx % L.nx, j = idx / L.nx;
+      const double a = lap_at(L, p, i, j);
+      q[idx] = a;
+      pq += p[idx] * a;
+    }
+    pq = block_sum(pq, red);
+    if (threadIdx.x == 0) bc[0] = pq;
+    __syncthreads();
+    const double alpha = rz / bc[0];
+    double rr = 0.0, rzn = 0.0;
+    for (int idx = threadIdx.x; idx < n; idx += T) {
+      x[idx] += alpha * p[idx];
+      const double rn = r[idx] - alpha * q[idx];
+      r[idx] = rn;
+      const double zn = L.dinv[idx] * rn;
+      z[idx] = zn;
+      rr += rn * rn;
+      rzn += rn * zn;
+    }
+    rr = block_sum(rr, red);
+    rzn = block_sum(rzn, red);
+    if (threadIdx.x == 0) { bc[0] = rr; bc[1] = rzn; }
+    __syncthreads();
+    ++it;
+    rel = bc[0] / ff;
+    if (rel <= tol2 || it >= maxit) break;
+    const double beta = bc[1] / rz;
+    rz = bc[1];
+    for (int idx = threadIdx.x; idx < n; idx += T) p[idx] = z[idx] + beta * p[idx];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { *out_it = it; *out_rel = rel; }
+}
+
+// ===========================================================================
+// elementwise and reductions
+// ===========================================================================
+__global__ void k_axpy(size_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
+// partial[b*4 + {0,1,2,3}] = max|x|, sum(w x), sum(w x^2), (unused) over a
+// range with per-point weight w(i, j) = wx[i] * wy[j] (w = 1 if wx null)
+__global__ void __launch_bounds__(256) k_reduce_stats(const double* __restrict__ x, int nx, int ny,
+                                                     const double* __restrict__ wx, const double* __restrict__ wy,
+                                                     double* __restrict__ part) {
+  __shared__ double red[4];
+  const size_t n = (size_t)nx * ny;
+  double mx = 0.0, s1 = 0.0, s2 = 0.0;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const double xv = x[idx];
+    mx = fmax(mx, fabs(xv));
+    const double w = wx ? wx[idx % nx] * wy[idx / nx] : 1.0;
+    s1 += w * xv;
+    s2 += w * xv * xv;
+  }
+  const double a = block_max(mx, red);
+  const double b = block_sum(s1, red);
+  const double c = block_sum(s2, red);
+  if (threadIdx.x == 0) { part[blockIdx.x * 4 + 0] = a; part[blockIdx.x * 4 + 1] = b; part[blockIdx.x * 4 + 2] = c; }
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+
+void launch_rhs_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,
+                        const double* u, const double* v, const double* p, int mode, double* out_u, double* out_v,
+                        const StageCombine& scu, const StageCombine& scv, double* vt_u, double* vt_v) {
+  dim3 b(64, 4);
+  hipLaunchKernelGGL(k_rhs_generic<0>, grid2(nux, nuy, b), b, 0, st, dCu, u, v, p, mode, out_u, scu, vt_u);
+  hipLaunchKernelGGL(k_rhs_generic<1>, grid2(nvx, nvy, b), b, 0, st, dCv, u, v, p, mode, out_v, scv, vt_v);
+}
+
+void launch_div_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int npx, int npy, const double* A,
+                        const double* B, double ca, double cb, double ycoef, double oscale, int nu, double* f) {
+  dim3 b(64, 4);
+  hipLaunchKernelGGL(k_div_generic, grid2(npx, npy, b), b, 0, st, dCu, dCv, npx, npy, A, B, ca, cb, ycoef, oscale,
+                     nu, f);
+}
+
+void launch_proj_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,
+                         int nu, const double* Vn, const double* Vt, const double* dp, double dt, double c,
+                         double* V) {
+  dim3 b(64, 4);
+  hipLaunchKernelGGL(k_proj_generic<0>, grid2(nux, nuy, b), b, 0, st, dCu, Vn, Vt, dp, dt, c, V);
+  hipLaunchKernelGGL(k_proj_generic<1>, grid2(nvx, nvy, b), b, 0, st, dCv, Vn + nu, Vt ? Vt + nu : nullptr, dp, dt,
+                     c, V + nu);
+}
+
+void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, double invRe, const double* u,
+                    const double* v, const double* p, int mode, double* out, const StageCombine& sc, double* vt) {
+  PerGrid G{Nx, Ny, hx, hy, invRe};
+  dim3 b(64, 4);
+  if (order == 2)
+    hipLaunchKernelGGL(k_rhs_per<2>, grid2(Nx, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
+  else
+    hipLaunchKernelGGL(k_rhs_per<4>, grid2(Nx, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
+}
+
+void launch_div_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* A, const double* B,
+                    double ca, double cb, double oscale, double* f) {
+  PerGrid G{Nx, Ny, hx, hy, 0.0};
+  dim3 b(64, 4);
+  if (order == 2)
+    hipLaunchKernelGGL(k_div_per<2>, grid2(Nx, Ny, b), b, 0, st, G, A, B, ca, cb, oscale, f);
+  else
+    hipLaunchKernelGGL(k_div_per<4>, grid2(Nx, Ny, b), b, 0, st, G, A, B, ca, cb, oscale, f);
+}
+
+void launch_proj_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* Vn,
+                     const double* Vt, const double* dp, double dt, double c, double* V) {
+  PerGrid G{Nx, Ny, hx, hy, 0.0};
+  dim3 b(64, 4);
+  if (order == 2)
+    hipLaunchKernelGGL(k_proj_per<2>, grid2(Nx, Ny, b), b, 0, st, G, Vn, Vt, dp, dt, c, V);
+  else
+    hipLaunchKernelGGL(k_proj_per<4>, grid2(Nx, Ny, b), b, 0, st, G, Vn, Vt, dp, dt, c, V);
+}
+
+static int ilog2(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+void launch_fft_poisson(hipStream_t st, const FFTPlan& P, const double* f, double* dp) {
+  const int Nx = P.Nx, Ny = P.Ny, Nh = P.Nh;
+  const int T = 256;
+  size_t shx = (size_t)(P.pow2x ? Nx : 2 * Nx) * sizeof(double2);
+  size_t shy = (size_t)(P.pow2y ? Ny : 2 * Ny) * sizeof(double2);
+  double2* H = (double2*)P.H;
+  double2* HT = (double2*)P.HT;
+  hipLaunchKernelGGL(k_fft_r2c_rows, dim3((Ny + 1) / 2), dim3(T), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
+                     (const double2*)P.twx, f, H);
+  dim3 tb(32, 8);
+  hipLaunchKernelGGL(k_transpose_c, dim3((Nh + 31) / 32, (Ny + 31) / 32), tb, 0, st, H, HT, Ny, Nh);
+  hipLaunchKernelGGL(k_fft_cols_solve, dim3(Nh), dim3(T), shy, st, Nx, Ny, P.pow2y, ilog2(Ny), (const double2*)P.twy,
+                     P.ax, P.ay, P.c, HT);
+  hipLaunchKernelGGL(k_transpose_c, dim3((Ny + 31) / 32, (Nh + 31) / 32), tb, 0, st, HT, H, Nh, Ny);
+  hipLaunchKernelGGL(k_fft_c2r_rows, dim3((Ny + 1) / 2), dim3(T), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
+                     (const double2*)P.twx, H, dp);
+}
+
+void launch_lap(hipStream_t st, const DLap& L, const double* p, double* q) {
+  dim3 b(64, 4);
+  hipLaunchKernelGGL(k_lap, grid2(L.nx, L.ny, b), b, 0, st, L, p, q);
+}
+
+void launch_cg_single(hipStream_t st, const DLap& L, const double* f, double* x, double* r, double* z, double* p,
+                      double* q, double tol2, int maxit, int x0_zero, int* out_it, double* out_rel) {
+  hipLaunchKernelGGL(k_cg_single, dim3(1), dim3(1024), 0, st, L, f, x, r, z, p, q, tol2, maxit, x0_zero, out_it,
+                     out_rel);
+}
+
+void launch_cg_init(hipStream_t st, const DLap& L, const CGStateH& S, const double* f, const double* x, double* r,
+                    double* z, double* p, int x0_zero, double tol2) {
+  CGState D{S.s, S.flag, S.part, S.nblk, S.maxit};
+  hipLaunchKernelGGL(k_cg_init, dim3(S.nblk), dim3(256), 0, st, L, D, f, x, r, z, p, x0_zero);
+  hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(256), 0, st, D, tol2);
+}
+
+void launch_cg_iter(hipStream_t st, const DLap& L, const CGStateH& S, double* x, double* r, double* z, double* p,
+                    double* q) {
+  CGState D{S.s, S.flag, S.part, S.nblk, S.maxit};
+  const size_t n = (size_t)L.nx * L.ny;
+  hipLaunchKernelGGL(k_cg_apply, dim3(S.nblk), dim3(256), 0, st, L, D, p, q);
+  hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k_cg_update, dim3(S.nblk), dim3(256), 0, st, L, D, p, q, x, r, z);
+  hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k_cg_dir, dim3(S.nblk), dim3(256), 0, st, D, n, z, p);
+}
+
+void launch_axpy(hipStream_t st, size_t n, double a, const double* x, double* y) {
+  int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, st, n, a, x, y);
+}
+
+void launch_reduce_stats(hipStream_t st, const double* x, int nx, int ny, const double* wx, const double* wy,
+                         double* part, int nblk) {
+  hipLaunchKernelGGL(k_reduce_stats, dim3(nblk), dim3(256), 0, st, x, nx, ny, wx, wy, part);
+}
+
+}  // namespace ins2d

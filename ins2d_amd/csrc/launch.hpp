@@ -1,0 +1,49 @@
+// Host launch wrappers of the kernels in kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+
+#include "device.hpp"
+
+namespace ins2d {
+
+// right-hand-side modes
+enum : int { RHS_WITH_P = 1, RHS_SCALE_OM = 2 };
+
+struct CGStateH {
+  double* s;
+  int* flag;
+  double* part;
+  int nblk;
+  int maxit;
+};
+
+void launch_rhs_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,
+                        const double* u, const double* v, const double* p, int mode, double* out_u, double* out_v,
+                        const StageCombine& scu, const StageCombine& scv, double* vt_u, double* vt_v);
+void launch_div_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int npx, int npy, const double* A,
+                        const double* B, double ca, double cb, double ycoef, double oscale, int nu, double* f);
+void launch_proj_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,
+                         int nu, const double* Vn, const double* Vt, const double* dp, double dt, double c, double* V);
+void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, double invRe, const double* u,
+                    const double* v, const double* p, int mode, double* out, const StageCombine& sc, double* vt);
+void launch_div_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* A, const double* B,
+                    double ca, double cb, double oscale, double* f);
+void launch_proj_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* Vn,
+                     const double* Vt, const double* dp, double dt, double c, double* V);
+void launch_fft_poisson(hipStream_t st, const FFTPlan& P, const double* f, double* dp);
+void launch_lap(hipStream_t st, const DLap& L, const double* p, double* q);
+void launch_cg_single(hipStream_t st, const DLap& L, const double* f, double* x, double* r, double* z, double* p,
+                      double* q, double tol2, int maxit, int x0_zero, int* out_it, double* out_rel);
+void launch_cg_init(hipStream_t st, const DLap& L, const CGStateH& S, const double* f, const double* x, double* r,
+                    double* z, double* p, int x0_zero, double tol2);
+void launch_cg_iter(hipStream_t st, const DLap& L, const CGStateH& S, double* x, double* r, double* z, double* p,
+                    double* q);
+void launch_axpy(hipStream_t st, size_t n, double a, const double* x, double* y);
+void launch_reduce_stats(hipStream_t st, const double* x, int nx, int ny, const double* wx, const double* wy,
+                         double* part, int nblk);
+
+}  // namespace ins2d

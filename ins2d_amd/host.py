@@ -1,0 +1,282 @@
+"""Host side of the drop-in: the reference's driver (main.m) and its
+``options`` plumbing restated over the libins2d_hip C ABI.
+
+``main(case_name)`` follows main.m:68-235: read the case parameters, build the
+mesh from the case's ``mesh`` function, take the boundary-condition types,
+build the operators (on the GPU), set the initial condition, sample the
+boundary values, run input_check and the explicit RK time loop
+(solver_unsteady.m:72-133), and record the per-step conservation log of
+process_iteration.m:112-119.  Everything numerical runs in libins2d_hip;
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import _lib
+from .cases import load as load_case
+from .rk import get_rk_method
+
+BC_KIND = {'dir': 0, 'per': 1, 'pres': 2, 'sym': 3}
+POISSON_FFT = 6
+
+
+class Ins2dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f'libins2d_hip error {code}: {msg}')
+        self.code = code
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check_bc(bct):
+    # boundary_conditions.m:17-40
+    allowed = {('u', 'left'): ('dir', 'per', 'pres'), ('u', 'right'): ('dir', 'per', 'pres'),
+               ('u', 'low'): ('dir', 'per', 'sym'), ('u', 'up'): ('dir', 'per', 'sym'),
+               ('v', 'left'): ('dir', 'per', 'sym'), ('v', 'right'): ('dir', 'per', 'sym'),
+               ('v', 'low'): ('dir', 'per', 'pres'), ('v', 'up'): ('dir', 'per', 'pres')}
+    for (c, side), ok in allowed.items():
+        if bct[c][side] not in ok:
+            raise ValueError(f'wrong BC for {c}-{side}')
+
+
+def bc_samples(case, opts, t):
+    """Boundary values sampled on the point sets of set_bc_vectors.m:53-71,
+    in the order the C ABI takes them."""
+    g = opts.grid
+    uBC, vBC = case.uBC, case.vBC
+    x, y, xin, yin, xp, yp = g.x, g.y, g.xin, g.yin, g.xp, g.yp
+    s = [uBC(x, y[:1], t, opts), uBC(x, y[-1:], t, opts),
+         uBC(xin, y[:1], t, opts), uBC(xin, y[-1:], t, opts),
+         uBC(x[:1], yp, t, opts), uBC(x[-1:], yp, t, opts),
+         vBC(x[:1], y, t, opts), vBC(x[-1:], y, t, opts),
+         vBC(xp, y[:1], t, opts), vBC(xp, y[-1:], t, opts),
+         vBC(x[:1], yin, t, opts), vBC(x[-1:], yin, t, opts)]
+    return [np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel()) for a in s]
+
+
+class Solver:
+    """One libins2d_hip handle: operators, state and solver buffers on the GPU."""
+
+    def __init__(self, x, y, bct, Re, RK='RK44', order4=0, poisson=POISSON_FFT, cg_tol=1e-8, cg_maxit=1000,
+                 p_add_solve=1, fft_symbol=0, kernels=0, alfa=81.0, beta=9 / 8):
+        lib = _lib.load()
+        self.lib = lib
+        _check_bc(bct)
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        A, b, c = get_rk_method(RK) if isinstance(RK, str) else RK
+        self._A = np.ascontiguousarray(A, dtype=np.float64)
+        self._b = np.ascontiguousarray(b, dtype=np.float64)
+        self._c = np.ascontiguousarray(c, dtype=np.float64)
+        d = _lib.CaseDesc()
+        d.Nx, d.Ny = len(self.x) - 1, len(self.y) - 1
+        d.x, d.y = _dp(self.x), _dp(self.y)
+        for k, side in enumerate(('left', 'right', 'low', 'up')):
+            d.bc_u[k] = BC_KIND[bct['u'][side]]
+            d.bc_v[k] = BC_KIND[bct['v'][side]]
+        d.Re = float(Re)
+        d.order4 = int(order4)
+        d.alfa, d.beta = float(alfa), float(beta)
+        d.rk_stages = len(self._b)
+        d.rk_A, d.rk_b, d.rk_c = _dp(self._A), _dp(self._b), _dp(self._c)
+        d.poisson = int(poisson)
+        d.fft_symbol = int(fft_symbol)
+        d.cg_tol = float(cg_tol)
+        d.cg_maxit = int(cg_maxit)
+        d.p_add_solve = int(p_add_solve)
+        d.kernels = int(kernels)
+        h = ctypes.c_void_p()
+        rc = lib.ins2d_create(ctypes.byref(d), ctypes.byref(h))
+        if rc != 0:
+            raise Ins2dError(rc, lib.ins2d_last_error(None).decode())
+        self.h = h
+        info = np.zeros(7, dtype=np.int32)
+        Nx, Ny = d.Nx, d.Ny
+        self.xin = np.zeros(Nx + 1)
+        self.yin = np.zeros(Ny + 1)
+        self._call(lib.ins2d_grid_info, info.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(self.xin), _dp(self.yin))
+        self.Nux_in, self.Nuy_in, self.Nvx_in, self.Nvy_in, self.Npx, self.Npy, self.path = (int(v) for v in info)
+        if self.path:   # periodic: interior points are x(1:end-1), y(1:end-1)
+            self.xin = self.x[:-1].copy()
+            self.yin = self.y[:-1].copy()
+        else:
+            self.xin = self.xin[:self.Nux_in]
+            self.yin = self.yin[:self.Nvy_in]
+        self.Nu = self.Nux_in * self.Nuy_in
+        self.Nv = self.Nvx_in * self.Nvy_in
+        self.NV = self.Nu + self.Nv
+        self.Np = self.Npx * self.Npy
+
+    # ------------------------------------------------------------------
+    def _call(self, fn, *args):
+        rc = fn(self.h, *args)
+        if rc != 0:
+            raise Ins2dError(rc, self.lib.ins2d_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self.lib.ins2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_bc(self, samples, t=0.0):
+        arrs = (ctypes.POINTER(ctypes.c_double) * 12)(*[_dp(s) for s in samples])
+        self._keep = samples
+        self._call(self.lib.ins2d_set_bc, ctypes.c_double(t), arrs)
+
+    def set_state(self, V=None, p=None):
+        V = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+        p = None if p is None else np.ascontiguousarray(p, dtype=np.float64)
+        if V is not None and V.size != self.NV:
+            raise ValueError('V has the wrong size')
+        if p is not None and p.size != self.Np:
+            raise ValueError('p has the wrong size')
+        self._call(self.lib.ins2d_set_state, None if V is None else _dp(V), None if p is None else _dp(p))
+
+    def get_state(self):
+        V = np.zeros(self.NV)
+        p = np.zeros(self.Np)
+        self._call(self.lib.ins2d_get_state, _dp(V), _dp(p))
+        return V, p
+
+    def rhs(self, V, p, t=0.0):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        F = np.zeros(self.NV)
+        mr = ctypes.c_double()
+        self._call(self.lib.ins2d_rhs, _dp(V), _dp(p), ctypes.c_double(t), _dp(F), ctypes.byref(mr))
+        return mr.value, F
+
+    def poisson(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        dp = np.zeros(self.Np)
+        it = ctypes.c_int()
+        rel = ctypes.c_double()
+        self._call(self.lib.ins2d_poisson, _dp(f), _dp(dp), ctypes.byref(it), ctypes.byref(rel))
+        return dp, it.value, rel.value
+
+    def input_check(self, t=0.0, p_initial=1):
+        self._call(self.lib.ins2d_input_check, ctypes.c_double(t), ctypes.c_int(p_initial))
+
+    def step(self, tn, dt, nsteps=1):
+        self._call(self.lib.ins2d_step, ctypes.c_double(tn), ctypes.c_double(dt), ctypes.c_int(nsteps))
+
+    def diagnostics(self, t=0.0):
+        out = np.zeros(5)
+        self._call(self.lib.ins2d_diagnostics, ctypes.c_double(t), _dp(out))
+        return tuple(float(v) for v in out)
+
+    def solver_stats(self):
+        it = ctypes.c_int()
+        rel = ctypes.c_double()
+        tot = ctypes.c_longlong()
+        self._call(self.lib.ins2d_solver_stats, ctypes.byref(it), ctypes.byref(rel), ctypes.byref(tot))
+        return it.value, rel.value, tot.value
+
+    def profile(self, enable):
+        self._call(self.lib.ins2d_profile, ctypes.c_int(int(enable)))
+
+    def profile_read(self, kind):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._call(self.lib.ins2d_profile_read, ctypes.c_int(kind), ctypes.byref(ms), ctypes.byref(n))
+        return ms.value, n.value
+
+    def sync(self):
+        self._call(self.lib.ins2d_sync)
+
+
+def _options(case, prm):
+    g = SimpleNamespace(x1=prm.get('x1', 0.0), x2=prm.get('x2', 1.0), y1=prm.get('y1', 0.0), y2=prm.get('y2', 1.0),
+                        Nx=prm['Nx'], Ny=prm['Ny'], sx=prm.get('sx', 1.0), sy=prm.get('sy', 1.0))
+    return SimpleNamespace(grid=g, prm=prm, case_obj=case, fluid=SimpleNamespace(Re=float(prm['Re'])))
+
+
+def setup(case, **overrides):
+    """main.m:68-186 up to the start of the solver: returns (solver, options,
+    V_start, p_start) with the operators built and the boundary samples set."""
+    if isinstance(case, str):
+        case = load_case(case, **overrides)
+    prm = dict(case.parameters())
+    opts = _options(case, prm)
+    x, y = case.mesh(opts)
+    x = np.asarray(x, float)
+    y = np.asarray(y, float)
+    g = opts.grid
+    g.x, g.y = x, y
+    g.xp = (x[:-1] + x[1:]) / 2
+    g.yp = (y[:-1] + y[1:]) / 2
+    g.hx, g.hy = np.diff(x), np.diff(y)
+    g.Nx, g.Ny = len(g.hx), len(g.hy)
+    bct = case.BCtype()
+    if int(bct.get('BC_unsteady', 0)):
+        raise NotImplementedError('unsteady boundary conditions are not on this path')
+    solver = Solver(x, y, bct, prm['Re'], RK=prm.get('RK', 'RK44'), order4=prm.get('order4', 0),
+                    poisson=prm.get('poisson', POISSON_FFT), cg_tol=prm.get('CG_acc', 1e-8),
+                    cg_maxit=prm.get('CG_maxit', 1000), p_add_solve=prm.get('p_add_solve', 1),
+                    fft_symbol=prm.get('fft_symbol', 0), kernels=prm.get('kernels', 0),
+                    alfa=prm.get('alfa', 81.0), beta=prm.get('beta', 9 / 8))
+    g.xin, g.yin = solver.xin, solver.yin
+    g.Nux_in, g.Nuy_in, g.Nvx_in, g.Nvy_in = solver.Nux_in, solver.Nuy_in, solver.Nvx_in, solver.Nvy_in
+    g.Npx, g.Npy = solver.Npx, solver.Npy
+    # operator_mesh.m:401-414 (column-major 2-D arrays)
+    g.xu = np.outer(g.xin, np.ones(g.Nuy_in)); g.yu = np.outer(np.ones(g.Nux_in), g.yp)
+    g.xv = np.outer(g.xp, np.ones(g.Nvy_in)); g.yv = np.outer(np.ones(g.Nvx_in), g.yin)
+    g.xpp = np.outer(g.xp, np.ones(g.Ny)); g.ypp = np.outer(np.ones(g.Nx), g.yp)
+    t = float(prm.get('t_start', 0.0))
+    u0, v0, p0 = case.IC(t, opts)
+    V0 = np.r_[np.asarray(u0, float).ravel(order='F'), np.asarray(v0, float).ravel(order='F')]
+    p0 = np.asarray(p0, float).ravel(order='F')
+    dt = prm.get('dt')
+    if dt is None:
+        dt = case.module.cfl_dt(opts, u0, v0)
+    opts.dt = float(dt)
+    t_end = prm.get('t_end')
+    opts.nt = int(prm['nsteps']) if t_end is None else int(math.ceil((t_end - t) / opts.dt - 1e-12))
+    opts.t_start = t
+    solver.set_bc(bc_samples(case, opts, t), t)
+    return solver, opts, V0, p0
+
+
+def main(case_name, nsteps=None, diagnostics=True, **overrides):
+    """main.m + solver_unsteady.m: returns (V, p, rows) where rows are the
+    convergence.txt records (n, dt, t, maxres, maxdiv, umom, vmom, k)."""
+    solver, opts, V0, p0 = setup(case_name, **overrides)
+    prm = opts.prm
+    t = opts.t_start
+    dt = opts.dt
+    solver.set_state(V0, p0)
+    solver.input_check(t, int(prm.get('p_initial', 1)))
+    rows = [(1, dt, t) + solver.diagnostics(t)] if diagnostics else []
+    nt = opts.nt if nsteps is None else nsteps
+    for n in range(2, nt + 2):
+        solver.step(t, dt, 1)
+        t = t + dt
+        if diagnostics:
+            rows.append((n, dt, t) + solver.diagnostics(t))
+    V, p = solver.get_state()
+    solver.close()
+    return V, p, rows
+
+
+def write_convergence(path, case_name, Nx, Ny, Re, rows):
+    """convergence.txt in the layout of create_files.m and
+    solver_unsteady.m:4-8 / process_iteration.m:112-119."""
+    with open(path, 'w') as fh:
+        fh.write(f'{case_name} {Nx} {Ny} {Re:.3e}\n')
+        fh.write('residual and conservation information\n')
+        fh.write('n            dt               t                res              maxdiv           umom             vmom             k\n')
+        for r in rows:
+            fh.write('%-10i %16.8e %16.8e %16.8e %16.8e %16.8e %16.8e %16.8e \n' % tuple(r))

@@ -1,0 +1,195 @@
+"""GPU parity: libins2d_hip (through its C ABI) against the CPU oracle on the
+same inputs, and against the reference-held fixtures.
+
+Tolerances: the plain FP64 path must match within 1e-12 relative
+(velocities; pressure modulo its constant for enclosed flows).  Pressure-CG
+runs use a tight CG tolerance here (the oracle solves directly), so the
+comparison is not limited by the solver tolerance."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from ins2d_amd import host
+from ins2d_amd.cases import load
+from oracle import ins2d_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def _say(*a):
+    print('[gpu-test]', *a, flush=True)
+
+
+def _case_periodic_noise(N=32, order4=0, Re=500.0, seed=3, **kw):
+    return load('taylor_green', Nx=N, Ny=N, Re=Re, order4=order4, noise=1e-2, seed=seed, dt=0.01, t_end=0.05, **kw)
+
+
+def _wall_case(kind, Nx, Ny, **kw):
+    c = load('LDC_unsteady', Nx=Nx, Ny=Ny, **kw)
+    if kind == 'outflow':
+        c.BCtype = lambda: {'BC_unsteady': 0, 'u': dict(left='dir', right='pres', low='dir', up='dir'),
+                            'v': dict(left='dir', right='sym', low='dir', up='dir')}
+        c.uBC = lambda x, y, t, o: np.ravel(np.add.outer(0 * np.atleast_1d(x), 4 * np.atleast_1d(y) * (1 - np.atleast_1d(y))) *
+                                            (np.abs(np.atleast_1d(x)) < 1e-12)[:, None], order='F')
+    return c
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize('name,kernels', [('ldc', 0), ('outflow', 0), ('periodic2', 0), ('periodic2', 1), ('periodic4', 0)])
+def test_rhs_matches_oracle(name, kernels):
+    _say('rhs', name, kernels)
+    if name == 'ldc':
+        case = _wall_case('ldc', 24, 20, poisson=3, CG_acc=1e-13, kernels=kernels)
+    elif name == 'outflow':
+        case = _wall_case('outflow', 30, 12, poisson=3, CG_acc=1e-13, sx=1.04, kernels=kernels)
+    else:
+        case = _case_periodic_noise(32, order4=1 if name == 'periodic4' else 0, kernels=kernels)
+    opts = O.build(case)
+    solver, hopts, V0, p0 = host.setup(case)
+    rng = np.random.default_rng(7)
+    V = rng.standard_normal(opts.grid.NV)
+    p = rng.standard_normal(opts.grid.Np)
+    mr, F = solver.rhs(V, p)
+    mro, Fo = O.F(V, V, p, 0.0, opts)
+    assert _rel(F, Fo) < RTOL
+    assert abs(mr - mro) <= RTOL * mro
+    solver.close()
+
+
+@pytest.mark.parametrize('name', ['fft2', 'fft4_ref', 'fft4_exact', 'fft_nonpow2', 'cg_ldc', 'cg_outflow', 'cg_large'])
+def test_poisson_solves_assembled_system(name):
+    _say('poisson', name)
+    fft_symbol = 0
+    if name == 'fft2':
+        case = _case_periodic_noise(64)
+    elif name.startswith('fft4'):
+        case = _case_periodic_noise(64, order4=1, fft_symbol=1 if name == 'fft4_exact' else 0)
+        fft_symbol = 1 if name == 'fft4_exact' else 0
+    elif name == 'fft_nonpow2':
+        case = load('shear_layer', Nx=40, Ny=24)
+    elif name == 'cg_ldc':
+        case = _wall_case('ldc', 40, 30, poisson=3, CG_acc=1e-12)
+    elif name == 'cg_outflow':
+        case = _wall_case('outflow', 48, 20, poisson=3, CG_acc=1e-12, sx=1.03)
+    else:
+        case = _wall_case('ldc', 256, 160, poisson=3, CG_acc=1e-11, CG_maxit=5000)
+    opts = O.build(case)
+    solver, _, _, _ = host.setup(case)
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal(opts.grid.Np)
+    singular = name != 'cg_outflow'
+    if singular:
+        f -= f.mean()
+    dp, it, rel = solver.poisson(f)
+    A = opts.discretization.A
+    r = A @ dp - f
+    if name == 'fft4_ref':
+        # the reference's FFT branch always uses the second-order symbol
+        # (operator_divergence.m:283), so the fourth-order system is not
+        # solved exactly; compare with the oracle's own FFT branch instead
+        dpo = O.pressure_poisson(f, 0.0, opts)
+        assert _rel(dp, dpo) < 1e-12
+    elif name.startswith('fft'):
+        assert np.abs(r).max() < 1e-11 * np.abs(f).max()
+        dpo = O.pressure_poisson(f, 0.0, opts) if not name.startswith('fft4') else None
+        if dpo is not None:
+            assert _rel(dp, dpo) < 1e-12
+    else:
+        tol = {'cg_ldc': 1e-12, 'cg_outflow': 1e-12, 'cg_large': 1e-11}[name]
+        assert rel <= tol and it > 0
+        assert np.linalg.norm(r) <= 2 * tol * np.linalg.norm(f)
+    solver.close()
+
+
+def test_cg_nonconvergence_is_reported():
+    _say('cg nonconvergence')
+    case = _wall_case('ldc', 32, 32, poisson=3, CG_acc=1e-14, CG_maxit=3)
+    solver, _, _, _ = host.setup(case)
+    f = np.random.default_rng(1).standard_normal(solver.Np)
+    f -= f.mean()
+    with pytest.raises(host.Ins2dError) as e:
+        solver.poisson(f)
+    assert e.value.code == 3
+    solver.close()
+
+
+def _compare_runs(case, nsteps, const_mode_p):
+    opts = O.build(case)
+    Vo, po, rows_o, _ = O.run(case, nsteps=nsteps)
+    V, p, rows = host.main(case, nsteps=nsteps)
+    assert _rel(V, Vo) < RTOL
+    if const_mode_p:
+        p = p - p.mean()
+        po = po - po.mean()
+    assert _rel(p, po) < 1e-11
+    ro, rg = np.asarray(rows_o), np.asarray(rows)
+    for c in (3, 7):   # maxres, kinetic energy
+        assert np.all(np.abs(rg[:, c] - ro[:, c]) <= 1e-11 * np.abs(ro[:, c]).max())
+    return rows
+
+
+def test_ldc_run_matches_oracle_and_reference_profiles():
+    """results/LDC_unsteady_1.000e+02_20x20_0: RK33P2, Re=100, 100 steps."""
+    _say('ldc 20x20 run')
+    case = load('LDC_unsteady', RK='RK33P2', poisson=3, CG_acc=1e-13, CG_maxit=2000)
+    rows = _compare_runs(case, 100, True)
+    d = os.path.join(GOLDEN, 'LDC_unsteady_1.000e+02_20x20_0')
+    ref = np.loadtxt(os.path.join(d, 'convergence.txt'), skiprows=3)
+    rows = np.asarray(rows)
+    for c in (3, 7):
+        sel = np.abs(ref[:, c]) > 0
+        assert np.max(np.abs(rows[sel, c] / ref[sel, c] - 1)) < 1.2e-8
+
+
+@pytest.mark.parametrize('kernels', [0, 1])
+def test_shear_layer_run_matches_oracle_and_reference_log(kernels):
+    """results/shear_layer01: inviscid, 40x40 (FFT of a non-power-of-two
+    length), RK44, u shifted by 1; first 50 steps."""
+    _say('shear layer 40x40 run', kernels)
+    case = load('shear_layer', Nx=40, Ny=40, u_shift=1.0, kernels=kernels)
+    rows = _compare_runs(case, 50, False)
+    ref = np.loadtxt(os.path.join(GOLDEN, 'shear_layer01', 'convergence.txt'), skiprows=3)[:51]
+    rows = np.asarray(rows)
+    for c in (3, 5, 7):
+        assert np.max(np.abs(rows[:, c] / ref[:, c] - 1)) < 1.2e-8
+
+
+def test_outflow_stretched_run_matches_oracle():
+    _say('outflow stretched run')
+    case = _wall_case('outflow', 32, 16, poisson=3, CG_acc=1e-13, CG_maxit=4000, sx=1.03, Re=200.0, dt=0.005)
+    _compare_runs(case, 20, False)
+
+
+@pytest.mark.parametrize('order4,rk', [(0, 'RK44'), (1, 'RK44'), (0, 'RK44C23'), (0, 'DOPRI6')])
+def test_periodic_run_matches_oracle(order4, rk):
+    _say('periodic run', order4, rk)
+    case = _case_periodic_noise(64, order4=order4, RK=rk)
+    _compare_runs(case, 10, False)
+
+
+def test_large_periodic_invariants():
+    """1024^2 Taylor-Green with noise, inviscid: after steps the field stays
+    discretely divergence-free, total momentum is conserved to round-off and
+    the kinetic energy drift over 20 steps is tiny (RK4 dissipates only at
+    O(dt^4) per unit time)."""
+    _say('1024^2 invariants')
+    case = load('taylor_green', Nx=1024, Ny=1024, Re=1e100)
+    solver, opts, V0, p0 = host.setup(case)
+    solver.set_state(V0, p0)
+    solver.input_check(0.0, 1)
+    d0 = solver.diagnostics(0.0)
+    assert d0[1] < 1e-12
+    solver.step(0.0, opts.dt, 20)
+    d1 = solver.diagnostics(20 * opts.dt)
+    assert d1[1] < 1e-12
+    assert abs(d1[2] - d0[2]) < 1e-10 and abs(d1[3] - d0[3]) < 1e-10
+    assert abs(d1[4] - d0[4]) / d0[4] < 1e-9
+    solver.close()
