@@ -360,79 +360,84 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 conjd(double2 a) { return make_double2(a.x, -a.y); }
 
-// in-place FFT of z[0..N) in LDS; INV selects exp(+2 pi i k n / N) (unscaled)
+// in-place FFT of z[0..N) in LDS; INV selects exp(+2 pi i k n / N) (unscaled).
+// Stockham autosort radix-4 passes (plus one radix-2 pass when log2 N is odd);
+// each thread owns at most 4 radix-4 (8 radix-2) butterflies per pass, held
+// in registers between the LDS read and the LDS write (N <= 16 * blockDim).
 template <bool INV>
-__device__ void lds_fft(double2* z, int N, int log2N, const double2* __restrict__ tw, double2* scratch) {
+__device__ void lds_fft(double2* z, int N, int log2N, const double2* __restrict__ tw) {
   const int T = blockDim.x;
-  if (N == 1) return;
-  // radix-4 Stockham passes, then one radix-2 pass if log2N is odd
+  const int tid = threadIdx.x;
   int Ns = 1;
-  int passes4 = log2N / 2;
+  const int passes4 = log2N / 2;
   for (int pass = 0; pass < passes4; ++pass) {
     const int Q = N / 4;
-    double2 vals[16];
-    int cnt = 0;
-    for (int jq = threadIdx.x; jq < Q; jq += T) {
-      const int k = jq % Ns;
-      double2 a0 = z[jq], a1 = z[jq + Q], a2 = z[jq + 2 * Q], a3 = z[jq + 3 * Q];
-      const int step = N / (Ns * 4);
-      int m1 = (k * step) % N, m2 = (2 * k * step) % N, m3 = (3 * k * step) % N;
-      double2 t1 = tw[m1], t2 = tw[m2], t3 = tw[m3];
-      if (INV) { t1 = conjd(t1); t2 = conjd(t2); t3 = conjd(t3); }
-      a1 = cmul(a1, t1); a2 = cmul(a2, t2); a3 = cmul(a3, t3);
-      const double2 s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = csub(a1, a3);
-      // -i * d13 (forward) or +i * d13 (inverse)
-      const double2 jd13 = INV ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
-      vals[cnt * 4 + 0] = cadd(s02, s13);
-      vals[cnt * 4 + 1] = cadd(d02, jd13);
-      vals[cnt * 4 + 2] = csub(s02, s13);
-      vals[cnt * 4 + 3] = csub(d02, jd13);
-      ++cnt;
-      if (cnt == 4) break;   // at most 4 butterflies per thread per pass (T >= N/16)
+    const int step = N / (Ns * 4);
+    double2 o0[4], o1[4], o2[4], o3[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int jq = tid + c * T;
+      if (jq < Q) {
+        const int k = jq % Ns;
+        double2 a0 = z[jq], a1 = z[jq + Q], a2 = z[jq + 2 * Q], a3 = z[jq + 3 * Q];
+        double2 t1 = tw[k * step], t2 = tw[2 * k * step], t3 = tw[3 * k * step];
+        if (INV) { t1 = conjd(t1); t2 = conjd(t2); t3 = conjd(t3); }
+        a1 = cmul(a1, t1); a2 = cmul(a2, t2); a3 = cmul(a3, t3);
+        const double2 s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = csub(a1, a3);
+        // -i * d13 (forward) or +i * d13 (inverse)
+        const double2 jd13 = INV ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
+        o0[c] = cadd(s02, s13);
+        o1[c] = cadd(d02, jd13);
+        o2[c] = csub(s02, s13);
+        o3[c] = csub(d02, jd13);
+      }
     }
     __syncthreads();
-    cnt = 0;
-    for (int jq = threadIdx.x; jq < Q; jq += T) {
-      const int k = jq % Ns;
-      const int o = (jq / Ns) * Ns * 4 + k;
-      z[o] = vals[cnt * 4 + 0];
-      z[o + Ns] = vals[cnt * 4 + 1];
-      z[o + 2 * Ns] = vals[cnt * 4 + 2];
-      z[o + 3 * Ns] = vals[cnt * 4 + 3];
-      ++cnt;
-      if (cnt == 4) break;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int jq = tid + c * T;
+      if (jq < Q) {
+        const int k = jq % Ns;
+        const int o = (jq / Ns) * Ns * 4 + k;
+        z[o] = o0[c];
+        z[o + Ns] = o1[c];
+        z[o + 2 * Ns] = o2[c];
+        z[o + 3 * Ns] = o3[c];
+      }
     }
     __syncthreads();
     Ns *= 4;
   }
   if (log2N & 1) {
     const int Q = N / 2;
-    double2 vals[16];
-    int cnt = 0;
-    for (int jq = threadIdx.x; jq < Q; jq += T) {
-      const int k = jq % Ns;
-      double2 a0 = z[jq], a1 = z[jq + Q];
-      double2 t1 = tw[(k * (N / (Ns * 2))) % N];
-      if (INV) t1 = conjd(t1);
-      a1 = cmul(a1, t1);
-      vals[cnt * 2 + 0] = cadd(a0, a1);
-      vals[cnt * 2 + 1] = csub(a0, a1);
-      ++cnt;
-      if (cnt == 8) break;
+    const int step = N / (Ns * 2);
+    double2 o0[8], o1[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int jq = tid + c * T;
+      if (jq < Q) {
+        const int k = jq % Ns;
+        double2 a0 = z[jq], a1 = z[jq + Q];
+        double2 t1 = tw[k * step];
+        if (INV) t1 = conjd(t1);
+        a1 = cmul(a1, t1);
+        o0[c] = cadd(a0, a1);
+        o1[c] = csub(a0, a1);
+      }
     }
     __syncthreads();
-    cnt = 0;
-    for (int jq = threadIdx.x; jq < Q; jq += T) {
-      const int k = jq % Ns;
-      const int o = (jq / Ns) * Ns * 2 + k;
-      z[o] = vals[cnt * 2 + 0];
-      z[o + Ns] = vals[cnt * 2 + 1];
-      ++cnt;
-      if (cnt == 8) break;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int jq = tid + c * T;
+      if (jq < Q) {
+        const int k = jq % Ns;
+        const int o = (jq / Ns) * Ns * 2 + k;
+        z[o] = o0[c];
+        z[o + Ns] = o1[c];
+      }
     }
     __syncthreads();
   }
-  (void)scratch;
 }
 
 // direct DFT for lengths that are not powers of two (O(N^2), N <= 4096)
@@ -458,7 +463,7 @@ __device__ void lds_dft(double2* z, double2* out, int N, const double2* __restri
 template <bool INV>
 __device__ __forceinline__ void lds_transform(double2* z, double2* scratch, int N, bool pow2, int log2N,
                                               const double2* __restrict__ tw) {
-  if (pow2) lds_fft<INV>(z, N, log2N, tw, scratch);
+  if (pow2) lds_fft<INV>(z, N, log2N, tw);
   else lds_dft<INV>(z, scratch, N, tw);
 }
 

@@ -765,6 +765,12 @@ def operator_divergence(opts):
         dx, dy = hx[0], hy[0]
         I, J = np.meshgrid(np.arange(g.Npx), np.arange(g.Npy), indexing='ij')
         Ahat = (dx * dy) * 4 * ((1 / dx ** 2) * np.sin(I * np.pi / g.Npx) ** 2 + (1 / dy ** 2) * np.sin(J * np.pi / g.Npy) ** 2)
+        if int(getattr(opts, 'prm', {}).get('fft_symbol', 0)) == 1:
+            # A is block-circulant on a periodic uniform grid: its eigenvalues
+            # are the 2-D DFT of its first column (exact for either order;
+            # the reference itself always uses the second-order closed form)
+            col = np.asarray(d.A[:, 0].todense()).ravel().reshape(g.Npx, g.Npy, order='F')
+            Ahat = -np.real(np.fft.fft2(col))
         Ahat[0, 0] = 1
         s.Ahat = Ahat
     if s.poisson == 1:
