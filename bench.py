@@ -36,7 +36,8 @@ def parse():
     ap.add_argument('--N', type=int, default=1024)
     ap.add_argument('--order4', type=int, default=0)
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--cpu-steps', type=int, default=2)
+    ap.add_argument('--cpu-steps', type=int, default=10)
+    ap.add_argument('--members', type=int, default=512)
     return ap.parse_args()
 
 
@@ -74,8 +75,38 @@ def cpu_baseline(case_args, steps):
                       f'nproc {os.cpu_count()}'}
 
 
+def bench_ensemble(a):
+    """Batched ensemble: 512 cavity/channel members of uneven size (48..512
+    cells on the long side, aspect 1..4), Re in [100, 5000], speed in
+    [0.5, 1.5], RK4, Jacobi-PCG pressure to 1e-10."""
+    from ins2d_amd import host
+    from ins2d_amd.cases import ensemble
+    members = ensemble.members(a.members)
+    t0 = time.perf_counter()
+    B = host.Batch(members)
+    B.input_check(1)
+    t_setup = time.perf_counter() - t0
+    B.step(a.warmup)
+    it0, _, cells = B.stats()
+    t0 = time.perf_counter()
+    B.step(a.steps)
+    el = time.perf_counter() - t0
+    it1, itmax, _ = B.stats()
+    value = cells * 4 * a.steps / el / 1e6
+    out = {'metric': 'Mcell-RK-stages/s per GPU', 'value': value, 'unit': 'Mcell-RK-stages/s', 'n_gpus': 1,
+           'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': 1e3 * el / a.steps, 'higher_is_better': True,
+           'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
+           'config': {'workload': f'ensemble of {a.members} cavity/channel members, RK44, Jacobi-PCG 1e-10',
+                      'cells': cells, 'setup_s': t_setup,
+                      'pcg_iterations_per_member_solve': (it1 - it0) / (a.members * 4 * a.steps)}}
+    B.close()
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
+    if a.case == 'ensemble':
+        return bench_ensemble(a)
     if a.gpus != 1:
         print(json.dumps({'error': 'this path runs on one GPU; multi-GPU sharding is described in DESIGN.md, '
                                    'not built', 'n_gpus': a.gpus}))

@@ -125,6 +125,28 @@ int ins2d_profile_read(const ins2d_handle* h, int kind, double* ms, long long* c
 /* wait for all work queued on the handle's stream */
 int ins2d_sync(ins2d_handle* h);
 
+/* ---- batched ensemble (no reference counterpart: many solver_unsteady runs
+ * of uneven size advanced together; second order, PCG pressure).  All
+ * instances share the Butcher tableau, cg_tol, cg_maxit and p_add_solve of
+ * descs[0]; each has its own grid, boundary kinds and Re. */
+typedef struct ins2d_batch ins2d_batch;
+int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out);
+void ins2d_batch_destroy(ins2d_batch* b);
+const char* ins2d_batch_last_error(const ins2d_batch* b);
+/* sizes[0..5] and interior grid lines of instance k (as ins2d_grid_info) */
+int ins2d_batch_info(const ins2d_batch* b, int k, int* sizes, double* xin, double* yin);
+/* boundary samples of instance k (same order as ins2d_set_bc) */
+int ins2d_batch_set_bc(ins2d_batch* b, int k, double t, const double* const* samples);
+int ins2d_batch_set_state(ins2d_batch* b, int k, const double* V, const double* p);
+int ins2d_batch_get_state(ins2d_batch* b, int k, double* V, double* p);
+/* input_check.m:92-128 for every instance */
+int ins2d_batch_input_check(ins2d_batch* b, int p_initial);
+/* nsteps of time_ERK.m for every instance, instance k with time step dt[k] */
+int ins2d_batch_step(ins2d_batch* b, const double* dt, int nsteps);
+/* accumulated PCG iterations over all instances and solves, the largest
+ * per-solve iteration count, total pressure cells */
+int ins2d_batch_stats(const ins2d_batch* b, long long* total_iters, int* max_iters, long long* cells);
+
 #ifdef __cplusplus
 }
 #endif

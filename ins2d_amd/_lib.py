@@ -39,6 +39,16 @@ SIGNATURES = {
     "ins2d_profile": (c_int, [c_void_p, c_int]),
     "ins2d_profile_read": (c_int, [c_void_p, c_int, P_double, ctypes.POINTER(ctypes.c_longlong)]),
     "ins2d_sync": (c_int, [c_void_p]),
+    "ins2d_batch_create": (c_int, [ctypes.POINTER(CaseDesc), c_int, ctypes.POINTER(c_void_p)]),
+    "ins2d_batch_destroy": (None, [c_void_p]),
+    "ins2d_batch_last_error": (c_char_p, [c_void_p]),
+    "ins2d_batch_info": (c_int, [c_void_p, c_int, P_int, P_double, P_double]),
+    "ins2d_batch_set_bc": (c_int, [c_void_p, c_int, c_double, ctypes.POINTER(P_double)]),
+    "ins2d_batch_set_state": (c_int, [c_void_p, c_int, P_double, P_double]),
+    "ins2d_batch_get_state": (c_int, [c_void_p, c_int, P_double, P_double]),
+    "ins2d_batch_input_check": (c_int, [c_void_p, c_int]),
+    "ins2d_batch_step": (c_int, [c_void_p, P_double, c_int]),
+    "ins2d_batch_stats": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong), P_int, ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 _LIB = None

@@ -1,0 +1,785 @@
+// Batched ensemble: many independent cavity/channel instances of uneven size
+// and Reynolds number advanced together.  Every phase of an RK stage is one
+// launch over a work list of 64x4 tiles that spans all instances; the
+// pressure solve is a batched Jacobi-PCG with per-instance scalars (converged
+// instances skip their tiles), run in chunks of iterations that are captured
+// once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ins2d.h"
+#include "device.hpp"
+#include "launch.hpp"
+#include "ops1d.hpp"
+#include "stencil.hpp"
+
+namespace ins2d {
+
+struct BInst {
+  const DComp* cu;
+  const DComp* cv;
+  DLap lap;
+  int nux, nuy, nvx, nvy, npx, npy;
+  long voff, poff;
+  int nu;
+  int pt0, pt1;   // range of pressure tiles of this instance
+};
+struct Tile {
+  int inst, comp, i0, j0;
+};
+
+constexpr int TX = 64, TY = 4;
+
+// ------------------------------------------------------------------ kernels
+__global__ void __launch_bounds__(256) k_b_rhs(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
+                                              const double* __restrict__ V, const double* __restrict__ P, int mode,
+                                              double* __restrict__ out, StageCombine sc,
+                                              double* __restrict__ vtemp) {
+  const Tile t = tiles[blockIdx.x];
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  const double* u = V + B.voff;
+  const double* v = u + B.nu;
+  const double* p = P + B.poff;
+  if (t.comp == 0) {
+    if (i >= B.nux || j >= B.nuy) return;
+    rhs_point<0>(*B.cu, u, v, p, mode, out ? out + B.voff : nullptr, sc, vtemp ? vtemp + B.voff : nullptr, B.voff,
+                 i, j);
+  } else {
+    if (i >= B.nvx || j >= B.nvy) return;
+    const long o = B.voff + B.nu;
+    rhs_point<1>(*B.cv, u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
+  }
+}
+
+// f = oscale * (M (ca A + cb B) + ycoef yM); with dt != null: ca = ycoef = 1/dt_k
+__global__ void __launch_bounds__(256) k_b_div(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
+                                              const double* __restrict__ A, const double* __restrict__ Bv,
+                                              const double* __restrict__ dt, double ca, double cb, double ycoef,
+                                              double oscale, double* __restrict__ f) {
+  const Tile t = tiles[blockIdx.x];
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  if (i >= B.npx || j >= B.npy) return;
+  double a = ca, y = ycoef;
+  if (dt) { a = 1.0 / dt[t.inst]; y = a; }
+  const double s = div_point(*B.cu, *B.cv, A + B.voff, Bv ? Bv + B.voff : nullptr, a, cb, y, B.nu, i, j);
+  f[B.poff + (long)i + (long)B.npx * j] = oscale * s;
+}
+
+// V = Vn + dt_k (Vt - c Om^-1 G dp)  (dt null: dt_k = 1)
+__global__ void __launch_bounds__(256) k_b_proj(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
+                                               const double* __restrict__ Vn, const double* __restrict__ Vt,
+                                               const double* __restrict__ dp, const double* __restrict__ dt, double c,
+                                               double* __restrict__ V) {
+  const Tile t = tiles[blockIdx.x];
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  const double d = dt ? dt[t.inst] : 1.0;
+  const double* pp = dp + B.poff;
+  if (t.comp == 0) {
+    if (i >= B.nux || j >= B.nuy) return;
+    const long o = B.voff;
+    V[o + (long)i + (long)B.nux * j] = proj_point(*B.cu, Vn + o, Vt ? Vt + o : nullptr, pp, d, c, i, j);
+  } else {
+    if (i >= B.nvx || j >= B.nvy) return;
+    const long o = B.voff + B.nu;
+    V[o + (long)i + (long)B.nvx * j] = proj_point(*B.cv, Vn + o, Vt ? Vt + o : nullptr, pp, d, c, i, j);
+  }
+}
+
+// ---- batched Jacobi-PCG; per-instance state: s[k*8 + {0 rz, 1 ff, 2 alpha,
+// 3 beta, 4 rel^2, 5 tol^2}], f[k*4 + {0 done, 1 iterations, 2 failed}]
+struct BCG {
+  double* s;
+  int* f;
+  double* part;   // 2 per pressure tile
+  int* nactive;
+  int maxit;
+};
+
+__global__ void __launch_bounds__(256) k_b_cg_init(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
+                                                  BCG S, const double* __restrict__ f, double* __restrict__ x,
+                                                  double* __restrict__ r, double* __restrict__ z,
+                                                  double* __restrict__ p) {
+  __shared__ double red[4];
+  const Tile t = tiles[blockIdx.x];
+  const BInst& B = I[t.inst];
+  const int tid = threadIdx.x + TX * threadIdx.y;
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  double ff = 0.0, rz = 0.0;
+  if (i < B.npx && j < B.npy) {
+    const long k = B.poff + (long)i + (long)B.npx * j;
+    const double rn = f[k];
+    x[k] = 0.0;
+    r[k] = rn;
+    const double zn = B.lap.dinv[(long)i + (long)B.npx * j] * rn;
+    z[k] = zn;
+    p[k] = zn;
+    ff = rn * rn;
+    rz = rn * zn;
+  }
+  (void)tid;
+  const double a = block_sum(ff, red);
+  const double b = block_sum(rz, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) { S.part[2 * blockIdx.x] = a; S.part[2 * blockIdx.x + 1] = b; }
+}
+
+__global__ void k_b_cg_init_fin(const BInst* __restrict__ I, BCG S, double tol2) {
+  __shared__ double red[4];
+  const int k = blockIdx.x;
+  const BInst& B = I[k];
+  double a = 0.0, b = 0.0;
+  for (int t = B.pt0 + threadIdx.x; t < B.pt1; t += blockDim.x) { a += S.part[2 * t]; b += S.part[2 * t + 1]; }
+  const double ff = block_sum(a, red);
+  const double rz = block_sum(b, red);
+  if (threadIdx.x == 0) {
+    S.s[8 * k + 0] = rz;
+    S.s[8 * k + 1] = ff;
+    S.s[8 * k + 4] = 1.0;
+    S.s[8 * k + 5] = tol2;
+    S.f[4 * k + 1] = 0;
+    S.f[4 * k + 2] = 0;
+    const int done = (ff <= tol2) ? 1 : 0;   // cg.F:222
+    S.f[4 * k + 0] = done;
+    if (done) atomicSub(S.nactive, 1);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_b_cg_apply(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
+                                                   BCG S, const double* __restrict__ p, double* __restrict__ q) {
+  __shared__ double red[4];
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  double acc = 0.0;
+  if (i < B.npx && j < B.npy) {
+    const long k = B.poff + (long)i + (long)B.npx * j;
+    const double a = lap_at(B.lap, p + B.poff, i, j);
+    q[k] = a;
+    acc = p[k] * a;
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) S.part[2 * blockIdx.x] = s;
+}
+
+__global__ void k_b_cg_alpha(const BInst* __restrict__ I, BCG S) {
+  __shared__ double red[4];
+  const int k = blockIdx.x;
+  if (S.f[4 * k]) return;
+  const BInst& B = I[k];
+  double a = 0.0;
+  for (int t = B.pt0 + threadIdx.x; t < B.pt1; t += blockDim.x) a += S.part[2 * t];
+  const double pq = block_sum(a, red);
+  if (threadIdx.x == 0) S.s[8 * k + 2] = S.s[8 * k + 0] / pq;
+}
+
+__global__ void __launch_bounds__(256) k_b_cg_update(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
+                                                    BCG S, const double* __restrict__ p,
+                                                    const double* __restrict__ q, double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ z) {
+  __shared__ double red[4];
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  const double alpha = S.s[8 * t.inst + 2];
+  double rr = 0.0, rz = 0.0;
+  if (i < B.npx && j < B.npy) {
+    const long k = B.poff + (long)i + (long)B.npx * j;
+    x[k] += alpha * p[k];
+    const double rn = r[k] - alpha * q[k];
+    r[k] = rn;
+    const double zn = B.lap.dinv[(long)i + (long)B.npx * j] * rn;
+    z[k] = zn;
+    rr = rn * rn;
+    rz = rn * zn;
+  }
+  const double a = block_sum(rr, red);
+  const double b = block_sum(rz, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) { S.part[2 * blockIdx.x] = a; S.part[2 * blockIdx.x + 1] = b; }
+}
+
+__global__ void k_b_cg_beta(const BInst* __restrict__ I, BCG S) {
+  __shared__ double red[4];
+  const int k = blockIdx.x;
+  if (S.f[4 * k]) return;
+  const BInst& B = I[k];
+  double a = 0.0, b = 0.0;
+  for (int t = B.pt0 + threadIdx.x; t < B.pt1; t += blockDim.x) { a += S.part[2 * t]; b += S.part[2 * t + 1]; }
+  const double rr = block_sum(a, red);
+  const double rz = block_sum(b, red);
+  if (threadIdx.x == 0) {
+    const int it = S.f[4 * k + 1] + 1;
+    S.f[4 * k + 1] = it;
+    const double rel = rr / S.s[8 * k + 1];
+    S.s[8 * k + 4] = rel;
+    int done = 0;
+    if (rel <= S.s[8 * k + 5]) {
+      done = 1;
+    } else if (it >= S.maxit) {
+      done = 1;
+      S.f[4 * k + 2] = 1;
+    }
+    S.s[8 * k + 3] = rz / S.s[8 * k + 0];
+    S.s[8 * k + 0] = rz;
+    if (done) {
+      S.f[4 * k + 0] = 1;
+      atomicSub(S.nactive, 1);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_b_cg_dir(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
+                                                 const double* __restrict__ z, double* __restrict__ p) {
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  if (i >= B.npx || j >= B.npy) return;
+  const long k = B.poff + (long)i + (long)B.npx * j;
+  p[k] = z[k] + S.s[8 * t.inst + 3] * p[k];
+}
+
+__global__ void k_b_set_int(int* x, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *x = v;
+}
+
+}  // namespace ins2d
+
+using namespace ins2d;
+
+namespace {
+
+struct BErr : std::runtime_error {
+  int code;
+  BErr(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define BHIP(x)                                                                                     \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) throw BErr(INS2D_E_HIP, std::string(#x ": ") + hipGetErrorString(e_));   \
+  } while (0)
+
+// single device buffer holding every operator array of the ensemble
+struct Arena {
+  std::vector<char> host;
+  char* dev = nullptr;
+  size_t cap = 0;
+  bool counting = true;
+  size_t used = 0;
+  template <typename T>
+  T* push(const std::vector<T>& v) {
+    const size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    used = (used + 255) & ~size_t(255);
+    size_t off = used;
+    used += bytes;
+    if (counting) return nullptr;
+    if (used > cap) throw std::runtime_error("arena overflow");
+    if (!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
+    return (T*)(dev + off);
+  }
+};
+
+DOp1 a_op(Arena& A, const HOp1& o) {
+  DOp1 d{};
+  d.nrows = o.nrows; d.ncols = o.ncols; d.K = o.K;
+  d.col = A.push(o.col);
+  d.val = A.push(o.val);
+  return d;
+}
+DKron a_kron(Arena& A, const HKron& k) { return DKron{a_op(A, k.X), a_op(A, k.Y)}; }
+
+}  // namespace
+
+struct ins2d_batch {
+  std::string err;
+  hipStream_t st = nullptr;
+  int n = 0;
+  std::vector<std::unique_ptr<Plan>> plans;
+  std::vector<BCSamples> samp;
+  std::vector<DComp> hcomp;   // 2 per instance, host copies with device pointers
+  DComp* dcomp = nullptr;
+  std::vector<BInst> hinst;
+  BInst* dinst = nullptr;
+  std::vector<Tile> uvt, pt;
+  Tile *duv = nullptr, *dpt = nullptr;
+  Arena ops;                  // operator arrays
+  std::vector<void*> allocs, yv_allocs;
+  long NVt = 0, Npt = 0;
+  double *V = nullptr, *Vb = nullptr, *P = nullptr, *Vt = nullptr, *F = nullptr, *dP = nullptr, *tV = nullptr;
+  std::vector<double*> kv;
+  double *x = nullptr, *r = nullptr, *z = nullptr, *pp = nullptr, *q = nullptr;
+  double* dt = nullptr;
+  BCG cg{};
+  double tol = 1e-10;
+  int maxit = 1000;
+  int s = 0;
+  std::vector<double> At, ct;
+  std::vector<bool> keep;
+  int p_add = 1;
+  hipGraphExec_t cg_chunk = nullptr;
+  hipGraph_t cg_graph = nullptr;
+  int chunk = 16;
+  long long total_it = 0;
+  int max_it = 0;
+  std::vector<int> last_it;
+  bool bc_dirty = false;
+};
+
+namespace {
+
+template <typename T>
+T* balloc(ins2d_batch* b, size_t n) {
+  void* p = nullptr;
+  BHIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  b->allocs.push_back(p);
+  return (T*)p;
+}
+
+void b_upload_bc(ins2d_batch* b) {
+  for (void* p : b->yv_allocs) (void)hipFree(p);
+  b->yv_allocs.clear();
+  // size, then fill one buffer with every boundary term of every instance
+  std::vector<std::pair<DYV*, std::vector<std::vector<double>>>> pending;
+  for (int k = 0; k < b->n; ++k) {
+    const Plan& P = *b->plans[k];
+    for (int c = 0; c < 2; ++c) {
+      const HComp& H = c == 0 ? P.cu : P.cv;
+      DComp& D = b->hcomp[2 * k + c];
+      auto add = [&](const HYVec& Y, DYV& dy) {
+        std::vector<std::vector<double>> ax, ay;
+        eval_yvec(Y, b->samp[k], ax, ay);
+        if (ax.size() > 4) throw BErr(INS2D_E_ARG, "too many boundary terms");
+        dy.nt = (int)ax.size();
+        std::vector<std::vector<double>> v;
+        for (size_t t = 0; t < ax.size(); ++t) { v.push_back(ax[t]); v.push_back(ay[t]); }
+        pending.push_back({&dy, v});
+      };
+      for (int t = 0; t < D.nconv; ++t) { add(H.conv[t].yI, D.conv[t].yI); add(H.conv[t].yA, D.conv[t].yA); }
+      for (int t = 0; t < D.ndiff; ++t) add(H.diff[t].yS, D.diff[t].yS);
+      add(H.yM, D.yM);
+    }
+  }
+  size_t total = 0;
+  for (auto& pe : pending)
+    for (auto& v : pe.second) total += v.size();
+  double* dbuf = nullptr;
+  BHIP(hipMalloc(&dbuf, std::max<size_t>(total, 1) * sizeof(double)));
+  b->yv_allocs.push_back(dbuf);
+  std::vector<double> hbuf(std::max<size_t>(total, 1));
+  size_t off = 0;
+  for (auto& pe : pending) {
+    DYV& dy = *pe.first;
+    for (int t = 0; t < 4; ++t) { dy.ax[t] = nullptr; dy.ay[t] = nullptr; }
+    for (int t = 0; t < dy.nt; ++t) {
+      const auto& ax = pe.second[2 * t];
+      const auto& ay = pe.second[2 * t + 1];
+      std::copy(ax.begin(), ax.end(), hbuf.begin() + off);
+      dy.ax[t] = dbuf + off;
+      off += ax.size();
+      std::copy(ay.begin(), ay.end(), hbuf.begin() + off);
+      dy.ay[t] = dbuf + off;
+      off += ay.size();
+    }
+  }
+  BHIP(hipMemcpy(dbuf, hbuf.data(), std::max<size_t>(total, 1) * sizeof(double), hipMemcpyHostToDevice));
+  BHIP(hipMemcpy(b->dcomp, b->hcomp.data(), b->hcomp.size() * sizeof(DComp), hipMemcpyHostToDevice));
+}
+
+void build_ops(ins2d_batch* b, Arena& A) {
+  for (int k = 0; k < b->n; ++k) {
+    const Plan& P = *b->plans[k];
+    for (int c = 0; c < 2; ++c) {
+      const HComp& H = c == 0 ? P.cu : P.cv;
+      DComp& D = b->hcomp[2 * k + c];
+      D.nx = H.nx; D.ny = H.ny;
+      D.nconv = (int)H.conv.size();
+      D.ndiff = (int)H.diff.size();
+      for (int t = 0; t < D.nconv; ++t) {
+        D.conv[t].D = a_kron(A, H.conv[t].D);
+        D.conv[t].I = a_kron(A, H.conv[t].I);
+        D.conv[t].A = a_kron(A, H.conv[t].A);
+        D.conv[t].icomp = H.conv[t].icomp;
+        D.conv[t].scale = H.conv[t].scale;
+      }
+      for (int t = 0; t < D.ndiff; ++t) {
+        D.diff[t].D = a_kron(A, H.diff[t].D[0]);
+        D.diff[t].S = a_kron(A, H.diff[t].S[0]);
+        D.diff[t].sscale = H.diff[t].sscale[0];
+      }
+      D.ox = A.push(H.ox);
+      D.oy = A.push(H.oy);
+      D.G = a_kron(A, H.G);
+      D.M = a_kron(A, H.M);
+    }
+    BInst& I = b->hinst[k];
+    I.lap.L1 = a_kron(A, P.L1);
+    I.lap.L2 = a_kron(A, P.L2);
+    std::vector<double> dinv(P.Adiag.size());
+    for (size_t q = 0; q < dinv.size(); ++q) dinv[q] = 1.0 / P.Adiag[q];
+    I.lap.dinv = A.push(dinv);
+    I.lap.nx = P.Npx;
+    I.lap.ny = P.Npy;
+  }
+}
+
+void launch_tiles_rhs(ins2d_batch* b, const double* V, const double* P, int mode, double* out, const StageCombine& sc,
+                      double* vt) {
+  hipLaunchKernelGGL(k_b_rhs, dim3((unsigned)b->uvt.size()), dim3(TX, TY), 0, b->st, b->dinst, b->duv, V, P, mode,
+                     out, sc, vt);
+}
+void launch_tiles_div(ins2d_batch* b, const double* A, const double* B, const double* dt, double ca, double cb,
+                      double yc, double os, double* f) {
+  hipLaunchKernelGGL(k_b_div, dim3((unsigned)b->pt.size()), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, A, B, dt, ca,
+                     cb, yc, os, f);
+}
+void launch_tiles_proj(ins2d_batch* b, const double* Vn, const double* Vt, const double* dp, const double* dt,
+                       double c, double* V) {
+  hipLaunchKernelGGL(k_b_proj, dim3((unsigned)b->uvt.size()), dim3(TX, TY), 0, b->st, b->dinst, b->duv, Vn, Vt, dp,
+                     dt, c, V);
+}
+
+void cg_iter_launches(ins2d_batch* b) {
+  const unsigned np = (unsigned)b->pt.size();
+  hipLaunchKernelGGL(k_b_cg_apply, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q);
+  hipLaunchKernelGGL(k_b_cg_alpha, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
+  hipLaunchKernelGGL(k_b_cg_update, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q, b->x,
+                     b->r, b->z);
+  hipLaunchKernelGGL(k_b_cg_beta, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
+  hipLaunchKernelGGL(k_b_cg_dir, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->z, b->pp);
+}
+
+// batched A dp = f for every instance (x0 = 0); result in b->x
+void b_poisson(ins2d_batch* b, const double* f) {
+  const unsigned np = (unsigned)b->pt.size();
+  hipLaunchKernelGGL(k_b_set_int, dim3(1), dim3(1), 0, b->st, b->cg.nactive, b->n);
+  hipLaunchKernelGGL(k_b_cg_init, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, f, b->x, b->r, b->z,
+                     b->pp);
+  hipLaunchKernelGGL(k_b_cg_init_fin, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tol * b->tol);
+  if (!b->cg_chunk) {
+    BHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    for (int c = 0; c < b->chunk; ++c) cg_iter_launches(b);
+    BHIP(hipStreamEndCapture(b->st, &b->cg_graph));
+    BHIP(hipGraphInstantiate(&b->cg_chunk, b->cg_graph, nullptr, nullptr, 0));
+  }
+  int active = b->n;
+  while (true) {
+    BHIP(hipGraphLaunch(b->cg_chunk, b->st));
+    BHIP(hipMemcpyAsync(&active, b->cg.nactive, sizeof(int), hipMemcpyDeviceToHost, b->st));
+    BHIP(hipStreamSynchronize(b->st));
+    if (active <= 0) break;
+  }
+  std::vector<int> fl(4 * (size_t)b->n);
+  BHIP(hipMemcpy(fl.data(), b->cg.f, fl.size() * sizeof(int), hipMemcpyDeviceToHost));
+  b->last_it.assign(b->n, 0);
+  for (int k = 0; k < b->n; ++k) {
+    b->last_it[k] = fl[4 * k + 1];
+    b->total_it += fl[4 * k + 1];
+    b->max_it = std::max(b->max_it, fl[4 * k + 1]);
+    if (fl[4 * k + 2]) throw BErr(INS2D_E_NOCONV, "CG did not converge in maxit iterations (instance " + std::to_string(k) + ")");
+  }
+}
+
+void b_additional(ins2d_batch* b, const double* V) {
+  launch_tiles_rhs(b, V, b->P, RHS_WITH_P | RHS_SCALE_OM, b->tV, StageCombine{0, 0.0, {0}, {nullptr}}, nullptr);
+  launch_tiles_div(b, b->tV, nullptr, nullptr, 1.0, 0.0, 0.0, 1.0, b->F);
+  b_poisson(b, b->F);
+  launch_axpy(b->st, (size_t)b->Npt, 1.0, b->x, b->P);
+}
+
+void b_step(ins2d_batch* b) {
+  double* Vn = b->V;
+  double* Vs = b->Vb;
+  for (int i = 0; i < b->s; ++i) {
+    const double* Vsrc = i == 0 ? Vn : Vs;
+    StageCombine sc{};
+    sc.aself = b->At[i * b->s + i];
+    for (int j = 0; j < i; ++j) {
+      const double a = b->At[i * b->s + j];
+      if (a != 0.0) { sc.a[sc.n] = a; sc.k[sc.n] = b->kv[j]; ++sc.n; }
+    }
+    launch_tiles_rhs(b, Vsrc, b->P, RHS_SCALE_OM, b->keep[i] ? b->kv[i] : nullptr, sc, b->Vt);
+    const double c = b->ct[i];
+    const double* dpp = b->P;
+    if (i > 0) {
+      launch_tiles_div(b, Vn, b->Vt, b->dt, 0.0, 1.0, 0.0, 1.0 / c, b->F);
+      b_poisson(b, b->F);
+      dpp = b->x;
+    }
+    launch_tiles_proj(b, Vn, b->Vt, dpp, b->dt, c, Vs);
+  }
+  std::swap(b->V, b->Vb);
+  if (b->p_add) b_additional(b, b->V);
+}
+
+int bfail(ins2d_batch* b, const std::exception& e, int code) {
+  if (b) b->err = e.what();
+  return code;
+}
+
+#define B_TRY try {
+#define B_CATCH(b)                                                  \
+  }                                                                 \
+  catch (const BErr& e) { return bfail(b, e, e.code); }             \
+  catch (const std::exception& e) { return bfail(b, e, INS2D_E_ARG); }
+
+std::string g_batch_err;
+
+}  // namespace
+
+extern "C" {
+
+const char* ins2d_batch_last_error(const ins2d_batch* b) { return b ? b->err.c_str() : g_batch_err.c_str(); }
+
+int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
+  if (!descs || n < 1 || !out) return INS2D_E_ARG;
+  *out = nullptr;
+  auto b = std::make_unique<ins2d_batch>();
+  try {
+    b->n = n;
+    const ins2d_case_desc& d0 = descs[0];
+    if (d0.rk_stages < 1 || d0.rk_stages > 8) throw BErr(INS2D_E_ARG, "bad tableau");
+    b->s = d0.rk_stages;
+    b->tol = d0.cg_tol;
+    b->maxit = d0.cg_maxit;
+    b->p_add = d0.p_add_solve;
+    for (int k = 0; k < n; ++k) {
+      const ins2d_case_desc& d = descs[k];
+      if (d.order4) throw BErr(INS2D_E_UNSUPPORTED, "the ensemble runs the second-order operators");
+      if (d.poisson == INS2D_POISSON_FFT) throw BErr(INS2D_E_UNSUPPORTED, "the ensemble solves the pressure by PCG");
+      if (d.rk_stages != b->s) throw BErr(INS2D_E_ARG, "all instances share one tableau");
+      for (int q = 0; q < b->s * b->s; ++q)
+        if (d.rk_A[q] != d0.rk_A[q]) throw BErr(INS2D_E_ARG, "all instances share one tableau");
+      CaseDesc cd;
+      cd.Nx = d.Nx; cd.Ny = d.Ny;
+      cd.x.assign(d.x, d.x + d.Nx + 1);
+      cd.y.assign(d.y, d.y + d.Ny + 1);
+      cd.u_left = d.bc_u[0]; cd.u_right = d.bc_u[1]; cd.u_low = d.bc_u[2]; cd.u_up = d.bc_u[3];
+      cd.v_left = d.bc_v[0]; cd.v_right = d.bc_v[1]; cd.v_low = d.bc_v[2]; cd.v_up = d.bc_v[3];
+      cd.Re = d.Re;
+      b->plans.emplace_back(new Plan());
+      build_plan(cd, *b->plans.back());
+    }
+    BHIP(hipStreamCreate(&b->st));
+    b->hcomp.assign(2 * (size_t)n, DComp{});
+    b->hinst.assign(n, BInst{});
+    b->samp.resize(n);
+    // operator arena: size, allocate, fill, upload
+    Arena A;
+    build_ops(b.get(), A);
+    A.cap = A.used + 256;
+    A.counting = false;
+    A.used = 0;
+    A.host.assign(A.cap, 0);
+    BHIP(hipMalloc((void**)&A.dev, A.cap));
+    b->allocs.push_back(A.dev);
+    build_ops(b.get(), A);
+    BHIP(hipMemcpy(A.dev, A.host.data(), A.cap, hipMemcpyHostToDevice));
+    // layout and tiles
+    long voff = 0, poff = 0;
+    for (int k = 0; k < n; ++k) {
+      const Plan& P = *b->plans[k];
+      BInst& I = b->hinst[k];
+      I.nux = P.Nux_in; I.nuy = P.Nuy_in; I.nvx = P.Nvx_in; I.nvy = P.Nvy_in; I.npx = P.Npx; I.npy = P.Npy;
+      I.nu = P.Nu;
+      I.voff = voff;
+      I.poff = poff;
+      voff += P.NV;
+      poff += P.Np;
+      for (int c = 0; c < 2; ++c) {
+        const int nx = c == 0 ? I.nux : I.nvx, ny = c == 0 ? I.nuy : I.nvy;
+        for (int j = 0; j < ny; j += TY)
+          for (int i = 0; i < nx; i += TX) b->uvt.push_back(Tile{k, c, i, j});
+      }
+      I.pt0 = (int)b->pt.size();
+      for (int j = 0; j < I.npy; j += TY)
+        for (int i = 0; i < I.npx; i += TX) b->pt.push_back(Tile{k, 2, i, j});
+      I.pt1 = (int)b->pt.size();
+      // samples default to zero
+      BCSamples& S = b->samp[k];
+      for (int s = 0; s < S_NSRC; ++s) S.s[s].assign(sample_len(P, s), 0.0);
+    }
+    b->NVt = voff;
+    b->Npt = poff;
+    b->dcomp = balloc<DComp>(b.get(), 2 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+      b->hinst[k].cu = b->dcomp + 2 * k;
+      b->hinst[k].cv = b->dcomp + 2 * k + 1;
+    }
+    b_upload_bc(b.get());
+    b->dinst = balloc<BInst>(b.get(), n);
+    BHIP(hipMemcpy(b->dinst, b->hinst.data(), n * sizeof(BInst), hipMemcpyHostToDevice));
+    b->duv = balloc<Tile>(b.get(), b->uvt.size());
+    BHIP(hipMemcpy(b->duv, b->uvt.data(), b->uvt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    b->dpt = balloc<Tile>(b.get(), b->pt.size());
+    BHIP(hipMemcpy(b->dpt, b->pt.data(), b->pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    // tableau (time_ERK.m:29,32)
+    const int s = b->s;
+    b->At.assign((size_t)s * s, 0.0);
+    b->ct.assign(s, 0.0);
+    for (int i = 0; i < s; ++i) {
+      for (int j = 0; j < s; ++j) b->At[i * s + j] = (i < s - 1) ? d0.rk_A[(i + 1) * s + j] : d0.rk_b[j];
+      b->ct[i] = (i < s - 1) ? d0.rk_c[i + 1] : 1.0;
+    }
+    b->keep.assign(s, false);
+    for (int j = 0; j < s; ++j)
+      for (int i = j + 1; i < s; ++i)
+        if (b->At[i * s + j] != 0.0) b->keep[j] = true;
+    // state and work buffers
+    b->V = balloc<double>(b.get(), b->NVt);
+    b->Vb = balloc<double>(b.get(), b->NVt);
+    b->Vt = balloc<double>(b.get(), b->NVt);
+    b->tV = balloc<double>(b.get(), b->NVt);
+    b->kv.assign(s, nullptr);
+    for (int j = 0; j < s; ++j)
+      if (b->keep[j]) b->kv[j] = balloc<double>(b.get(), b->NVt);
+    b->P = balloc<double>(b.get(), b->Npt);
+    b->F = balloc<double>(b.get(), b->Npt);
+    b->x = balloc<double>(b.get(), b->Npt);
+    b->r = balloc<double>(b.get(), b->Npt);
+    b->z = balloc<double>(b.get(), b->Npt);
+    b->pp = balloc<double>(b.get(), b->Npt);
+    b->q = balloc<double>(b.get(), b->Npt);
+    b->dt = balloc<double>(b.get(), n);
+    b->cg.s = balloc<double>(b.get(), 8 * (size_t)n);
+    b->cg.f = balloc<int>(b.get(), 4 * (size_t)n);
+    b->cg.part = balloc<double>(b.get(), 2 * b->pt.size());
+    b->cg.nactive = balloc<int>(b.get(), 1);
+    b->cg.maxit = b->maxit;
+    BHIP(hipMemset(b->V, 0, sizeof(double) * b->NVt));
+    BHIP(hipMemset(b->P, 0, sizeof(double) * b->Npt));
+    BHIP(hipDeviceSynchronize());
+  } catch (const BErr& e) {
+    g_batch_err = e.what();
+    for (void* p : b->allocs) (void)hipFree(p);
+    for (void* p : b->yv_allocs) (void)hipFree(p);
+    b->allocs.clear();
+    b->yv_allocs.clear();
+    if (b->st) (void)hipStreamDestroy(b->st);
+    b->st = nullptr;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_batch_err = e.what();
+    for (void* p : b->allocs) (void)hipFree(p);
+    for (void* p : b->yv_allocs) (void)hipFree(p);
+    b->allocs.clear();
+    b->yv_allocs.clear();
+    if (b->st) (void)hipStreamDestroy(b->st);
+    b->st = nullptr;
+    return INS2D_E_ARG;
+  }
+  *out = b.release();
+  return INS2D_OK;
+}
+
+void ins2d_batch_destroy(ins2d_batch* b) {
+  if (!b) return;
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->cg_chunk) (void)hipGraphExecDestroy(b->cg_chunk);
+  if (b->cg_graph) (void)hipGraphDestroy(b->cg_graph);
+  for (void* p : b->allocs) (void)hipFree(p);
+  for (void* p : b->yv_allocs) (void)hipFree(p);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
+int ins2d_batch_info(const ins2d_batch* b, int k, int* sizes, double* xin, double* yin) {
+  if (!b || k < 0 || k >= b->n || !sizes) return INS2D_E_ARG;
+  const Plan& P = *b->plans[k];
+  sizes[0] = P.Nux_in; sizes[1] = P.Nuy_in; sizes[2] = P.Nvx_in; sizes[3] = P.Nvy_in; sizes[4] = P.Npx;
+  sizes[5] = P.Npy;
+  if (xin) std::copy(P.xin.begin(), P.xin.end(), xin);
+  if (yin) std::copy(P.yin.begin(), P.yin.end(), yin);
+  return INS2D_OK;
+}
+
+int ins2d_batch_set_bc(ins2d_batch* b, int k, double t, const double* const* samples) {
+  if (!b || k < 0 || k >= b->n || !samples) return INS2D_E_ARG;
+  (void)t;
+  B_TRY
+  BCSamples& S = b->samp[k];
+  for (int s = 0; s < S_NSRC; ++s) {
+    if (!samples[s]) throw BErr(INS2D_E_ARG, "missing boundary sample");
+    std::copy(samples[s], samples[s] + S.s[s].size(), S.s[s].begin());
+  }
+  b->bc_dirty = true;
+  return INS2D_OK;
+  B_CATCH(b)
+}
+
+int ins2d_batch_set_state(ins2d_batch* b, int k, const double* V, const double* p) {
+  if (!b || k < 0 || k >= b->n) return INS2D_E_ARG;
+  B_TRY
+  const BInst& I = b->hinst[k];
+  const Plan& P = *b->plans[k];
+  if (V) BHIP(hipMemcpy(b->V + I.voff, V, sizeof(double) * P.NV, hipMemcpyHostToDevice));
+  if (p) BHIP(hipMemcpy(b->P + I.poff, p, sizeof(double) * P.Np, hipMemcpyHostToDevice));
+  return INS2D_OK;
+  B_CATCH(b)
+}
+
+int ins2d_batch_get_state(ins2d_batch* b, int k, double* V, double* p) {
+  if (!b || k < 0 || k >= b->n) return INS2D_E_ARG;
+  B_TRY
+  BHIP(hipStreamSynchronize(b->st));
+  const BInst& I = b->hinst[k];
+  const Plan& P = *b->plans[k];
+  if (V) BHIP(hipMemcpy(V, b->V + I.voff, sizeof(double) * P.NV, hipMemcpyDeviceToHost));
+  if (p) BHIP(hipMemcpy(p, b->P + I.poff, sizeof(double) * P.Np, hipMemcpyDeviceToHost));
+  return INS2D_OK;
+  B_CATCH(b)
+}
+
+int ins2d_batch_input_check(ins2d_batch* b, int p_initial) {
+  if (!b) return INS2D_E_ARG;
+  B_TRY
+  BHIP(hipStreamSynchronize(b->st));
+  b_upload_bc(b);
+  b->bc_dirty = false;
+  // initial projection of every instance (input_check.m:104-107); an
+  // instance that is already divergence-free gets dp ~ 0
+  launch_tiles_div(b, b->V, nullptr, nullptr, 1.0, 0.0, 1.0, 1.0, b->F);
+  b_poisson(b, b->F);
+  BHIP(hipMemcpyAsync(b->Vb, b->V, sizeof(double) * b->NVt, hipMemcpyDeviceToDevice, b->st));
+  launch_tiles_proj(b, b->Vb, nullptr, b->x, nullptr, 1.0, b->V);
+  if (p_initial) b_additional(b, b->V);
+  BHIP(hipStreamSynchronize(b->st));
+  return INS2D_OK;
+  B_CATCH(b)
+}
+
+int ins2d_batch_step(ins2d_batch* b, const double* dt, int nsteps) {
+  if (!b || !dt || nsteps < 0) return INS2D_E_ARG;
+  B_TRY
+  for (int k = 0; k < b->n; ++k)
+    if (!(dt[k] > 0)) throw BErr(INS2D_E_ARG, "time steps must be positive");
+  if (b->bc_dirty) {
+    BHIP(hipStreamSynchronize(b->st));
+    b_upload_bc(b);
+    b->bc_dirty = false;
+  }
+  BHIP(hipMemcpyAsync(b->dt, dt, sizeof(double) * b->n, hipMemcpyHostToDevice, b->st));
+  for (int n = 0; n < nsteps; ++n) b_step(b);
+  BHIP(hipStreamSynchronize(b->st));
+  return INS2D_OK;
+  B_CATCH(b)
+}
+
+int ins2d_batch_stats(const ins2d_batch* b, long long* total_iters, int* max_iters, long long* cells) {
+  if (!b) return INS2D_E_ARG;
+  if (total_iters) *total_iters = b->total_it;
+  if (max_iters) *max_iters = b->max_it;
+  if (cells) *cells = b->Npt;
+  return INS2D_OK;
+}
+
+}  // extern "C"

@@ -12,55 +12,9 @@
 //  * reductions for the per-step diagnostics.
 #include "device.hpp"
 #include "launch.hpp"
+#include "stencil.hpp"
 
 namespace ins2d {
-
-// ===========================================================================
-// kron-factored helpers
-// ===========================================================================
-__device__ __forceinline__ double kron_at(const DKron& K, const double* __restrict__ in, int f, int g) {
-  const int KY = K.Y.K, KX = K.X.K, ld = K.X.ncols;
-  const int* cy = K.Y.col + (size_t)g * KY;
-  const double* wy = K.Y.val + (size_t)g * KY;
-  const int* cx = K.X.col + (size_t)f * KX;
-  const double* wx = K.X.val + (size_t)f * KX;
-  double acc = 0.0;
-  for (int a = 0; a < KY; ++a) {
-    const double* row = in + (size_t)cy[a] * ld;
-    double s = 0.0;
-    for (int b = 0; b < KX; ++b) s += wx[b] * row[cx[b]];
-    acc += wy[a] * s;
-  }
-  return acc;
-}
-
-// kron applied to (ca * A + cb * B) without forming the sum
-__device__ __forceinline__ double kron_at2(const DKron& K, const double* __restrict__ A, double ca,
-                                           const double* __restrict__ B, double cb, int f, int g) {
-  const int KY = K.Y.K, KX = K.X.K, ld = K.X.ncols;
-  const int* cy = K.Y.col + (size_t)g * KY;
-  const double* wy = K.Y.val + (size_t)g * KY;
-  const int* cx = K.X.col + (size_t)f * KX;
-  const double* wx = K.X.val + (size_t)f * KX;
-  double acc = 0.0;
-  for (int a = 0; a < KY; ++a) {
-    const size_t ro = (size_t)cy[a] * ld;
-    double s = 0.0;
-    if (B) {
-      for (int b = 0; b < KX; ++b) s += wx[b] * (ca * A[ro + cx[b]] + cb * B[ro + cx[b]]);
-    } else {
-      for (int b = 0; b < KX; ++b) s += wx[b] * (ca * A[ro + cx[b]]);
-    }
-    acc += wy[a] * s;
-  }
-  return acc;
-}
-
-__device__ __forceinline__ double yv_at(const DYV& Y, int f, int g) {
-  double s = 0.0;
-  for (int t = 0; t < Y.nt; ++t) s += Y.ax[t][f] * Y.ay[t][g];
-  return s;
-}
 
 // ===========================================================================
 // generic momentum right-hand side
@@ -76,58 +30,7 @@ __global__ void __launch_bounds__(256) k_rhs_generic(const DComp* __restrict__ C
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= C.nx || j >= C.ny) return;
-  const double* own = COMP == 0 ? u : v;
-  double acc = 0.0;
-  for (int t = 0; t < C.nconv; ++t) {
-    const DConv& T = C.conv[t];
-    const double* cf = T.icomp == 0 ? u : v;
-    const int KDY = T.D.Y.K, KDX = T.D.X.K;
-    double s = 0.0;
-    for (int a = 0; a < KDY; ++a) {
-      const int g = T.D.Y.col[(size_t)j * KDY + a];
-      const double dy = T.D.Y.val[(size_t)j * KDY + a];
-      if (dy == 0.0) continue;
-      double sx = 0.0;
-      for (int b = 0; b < KDX; ++b) {
-        const int f = T.D.X.col[(size_t)i * KDX + b];
-        const double dx = T.D.X.val[(size_t)i * KDX + b];
-        if (dx == 0.0) continue;
-        const double fi = kron_at(T.I, cf, f, g) + yv_at(T.yI, f, g);
-        const double fa = kron_at(T.A, own, f, g) + yv_at(T.yA, f, g);
-        sx += dx * (fi * fa);
-      }
-      s += dy * sx;
-    }
-    acc -= T.scale * s;
-  }
-  for (int t = 0; t < C.ndiff; ++t) {
-    const DDiff& T = C.diff[t];
-    const int KDY = T.D.Y.K, KDX = T.D.X.K;
-    double s = 0.0;
-    for (int a = 0; a < KDY; ++a) {
-      const int g = T.D.Y.col[(size_t)j * KDY + a];
-      const double dy = T.D.Y.val[(size_t)j * KDY + a];
-      if (dy == 0.0) continue;
-      double sx = 0.0;
-      for (int b = 0; b < KDX; ++b) {
-        const int f = T.D.X.col[(size_t)i * KDX + b];
-        const double dx = T.D.X.val[(size_t)i * KDX + b];
-        if (dx == 0.0) continue;
-        sx += dx * (T.sscale * kron_at(T.S, own, f, g) + yv_at(T.yS, f, g));
-      }
-      s += dy * sx;
-    }
-    acc += s;
-  }
-  if (mode & RHS_WITH_P) acc -= kron_at(C.G, p, i, j);
-  if (mode & RHS_SCALE_OM) acc = acc / (C.ox[i] * C.oy[j]);
-  const size_t idx = (size_t)i + (size_t)C.nx * j;
-  if (out) out[idx] = acc;
-  if (vtemp) {
-    double vt = sc.aself * acc;
-    for (int q = 0; q < sc.n; ++q) vt += sc.a[q] * sc.k[q][idx];
-    vtemp[idx] = vt;
-  }
+  rhs_point<COMP>(C, u, v, p, mode, out, sc, vtemp, 0, i, j);
 }
 
 // f = oscale * ( Mu (ca A_u + cb B_u) + Mv (ca A_v + cb B_v) + ycoef yM )
@@ -138,10 +41,7 @@ __global__ void __launch_bounds__(256) k_div_generic(const DComp* __restrict__ C
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= npx || j >= npy) return;
-  double s = kron_at2(Cu->M, A, ca, B, cb, i, j);
-  s += kron_at2(Cv->M, A + nu, ca, B ? B + nu : nullptr, cb, i, j);
-  if (ycoef != 0.0) s += ycoef * (yv_at(Cu->yM, i, j) + yv_at(Cv->yM, i, j));
-  f[(size_t)i + (size_t)npx * j] = oscale * s;
+  f[(size_t)i + (size_t)npx * j] = oscale * div_point(*Cu, *Cv, A, B, ca, cb, ycoef, nu, i, j);
 }
 
 // V = Vn + dt * (Vtemp - c * Om^-1 G dp)      (time_ERK.m:132)
@@ -153,10 +53,7 @@ __global__ void __launch_bounds__(256) k_proj_generic(const DComp* __restrict__ 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= C.nx || j >= C.ny) return;
-  const size_t idx = (size_t)i + (size_t)C.nx * j;
-  const double g = kron_at(C.G, dp, i, j) / (C.ox[i] * C.oy[j]);
-  const double vt = Vt ? Vt[idx] : 0.0;
-  V[idx] = Vn[idx] + dt * (vt - c * g);
+  V[(size_t)i + (size_t)C.nx * j] = proj_point(C, Vn, Vt, dp, dt, c, i, j);
 }
 
 // ===========================================================================
@@ -560,39 +457,11 @@ __global__ void __launch_bounds__(256) k_fft_cols_solve(int Nx, int Ny, bool pow
 // ===========================================================================
 // Laplacian and Jacobi-PCG (cg.F:158-280 stopping rule ||r||^2/||f||^2 <= tol^2)
 // ===========================================================================
-__device__ __forceinline__ double lap_at(const DLap& L, const double* __restrict__ p, int i, int j) {
-  return kron_at(L.L1, p, i, j) + kron_at(L.L2, p, i, j);
-}
-
 __global__ void __launch_bounds__(256) k_lap(DLap L, const double* __restrict__ p, double* __restrict__ q) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= L.nx || j >= L.ny) return;
   q[(size_t)i + (size_t)L.nx * j] = lap_at(L, p, i, j);
-}
-
-// block reduction of a double (sum) -> returns on thread 0
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
-  return s;
-}
-__device__ __forceinline__ double block_max(double v, double* red) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s = fmax(s, red[w]);
-  return s;
 }
 
 // CG state on the device: s[0]=rz, s[1]=nf2 (|f|^2), s[2]=alpha, s[3]=beta,

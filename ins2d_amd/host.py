@@ -204,11 +204,8 @@ def _options(case, prm):
     return SimpleNamespace(grid=g, prm=prm, case_obj=case, fluid=SimpleNamespace(Re=float(prm['Re'])))
 
 
-def setup(case, **overrides):
-    """main.m:68-186 up to the start of the solver: returns (solver, options,
-    V_start, p_start) with the operators built and the boundary samples set."""
-    if isinstance(case, str):
-        case = load_case(case, **overrides)
+def _prepare(case):
+    """Parameters, options and mesh of a case (main.m:68-142)."""
     prm = dict(case.parameters())
     opts = _options(case, prm)
     x, y = case.mesh(opts)
@@ -223,15 +220,17 @@ def setup(case, **overrides):
     bct = case.BCtype()
     if int(bct.get('BC_unsteady', 0)):
         raise NotImplementedError('unsteady boundary conditions are not on this path')
-    solver = Solver(x, y, bct, prm['Re'], RK=prm.get('RK', 'RK44'), order4=prm.get('order4', 0),
-                    poisson=prm.get('poisson', POISSON_FFT), cg_tol=prm.get('CG_acc', 1e-8),
-                    cg_maxit=prm.get('CG_maxit', 1000), p_add_solve=prm.get('p_add_solve', 1),
-                    fft_symbol=prm.get('fft_symbol', 0), kernels=prm.get('kernels', 0),
-                    alfa=prm.get('alfa', 81.0), beta=prm.get('beta', 9 / 8))
-    g.xin, g.yin = solver.xin, solver.yin
-    g.Nux_in, g.Nuy_in, g.Nvx_in, g.Nvy_in = solver.Nux_in, solver.Nuy_in, solver.Nvx_in, solver.Nvy_in
-    g.Npx, g.Npy = solver.Npx, solver.Npy
-    # operator_mesh.m:401-414 (column-major 2-D arrays)
+    _check_bc(bct)
+    return prm, opts, bct
+
+
+def _finish(case, opts, sizes, xin, yin):
+    """Grid metrics, initial condition, time step and boundary samples
+    (operator_mesh.m:401-414, initialize.m, set_bc_vectors.m:53-71)."""
+    g = opts.grid
+    prm = opts.prm
+    g.xin, g.yin = xin, yin
+    g.Nux_in, g.Nuy_in, g.Nvx_in, g.Nvy_in, g.Npx, g.Npy = sizes[:6]
     g.xu = np.outer(g.xin, np.ones(g.Nuy_in)); g.yu = np.outer(np.ones(g.Nux_in), g.yp)
     g.xv = np.outer(g.xp, np.ones(g.Nvy_in)); g.yv = np.outer(np.ones(g.Nvx_in), g.yin)
     g.xpp = np.outer(g.xp, np.ones(g.Ny)); g.ypp = np.outer(np.ones(g.Nx), g.yp)
@@ -246,7 +245,122 @@ def setup(case, **overrides):
     t_end = prm.get('t_end')
     opts.nt = int(prm['nsteps']) if t_end is None else int(math.ceil((t_end - t) / opts.dt - 1e-12))
     opts.t_start = t
-    solver.set_bc(bc_samples(case, opts, t), t)
+    return V0, p0, bc_samples(case, opts, t)
+
+
+class Batch:
+    """Many cases advanced together by the batched ensemble of libins2d_hip
+    (second order, PCG pressure, one shared tableau)."""
+
+    def __init__(self, cases):
+        lib = _lib.load()
+        self.lib = lib
+        self.cases = cases
+        self.opts, descs, keep = [], [], []
+        RK = None
+        for c in cases:
+            prm, opts, bct = _prepare(c)
+            RK = RK or prm.get('RK', 'RK44')
+            if prm.get('RK', 'RK44') != RK:
+                raise ValueError('all ensemble members share one RK method')
+            A, b, cc = get_rk_method(RK)
+            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (opts.grid.x, opts.grid.y, A, b, cc)]
+            keep.append(arrs)
+            d = _lib.CaseDesc()
+            d.Nx, d.Ny = opts.grid.Nx, opts.grid.Ny
+            d.x, d.y = _dp(arrs[0]), _dp(arrs[1])
+            for k, side in enumerate(('left', 'right', 'low', 'up')):
+                d.bc_u[k] = BC_KIND[bct['u'][side]]
+                d.bc_v[k] = BC_KIND[bct['v'][side]]
+            d.Re = float(prm['Re'])
+            d.rk_stages = len(b)
+            d.rk_A, d.rk_b, d.rk_c = _dp(arrs[2]), _dp(arrs[3]), _dp(arrs[4])
+            d.poisson = int(prm.get('poisson', 3))
+            d.cg_tol = float(prm.get('CG_acc', 1e-10))
+            d.cg_maxit = int(prm.get('CG_maxit', 1000))
+            d.p_add_solve = int(prm.get('p_add_solve', 1))
+            descs.append(d)
+            self.opts.append(opts)
+        self._keep = keep
+        arr = (_lib.CaseDesc * len(descs))(*descs)
+        h = ctypes.c_void_p()
+        rc = lib.ins2d_batch_create(arr, len(descs), ctypes.byref(h))
+        if rc != 0:
+            raise Ins2dError(rc, lib.ins2d_batch_last_error(None).decode())
+        self.h = h
+        self.n = len(cases)
+        self.V0, self.p0, self.sizes = [], [], []
+        self._samples = []
+        for k, (c, opts) in enumerate(zip(cases, self.opts)):
+            sz = np.zeros(6, dtype=np.int32)
+            xin = np.zeros(opts.grid.Nx + 1)
+            yin = np.zeros(opts.grid.Ny + 1)
+            self._call(lib.ins2d_batch_info, ctypes.c_int(k), sz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                       _dp(xin), _dp(yin))
+            sizes = [int(v) for v in sz]
+            V0, p0, samples = _finish(c, opts, sizes, xin[:sizes[0]], yin[:sizes[3]])
+            self.sizes.append(sizes)
+            self.V0.append(V0)
+            self.p0.append(p0)
+            arrs = (ctypes.POINTER(ctypes.c_double) * 12)(*[_dp(s) for s in samples])
+            self._samples.append(samples)
+            self._call(lib.ins2d_batch_set_bc, ctypes.c_int(k), ctypes.c_double(opts.t_start), arrs)
+            self._call(lib.ins2d_batch_set_state, ctypes.c_int(k), _dp(V0), _dp(p0))
+        self.dt = np.array([o.dt for o in self.opts], dtype=np.float64)
+        self.cells = sum(s[4] * s[5] for s in self.sizes)
+
+    def _call(self, fn, *args):
+        rc = fn(self.h, *args)
+        if rc != 0:
+            raise Ins2dError(rc, self.lib.ins2d_batch_last_error(self.h).decode())
+
+    def input_check(self, p_initial=1):
+        self._call(self.lib.ins2d_batch_input_check, ctypes.c_int(p_initial))
+
+    def step(self, nsteps=1):
+        self._call(self.lib.ins2d_batch_step, _dp(self.dt), ctypes.c_int(nsteps))
+
+    def get_state(self, k):
+        s = self.sizes[k]
+        V = np.zeros(s[0] * s[1] + s[2] * s[3])
+        p = np.zeros(s[4] * s[5])
+        self._call(self.lib.ins2d_batch_get_state, ctypes.c_int(k), _dp(V), _dp(p))
+        return V, p
+
+    def stats(self):
+        tot = ctypes.c_longlong()
+        mx = ctypes.c_int()
+        cells = ctypes.c_longlong()
+        self._call(self.lib.ins2d_batch_stats, ctypes.byref(tot), ctypes.byref(mx), ctypes.byref(cells))
+        return tot.value, mx.value, cells.value
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self.lib.ins2d_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def setup(case, **overrides):
+    """main.m:68-186 up to the start of the solver: returns (solver, options,
+    V_start, p_start) with the operators built and the boundary samples set."""
+    if isinstance(case, str):
+        case = load_case(case, **overrides)
+    prm, opts, bct = _prepare(case)
+    g = opts.grid
+    solver = Solver(g.x, g.y, bct, prm['Re'], RK=prm.get('RK', 'RK44'), order4=prm.get('order4', 0),
+                    poisson=prm.get('poisson', POISSON_FFT), cg_tol=prm.get('CG_acc', 1e-8),
+                    cg_maxit=prm.get('CG_maxit', 1000), p_add_solve=prm.get('p_add_solve', 1),
+                    fft_symbol=prm.get('fft_symbol', 0), kernels=prm.get('kernels', 0),
+                    alfa=prm.get('alfa', 81.0), beta=prm.get('beta', 9 / 8))
+    sizes = [solver.Nux_in, solver.Nuy_in, solver.Nvx_in, solver.Nvy_in, solver.Npx, solver.Npy]
+    V0, p0, samples = _finish(case, opts, sizes, solver.xin, solver.yin)
+    solver.set_bc(samples, opts.t_start)
     return solver, opts, V0, p0
 
 
