@@ -33,8 +33,7 @@ def parse():
     ap.add_argument('--steps', type=int, default=200)
     ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--case', default='taylor_green')
-    ap.add_argument('--N', type=int, default=1024)
-    ap.add_argument('--order4', type=int, default=0)
+    ap.add_argument('--N', type=int, default=0)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-steps', type=int, default=10)
     ap.add_argument('--members', type=int, default=512)
@@ -103,6 +102,20 @@ def bench_ensemble(a):
     print(json.dumps(out))
 
 
+CASES = {
+    # workload name -> (case plugin, overrides, description)
+    'taylor_green': ('taylor_green', lambda N: dict(Nx=N or 1024, Ny=N or 1024),
+                     'Taylor-Green + noise, periodic {Nx}x{Ny}, order 2, RK44, FFT pressure, Re=2000'),
+    'decaying_turbulence': ('decaying_turbulence', lambda N: dict(Nx=N or 4096, Ny=N or 4096, order4=1, fft_symbol=1),
+                            'decaying turbulence, periodic {Nx}x{Ny}, order 4, RK44, FFT pressure (exact symbol), Re=1e4'),
+    'LDC_unsteady': ('LDC_unsteady', lambda N: dict(Nx=N or 64, Ny=N or 64, Re=1000.0, dt=1e-3, RK='RK44', poisson=3,
+                                                     CG_acc=1e-10, CG_maxit=20000),
+                     'lid-driven cavity {Nx}x{Ny}, order 2, RK44, dt=1e-3, Jacobi-PCG 1e-10, Re=1000'),
+    'BFS_unsteady': ('BFS_unsteady', lambda N: dict(Nx=N or 1024, Ny=(N // 4 if N else 256)),
+                     'backward-facing step {Nx}x{Ny} tanh-stretched, order 2, RK44, Jacobi-PCG 1e-10, Re=800'),
+}
+
+
 def main():
     a = parse()
     if a.case == 'ensemble':
@@ -118,15 +131,14 @@ def main():
         have_torch = False
     from ins2d_amd import host
 
-    case_args = dict(name=a.case, Nx=a.N, Ny=a.N, order4=a.order4)
-    if a.case == 'decaying_turbulence':
-        case_args.update(fft_symbol=1)
-    solver, opts, V0, p0 = host.setup(case_args.pop('name'), **case_args)
-    case_args['name'] = a.case
+    plugin, mk, desc = CASES[a.case]
+    over = mk(a.N)
+    solver, opts, V0, p0 = host.setup(plugin, **over)
     solver.set_state(V0, p0)
     solver.input_check(0.0, 1)
     dt = opts.dt
-    stages = len(host.get_rk_method(opts.prm.get('RK', 'RK44'))[1])
+    RK = opts.prm.get('RK', 'RK44')
+    stages = len(host.get_rk_method(RK)[1])
     cells = solver.Np
     t = 0.0
     solver.step(t, dt, a.warmup)
@@ -134,17 +146,19 @@ def main():
     solver.sync()
     if have_torch:
         torch.cuda.synchronize()
+    it0 = solver.solver_stats()[2]
     t0 = time.perf_counter()
     solver.step(t, dt, a.steps)
     solver.sync()
     if have_torch:
         torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    it1 = solver.solver_stats()[2]
     t += a.steps * dt
     value = cells * stages * a.steps / el / 1e6
 
     # per-kernel timing with HIP events on the library's stream (separate pass)
-    nprof = max(5, min(a.steps, 50))
+    nprof = max(2, min(a.steps, 50))
     solver.profile(True)
     solver.step(t, dt, nprof)
     solver.sync()
@@ -154,12 +168,13 @@ def main():
         ms, n = solver.profile_read(k)
         prof[nm] = {'ms_total': ms, 'launches': n, 'us_per_launch': (1e3 * ms / n) if n else None}
     solver.profile(False)
+    t += nprof * dt
     diag = solver.diagnostics(t)
 
     # algorithmic bytes of the momentum RHS kernel (fused stage combination):
     # reads V (2 fields), writes k_i when a later stage needs it and Vtemp
     # (2 fields each), reads the earlier k_j of the stage's tableau row.
-    A, b, c = host.get_rk_method(opts.prm.get('RK', 'RK44'))
+    A, b, c = host.get_rk_method(RK)
     s = len(b)
     At = np.vstack([A[1:], b[None, :]])
     keep = [any(At[i, j] != 0 for i in range(j + 1, s)) for j in range(s)]
@@ -183,8 +198,7 @@ def main():
         'vs_baseline': None,
         'dtype': 'f64',
         'data': 'synthetic',
-        'config': {'workload': f'{a.case} {opts.grid.Nx}x{opts.grid.Ny} periodic, order {4 if a.order4 else 2}, '
-                               f'{opts.prm.get("RK", "RK44")}, FFT pressure, Re={opts.prm["Re"]:g}, dt={dt:.4e}',
+        'config': {'workload': desc.format(Nx=opts.grid.Nx, Ny=opts.grid.Ny) + f', dt={dt:.4e}',
                    'cells': cells, 'rk_stages': stages, 'kernel_path': solver.path},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': (achieved / HBM_PEAK_GBS) if achieved else None, 'traffic': None,
@@ -193,9 +207,17 @@ def main():
         'kernels': prof,
         'diagnostics': {'maxres': diag[0], 'maxdiv': diag[1], 'k': diag[4]},
     }
+    if it1 > it0:
+        out['config']['pcg_iterations_per_solve'] = (it1 - it0) / (a.steps * (s - 1 + 1))
     solver.close()
     if not a.no_cpu_baseline:
-        out['cpu_baseline'] = cpu_baseline(case_args, a.cpu_steps)
+        cpu_over = dict(over)
+        if cpu_over.get('poisson', 6) != 6:
+            cpu_over['poisson'] = 1   # the reference's default direct solve on the CPU side
+        if plugin == 'decaying_turbulence':
+            # the CPU sample runs the same case on a 1024^2 grid (per-cell cost)
+            cpu_over.update(Nx=min(opts.grid.Nx, 1024), Ny=min(opts.grid.Ny, 1024))
+        out['cpu_baseline'] = cpu_baseline(dict(name=plugin, **cpu_over), a.cpu_steps)
     print(json.dumps(out))
 
 
