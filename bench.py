@@ -27,6 +27,10 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(*a):
+    print('[bench]', *a, file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -51,27 +55,48 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(case_args, steps):
+def cpu_baseline(case_args, steps, timeout_s=240):
     """The CPU oracle (numpy/scipy restatement of the reference algorithm)
-    timed on a bounded sample of the same workload."""
+    timed on a bounded sample of the same workload, in a single-threaded
+    child process with its own time limit."""
+    import subprocess
+    env = dict(os.environ)
+    for k in ('OMP_NUM_THREADS', 'OPENBLAS_NUM_THREADS', 'MKL_NUM_THREADS'):
+        env[k] = '1'
+    cmd = [sys.executable, '-u', os.path.abspath(__file__), '--cpu-child', json.dumps(case_args), str(steps)]
+    try:
+        out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=timeout_s)
+        sys.stderr.write(out.stderr)
+        return json.loads(out.stdout.strip().splitlines()[-1])
+    except subprocess.TimeoutExpired:
+        return {'value': None, 'unit': 'Mcell-RK-stages/s', 'cores': 1, 'kind': 'port',
+                'sample': f'CPU oracle sample did not finish within {timeout_s} s'}
+
+
+def cpu_child(case_json, steps):
     from ins2d_amd.cases import load
     from oracle import ins2d_oracle as O
+    case_args = json.loads(case_json)
     case = load(**case_args)
     t0 = time.perf_counter()
     opts = O.build(case)
     V, p, _ = O.input_check(opts)
     t_setup = time.perf_counter() - t0
+    progress(f'cpu oracle setup {t_setup:.1f} s')
     dt = opts.time.dt
     t0 = time.perf_counter()
     for n in range(steps):
         V, p = O.time_ERK(V, p, n * dt, dt, opts)
+        progress(f'cpu oracle step {n + 1}/{steps}')
     el = time.perf_counter() - t0
     stages = len(O.get_rk_method(opts.time.RK)[1])
     cells = opts.grid.Np
-    return {'value': cells * stages * steps / el / 1e6, 'unit': 'Mcell-RK-stages/s', 'cores': 1, 'kind': 'port',
-            'sample': f'{steps} RK steps of the same {opts.grid.Nx}x{opts.grid.Ny} case '
-                      f'({el:.1f} s, setup {t_setup:.1f} s excluded), numpy/scipy single thread, CPU {cpu_model()}, '
-                      f'nproc {os.cpu_count()}'}
+    solver = {1: 'direct sparse LU', 6: 'FFT', 3: 'IC(0)-PCG', 4: 'IC(0)-PCG'}[opts.solversettings.poisson]
+    print(json.dumps({'value': cells * stages * steps / el / 1e6, 'unit': 'Mcell-RK-stages/s', 'cores': 1,
+                      'kind': 'port',
+                      'sample': f'{steps} RK steps of the {opts.grid.Nx}x{opts.grid.Ny} case, {solver} pressure '
+                                f'({el:.1f} s, setup {t_setup:.1f} s excluded), numpy/scipy single thread, '
+                                f'CPU {cpu_model()}, nproc {os.cpu_count()}'}))
 
 
 def bench_ensemble(a):
@@ -133,9 +158,11 @@ def main():
 
     plugin, mk, desc = CASES[a.case]
     over = mk(a.N)
+    progress('setup', a.case, over)
     solver, opts, V0, p0 = host.setup(plugin, **over)
     solver.set_state(V0, p0)
     solver.input_check(0.0, 1)
+    progress('input_check done')
     dt = opts.dt
     RK = opts.prm.get('RK', 'RK44')
     stages = len(host.get_rk_method(RK)[1])
@@ -147,6 +174,7 @@ def main():
     if have_torch:
         torch.cuda.synchronize()
     it0 = solver.solver_stats()[2]
+    progress('warmup done; timing', a.steps, 'steps')
     t0 = time.perf_counter()
     solver.step(t, dt, a.steps)
     solver.sync()
@@ -157,6 +185,7 @@ def main():
     t += a.steps * dt
     value = cells * stages * a.steps / el / 1e6
 
+    progress(f'timed region {el:.3f} s')
     # per-kernel timing with HIP events on the library's stream (separate pass)
     nprof = max(2, min(a.steps, 50))
     solver.profile(True)
@@ -210,7 +239,9 @@ def main():
     if it1 > it0:
         out['config']['pcg_iterations_per_solve'] = (it1 - it0) / (a.steps * (s - 1 + 1))
     solver.close()
+    progress('profile pass done')
     if not a.no_cpu_baseline:
+        progress('cpu baseline')
         cpu_over = dict(over)
         if cpu_over.get('poisson', 6) != 6:
             cpu_over['poisson'] = 1   # the reference's default direct solve on the CPU side
@@ -222,4 +253,7 @@ def main():
 
 
 if __name__ == '__main__':
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == '--cpu-child':
+        cpu_child(sys.argv[2], int(sys.argv[3]))
+    else:
+        main()
