@@ -65,8 +65,8 @@ def cpu_baseline(case_args, steps, timeout_s=240):
         env[k] = '1'
     cmd = [sys.executable, '-u', os.path.abspath(__file__), '--cpu-child', json.dumps(case_args), str(steps)]
     try:
-        out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=timeout_s)
-        sys.stderr.write(out.stderr)
+        # the child's progress lines go straight to this process's stderr
+        out = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True, timeout=timeout_s)
         return json.loads(out.stdout.strip().splitlines()[-1])
     except subprocess.TimeoutExpired:
         return {'value': None, 'unit': 'Mcell-RK-stages/s', 'cores': 1, 'kind': 'port',
@@ -74,6 +74,7 @@ def cpu_baseline(case_args, steps, timeout_s=240):
 
 
 def cpu_child(case_json, steps):
+    progress('cpu oracle child started')
     from ins2d_amd.cases import load
     from oracle import ins2d_oracle as O
     case_args = json.loads(case_json)

@@ -147,6 +147,47 @@ int ins2d_batch_step(ins2d_batch* b, const double* dt, int nsteps);
  * per-solve iteration count, total pressure cells */
 int ins2d_batch_stats(const ins2d_batch* b, long long* total_iters, int* max_iters, long long* cells);
 
+/* ---- learned Poisson warm start (no reference counterpart).  A U-shaped
+ * CNN (4 levels, 16/32/64/64 channels, 3x3 convolutions, BF16 weights and
+ * activations, FP32 accumulation on the matrix cores) maps a Poisson
+ * right-hand side f on an H x W pressure grid (x fastest, H, W multiples of 8)
+ * to p0 = out_scale * rms(f) * net(f / rms(f)), the start vector of PCG.
+ * Parameters are one flat FP32 vector (ins2d_nn_num_params entries). */
+typedef struct ins2d_nn ins2d_nn;
+int ins2d_nn_create(int H, int W, ins2d_nn** out);
+void ins2d_nn_destroy(ins2d_nn* n);
+const char* ins2d_nn_last_error(const ins2d_nn* n);
+long ins2d_nn_num_params(const ins2d_nn* n);
+int ins2d_nn_set_params(ins2d_nn* n, const float* params, double out_scale);
+int ins2d_nn_get_params(ins2d_nn* n, float* params);
+/* host arrays of H*W doubles */
+int ins2d_nn_forward(ins2d_nn* n, const double* f, double* p);
+/* device arrays, queued on the given HIP stream (NULL: the null stream) */
+int ins2d_nn_forward_dev(ins2d_nn* n, const double* f, double* p, void* stream);
+/* mean-squared-error loss of net(f_b / rms(f_b)) against p_b / (out_scale
+ * rms(f_b)) over a batch of device arrays; gradients into the gradient
+ * vector; mode 1 also takes one Adam step (lr, beta 0.9 / 0.999) */
+int ins2d_nn_train_step(ins2d_nn* n, const double* const* f, const double* const* p, int batch, float lr, int mode,
+                        double* loss_out, void* stream);
+int ins2d_nn_get_grad(ins2d_nn* n, float* grad);
+/* training pairs held by the network object in device memory: reserve
+ * room for count pairs, upload pair k from host arrays, and train on the
+ * pairs idx[0..batch) (as ins2d_nn_train_step) */
+int ins2d_nn_data_reserve(ins2d_nn* n, int count);
+int ins2d_nn_data_set(ins2d_nn* n, int k, const double* f, const double* p);
+int ins2d_nn_train(ins2d_nn* n, const int* idx, int batch, float lr, int mode, double* loss_out);
+/* one convolution layer's kernels on host FP32 arrays (inputs, weights and
+ * gradients rounded to BF16 where the kernels read BF16): out = act(conv(x)
+ * + bias); din, dw, db = the input, weight and bias gradients for the output
+ * gradient g.  Layouts as the network's: x (H, W, C) channels innermost,
+ * w[t][cout][ca + cb] with t = ks * (dy + r) + (dx + r). */
+int ins2d_nn_layer_check(int H, int W, int ca, int cb, int cout, int ks, int relu, const float* xa, const float* xb,
+                         const float* w, const float* bias, const float* g, float* out, float* din, float* dw,
+                         float* db);
+/* one 16x16x16 BF16 matrix-core product D = A B of row-major 16x16 FP32
+ * matrices rounded to BF16 (checks the operand layout the kernels use) */
+int ins2d_nn_mfma_probe(const float* A, const float* B, float* D);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(HERE, "libins2d_hip.so")
 c_int, c_double, c_void_p, c_char_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_char_p
 P_double = ctypes.POINTER(ctypes.c_double)
 P_int = ctypes.POINTER(ctypes.c_int)
+P_float = ctypes.POINTER(ctypes.c_float)
 
 
 class CaseDesc(ctypes.Structure):
@@ -49,6 +50,23 @@ SIGNATURES = {
     "ins2d_batch_input_check": (c_int, [c_void_p, c_int]),
     "ins2d_batch_step": (c_int, [c_void_p, P_double, c_int]),
     "ins2d_batch_stats": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong), P_int, ctypes.POINTER(ctypes.c_longlong)]),
+    "ins2d_nn_create": (c_int, [c_int, c_int, ctypes.POINTER(c_void_p)]),
+    "ins2d_nn_destroy": (None, [c_void_p]),
+    "ins2d_nn_last_error": (c_char_p, [c_void_p]),
+    "ins2d_nn_num_params": (ctypes.c_long, [c_void_p]),
+    "ins2d_nn_set_params": (c_int, [c_void_p, P_float, c_double]),
+    "ins2d_nn_get_params": (c_int, [c_void_p, P_float]),
+    "ins2d_nn_forward": (c_int, [c_void_p, P_double, P_double]),
+    "ins2d_nn_forward_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ins2d_nn_train_step": (c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int,
+                                    ctypes.c_float, c_int, P_double, c_void_p]),
+    "ins2d_nn_get_grad": (c_int, [c_void_p, P_float]),
+    "ins2d_nn_data_reserve": (c_int, [c_void_p, c_int]),
+    "ins2d_nn_data_set": (c_int, [c_void_p, c_int, P_double, P_double]),
+    "ins2d_nn_train": (c_int, [c_void_p, P_int, c_int, ctypes.c_float, c_int, P_double]),
+    "ins2d_nn_layer_check": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, P_float, P_float, P_float,
+                                     P_float, P_float, P_float, P_float, P_float, P_float]),
+    "ins2d_nn_mfma_probe": (c_int, [P_float, P_float, P_float]),
 }
 
 _LIB = None

@@ -1,0 +1,892 @@
+// Learned Poisson warm start: a 4-level U-shaped CNN (16/32/64/64 channels,
+// 3x3 convolutions, BF16 weights and activations, FP32 accumulation on the
+// MFMA units) with hand-written forward and backward kernels, a trainer step
+// (MSE loss, Adam) and the inference entry used by the pressure solver.
+//
+// Tensors are NHWC (x fastest over pixels, channels innermost), H = Npy rows,
+// W = Npx columns, so a pressure field in the solver's layout is a 1-channel
+// tensor.  A 3x3 convolution is an implicit GEMM: M = pixels, N = output
+// channels, K = 9 * input channels, computed with v_mfma_f32_16x16x16_bf16.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ins2d.h"
+#include "stencil.hpp"
+
+namespace ins2d {
+namespace nn {
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+
+// one input of a convolution (concatenation = two inputs)
+struct In {
+  const void* p;
+  int c;        // channels
+  int fp32;     // 1: float storage, 0: bf16
+};
+
+struct ConvArgs {
+  In a, b;              // channels [0, a.c) from a, [a.c, a.c + b.c) from b
+  int H, W, cout, ks;   // ks 3 (padding 1) or 1
+  const uint16_t* w;    // [taps][cout][cin] bf16
+  const float* bias;    // cout (or null)
+  void* out;
+  int out_fp32;
+  int relu;
+};
+
+__device__ __forceinline__ float load_ch(const In& s, long pix, int c) {
+  if (s.fp32) return ((const float*)s.p)[pix * s.c + c];
+  return bf2f(((const uint16_t*)s.p)[pix * s.c + c]);
+}
+
+// out[y][x][co] = act(bias[co] + sum_t sum_ci in[y+dy][x+dx][ci] w[t][co][ci])
+// grid (ceil(W/64), H, ceil(cout/64)); 256 threads = 4 waves; wave w owns
+// 16 pixels x 64 output channels (channels 64 * blockIdx.z + [0, 64))
+__global__ void __launch_bounds__(256) k_conv(ConvArgs A) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int y = blockIdx.y;
+  const int xw = blockIdx.x * 64 + wave * 16;
+  const int cin = A.a.c + A.b.c;
+  const int cob = 64 * blockIdx.z;
+  const int ntile = min(4, (A.cout - cob + 15) / 16);
+  float4_t acc[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) acc[n] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const int taps = A.ks * A.ks;
+  const int r = A.ks / 2;
+  const int xa = xw + (lane & 15);     // A-operand pixel of this lane
+  const int kq = 4 * (lane >> 4);      // K sub-index of this lane
+  for (int t = 0; t < taps; ++t) {
+    const int yy = y + t / A.ks - r, xx = xa + t % A.ks - r;
+    const bool inb = yy >= 0 && yy < A.H && xx >= 0 && xx < A.W;
+    const long pix = (long)yy * A.W + xx;
+    for (int c0 = 0; c0 < cin; c0 += 16) {
+      short4_t av;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int ci = c0 + kq + i;
+        float v = 0.f;
+        if (inb && ci < cin) v = ci < A.a.c ? load_ch(A.a, pix, ci) : load_ch(A.b, pix, ci - A.a.c);
+        av[i] = (short)f2bf(v);
+      }
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        if (n >= ntile) break;
+        const int co = cob + 16 * n + (lane & 15);
+        short4_t bv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int ci = c0 + kq + i;
+          bv[i] = (co < A.cout && ci < cin) ? (short)A.w[((long)t * A.cout + co) * cin + ci] : (short)0;
+        }
+        acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[n], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    if (n >= ntile) break;
+    const int co = cob + 16 * n + (lane & 15);
+    if (co >= A.cout) continue;
+    const float bias = A.bias ? A.bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int x = xw + 4 * (lane >> 4) + i;
+      if (x >= A.W) continue;
+      float v = acc[n][i] + bias;
+      if (A.relu) v = fmaxf(v, 0.f);
+      const long o = ((long)y * A.W + x) * A.cout + co;
+      if (A.out_fp32) ((float*)A.out)[o] = v;
+      else ((uint16_t*)A.out)[o] = f2bf(v);
+    }
+  }
+}
+
+// dw[t][co][ci] += sum_p g[p][co] in[p + off_t][ci]   (fp32 atomics)
+// grid (chunks, taps, tiles_co * tiles_ci); 64 threads
+__global__ void __launch_bounds__(64) k_conv_dw(In a, In b, const float* __restrict__ g, int H, int W, int cout,
+                                               int ks, int chunk, float* __restrict__ dw) {
+  const int lane = threadIdx.x;
+  const int cin = a.c + b.c;
+  const int tci = (cin + 15) / 16;
+  const int co0 = 16 * (blockIdx.z / tci), ci0 = 16 * (blockIdx.z % tci);
+  const int t = blockIdx.y;
+  const int r = ks / 2;
+  const int dy = t / ks - r, dx = t % ks - r;
+  const long npix = (long)H * W;
+  const long p0 = (long)blockIdx.x * chunk;
+  const long p1 = std::min<long>(p0 + chunk, npix);
+  float4_t acc = float4_t{0.f, 0.f, 0.f, 0.f};
+  const int com = co0 + (lane & 15);   // A row (output channel)
+  const int cin_n = ci0 + (lane & 15); // B column (input channel)
+  const int kq = 4 * (lane >> 4);
+  for (long p = p0; p < p1; p += 16) {
+    short4_t av, bv;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long pp = p + kq + i;
+      float ga = 0.f, xb = 0.f;
+      if (pp < p1) {
+        if (com < cout) ga = g[pp * cout + com];
+        const int y = (int)(pp / W), x = (int)(pp % W);
+        const int yy = y + dy, xx = x + dx;
+        if (cin_n < cin && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+          const long sp = (long)yy * W + xx;
+          xb = cin_n < a.c ? load_ch(a, sp, cin_n) : load_ch(b, sp, cin_n - a.c);
+        }
+      }
+      av[i] = (short)f2bf(ga);
+      bv[i] = (short)f2bf(xb);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int co = co0 + 4 * (lane >> 4) + i;
+    const int ci = ci0 + (lane & 15);
+    if (co < cout && ci < cin) atomicAdd(&dw[((long)t * cout + co) * cin + ci], acc[i]);
+  }
+}
+
+// db[co] += sum_p g[p][co]
+__global__ void k_bias_grad(const float* __restrict__ g, long npix, int cout, float* __restrict__ db) {
+  __shared__ float red[256];
+  const int co = blockIdx.y;
+  float s = 0.f;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < npix; p += (long)gridDim.x * blockDim.x)
+    s += g[p * cout + co];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(&db[co], red[0]);
+}
+
+// g = dout .* (out > 0) (ReLU backward; out is the stored bf16 activation)
+__global__ void k_relu_bwd(const float* __restrict__ dout, const uint16_t* __restrict__ out, long n,
+                           float* __restrict__ g) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    g[i] = bf2f(out[i]) > 0.f ? dout[i] : 0.f;
+}
+
+// 2x2 average pool (H, W even), bf16 -> bf16
+__global__ void k_pool(const uint16_t* __restrict__ in, int H, int W, int C, uint16_t* __restrict__ out) {
+  const int Ho = H / 2, Wo = W / 2;
+  const long n = (long)Ho * Wo * C;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long px = i / C;
+    const int x = (int)(px % Wo), y = (int)(px / Wo);
+    const long b = ((long)(2 * y) * W + 2 * x) * C + c;
+    const float s = bf2f(in[b]) + bf2f(in[b + C]) + bf2f(in[b + (long)W * C]) + bf2f(in[b + (long)W * C + C]);
+    out[i] = f2bf(0.25f * s);
+  }
+}
+// dIn[y][x] += dOut[y/2][x/2] / 4
+__global__ void k_pool_bwd(const float* __restrict__ dout, int H, int W, int C, float* __restrict__ din) {
+  const long n = (long)H * W * C;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long px = i / C;
+    const int x = (int)(px % W), y = (int)(px / W);
+    din[i] += 0.25f * dout[((long)(y / 2) * (W / 2) + x / 2) * C + c];
+  }
+}
+// nearest upsampling x2: out[y][x] = in[y/2][x/2]
+__global__ void k_up(const uint16_t* __restrict__ in, int Hs, int Ws, int C, uint16_t* __restrict__ out) {
+  const int H = 2 * Hs, W = 2 * Ws;
+  const long n = (long)H * W * C;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long px = i / C;
+    const int x = (int)(px % W), y = (int)(px / W);
+    out[i] = in[((long)(y / 2) * Ws + x / 2) * C + c];
+  }
+}
+// dIn[y][x] = sum of the 4 dOut children
+__global__ void k_up_bwd(const float* __restrict__ dout, int Hs, int Ws, int C, float* __restrict__ din) {
+  const int W = 2 * Ws;
+  const long n = (long)Hs * Ws * C;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long px = i / C;
+    const int x = (int)(px % Ws), y = (int)(px / Ws);
+    const long b = ((long)(2 * y) * W + 2 * x) * C + c;
+    din[i] = dout[b] + dout[b + C] + dout[b + (long)W * C] + dout[b + (long)W * C + C];
+  }
+}
+// split a (ca + cb)-channel gradient: da (=) first ca channels, db (+=) rest
+__global__ void k_split(const float* __restrict__ g, long npix, int ca, int cb, float* __restrict__ da,
+                        float* __restrict__ db) {
+  const int C = ca + cb;
+  const long n = npix * C;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long px = i / C;
+    if (c < ca) da[px * ca + c] = g[i];
+    else db[px * cb + (c - ca)] += g[i];
+  }
+}
+// weight transform for the backward-data convolution:
+// wt[8 - t][ci][co] = w[t][co][ci] (taps flipped, channels swapped)
+__global__ void k_wflip(const float* __restrict__ w, int taps, int cout, int cin, uint16_t* __restrict__ wt,
+                        uint16_t* __restrict__ wb) {
+  const long n = (long)taps * cout * cin;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int ci = (int)(i % cin);
+    const int co = (int)((i / cin) % cout);
+    const int t = (int)(i / ((long)cin * cout));
+    wb[i] = f2bf(w[i]);
+    wt[((long)(taps - 1 - t) * cin + ci) * cout + co] = f2bf(w[i]);
+  }
+}
+// network input: x = f / s (fp64 -> 4-channel bf16 [x, 0, 0, 0]); s = rms(f)
+__global__ void k_input(const double* __restrict__ f, long n, const double* __restrict__ s, uint16_t* __restrict__ x) {
+  const double inv = 1.0 / fmax(s[0], 1e-300);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    x[4 * i] = f2bf((float)(f[i] * inv));
+    x[4 * i + 1] = 0;
+    x[4 * i + 2] = 0;
+    x[4 * i + 3] = 0;
+  }
+}
+// p = scale * s * y (network output fp32 -> fp64)
+__global__ void k_output(const float* __restrict__ y, long n, const double* __restrict__ s, double scale,
+                         double* __restrict__ p) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = scale * s[0] * (double)y[i];
+}
+// s = sqrt(mean(f^2)) (single block)
+__global__ void k_rms(const double* __restrict__ f, long n, double* __restrict__ s) {
+  __shared__ double red[16];
+  double a = 0.0;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) a += f[i] * f[i];
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) s[0] = sqrt(a / (double)n);
+}
+// MSE loss of y against target t / (scale s): dy = 2 (y - tt) / n; loss += sum (y-tt)^2 / n
+__global__ void k_mse(const float* __restrict__ y, const double* __restrict__ t, long n, const double* __restrict__ s,
+                      double scale, float* __restrict__ dy, double* __restrict__ loss) {
+  __shared__ double red[16];
+  const double inv = 1.0 / (scale * fmax(s[0], 1e-300));
+  double a = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double d = (double)y[i] - t[i] * inv;
+    dy[i] = (float)(2.0 * d / (double)n);
+    a += d * d / (double)n;
+  }
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) atomicAdd(loss, a);
+}
+// Adam (fp32 master parameters)
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       long n, float lr, float b1, float b2, float eps, float c1, float c2, float gscale) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float gi = g[i] * gscale;
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+__global__ void k_zero_f(float* x, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = 0.f;
+}
+__global__ void k_zero_d(double* x, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = 0.0;
+}
+
+// one 16x16x16 bf16 MFMA of fp32 inputs rounded to bf16: D = A (16x16) B (16x16)
+// with A, B, D row-major; used to check the operand layout the kernels assume
+__global__ void k_mfma_probe(const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ D) {
+  const int lane = threadIdx.x;
+  short4_t av, bv;
+  for (int i = 0; i < 4; ++i) {
+    av[i] = (short)f2bf(A[(lane & 15) * 16 + 4 * (lane >> 4) + i]);
+    bv[i] = (short)f2bf(B[(4 * (lane >> 4) + i) * 16 + (lane & 15)]);
+  }
+  float4_t acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, float4_t{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  for (int i = 0; i < 4; ++i) D[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[i];
+}
+
+// ---------------------------------------------------------------------------
+// network definition
+// ---------------------------------------------------------------------------
+struct LayerDef {
+  int level, cin_a, cin_b, cout, ks, relu;
+};
+// c1a c1b | c2a c2b | c3a c3b | c4a c4b | d3a d3b | d2a d2b | d1a d1b | out
+static const LayerDef kLayers[15] = {
+    {0, 4, 0, 16, 3, 1},  {0, 16, 0, 16, 3, 1},  {1, 16, 0, 32, 3, 1}, {1, 32, 0, 32, 3, 1},
+    {2, 32, 0, 64, 3, 1}, {2, 64, 0, 64, 3, 1},  {3, 64, 0, 64, 3, 1}, {3, 64, 0, 64, 3, 1},
+    {2, 64, 64, 64, 3, 1}, {2, 64, 0, 64, 3, 1}, {1, 64, 32, 32, 3, 1}, {1, 32, 0, 32, 3, 1},
+    {0, 32, 16, 16, 3, 1}, {0, 16, 0, 16, 3, 1}, {0, 16, 0, 1, 1, 0}};
+constexpr int kNL = 15;
+
+inline long wcount(const LayerDef& L) { return (long)L.ks * L.ks * L.cout * (L.cin_a + L.cin_b); }
+
+}  // namespace nn
+}  // namespace ins2d
+
+using namespace ins2d;
+using namespace ins2d::nn;
+
+struct ins2d_nn {
+  std::string err;
+  int H = 0, W = 0;
+  int Hl[4], Wl[4];
+  long nparam = 0;
+  std::vector<long> woff, boff;   // offsets of each layer's weights/bias in the flat vector
+  std::vector<void*> allocs;
+  float *param = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+  uint16_t *wbf = nullptr, *wtbf = nullptr;   // bf16 weights and flipped weights
+  uint16_t* act[kNL];                          // layer outputs (bf16)
+  float* out = nullptr;                        // network output (fp32, 1 channel)
+  uint16_t* x0 = nullptr;                      // input (4 channels)
+  uint16_t *pool1 = nullptr, *pool2 = nullptr, *pool3 = nullptr, *up3 = nullptr, *up2 = nullptr, *up1 = nullptr;
+  float *g0 = nullptr, *g1 = nullptr, *gs1 = nullptr, *gs2 = nullptr, *gs3 = nullptr, *gtmp = nullptr;
+  double* scale = nullptr;   // rms of the current input
+  double* loss = nullptr;
+  double out_scale = 1.0;    // p = out_scale * rms(f) * y
+  double* data = nullptr;    // training pairs: [k][0] = f, [k][1] = p (H*W each)
+  int data_count = 0;
+  long step = 0;
+  bool weights_ready = false;
+};
+
+namespace {
+
+#define NNCHK(x)                                                                                 \
+  do {                                                                                           \
+    hipError_t e_ = (x);                                                                         \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+T* nalloc(ins2d_nn* n, size_t count) {
+  void* p = nullptr;
+  NNCHK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+  n->allocs.push_back(p);
+  return (T*)p;
+}
+
+inline unsigned nb(long n) { return (unsigned)std::min<long>((n + 255) / 256, 8192); }
+
+void conv(hipStream_t st, ins2d_nn* N, int l, In a, In b, void* out, int out_fp32, const uint16_t* w,
+          const float* bias, int relu, int cout_override = -1) {
+  const LayerDef& L = kLayers[l];
+  ConvArgs A;
+  A.a = a;
+  A.b = b;
+  A.H = N->Hl[L.level];
+  A.W = N->Wl[L.level];
+  A.cout = cout_override > 0 ? cout_override : L.cout;
+  A.ks = L.ks;
+  A.w = w;
+  A.bias = bias;
+  A.out = out;
+  A.out_fp32 = out_fp32;
+  A.relu = relu;
+  hipLaunchKernelGGL(k_conv, dim3((A.W + 63) / 64, A.H, (A.cout + 63) / 64), dim3(256), 0, st, A);
+}
+
+In bfin(const uint16_t* p, int c) { return In{p, c, 0}; }
+In f32in(const float* p, int c) { return In{p, c, 1}; }
+In none() { return In{nullptr, 0, 0}; }
+
+void refresh_weights(hipStream_t st, ins2d_nn* N) {
+  for (int l = 0; l < kNL; ++l) {
+    const LayerDef& L = kLayers[l];
+    const long n = wcount(L);
+    hipLaunchKernelGGL(k_wflip, dim3(nb(n)), dim3(256), 0, st, N->param + N->woff[l], L.ks * L.ks, L.cout,
+                       L.cin_a + L.cin_b, N->wtbf + N->woff[l], N->wbf + N->woff[l]);
+  }
+  N->weights_ready = true;
+}
+
+// forward pass on a prepared input tensor N->x0; output in N->out (fp32)
+void forward(hipStream_t st, ins2d_nn* N) {
+  uint16_t** A = N->act;
+  auto W = [&](int l) { return N->wbf + N->woff[l]; };
+  auto B = [&](int l) { return N->param + N->boff[l]; };
+  conv(st, N, 0, bfin(N->x0, 4), none(), A[0], 0, W(0), B(0), 1);
+  conv(st, N, 1, bfin(A[0], 16), none(), A[1], 0, W(1), B(1), 1);   // s1
+  hipLaunchKernelGGL(k_pool, dim3(nb((long)N->Hl[1] * N->Wl[1] * 16)), dim3(256), 0, st, A[1], N->Hl[0], N->Wl[0], 16, N->pool1);
+  conv(st, N, 2, bfin(N->pool1, 16), none(), A[2], 0, W(2), B(2), 1);
+  conv(st, N, 3, bfin(A[2], 32), none(), A[3], 0, W(3), B(3), 1);   // s2
+  hipLaunchKernelGGL(k_pool, dim3(nb((long)N->Hl[2] * N->Wl[2] * 32)), dim3(256), 0, st, A[3], N->Hl[1], N->Wl[1], 32, N->pool2);
+  conv(st, N, 4, bfin(N->pool2, 32), none(), A[4], 0, W(4), B(4), 1);
+  conv(st, N, 5, bfin(A[4], 64), none(), A[5], 0, W(5), B(5), 1);   // s3
+  hipLaunchKernelGGL(k_pool, dim3(nb((long)N->Hl[3] * N->Wl[3] * 64)), dim3(256), 0, st, A[5], N->Hl[2], N->Wl[2], 64, N->pool3);
+  conv(st, N, 6, bfin(N->pool3, 64), none(), A[6], 0, W(6), B(6), 1);
+  conv(st, N, 7, bfin(A[6], 64), none(), A[7], 0, W(7), B(7), 1);   // bottleneck
+  hipLaunchKernelGGL(k_up, dim3(nb((long)N->Hl[2] * N->Wl[2] * 64)), dim3(256), 0, st, A[7], N->Hl[3], N->Wl[3], 64, N->up3);
+  conv(st, N, 8, bfin(N->up3, 64), bfin(A[5], 64), A[8], 0, W(8), B(8), 1);
+  conv(st, N, 9, bfin(A[8], 64), none(), A[9], 0, W(9), B(9), 1);
+  hipLaunchKernelGGL(k_up, dim3(nb((long)N->Hl[1] * N->Wl[1] * 64)), dim3(256), 0, st, A[9], N->Hl[2], N->Wl[2], 64, N->up2);
+  conv(st, N, 10, bfin(N->up2, 64), bfin(A[3], 32), A[10], 0, W(10), B(10), 1);
+  conv(st, N, 11, bfin(A[10], 32), none(), A[11], 0, W(11), B(11), 1);
+  hipLaunchKernelGGL(k_up, dim3(nb((long)N->Hl[0] * N->Wl[0] * 32)), dim3(256), 0, st, A[11], N->Hl[1], N->Wl[1], 32, N->up1);
+  conv(st, N, 12, bfin(N->up1, 32), bfin(A[1], 16), A[12], 0, W(12), B(12), 1);
+  conv(st, N, 13, bfin(A[12], 16), none(), A[13], 0, W(13), B(13), 1);
+  conv(st, N, 14, bfin(A[13], 16), none(), N->out, 1, W(14), B(14), 0);
+}
+
+// backward pass for the loss gradient dy (N->g0, fp32, 1 channel at level 0);
+// accumulates parameter gradients into N->grad
+void backward(hipStream_t st, ins2d_nn* N) {
+  uint16_t** A = N->act;
+  auto WT = [&](int l) { return N->wtbf + N->woff[l]; };
+  auto GW = [&](int l) { return N->grad + N->woff[l]; };
+  auto GB = [&](int l) { return N->grad + N->boff[l]; };
+  const long npx[4] = {(long)N->Hl[0] * N->Wl[0], (long)N->Hl[1] * N->Wl[1], (long)N->Hl[2] * N->Wl[2],
+                       (long)N->Hl[3] * N->Wl[3]};
+  // dW, db of layer l given output gradient g (already through the activation)
+  auto wgrad = [&](int l, In a, In b, const float* g) {
+    const LayerDef& L = kLayers[l];
+    const int lv = L.level;
+    const int chunk = 4096;
+    const int tci = (L.cin_a + L.cin_b + 15) / 16, tco = (L.cout + 15) / 16;
+    hipLaunchKernelGGL(k_conv_dw, dim3((unsigned)((npx[lv] + chunk - 1) / chunk), L.ks * L.ks, tci * tco), dim3(64), 0,
+                       st, a, b, g, N->Hl[lv], N->Wl[lv], L.cout, L.ks, chunk, GW(l));
+    hipLaunchKernelGGL(k_bias_grad, dim3(std::min<long>(64, (npx[lv] + 255) / 256), L.cout), dim3(256), 0, st, g,
+                       npx[lv], L.cout, GB(l));
+  };
+  // input gradient of layer l: din = conv(g, flipped W), no bias, fp32
+  auto dgrad = [&](int l, const float* g, float* din) {
+    const LayerDef& L = kLayers[l];
+    ConvArgs C;
+    C.a = f32in(g, L.cout);
+    C.b = none();
+    C.H = N->Hl[L.level];
+    C.W = N->Wl[L.level];
+    C.cout = L.cin_a + L.cin_b;
+    C.ks = L.ks;
+    C.w = WT(l);
+    C.bias = nullptr;
+    C.out = din;
+    C.out_fp32 = 1;
+    C.relu = 0;
+    hipLaunchKernelGGL(k_conv, dim3((C.W + 63) / 64, C.H, (C.cout + 63) / 64), dim3(256), 0, st, C);
+  };
+  auto relub = [&](float* g, const uint16_t* out, long n) {
+    hipLaunchKernelGGL(k_relu_bwd, dim3(nb(n)), dim3(256), 0, st, g, out, n, g);
+  };
+  float* g0 = N->g0;     // level-0 scratch (up to 48 channels)
+  float* g1 = N->g1;
+  // out layer (linear, 16 -> 1, ks 1)
+  wgrad(14, bfin(A[13], 16), none(), g0);
+  dgrad(14, g0, g1);                         // g1: d A13 (16 ch)
+  relub(g1, A[13], npx[0] * 16);
+  wgrad(13, bfin(A[12], 16), none(), g1);
+  dgrad(13, g1, g0);                         // d A12 (16)
+  relub(g0, A[12], npx[0] * 16);
+  wgrad(12, bfin(N->up1, 32), bfin(A[1], 16), g0);
+  dgrad(12, g0, g1);                         // d [up1, s1] (48 ch)
+  // split: gtmp = d up1 (32 ch @ L0), gs1 += d s1 (16 ch @ L0)
+  hipLaunchKernelGGL(k_zero_f, dim3(nb(npx[0] * 16)), dim3(256), 0, st, N->gs1, npx[0] * 16);
+  hipLaunchKernelGGL(k_split, dim3(nb(npx[0] * 48)), dim3(256), 0, st, g1, npx[0], 32, 16, N->gtmp, N->gs1);
+  // up1 backward -> d A11 (32 ch @ L1)
+  hipLaunchKernelGGL(k_up_bwd, dim3(nb(npx[1] * 32)), dim3(256), 0, st, N->gtmp, N->Hl[1], N->Wl[1], 32, g0);
+  relub(g0, A[11], npx[1] * 32);
+  wgrad(11, bfin(A[10], 32), none(), g0);
+  dgrad(11, g0, g1);                         // d A10 (32 @ L1)
+  relub(g1, A[10], npx[1] * 32);
+  wgrad(10, bfin(N->up2, 64), bfin(A[3], 32), g1);
+  dgrad(10, g1, g0);                         // d [up2, s2] (96 @ L1)
+  hipLaunchKernelGGL(k_zero_f, dim3(nb(npx[1] * 32)), dim3(256), 0, st, N->gs2, npx[1] * 32);
+  hipLaunchKernelGGL(k_split, dim3(nb(npx[1] * 96)), dim3(256), 0, st, g0, npx[1], 64, 32, N->gtmp, N->gs2);
+  hipLaunchKernelGGL(k_up_bwd, dim3(nb(npx[2] * 64)), dim3(256), 0, st, N->gtmp, N->Hl[2], N->Wl[2], 64, g1);
+  relub(g1, A[9], npx[2] * 64);              // d A9 (64 @ L2)
+  wgrad(9, bfin(A[8], 64), none(), g1);
+  dgrad(9, g1, g0);
+  relub(g0, A[8], npx[2] * 64);
+  wgrad(8, bfin(N->up3, 64), bfin(A[5], 64), g0);
+  dgrad(8, g0, g1);                          // d [up3, s3] (128 @ L2)
+  hipLaunchKernelGGL(k_zero_f, dim3(nb(npx[2] * 64)), dim3(256), 0, st, N->gs3, npx[2] * 64);
+  hipLaunchKernelGGL(k_split, dim3(nb(npx[2] * 128)), dim3(256), 0, st, g1, npx[2], 64, 64, N->gtmp, N->gs3);
+  hipLaunchKernelGGL(k_up_bwd, dim3(nb(npx[3] * 64)), dim3(256), 0, st, N->gtmp, N->Hl[3], N->Wl[3], 64, g0);
+  relub(g0, A[7], npx[3] * 64);              // d A7 (64 @ L3)
+  wgrad(7, bfin(A[6], 64), none(), g0);
+  dgrad(7, g0, g1);
+  relub(g1, A[6], npx[3] * 64);
+  wgrad(6, bfin(N->pool3, 64), none(), g1);
+  dgrad(6, g1, g0);                          // d pool3 (64 @ L3)
+  // pool3 backward adds into d s3 (gs3, 64 @ L2)
+  hipLaunchKernelGGL(k_pool_bwd, dim3(nb(npx[2] * 64)), dim3(256), 0, st, g0, N->Hl[2], N->Wl[2], 64, N->gs3);
+  relub(N->gs3, A[5], npx[2] * 64);          // d A5
+  wgrad(5, bfin(A[4], 64), none(), N->gs3);
+  dgrad(5, N->gs3, g0);
+  relub(g0, A[4], npx[2] * 64);
+  wgrad(4, bfin(N->pool2, 32), none(), g0);
+  dgrad(4, g0, g1);                          // d pool2 (32 @ L2)
+  hipLaunchKernelGGL(k_pool_bwd, dim3(nb(npx[1] * 32)), dim3(256), 0, st, g1, N->Hl[1], N->Wl[1], 32, N->gs2);
+  relub(N->gs2, A[3], npx[1] * 32);
+  wgrad(3, bfin(A[2], 32), none(), N->gs2);
+  dgrad(3, N->gs2, g0);
+  relub(g0, A[2], npx[1] * 32);
+  wgrad(2, bfin(N->pool1, 16), none(), g0);
+  dgrad(2, g0, g1);                          // d pool1 (16 @ L1)
+  hipLaunchKernelGGL(k_pool_bwd, dim3(nb(npx[0] * 16)), dim3(256), 0, st, g1, N->Hl[0], N->Wl[0], 16, N->gs1);
+  relub(N->gs1, A[1], npx[0] * 16);
+  wgrad(1, bfin(A[0], 16), none(), N->gs1);
+  dgrad(1, N->gs1, g0);
+  relub(g0, A[0], npx[0] * 16);
+  wgrad(0, bfin(N->x0, 4), none(), g0);
+}
+
+void prepare_input(hipStream_t st, ins2d_nn* N, const double* f) {
+  const long n = (long)N->H * N->W;
+  hipLaunchKernelGGL(k_rms, dim3(1), dim3(1024), 0, st, f, n, N->scale);
+  hipLaunchKernelGGL(k_input, dim3(nb(n)), dim3(256), 0, st, f, n, N->scale, N->x0);
+}
+
+int nn_fail(ins2d_nn* N, const std::exception& e) {
+  if (N) N->err = e.what();
+  return INS2D_E_HIP;
+}
+
+std::string g_nn_err;
+
+}  // namespace
+
+extern "C" {
+
+int ins2d_nn_create(int H, int W, ins2d_nn** out) {
+  if (!out || H < 16 || W < 16 || H % 8 || W % 8) {
+    g_nn_err = "network grid must be at least 16x16 with sides divisible by 8";
+    return INS2D_E_ARG;
+  }
+  auto N = std::make_unique<ins2d_nn>();
+  try {
+    N->H = H;
+    N->W = W;
+    for (int l = 0; l < 4; ++l) { N->Hl[l] = H >> l; N->Wl[l] = W >> l; }
+    long off = 0;
+    for (int l = 0; l < kNL; ++l) {
+      N->woff.push_back(off);
+      off += wcount(kLayers[l]);
+      N->boff.push_back(off);
+      off += kLayers[l].cout;
+    }
+    N->nparam = off;
+    N->param = nalloc<float>(N.get(), off);
+    N->grad = nalloc<float>(N.get(), off);
+    N->m = nalloc<float>(N.get(), off);
+    N->v = nalloc<float>(N.get(), off);
+    N->wbf = nalloc<uint16_t>(N.get(), off);
+    N->wtbf = nalloc<uint16_t>(N.get(), off);
+    const long p0 = (long)H * W;
+    for (int l = 0; l < kNL; ++l) {
+      const LayerDef& L = kLayers[l];
+      N->act[l] = nalloc<uint16_t>(N.get(), (p0 >> (2 * L.level)) * L.cout);
+    }
+    N->out = nalloc<float>(N.get(), p0);
+    N->x0 = nalloc<uint16_t>(N.get(), p0 * 4);
+    N->pool1 = nalloc<uint16_t>(N.get(), (p0 >> 2) * 16);
+    N->pool2 = nalloc<uint16_t>(N.get(), (p0 >> 4) * 32);
+    N->pool3 = nalloc<uint16_t>(N.get(), (p0 >> 6) * 64);
+    N->up3 = nalloc<uint16_t>(N.get(), (p0 >> 4) * 64);
+    N->up2 = nalloc<uint16_t>(N.get(), (p0 >> 2) * 64);
+    N->up1 = nalloc<uint16_t>(N.get(), p0 * 32);
+    N->g0 = nalloc<float>(N.get(), p0 * 48);
+    N->g1 = nalloc<float>(N.get(), p0 * 48);
+    N->gtmp = nalloc<float>(N.get(), p0 * 32);
+    N->gs1 = nalloc<float>(N.get(), p0 * 16);
+    N->gs2 = nalloc<float>(N.get(), (p0 >> 2) * 32);
+    N->gs3 = nalloc<float>(N.get(), (p0 >> 4) * 64);
+    N->scale = nalloc<double>(N.get(), 1);
+    N->loss = nalloc<double>(N.get(), 1);
+    NNCHK(hipMemset(N->param, 0, sizeof(float) * off));
+    NNCHK(hipMemset(N->m, 0, sizeof(float) * off));
+    NNCHK(hipMemset(N->v, 0, sizeof(float) * off));
+    NNCHK(hipDeviceSynchronize());
+  } catch (const std::exception& e) {
+    g_nn_err = e.what();
+    for (void* p : N->allocs) (void)hipFree(p);
+    return INS2D_E_HIP;
+  }
+  *out = N.release();
+  return INS2D_OK;
+}
+
+void ins2d_nn_destroy(ins2d_nn* N) {
+  if (!N) return;
+  (void)hipDeviceSynchronize();
+  for (void* p : N->allocs) (void)hipFree(p);
+  if (N->data) (void)hipFree(N->data);
+  delete N;
+}
+
+const char* ins2d_nn_last_error(const ins2d_nn* N) { return N ? N->err.c_str() : g_nn_err.c_str(); }
+
+long ins2d_nn_num_params(const ins2d_nn* N) { return N ? N->nparam : 0; }
+
+int ins2d_nn_set_params(ins2d_nn* N, const float* params, double out_scale) {
+  if (!N || !params) return INS2D_E_ARG;
+  try {
+    NNCHK(hipMemcpy(N->param, params, sizeof(float) * N->nparam, hipMemcpyHostToDevice));
+    N->out_scale = out_scale;
+    refresh_weights(nullptr, N);
+    NNCHK(hipDeviceSynchronize());
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+  return INS2D_OK;
+}
+
+int ins2d_nn_get_params(ins2d_nn* N, float* params) {
+  if (!N || !params) return INS2D_E_ARG;
+  try {
+    NNCHK(hipMemcpy(params, N->param, sizeof(float) * N->nparam, hipMemcpyDeviceToHost));
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+  return INS2D_OK;
+}
+
+// p = out_scale * rms(f) * U(f / rms(f)); f, p device arrays (H*W doubles)
+int ins2d_nn_forward_dev(ins2d_nn* N, const double* f, double* p, void* stream) {
+  if (!N || !f || !p) return INS2D_E_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  try {
+    const long n = (long)N->H * N->W;
+    prepare_input(st, N, f);
+    forward(st, N);
+    hipLaunchKernelGGL(k_output, dim3(nb(n)), dim3(256), 0, st, N->out, n, N->scale, N->out_scale, p);
+    NNCHK(hipGetLastError());
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+  return INS2D_OK;
+}
+
+// host-array forward (tests)
+int ins2d_nn_forward(ins2d_nn* N, const double* f, double* p) {
+  if (!N || !f || !p) return INS2D_E_ARG;
+  try {
+    const long n = (long)N->H * N->W;
+    double *df = nullptr, *dp = nullptr;
+    NNCHK(hipMalloc(&df, n * sizeof(double)));
+    NNCHK(hipMalloc(&dp, n * sizeof(double)));
+    NNCHK(hipMemcpy(df, f, n * sizeof(double), hipMemcpyHostToDevice));
+    int rc = ins2d_nn_forward_dev(N, df, dp, nullptr);
+    NNCHK(hipDeviceSynchronize());
+    NNCHK(hipMemcpy(p, dp, n * sizeof(double), hipMemcpyDeviceToHost));
+    (void)hipFree(df);
+    (void)hipFree(dp);
+    return rc;
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+}
+
+// loss and parameter gradients (accumulated over a batch of device samples
+// f[b], p[b]); mode 0: gradient only (into grad, zeroed first); mode 1: also
+// one Adam step with learning rate lr.  loss_out: mean loss over the batch.
+int ins2d_nn_train_step(ins2d_nn* N, const double* const* f, const double* const* p, int batch, float lr, int mode,
+                        double* loss_out, void* stream) {
+  if (!N || !f || !p || batch < 1) return INS2D_E_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  try {
+    const long n = (long)N->H * N->W;
+    hipLaunchKernelGGL(k_zero_f, dim3(nb(N->nparam)), dim3(256), 0, st, N->grad, N->nparam);
+    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, N->loss, 1L);
+    for (int b = 0; b < batch; ++b) {
+      prepare_input(st, N, f[b]);
+      forward(st, N);
+      hipLaunchKernelGGL(k_mse, dim3(nb(n)), dim3(256), 0, st, N->out, p[b], n, N->scale, N->out_scale, N->g0, N->loss);
+      backward(st, N);
+    }
+    if (mode == 1) {
+      N->step += 1;
+      const float b1 = 0.9f, b2 = 0.999f;
+      const float c1 = 1.f - powf(b1, (float)N->step), c2 = 1.f - powf(b2, (float)N->step);
+      hipLaunchKernelGGL(k_adam, dim3(nb(N->nparam)), dim3(256), 0, st, N->param, N->grad, N->m, N->v, N->nparam, lr,
+                         b1, b2, 1e-8f, c1, c2, 1.0f / batch);
+      refresh_weights(st, N);
+    }
+    double l = 0.0;
+    NNCHK(hipMemcpyAsync(&l, N->loss, sizeof(double), hipMemcpyDeviceToHost, st));
+    NNCHK(hipStreamSynchronize(st));
+    if (loss_out) *loss_out = l / batch;
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+  return INS2D_OK;
+}
+
+int ins2d_nn_data_reserve(ins2d_nn* N, int count) {
+  if (!N || count < 0) return INS2D_E_ARG;
+  try {
+    NNCHK(hipDeviceSynchronize());
+    if (N->data) NNCHK(hipFree(N->data));
+    N->data = nullptr;
+    N->data_count = 0;
+    if (count > 0) {
+      NNCHK(hipMalloc(&N->data, sizeof(double) * 2 * (size_t)count * N->H * N->W));
+      N->data_count = count;
+    }
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+  return INS2D_OK;
+}
+
+int ins2d_nn_data_set(ins2d_nn* N, int k, const double* f, const double* p) {
+  if (!N || !f || !p) return INS2D_E_ARG;
+  if (k < 0 || k >= N->data_count) {
+    N->err = "training pair index out of range";
+    return INS2D_E_ARG;
+  }
+  try {
+    const size_t n = (size_t)N->H * N->W;
+    NNCHK(hipMemcpy(N->data + 2 * n * k, f, n * sizeof(double), hipMemcpyHostToDevice));
+    NNCHK(hipMemcpy(N->data + 2 * n * k + n, p, n * sizeof(double), hipMemcpyHostToDevice));
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+  return INS2D_OK;
+}
+
+int ins2d_nn_train(ins2d_nn* N, const int* idx, int batch, float lr, int mode, double* loss_out) {
+  if (!N || !idx || batch < 1) return INS2D_E_ARG;
+  const size_t n = (size_t)N->H * N->W;
+  std::vector<const double*> f(batch), p(batch);
+  for (int b = 0; b < batch; ++b) {
+    if (idx[b] < 0 || idx[b] >= N->data_count) {
+      N->err = "training pair index out of range";
+      return INS2D_E_ARG;
+    }
+    f[b] = N->data + 2 * n * idx[b];
+    p[b] = f[b] + n;
+  }
+  return ins2d_nn_train_step(N, f.data(), p.data(), batch, lr, mode, loss_out, nullptr);
+}
+
+int ins2d_nn_get_grad(ins2d_nn* N, float* grad) {
+  if (!N || !grad) return INS2D_E_ARG;
+  try {
+    NNCHK(hipMemcpy(grad, N->grad, sizeof(float) * N->nparam, hipMemcpyDeviceToHost));
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+  return INS2D_OK;
+}
+
+// One convolution layer's kernels on host data (FP32 arrays rounded to BF16
+// where the kernels read BF16): forward out = conv(x) (+ bias, ReLU), and for
+// an output gradient g: din = conv^T(g), dw, db.  xa: H*W*ca, xb: H*W*cb,
+// w: ks*ks*cout*(ca+cb), g/out: H*W*cout, din: H*W*(ca+cb).
+int ins2d_nn_layer_check(int H, int W, int ca, int cb, int cout, int ks, int relu, const float* xa, const float* xb,
+                         const float* w, const float* bias, const float* g, float* out, float* din, float* dw,
+                         float* db) {
+  if (H < 1 || W < 1 || ca < 1 || cb < 0 || cout < 1 || (ks != 1 && ks != 3) || !xa || (cb && !xb) || !w || !bias ||
+      !g || !out || !din || !dw || !db)
+    return INS2D_E_ARG;
+  const long np_ = (long)H * W, cin = ca + cb, nw = (long)ks * ks * cout * cin;
+  std::vector<void*> mem;
+  auto dev = [&](size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess) throw std::runtime_error("hipMalloc");
+    mem.push_back(p);
+    return p;
+  };
+  int rc = INS2D_OK;
+  try {
+    float* dxa = (float*)dev(np_ * ca * 4);
+    float* dxb = (float*)dev(np_ * std::max(cb, 1) * 4);
+    float* dwf = (float*)dev(nw * 4);
+    uint16_t* dwb = (uint16_t*)dev(nw * 2);
+    uint16_t* dwt = (uint16_t*)dev(nw * 2);
+    float* dbias = (float*)dev(cout * 4);
+    float* dg = (float*)dev(np_ * cout * 4);
+    float* dout = (float*)dev(np_ * cout * 4);
+    float* ddin = (float*)dev(np_ * cin * 4);
+    float* ddw = (float*)dev(nw * 4);
+    float* ddb = (float*)dev(cout * 4);
+    NNCHK(hipMemcpy(dxa, xa, np_ * ca * 4, hipMemcpyHostToDevice));
+    if (cb) NNCHK(hipMemcpy(dxb, xb, np_ * cb * 4, hipMemcpyHostToDevice));
+    NNCHK(hipMemcpy(dwf, w, nw * 4, hipMemcpyHostToDevice));
+    NNCHK(hipMemcpy(dbias, bias, cout * 4, hipMemcpyHostToDevice));
+    NNCHK(hipMemcpy(dg, g, np_ * cout * 4, hipMemcpyHostToDevice));
+    NNCHK(hipMemset(ddw, 0, nw * 4));
+    NNCHK(hipMemset(ddb, 0, cout * 4));
+    hipLaunchKernelGGL(k_wflip, dim3(nb(nw)), dim3(256), 0, nullptr, dwf, ks * ks, cout, (int)cin, dwt, dwb);
+    ConvArgs A;
+    A.a = f32in(dxa, ca);
+    A.b = cb ? f32in(dxb, cb) : none();
+    A.H = H;
+    A.W = W;
+    A.cout = cout;
+    A.ks = ks;
+    A.w = dwb;
+    A.bias = dbias;
+    A.out = dout;
+    A.out_fp32 = 1;
+    A.relu = relu;
+    hipLaunchKernelGGL(k_conv, dim3((W + 63) / 64, H, (cout + 63) / 64), dim3(256), 0, nullptr, A);
+    ConvArgs C = A;
+    C.a = f32in(dg, cout);
+    C.b = none();
+    C.cout = (int)cin;
+    C.w = dwt;
+    C.bias = nullptr;
+    C.out = ddin;
+    C.relu = 0;
+    hipLaunchKernelGGL(k_conv, dim3((W + 63) / 64, H, ((int)cin + 63) / 64), dim3(256), 0, nullptr, C);
+    const int chunk = 4096, tci = ((int)cin + 15) / 16, tco = (cout + 15) / 16;
+    hipLaunchKernelGGL(k_conv_dw, dim3((unsigned)((np_ + chunk - 1) / chunk), ks * ks, tci * tco), dim3(64), 0, nullptr,
+                       A.a, A.b, dg, H, W, cout, ks, chunk, ddw);
+    hipLaunchKernelGGL(k_bias_grad, dim3(std::min<long>(64, (np_ + 255) / 256), cout), dim3(256), 0, nullptr, dg, np_,
+                       cout, ddb);
+    NNCHK(hipGetLastError());
+    NNCHK(hipDeviceSynchronize());
+    NNCHK(hipMemcpy(out, dout, np_ * cout * 4, hipMemcpyDeviceToHost));
+    NNCHK(hipMemcpy(din, ddin, np_ * cin * 4, hipMemcpyDeviceToHost));
+    NNCHK(hipMemcpy(dw, ddw, nw * 4, hipMemcpyDeviceToHost));
+    NNCHK(hipMemcpy(db, ddb, cout * 4, hipMemcpyDeviceToHost));
+  } catch (const std::exception& e) {
+    g_nn_err = e.what();
+    rc = INS2D_E_HIP;
+  }
+  for (void* p : mem) (void)hipFree(p);
+  return rc;
+}
+
+int ins2d_nn_mfma_probe(const float* A, const float* B, float* D) {
+  if (!A || !B || !D) return INS2D_E_ARG;
+  float *dA = nullptr, *dB = nullptr, *dD = nullptr;
+  if (hipMalloc(&dA, 256 * 4) != hipSuccess || hipMalloc(&dB, 256 * 4) != hipSuccess ||
+      hipMalloc(&dD, 256 * 4) != hipSuccess)
+    return INS2D_E_HIP;
+  (void)hipMemcpy(dA, A, 256 * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(dB, B, 256 * 4, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, nullptr, dA, dB, dD);
+  hipError_t e = hipMemcpy(D, dD, 256 * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dD);
+  return e == hipSuccess ? INS2D_OK : INS2D_E_HIP;
+}
+
+}  // extern "C"
