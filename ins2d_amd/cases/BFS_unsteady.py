@@ -5,7 +5,10 @@ half of x = 0 (the lower half is the step face, a wall), 'pres' outflow at
 x = L (p_inf = 0), no-slip walls.  Default: 1024 x 256 cells, x uniform,
 y clustered towards both walls by a tanh map with stretching factor 1.5 (a
 case-defined mesh: mesh_generation.m accepts any x, y from the case),
-Re = 800, RK44, Jacobi-PCG pressure to 1e-10.  The reference's convective
+Re = 800, RK44, direct pressure solve (poisson = 1, as
+BFS_unsteady_parameters.m:104 selects; fast diagonalization on the GPU); with
+poisson = 3 the PCG settings below apply (relative residual 1e-10).  The
+reference's convective
 outflow variant is not on the path; the outflow here is the reference's
 'pres' condition.
 """
@@ -17,7 +20,7 @@ from ._common import tanh_mesh_1d
 def parameters():
     return dict(Re=800.0, Nx=1024, Ny=256, x1=0.0, x2=10.0, y1=-0.5, y2=0.5, sx=1.0, sy=1.0, stretch=1.5,
                 order4=0, dt=None, CFL=0.5, t_start=0.0, nsteps=200, RK='RK44',
-                poisson=3, CG_acc=1e-10, CG_maxit=50000, p_initial=1, p_add_solve=1)
+                poisson=1, CG_acc=1e-10, CG_maxit=50000, p_initial=1, p_add_solve=1)
 
 
 def BCtype():

@@ -57,6 +57,8 @@ struct ins2d_handle {
   // FFT
   FFTPlan fft{};
   bool has_fft = false;
+  // fast-diagonalization direct solve (poisson = direct)
+  FDMPlan fdm{};
   // state and work buffers
   double *V = nullptr, *Vb = nullptr, *p = nullptr, *Vt = nullptr, *f = nullptr, *dp = nullptr, *tV = nullptr;
   std::vector<double*> kv;
@@ -269,6 +271,12 @@ void poisson(ins2d_handle* h, const double* f, double* dp) {
   ProfScope ps(h, 2);
   if (h->poisson == INS2D_POISSON_FFT) {
     launch_fft_poisson(h->st, h->fft, f, dp);
+    return;
+  }
+  if (h->poisson == INS2D_POISSON_DIRECT) {
+    launch_fdm_poisson(h->st, h->fdm, f, dp, h->cr, h->cz);
+    h->last_it = 0;
+    h->last_rel = 0.0;
     return;
   }
   const double tol2 = h->tol * h->tol;
@@ -597,6 +605,20 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
       h->cg.flag = dalloc<int>(h.get(), 4);
       h->cg.part = dalloc<double>(h.get(), 2 * (size_t)h->cg.nblk);
       h->cg.maxit = h->maxit;
+      if (h->poisson == INS2D_POISSON_DIRECT) {
+        FDMHost F;
+        build_fdm_host(*h->plan, F);
+        h->fdm.nx = F.nx;
+        h->fdm.ny = F.ny;
+        h->fdm.nx0 = F.nx0;
+        h->fdm.ny0 = F.ny0;
+        h->fdm.Vx = dupload(h.get(), F.Vx);
+        h->fdm.VxT = dupload(h.get(), F.VxT);
+        h->fdm.Vy = dupload(h.get(), F.Vy);
+        h->fdm.VyT = dupload(h.get(), F.VyT);
+        h->fdm.lx = dupload(h.get(), F.lx);
+        h->fdm.ly = dupload(h.get(), F.ly);
+      }
     }
     h->nred = (int)std::min<long>(1024, std::max<long>(1, ((long)std::max(h->NV, h->Np) + 255) / 256));
     h->part = dalloc<double>(h.get(), 4 * (size_t)h->nred);

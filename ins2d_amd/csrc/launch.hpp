@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <vector>
 
 #include "device.hpp"
 
@@ -43,6 +44,14 @@ void launch_cg_init(hipStream_t st, const DLap& L, const CGStateH& S, const doub
 void launch_cg_iter(hipStream_t st, const DLap& L, const CGStateH& S, double* x, double* r, double* z, double* p,
                     double* q);
 void launch_axpy(hipStream_t st, size_t n, double a, const double* x, double* y);
+
+// fast-diagonalization direct Poisson solve (fdm.hip; factors from
+// build_fdm_host in fdm_host.cpp)
+struct FDMPlan {
+  int nx, ny, nx0, ny0;
+  const double *Vx, *VxT, *Vy, *VyT, *lx, *ly;
+};
+void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2);
 void launch_reduce_stats(hipStream_t st, const double* x, int nx, int ny, const double* wx, const double* wy,
                          double* part, int nblk);
 

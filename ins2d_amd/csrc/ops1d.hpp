@@ -154,6 +154,15 @@ struct Plan {
 
 void build_plan(const CaseDesc& cd, Plan& P);
 
+// factors of the fast-diagonalization direct Poisson solve (fdm_host.cpp):
+// Lx Vx = diag(hx) Vx diag(lx), Vx^T diag(hx) Vx = I, the same in y; (nx0,
+// ny0) is the dropped null mode of a singular A (-1 when A is nonsingular)
+struct FDMHost {
+  int nx = 0, ny = 0, nx0 = -1, ny0 = -1;
+  std::vector<double> Vx, VxT, Vy, VyT, lx, ly;
+};
+void build_fdm_host(const Plan& P, FDMHost& F);
+
 // sampled boundary values (set_bc_vectors.m:53-79), host copies
 struct BCSamples {
   std::vector<double> s[S_NSRC];

@@ -4,6 +4,8 @@
 // compare them with the CPU oracle's assembled matrices.
 //
 // stdin: Nx Ny | x[0..Nx] | y[0..Ny] | 8 BC kinds | Re | 12 samples (n v...)
+// With the argument --fdm it also prints the fast-diagonalization factors of
+// the pressure matrix (fdm_host.cpp).
 #include <cstdio>
 #include <iostream>
 #include <string>
@@ -80,7 +82,8 @@ static void comp(const char* name, const HComp& C, const BCSamples& S) {
   printf("}");
 }
 
-int main() {
+int main(int argc, char** argv) {
+  const bool fdm = argc > 1 && std::string(argv[1]) == "--fdm";
   CaseDesc cd;
   std::cin >> cd.Nx >> cd.Ny;
   cd.x.resize(cd.Nx + 1);
@@ -124,6 +127,24 @@ int main() {
   kr("L2", P.L2);
   printf(",\"Adiag\":");
   vec(P.Adiag);
+  if (fdm) {
+    FDMHost F;
+    try {
+      build_fdm_host(P, F);
+    } catch (const std::exception& e) {
+      printf(",\"fdm_error\":\"%s\"}\n", e.what());
+      return 0;
+    }
+    printf(",\"fdm\":{\"nx0\":%d,\"ny0\":%d,\"Vx\":", F.nx0, F.ny0);
+    vec(F.Vx);
+    printf(",\"Vy\":");
+    vec(F.Vy);
+    printf(",\"lx\":");
+    vec(F.lx);
+    printf(",\"ly\":");
+    vec(F.ly);
+    printf("}");
+  }
   printf("}\n");
   return 0;
 }

@@ -36,6 +36,9 @@ def _wall_case(kind, Nx, Ny, **kw):
                             'v': dict(left='dir', right='sym', low='dir', up='dir')}
         c.uBC = lambda x, y, t, o: np.ravel(np.add.outer(0 * np.atleast_1d(x), 4 * np.atleast_1d(y) * (1 - np.atleast_1d(y))) *
                                             (np.abs(np.atleast_1d(x)) < 1e-12)[:, None], order='F')
+    elif kind == 'channel':
+        c.BCtype = lambda: {'BC_unsteady': 0, 'u': dict(left='per', right='per', low='dir', up='dir'),
+                            'v': dict(left='per', right='per', low='dir', up='dir')}
     return c
 
 
@@ -64,7 +67,8 @@ def test_rhs_matches_oracle(name, kernels):
     solver.close()
 
 
-@pytest.mark.parametrize('name', ['fft2', 'fft4_ref', 'fft4_exact', 'fft_nonpow2', 'cg_ldc', 'cg_outflow', 'cg_large'])
+@pytest.mark.parametrize('name', ['fft2', 'fft4_ref', 'fft4_exact', 'fft_nonpow2', 'cg_ldc', 'cg_outflow', 'cg_large',
+                                  'direct_ldc', 'direct_outflow', 'direct_channel', 'direct_bfs'])
 def test_poisson_solves_assembled_system(name):
     _say('poisson', name)
     fft_symbol = 0
@@ -79,13 +83,21 @@ def test_poisson_solves_assembled_system(name):
         case = _wall_case('ldc', 40, 30, poisson=3, CG_acc=1e-12)
     elif name == 'cg_outflow':
         case = _wall_case('outflow', 48, 20, poisson=3, CG_acc=1e-12, sx=1.03)
-    else:
+    elif name == 'cg_large':
         case = _wall_case('ldc', 256, 160, poisson=3, CG_acc=1e-11, CG_maxit=5000)
+    elif name == 'direct_ldc':
+        case = _wall_case('ldc', 40, 30, poisson=1, sx=1.05, sy=1.05)
+    elif name == 'direct_outflow':
+        case = _wall_case('outflow', 48, 20, poisson=1, sx=1.03)
+    elif name == 'direct_channel':
+        case = _wall_case('channel', 36, 20, poisson=1, sy=1.04)
+    else:
+        case = load('BFS_unsteady', Nx=256, Ny=64, poisson=1)
     opts = O.build(case)
     solver, _, _, _ = host.setup(case)
     rng = np.random.default_rng(5)
     f = rng.standard_normal(opts.grid.Np)
-    singular = name != 'cg_outflow'
+    singular = name not in ('cg_outflow', 'direct_outflow', 'direct_bfs')
     if singular:
         f -= f.mean()
     dp, it, rel = solver.poisson(f)
@@ -102,6 +114,20 @@ def test_poisson_solves_assembled_system(name):
         # the oracle's exact symbol is the DFT of its assembled A
         dpo = O.pressure_poisson(f, 0.0, opts)
         assert _rel(dp, dpo) < 1e-12
+    elif name.startswith('direct'):
+        # fast diagonalization against the oracle's sparse LU (pressure
+        # modulo its constant where A is singular); normwise backward error
+        # at the level of a backward-stable direct solve
+        An = abs(A).sum(axis=1).max()
+        berr = np.abs(r).max() / (An * np.abs(dp).max() + np.abs(f).max())
+        _say('direct', name, 'backward error', berr)
+        assert berr < 1e-14
+        assert np.abs(r).max() < 1e-11 * np.abs(f).max()
+        dpo = O.pressure_poisson(f, 0.0, opts)
+        if singular:
+            dp, dpo = dp - dp.mean(), dpo - dpo.mean()
+        _say('direct', name, 'rel', _rel(dp, dpo))
+        assert _rel(dp, dpo) < 1e-11
     else:
         tol = {'cg_ldc': 1e-12, 'cg_outflow': 1e-12, 'cg_large': 1e-11}[name]
         assert rel <= tol and it > 0
@@ -136,10 +162,13 @@ def _compare_runs(case, nsteps, const_mode_p):
     return rows
 
 
-def test_ldc_run_matches_oracle_and_reference_profiles():
-    """results/LDC_unsteady_1.000e+02_20x20_0: RK33P2, Re=100, 100 steps."""
-    _say('ldc 20x20 run')
-    case = load('LDC_unsteady', RK='RK33P2', poisson=3, CG_acc=1e-13, CG_maxit=2000)
+@pytest.mark.parametrize('poisson', [1, 3])
+def test_ldc_run_matches_oracle_and_reference_profiles(poisson):
+    """results/LDC_unsteady_1.000e+02_20x20_0: RK33P2, Re=100, 100 steps,
+    with the reference's direct solve (fast diagonalization here) and with
+    tight PCG."""
+    _say('ldc 20x20 run', poisson)
+    case = load('LDC_unsteady', RK='RK33P2', poisson=poisson, CG_acc=1e-13, CG_maxit=2000)
     rows = _compare_runs(case, 100, True)
     d = os.path.join(GOLDEN, 'LDC_unsteady_1.000e+02_20x20_0')
     ref = np.loadtxt(os.path.join(d, 'convergence.txt'), skiprows=3)
@@ -162,10 +191,19 @@ def test_shear_layer_run_matches_oracle_and_reference_log(kernels):
         assert np.max(np.abs(rows[:, c] / ref[:, c] - 1)) < 1.2e-8
 
 
-def test_outflow_stretched_run_matches_oracle():
-    _say('outflow stretched run')
-    case = _wall_case('outflow', 32, 16, poisson=3, CG_acc=1e-13, CG_maxit=4000, sx=1.03, Re=200.0, dt=0.005)
+@pytest.mark.parametrize('poisson', [1, 3])
+def test_outflow_stretched_run_matches_oracle(poisson):
+    _say('outflow stretched run', poisson)
+    case = _wall_case('outflow', 32, 16, poisson=poisson, CG_acc=1e-13, CG_maxit=4000, sx=1.03, Re=200.0, dt=0.005)
     _compare_runs(case, 20, False)
+
+
+def test_bfs_run_matches_oracle():
+    """Backward-facing step (BFS_unsteady case files: step inflow, 'pres'
+    outflow), tanh-stretched 128x32, the reference's direct solve."""
+    _say('bfs 128x32 run')
+    case = load('BFS_unsteady', Nx=128, Ny=32, poisson=1)
+    _compare_runs(case, 10, False)
 
 
 @pytest.mark.parametrize('order4,rk', [(0, 'RK44'), (1, 'RK44'), (0, 'RK44C23'), (0, 'DOPRI6')])
