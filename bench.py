@@ -41,6 +41,7 @@ def parse():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-steps', type=int, default=10)
     ap.add_argument('--members', type=int, default=512)
+    ap.add_argument('--train-steps', type=int, default=1500)
     return ap.parse_args()
 
 
@@ -128,6 +129,50 @@ def bench_ensemble(a):
     print(json.dumps(out))
 
 
+def bench_poisson_learned(a):
+    """Per-solve pressure time, learned vs plain, on the backward-facing step
+    1024x256 (tanh-stretched, Re 800 grid), PCG to a true relative residual
+    of 1e-10.  Training pairs come from plain-solver runs at Re 200..1200,
+    the held-out right-hand sides from runs at Re 300, 700, 1100 at later
+    times.  Plain = Jacobi-PCG from x0 = 0; learned = network start vector
+    (smoothed) + PCG with the two-level coarse correction; the two parts are
+    also timed alone."""
+    from ins2d_amd import host, learn
+    Nx, Ny = (a.N or 1024), (a.N // 4 if a.N else 256)
+    t0 = time.perf_counter()
+    pairs = learn.generate_pairs(Nx, Ny, [200.0, 400.0, 600.0, 800.0, 1000.0, 1200.0], steps=600, every=6, skip=100,
+                                 progress=progress)
+    held = learn.generate_pairs(Nx, Ny, [300.0, 700.0, 1100.0], steps=340, every=10, skip=400, progress=progress)
+    t_data = time.perf_counter() - t0
+    net = learn.make_net(Nx, Ny, out_scale=learn.out_scale_of(pairs), seed=0)
+    t0 = time.perf_counter()
+    losses = learn.train(net, pairs, steps=a.train_steps, batch=4, lr=2e-3, progress=progress, every=100)
+    t_train = time.perf_counter() - t0
+    solver, _, _, _ = host.setup('BFS_unsteady', Nx=Nx, Ny=Ny, poisson=3, CG_acc=1e-10, CG_maxit=100000)
+    rhs = [h[0] for h in held]
+    variants = [('plain', False, False), ('two_level', False, True), ('warm_start', True, False),
+                ('learned', True, True)]
+    res = learn.evaluate(solver, net, rhs, variants, progress=progress)
+    med = {k: float(np.median(v['ms'])) for k, v in res.items()}
+    out = {'metric': 'Poisson time learned vs plain PCG @1e-10 residual', 'value': med['plain'] / med['learned'],
+           'unit': 'x (median plain / median learned per-solve GPU time)', 'n_gpus': 1, 'steps': len(rhs),
+           'warmup': 1, 'ms_per_step': med['learned'], 'higher_is_better': True, 'scaling': 'weak',
+           'vs_baseline': None, 'dtype': 'f64 (solver), bf16/fp32 (network)', 'data': 'synthetic',
+           'config': {'workload': f'backward-facing step {Nx}x{Ny} tanh-stretched, pressure RHS from plain runs',
+                      'training_pairs': len(pairs), 'held_out_rhs': len(rhs), 'train_steps': a.train_steps,
+                      'train_batch': 4, 'train_s': t_train, 'data_s': t_data,
+                      'loss_first': losses[0], 'loss_last': float(np.mean(losses[-20:]))},
+           'variants': {k: {'median_ms': med[k], 'median_iterations': float(np.median(v['it'])),
+                            'max_true_relres': float(np.max(v['rel'])), 'fallbacks': v['fallbacks']}
+                        for k, v in res.items()},
+           'network_gain_over_two_level': med['two_level'] / med['learned']}
+    agg = solver.accel_stats()[1]
+    out['config']['aggregate'] = agg
+    solver.close()
+    net.close()
+    print(json.dumps(out))
+
+
 CASES = {
     # workload name -> (case plugin, overrides, description)
     'taylor_green': ('taylor_green', lambda N: dict(Nx=N or 1024, Ny=N or 1024),
@@ -153,6 +198,8 @@ def main():
     a = parse()
     if a.case == 'ensemble':
         return bench_ensemble(a)
+    if a.case == 'poisson_learned':
+        return bench_poisson_learned(a)
     if a.gpus != 1:
         print(json.dumps({'error': 'this path runs on one GPU; multi-GPU sharding is described in DESIGN.md, '
                                    'not built', 'n_gpus': a.gpus}))

@@ -115,6 +115,14 @@ int ins2d_diagnostics(ins2d_handle* h, double t, double* out);
 /* last Poisson solve: iterations and relative residual (CG modes) */
 int ins2d_solver_stats(const ins2d_handle* h, int* iters, double* relres, long long* total_iters);
 
+/* true relative residual ||f - A x|| / ||f|| of a pressure solve */
+int ins2d_residual(ins2d_handle* h, const double* f, const double* x, double* relres);
+
+/* the right-hand side of the pressure equation for the resident velocity,
+ * f = M Om^-1 (-conv + diff) (pressure_additional_solve.m:19-22 with p = 0):
+ * its solution is the pressure consistent with V (second order) */
+int ins2d_pressure_rhs(ins2d_handle* h, double t, double* f);
+
 /* kernel timing with HIP events on the handle's stream:
  * enable != 0 starts accumulating; read returns the accumulated milliseconds
  * and launch count of one kernel class (0 momentum RHS, 1 divergence,
@@ -160,6 +168,8 @@ const char* ins2d_nn_last_error(const ins2d_nn* n);
 long ins2d_nn_num_params(const ins2d_nn* n);
 int ins2d_nn_set_params(ins2d_nn* n, const float* params, double out_scale);
 int ins2d_nn_get_params(ins2d_nn* n, float* params);
+/* He-normal weights from a seed, zero biases, fresh optimizer state */
+int ins2d_nn_init(ins2d_nn* n, unsigned long long seed, double out_scale);
 /* host arrays of H*W doubles */
 int ins2d_nn_forward(ins2d_nn* n, const double* f, double* p);
 /* device arrays, queued on the given HIP stream (NULL: the null stream) */
@@ -187,6 +197,20 @@ int ins2d_nn_layer_check(int H, int W, int ca, int cb, int cout, int ks, int rel
 /* one 16x16x16 BF16 matrix-core product D = A B of row-major 16x16 FP32
  * matrices rounded to BF16 (checks the operand layout the kernels use) */
 int ins2d_nn_mfma_probe(const float* A, const float* B, float* D);
+/* H (rows, y) and W (columns, x) the network was created for */
+int ins2d_nn_shape(const ins2d_nn* n, int* H, int* W);
+
+/* ---- accelerated PCG for the pressure modes 3/4 (no reference
+ * counterpart).  flags bit 0: start from the network's p0 (nn on the handle's
+ * pressure grid; when ||f - A p0|| > ||f|| the solve falls back to x0 = 0);
+ * bit 1: two-level preconditioner, Jacobi plus an aggregation coarse-grid
+ * correction over agg x agg cells solved exactly (agg 0: automatic).  The
+ * stopping rule stays ||r|| <= cg_tol ||f|| (cg.F:247); ins2d_solver_stats then
+ * reports the true relative residual ||f - A x|| / ||f||.  flags 0 restores
+ * the plain PCG.  The network is borrowed and must outlive its use. */
+int ins2d_set_poisson_accel(ins2d_handle* h, ins2d_nn* nn, int flags, int agg);
+/* number of warm-start fallbacks so far, aggregate size in use */
+int ins2d_poisson_accel_stats(const ins2d_handle* h, long long* fallbacks, int* agg);
 
 #ifdef __cplusplus
 }

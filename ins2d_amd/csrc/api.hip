@@ -59,6 +59,11 @@ struct ins2d_handle {
   bool has_fft = false;
   // fast-diagonalization direct solve (poisson = direct)
   FDMPlan fdm{};
+  // accelerated PCG: 1 learned warm start, 2 two-level preconditioner
+  int accel = 0;
+  ins2d_nn* nn = nullptr;
+  TLPlan tl{};
+  long long fallbacks = 0;
   // state and work buffers
   double *V = nullptr, *Vb = nullptr, *p = nullptr, *Vt = nullptr, *f = nullptr, *dp = nullptr, *tV = nullptr;
   std::vector<double*> kv;
@@ -266,6 +271,58 @@ void proj(ins2d_handle* h, const double* Vn, const double* Vt, const double* dp,
     launch_proj_generic(h->st, h->dCu, h->dCv, h->nux, h->nuy, h->nvx, h->nvy, h->Nu, Vn, Vt, dp, dt, c, V);
 }
 
+// PCG with the learned start vector and/or the two-level preconditioner;
+// the stopping rule and tolerance are the plain solver's (cg.F:247)
+void accel_poisson(ins2d_handle* h, const double* f, double* dp) {
+  const double tol2 = h->tol * h->tol;
+  int x0_zero = 1;
+  if (h->accel & 1) {
+    if (ins2d_nn_forward_dev(h->nn, f, dp, h->st) != INS2D_OK)
+      throw HipError(std::string("warm start: ") + ins2d_nn_last_error(h->nn));
+    // the network's BF16 rounding noise is rough and A amplifies rough
+    // error most: damped Jacobi sweeps remove it before CG starts; with the
+    // two-level preconditioner the smooth (global) error of the start vector
+    // is also corrected on the coarse grid, between the sweeps
+    launch_tl_smooth(h->st, h->tl, f, dp, h->cz, 12, 0.6);
+    if (h->tl.coarse) {
+      launch_tl_coarse_correct(h->st, h->tl, f, dp, h->cr);
+      launch_tl_smooth(h->st, h->tl, f, dp, h->cz, 12, 0.6);
+    }
+    x0_zero = 0;
+  } else {
+    HIPCHK(hipMemsetAsync(dp, 0, sizeof(double) * h->Np, h->st));
+  }
+  launch_tl_init(h->st, h->tl, f, dp, h->cr, h->cp, x0_zero, tol2);
+  if (!x0_zero) {
+    double sc[8];
+    HIPCHK(hipMemcpyAsync(sc, h->tl.s, sizeof(sc), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (sc[6] > sc[1]) {
+      // the warm start raised the residual above ||f||: start from x0 = 0
+      ++h->fallbacks;
+      HIPCHK(hipMemsetAsync(dp, 0, sizeof(double) * h->Np, h->st));
+      launch_tl_init(h->st, h->tl, f, dp, h->cr, h->cp, 1, tol2);
+    }
+  }
+  int flag[3] = {0, 0, 0};
+  HIPCHK(hipMemcpyAsync(flag, h->tl.flag, 3 * sizeof(int), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  while (!flag[0]) {
+    for (int k = 0; k < 16; ++k) launch_tl_iter(h->st, h->tl, dp, h->cr, h->cp, h->cq);
+    HIPCHK(hipMemcpyAsync(flag, h->tl.flag, 3 * sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+  }
+  // report the true relative residual ||f - A x|| / ||f|| of the result
+  launch_tl_true_residual(h->st, h->tl, f, dp, h->cr);
+  double rel = 0;
+  HIPCHK(hipMemcpyAsync(&rel, h->tl.s + 7, sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->last_it = flag[1];
+  h->last_rel = std::sqrt(rel);
+  h->total_it += flag[1];
+  if (flag[2]) throw NoConv("CG did not converge in maxit iterations");
+}
+
 // A dp = f, x0 = 0 (pressure_poisson.m:12)
 void poisson(ins2d_handle* h, const double* f, double* dp) {
   ProfScope ps(h, 2);
@@ -280,6 +337,10 @@ void poisson(ins2d_handle* h, const double* f, double* dp) {
     return;
   }
   const double tol2 = h->tol * h->tol;
+  if (h->accel) {
+    accel_poisson(h, f, dp);
+    return;
+  }
   if (h->Np <= 32768) {
     launch_cg_single(h->st, h->lap, f, dp, h->cr, h->cz, h->cp, h->cq, tol2, h->maxit, 1, h->d_it, h->d_rel);
     int it = 0;
@@ -795,6 +856,101 @@ int ins2d_diagnostics(ins2d_handle* h, double t, double* out) {
   out[2] = umom;
   out[3] = vmom;
   out[4] = k;
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_set_poisson_accel(ins2d_handle* h, ins2d_nn* nn, int flags, int agg) {
+  if (!h || flags < 0 || flags > 3) return INS2D_E_ARG;
+  API_TRY
+  if (flags == 0) {
+    h->accel = 0;
+    h->nn = nullptr;
+    return INS2D_OK;
+  }
+  if (h->poisson != INS2D_POISSON_CG && h->poisson != INS2D_POISSON_CG_IC)
+    throw Unsupported("Poisson acceleration applies to the PCG modes");
+  if ((flags & 1) && !nn) throw ArgError("warm start needs a network");
+  if (flags & 1) {
+    int H = 0, W = 0;
+    ins2d_nn_shape(nn, &H, &W);
+    if (H != h->Ny || W != h->Nx) throw ArgError("network grid does not match the pressure grid");
+  }
+  const int nx = h->lap.nx, ny = h->lap.ny;
+  auto ncoarse = [&](int a) { return (long)((nx + a - 1) / a) * ((ny + a - 1) / a); };
+  if (agg == 0) {
+    for (int a : {8, 16, 32})
+      if (ncoarse(a) <= kTLMaxCoarse) { agg = a; break; }
+    if (agg == 0) throw Unsupported("grid too large for the one-workgroup coarse solve");
+  }
+  if (agg != 4 && agg != 8 && agg != 16 && agg != 32) throw ArgError("aggregate size must be 4, 8, 16 or 32");
+  if (ncoarse(agg) > kTLMaxCoarse) throw Unsupported("too many coarse unknowns for this aggregate size");
+  TLPlan T{};
+  T.L = h->lap;
+  T.maxit = h->maxit;
+  T.coarse = (flags & 2) ? 1 : 0;
+  T.C.agg = agg;
+  T.C.ncx = (nx + agg - 1) / agg;
+  T.C.ncy = (ny + agg - 1) / agg;
+  T.C.nx0 = T.C.ny0 = -1;
+  if (flags & 2) {
+    FDMHost C;
+    build_coarse_host(*h->plan, agg, C);
+    if (C.nx != T.C.ncx || C.ny != T.C.ncy) throw std::runtime_error("coarse grid mismatch");
+    T.C.Vx = dupload(h, C.Vx);
+    T.C.Vy = dupload(h, C.Vy);
+    T.C.lx = dupload(h, C.lx);
+    T.C.ly = dupload(h, C.ly);
+    T.C.nx0 = C.nx0;
+    T.C.ny0 = C.ny0;
+  }
+  const size_t nc = (size_t)T.C.ncx * T.C.ncy;
+  T.C.rc = dalloc<double>(h, nc);
+  T.C.ec = dalloc<double>(h, nc);
+  T.s = dalloc<double>(h, 8);
+  T.flag = dalloc<int>(h, 4);
+  T.part = dalloc<double>(h, 3 * nc);
+  HIPCHK(hipMemset(T.s, 0, 8 * sizeof(double)));
+  HIPCHK(hipMemset(T.flag, 0, 4 * sizeof(int)));
+  h->tl = T;
+  h->accel = flags;
+  h->nn = (flags & 1) ? nn : nullptr;
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_poisson_accel_stats(const ins2d_handle* h, long long* fallbacks, int* agg) {
+  if (!h) return INS2D_E_ARG;
+  if (fallbacks) *fallbacks = h->fallbacks;
+  if (agg) *agg = h->accel ? h->tl.C.agg : 0;
+  return INS2D_OK;
+}
+
+int ins2d_residual(ins2d_handle* h, const double* f, const double* x, double* relres) {
+  if (!h || !f || !x || !relres) return INS2D_E_ARG;
+  API_TRY
+  if (!h->plan) throw Unsupported("the residual needs the second-order pressure matrix");
+  HIPCHK(hipMemcpyAsync(h->f, f, sizeof(double) * h->Np, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->dp, x, sizeof(double) * h->Np, hipMemcpyHostToDevice, h->st));
+  launch_lap(h->st, h->lap, h->dp, h->tV);
+  launch_axpy(h->st, (size_t)h->Np, -1.0, h->f, h->tV);
+  double a[3], b[3];
+  stats_reduce(h, h->tV, h->Nx, h->Ny, nullptr, nullptr, a);
+  stats_reduce(h, h->f, h->Nx, h->Ny, nullptr, nullptr, b);
+  *relres = b[2] > 0.0 ? std::sqrt(a[2] / b[2]) : std::sqrt(a[2]);
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_pressure_rhs(ins2d_handle* h, double t, double* f) {
+  if (!h || !f) return INS2D_E_ARG;
+  (void)t;
+  API_TRY
+  if (h->order4) throw Unsupported("pressure right-hand side is built for second order");
+  rhs(h, h->V, h->p, RHS_SCALE_OM, h->tV, kNoCombine, nullptr);
+  div(h, h->tV, nullptr, 1.0, 0.0, 0.0, 1.0, h->f);
+  HIPCHK(hipMemcpyAsync(f, h->f, sizeof(double) * h->Np, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
   return INS2D_OK;
   API_CATCH(h)
 }

@@ -157,11 +157,11 @@ std::vector<double> dense_of(const HOp1& o) {
   return M;
 }
 
-// generalized problem L v = lam diag(h) v, V^T diag(h) V = I
-void gen_eig(const HOp1& L, const std::vector<double>& h, std::vector<double>& V, std::vector<double>& lam) {
-  const int n = L.nrows;
-  if (L.ncols != n || (int)h.size() != n) throw std::runtime_error("fast diagonalization: bad 1-D operator");
-  std::vector<double> S = dense_of(L);
+// generalized problem L v = lam diag(h) v, V^T diag(h) V = I (L dense n x n)
+void gen_eig_dense(int n, const std::vector<double>& Ld, const std::vector<double>& h, std::vector<double>& V,
+                   std::vector<double>& lam) {
+  if ((int)h.size() != n || Ld.size() != (size_t)n * n) throw std::runtime_error("fast diagonalization: bad 1-D operator");
+  std::vector<double> S = Ld;
   double amax = 0.0, asym = 0.0;
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) {
@@ -181,7 +181,6 @@ void gen_eig(const HOp1& L, const std::vector<double>& h, std::vector<double>& V
   // zero eigenvalue only to within eps * max|lam|, which next to a small
   // eigenvalue of the other direction would perturb the solve; use the exact
   // pair (0, constant / sqrt(sum h)) instead
-  const std::vector<double> Ld = dense_of(L);
   double rmax = 0.0;
   for (int i = 0; i < n; ++i) {
     double r = 0.0;
@@ -199,7 +198,42 @@ void gen_eig(const HOp1& L, const std::vector<double>& h, std::vector<double>& V
   }
 }
 
+void gen_eig(const HOp1& L, const std::vector<double>& h, std::vector<double>& V, std::vector<double>& lam) {
+  if (L.ncols != L.nrows) throw std::runtime_error("fast diagonalization: bad 1-D operator");
+  gen_eig_dense(L.nrows, dense_of(L), h, V, lam);
+}
+
+// Galerkin coarsening of a 1-D factor over aggregates of b consecutive
+// points: Lc = P^T L P, hc = P^T h (P the 0/1 aggregation matrix)
+void coarsen(const HOp1& L, const std::vector<double>& h, int b, std::vector<double>& Lc, std::vector<double>& hc,
+             int& nc) {
+  const int n = L.nrows;
+  nc = (n + b - 1) / b;
+  Lc.assign((size_t)nc * nc, 0.0);
+  hc.assign(nc, 0.0);
+  for (int i = 0; i < n; ++i) {
+    hc[i / b] += h[i];
+    for (int k = 0; k < L.K; ++k) {
+      const int j = L.col[(size_t)i * L.K + k];
+      Lc[(size_t)(i / b) + (size_t)nc * (j / b)] += L.val[(size_t)i * L.K + k];
+    }
+  }
+}
+
 }  // namespace
+
+void build_coarse_host(const Plan& P, int agg, FDMHost& C) {
+  std::vector<double> Lcx, Lcy, hcx, hcy;
+  coarsen(P.L1.X, P.hx, agg, Lcx, hcx, C.nx);
+  coarsen(P.L2.Y, P.hy, agg, Lcy, hcy, C.ny);
+  gen_eig_dense(C.nx, Lcx, hcx, C.Vx, C.lx);
+  gen_eig_dense(C.ny, Lcy, hcy, C.Vy, C.ly);
+  C.nx0 = C.ny0 = -1;
+  if (!P.pres_bc) {
+    C.nx0 = (int)(std::min_element(C.lx.begin(), C.lx.end()) - C.lx.begin());
+    C.ny0 = (int)(std::min_element(C.ly.begin(), C.ly.end()) - C.ly.begin());
+  }
+}
 
 void build_fdm_host(const Plan& P, FDMHost& F) {
   F.nx = P.Npx;

@@ -52,6 +52,33 @@ struct FDMPlan {
   const double *Vx, *VxT, *Vy, *VyT, *lx, *ly;
 };
 void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2);
+
+// accelerated PCG (pcg2.hip): optional warm start and two-level preconditioner
+constexpr int kTLMaxCoarse = 2048;   // coarse unknowns solved in one workgroup
+struct TLCoarse {
+  int ncx, ncy, agg, nx0, ny0;
+  const double *Vx, *Vy, *lx, *ly;   // coarse fast-diagonalization factors
+  double* rc;                        // aggregate sums of r (ncx * ncy)
+  double* ec;                        // coarse correction
+};
+struct TLPlan {
+  DLap L;
+  TLCoarse C;
+  double* s;      // 8 scalars
+  int* flag;      // 4 flags
+  double* part;   // 3 per aggregate block
+  int maxit;
+  int coarse;
+};
+void launch_tl_init(hipStream_t st, const TLPlan& P, const double* f, const double* x, double* r, double* pdir,
+                    int x0_zero, double tol2);
+void launch_tl_iter(hipStream_t st, const TLPlan& P, double* x, double* r, double* pdir, double* q);
+void launch_tl_true_residual(hipStream_t st, const TLPlan& P, const double* f, const double* x, double* r);
+// x += P Ac^-1 P^T (f - A x) (needs the coarse factors; r: work array)
+void launch_tl_coarse_correct(hipStream_t st, const TLPlan& P, const double* f, double* x, double* r);
+// 2 * sweeps damped Jacobi sweeps on x (tmp: work array)
+void launch_tl_smooth(hipStream_t st, const TLPlan& P, const double* f, double* x, double* tmp, int sweeps,
+                      double omega);
 void launch_reduce_stats(hipStream_t st, const double* x, int nx, int ny, const double* wx, const double* wy,
                          double* part, int nblk);
 

@@ -162,6 +162,10 @@ struct FDMHost {
   std::vector<double> Vx, VxT, Vy, VyT, lx, ly;
 };
 void build_fdm_host(const Plan& P, FDMHost& F);
+// the same factors for the Galerkin coarse operator P^T A P with aggregates
+// of agg x agg cells (the last aggregate of a row or column may be smaller);
+// VxT/VyT are left empty
+void build_coarse_host(const Plan& P, int agg, FDMHost& C);
 
 // sampled boundary values (set_bc_vectors.m:53-79), host copies
 struct BCSamples {

@@ -641,6 +641,13 @@ const char* ins2d_nn_last_error(const ins2d_nn* N) { return N ? N->err.c_str() :
 
 long ins2d_nn_num_params(const ins2d_nn* N) { return N ? N->nparam : 0; }
 
+int ins2d_nn_shape(const ins2d_nn* N, int* H, int* W) {
+  if (!N) return INS2D_E_ARG;
+  if (H) *H = N->H;
+  if (W) *W = N->W;
+  return INS2D_OK;
+}
+
 int ins2d_nn_set_params(ins2d_nn* N, const float* params, double out_scale) {
   if (!N || !params) return INS2D_E_ARG;
   try {
@@ -652,6 +659,39 @@ int ins2d_nn_set_params(ins2d_nn* N, const float* params, double out_scale) {
     return nn_fail(N, e);
   }
   return INS2D_OK;
+}
+
+// He-normal weights (std sqrt(2 / fan_in) before a ReLU, sqrt(1 / fan_in)
+// for the linear output layer), zero biases, fresh Adam state
+int ins2d_nn_init(ins2d_nn* N, unsigned long long seed, double out_scale) {
+  if (!N) return INS2D_E_ARG;
+  std::vector<float> th(N->nparam, 0.f);
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+  auto next = [&]() {   // splitmix64
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  auto unif = [&]() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+  for (int l = 0; l < kNL; ++l) {
+    const LayerDef& L = kLayers[l];
+    const double fan_in = (double)L.ks * L.ks * (L.cin_a + L.cin_b);
+    const double sd = std::sqrt((L.relu ? 2.0 : 1.0) / fan_in);
+    const long n = wcount(L);
+    for (long k = 0; k < n; ++k) {
+      const double g = std::sqrt(-2.0 * std::log(unif())) * std::cos(6.283185307179586 * unif());   // Box-Muller
+      th[N->woff[l] + k] = (float)(sd * g);
+    }
+  }
+  try {
+    NNCHK(hipMemset(N->m, 0, sizeof(float) * N->nparam));
+    NNCHK(hipMemset(N->v, 0, sizeof(float) * N->nparam));
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+  N->step = 0;
+  return ins2d_nn_set_params(N, th.data(), out_scale);
 }
 
 int ins2d_nn_get_params(ins2d_nn* N, float* params) {

@@ -185,6 +185,35 @@ class Solver:
         self._call(self.lib.ins2d_solver_stats, ctypes.byref(it), ctypes.byref(rel), ctypes.byref(tot))
         return it.value, rel.value, tot.value
 
+    def pressure_rhs(self, t=0.0):
+        """f = M Om^-1 (-conv + diff) for the resident velocity: the right-hand
+        side whose solution is the consistent pressure."""
+        f = np.zeros(self.Np)
+        self._call(self.lib.ins2d_pressure_rhs, ctypes.c_double(t), _dp(f))
+        return f
+
+    def residual(self, f, x):
+        """True relative residual ||f - A x|| / ||f||."""
+        rel = ctypes.c_double()
+        self._call(self.lib.ins2d_residual, _dp(np.ascontiguousarray(f, dtype=np.float64)),
+                   _dp(np.ascontiguousarray(x, dtype=np.float64)), ctypes.byref(rel))
+        return rel.value
+
+    def set_poisson_accel(self, net=None, warm_start=False, two_level=False, agg=0):
+        """Accelerated PCG (modes 3/4): learned start vector from ``net``
+        (a warmstart.WarmStartNet on this pressure grid) and/or the two-level
+        preconditioner; both off restores the plain PCG."""
+        flags = (1 if warm_start else 0) | (2 if two_level else 0)
+        self._accel_net = net if warm_start else None   # keep the network alive
+        self._call(self.lib.ins2d_set_poisson_accel, net._h if warm_start else None, ctypes.c_int(flags),
+                   ctypes.c_int(agg))
+
+    def accel_stats(self):
+        fb = ctypes.c_longlong()
+        agg = ctypes.c_int()
+        self._call(self.lib.ins2d_poisson_accel_stats, ctypes.byref(fb), ctypes.byref(agg))
+        return fb.value, agg.value
+
     def profile(self, enable):
         self._call(self.lib.ins2d_profile, ctypes.c_int(int(enable)))
 

@@ -48,6 +48,20 @@ class WarmStartNet:
         self.out_scale = float(out_scale)
         self._check(self._lib.ins2d_nn_set_params(self._h, th.ctypes.data_as(_lib.P_float), self.out_scale))
 
+    def init(self, seed=0, out_scale=1.0):
+        """He-normal weights, zero biases, fresh Adam state."""
+        self.out_scale = float(out_scale)
+        self._check(self._lib.ins2d_nn_init(self._h, ctypes.c_ulonglong(int(seed)), self.out_scale))
+
+    def save(self, path):
+        np.savez(path, theta=self.get_params(), out_scale=self.out_scale, H=self.H, W=self.W)
+
+    def load(self, path):
+        d = np.load(path)
+        if int(d['H']) != self.H or int(d['W']) != self.W:
+            raise ValueError('saved network is for another grid')
+        self.set_params(d['theta'], float(d['out_scale']))
+
     def get_params(self):
         th = np.empty(self.nparam, dtype=np.float32)
         self._check(self._lib.ins2d_nn_get_params(self._h, th.ctypes.data_as(_lib.P_float)))
