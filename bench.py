@@ -101,6 +101,29 @@ def cpu_child(case_json, steps):
                                 f'CPU {cpu_model()}, nproc {os.cpu_count()}'}))
 
 
+def copy_calibration():
+    """A plain 256 MiB device-to-device copy, three times: the reference
+    transfer for scaling the FETCH_SIZE / WRITE_SIZE hardware counters."""
+    import torch
+    n = 64 * 1024 * 1024
+    x = torch.ones(n, dtype=torch.float32, device='cuda')
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y = x.clone()
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_bytes': 4 * n, 'copies': 3, 'check': float(y[-1])}))
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of the roofline kernel from the committed
+    FETCH_SIZE / WRITE_SIZE counter runs (profiles/pmc_traffic.json)."""
+    try:
+        with open(os.path.join(ROOT, 'profiles', 'pmc_traffic.json')) as fh:
+            return json.load(fh).get(key, {}).get('bytes_per_launch')
+    except (OSError, ValueError):
+        return None
+
+
 def bench_ensemble(a):
     """Batched ensemble: 512 cavity/channel members of uneven size (48..512
     cells on the long side, aspect 1..4), Re in [100, 5000], speed in
@@ -285,7 +308,10 @@ def main():
         'config': {'workload': desc.format(Nx=opts.grid.Nx, Ny=opts.grid.Ny) + f', dt={dt:.4e}',
                    'cells': cells, 'rk_stages': stages, 'kernel_path': solver.path},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': (achieved / HBM_PEAK_GBS) if achieved else None, 'traffic': None,
+                     'frac': (achieved / HBM_PEAK_GBS) if achieved else None,
+                     'traffic': measured_traffic(f'{a.case}_{opts.grid.Nx}') if opts.grid.Nx == opts.grid.Ny else None,
+                     'traffic_unit': 'bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)',
+                     'algorithmic_bytes_per_launch': bytes_per_stage_cell * cells,
                      'kernel': 'momentum RHS + stage combination',
                      'bytes_per_cell': bytes_per_stage_cell},
         'kernels': prof,
@@ -312,5 +338,7 @@ def main():
 if __name__ == '__main__':
     if len(sys.argv) > 1 and sys.argv[1] == '--cpu-child':
         cpu_child(sys.argv[2], int(sys.argv[3]))
+    elif len(sys.argv) > 1 and sys.argv[1] == '--copy-calibration':
+        copy_calibration()
     else:
         main()
