@@ -115,6 +115,12 @@ int ins2d_diagnostics(ins2d_handle* h, double t, double* out);
 /* last Poisson solve: iterations and relative residual (CG modes) */
 int ins2d_solver_stats(const ins2d_handle* h, int* iters, double* relres, long long* total_iters);
 
+/* one FP64 16x16x4 matrix-core product of row-major A (16 x 4) and B
+ * (4 x 16) with the operand layout of the direct solve's GEMM; Draw receives
+ * each lane's four accumulator values (lane * 4 + r), which pins the result
+ * layout the GEMM assumes (lane l, register r = D[l / 16 + 4 r][l % 16]) */
+int ins2d_fdm_mfma_probe(const double* A, const double* B, double* Draw);
+
 /* true relative residual ||f - A x|| / ||f|| of a pressure solve */
 int ins2d_residual(ins2d_handle* h, const double* f, const double* x, double* relres);
 

@@ -73,6 +73,7 @@ SIGNATURES = {
     "ins2d_poisson_accel_stats": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong), P_int]),
     "ins2d_pressure_rhs": (c_int, [c_void_p, c_double, P_double]),
     "ins2d_residual": (c_int, [c_void_p, P_double, P_double, P_double]),
+    "ins2d_fdm_mfma_probe": (c_int, [P_double, P_double, P_double]),
 }
 
 _LIB = None

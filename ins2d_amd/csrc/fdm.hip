@@ -25,6 +25,7 @@
 #include <stdexcept>
 #include <vector>
 
+#include "../../include/ins2d.h"
 #include "launch.hpp"
 #include "ops1d.hpp"
 
@@ -32,72 +33,92 @@ namespace ins2d {
 
 namespace {
 
-// C = A B (column-major; M x K times K x N), 64 x 64 tile per workgroup,
-// 4 x 4 outputs per thread; EPI 1 scales C_ij by -1 / (lx_i + ly_j) and zeroes
-// the null mode (nx0, ny0)
+// C = A B (column-major; M x K times K x N) on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64): 32 x 32 tile per workgroup (enough workgroups to
+// fill the chip at the solve's sizes), 4 waves of one 16 x 16 MFMA tile, K
+// staged through LDS 32 at a time.  EPI 1 scales C_ij by -1 / (lx_i + ly_j) and
+// zeroes the null mode (nx0, ny0).
+// Operand layout of the 16x16x4 instruction: lane l holds A[l % 16][l / 16]
+// and B[l / 16][l % 16]; the result lane l, register r holds
+// D[l / 16 + 4 r][l % 16] (pinned by a GPU test through ins2d_fdm_mfma_probe).
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
 template <int EPI>
 __global__ void __launch_bounds__(256) k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
                                                const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc,
                                                const double* __restrict__ lx, const double* __restrict__ ly, int nx0,
                                                int ny0) {
-  __shared__ double As[16][64];
-  __shared__ double Bs[16][65];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
-  double acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  __shared__ double As[32][33];   // [k][m]
+  __shared__ double Bs[32][33];   // [k][n]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+  const int wm = (wave & 1) * 16, wn = (wave >> 1) * 16;
+  double4_t acc = double4_t{0.0, 0.0, 0.0, 0.0};
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += 32) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int e = threadIdx.x + 256 * r;
-      const int mm = e & 63, kk = e >> 6;
+      const int mm = e & 31, kk = e >> 5;      // A: consecutive threads along m (contiguous)
       const int gm = m0 + mm, gk = k0 + kk;
       As[kk][mm] = (gm < M && gk < K) ? A[(size_t)gm + (size_t)lda * gk] : 0.0;
-      const int kb = e & 15, nn = e >> 4;
+      const int kb = e & 31, nn = e >> 5;      // B: consecutive threads along k (contiguous)
       const int gk2 = k0 + kb, gn = n0 + nn;
       Bs[kb][nn] = (gk2 < K && gn < N) ? B[(size_t)gk2 + (size_t)ldb * gn] : 0.0;
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      double a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][ty + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    for (int ks = 0; ks < 32; ks += 4) {
+      const double a = As[ks + lk][wm + lr];
+      const double b = Bs[ks + lk][wn + lr];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
     __syncthreads();
   }
+  const int gn = n0 + wn + lr;
+  if (gn >= N) return;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int gn = n0 + ty + 16 * j;
-    if (gn >= N) continue;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int gm = m0 + tx + 16 * i;
-      if (gm >= M) continue;
-      double v = acc[i][j];
-      if (EPI == 1) v = (gm == nx0 && gn == ny0) ? 0.0 : -v / (lx[gm] + ly[gn]);
-      C[(size_t)gm + (size_t)ldc * gn] = v;
-    }
+  for (int r = 0; r < 4; ++r) {
+    const int gm = m0 + wm + lk + 4 * r;
+    if (gm >= M) continue;
+    double v = acc[r];
+    if (EPI == 1) v = (gm == nx0 && gn == ny0) ? 0.0 : -v / (lx[gm] + ly[gn]);
+    C[(size_t)gm + (size_t)ldc * gn] = v;
   }
 }
 
 template <int EPI>
 void gemm(hipStream_t st, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
           const double* lx = nullptr, const double* ly = nullptr, int nx0 = -1, int ny0 = -1) {
-  hipLaunchKernelGGL(k_dgemm<EPI>, dim3((M + 63) / 64, (N + 63) / 64), dim3(256), 0, st, M, N, K, A, lda, B, ldb, C,
+  hipLaunchKernelGGL(k_dgemm<EPI>, dim3((M + 31) / 32, (N + 31) / 32), dim3(256), 0, st, M, N, K, A, lda, B, ldb, C,
                      ldc, lx, ly, nx0, ny0);
 }
 
+// one v_mfma_f64_16x16x4f64 with the operand layout the GEMM uses; the raw
+// accumulator of every lane is written out (Draw[lane * 4 + r])
+__global__ void k_mfma64_probe(const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ Draw) {
+  const int l = threadIdx.x;
+  const double a = A[(l & 15) * 4 + (l >> 4)];    // A row-major 16 x 4
+  const double b = B[(l >> 4) * 16 + (l & 15)];   // B row-major 4 x 16
+  double4_t acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, double4_t{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) Draw[l * 4 + r] = acc[r];
+}
+
 }  // namespace
+
+int fdm_mfma_probe(const double* A, const double* B, double* Draw) {
+  double *dA = nullptr, *dB = nullptr, *dD = nullptr;
+  if (hipMalloc(&dA, 64 * 8) != hipSuccess || hipMalloc(&dB, 64 * 8) != hipSuccess || hipMalloc(&dD, 256 * 8) != hipSuccess)
+    return 2;
+  (void)hipMemcpy(dA, A, 64 * 8, hipMemcpyHostToDevice);
+  (void)hipMemcpy(dB, B, 64 * 8, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_mfma64_probe, dim3(1), dim3(64), 0, nullptr, dA, dB, dD);
+  hipError_t e = hipMemcpy(Draw, dD, 256 * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dD);
+  return e == hipSuccess ? 0 : 2;
+}
 
 // dp = A^-1 f; w1, w2: Npx*Npy work arrays
 void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2) {
@@ -109,3 +130,8 @@ void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, doubl
 }
 
 }  // namespace ins2d
+
+extern "C" int ins2d_fdm_mfma_probe(const double* A, const double* B, double* Draw) {
+  if (!A || !B || !Draw) return INS2D_E_ARG;
+  return ins2d::fdm_mfma_probe(A, B, Draw);
+}

@@ -233,3 +233,23 @@ def test_large_periodic_invariants():
     assert abs(d1[2] - d0[2]) < 1e-10 and abs(d1[3] - d0[3]) < 1e-10
     assert abs(d1[4] - d0[4]) / d0[4] < 1e-9
     solver.close()
+
+
+def test_fp64_matrix_core_result_layout():
+    """The direct solve's GEMM reads the accumulator of lane l, register r as
+    D[l / 16 + 4 r][l % 16]; integer operands make every product exact."""
+    import ctypes
+    from ins2d_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(0)
+    A = rng.integers(-9, 10, (16, 4)).astype(np.float64)
+    B = rng.integers(-9, 10, (4, 16)).astype(np.float64)
+    D = np.zeros(256)
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert lib.ins2d_fdm_mfma_probe(P(np.ascontiguousarray(A)), P(np.ascontiguousarray(B)), P(D)) == 0
+    ref = A @ B
+    raw = D.reshape(64, 4)
+    mapped = np.array([[raw[l, r] for r in range(4)] for l in range(64)])
+    expect = np.array([[ref[(l // 16) + 4 * r, l % 16] for r in range(4)] for l in range(64)])
+    _say('fp64 mfma layout as assumed:', np.array_equal(mapped, expect))
+    assert np.array_equal(mapped, expect)
