@@ -99,16 +99,41 @@ __device__ __forceinline__ void rhs_per2(const PerGrid& G, const double* __restr
 
 // fourth order (alfa = 81, beta = 9/8): alfa * (1h operators) - (3h operators)
 // (convection.m:38-43, operator_convection_diffusion.m:463-470)
-__device__ __forceinline__ void rhs_per4(const PerGrid& G, const double* __restrict__ u, const double* __restrict__ v,
-                                         int i, int j, double& fu, double& fv) {
-  const int Nx = G.Nx, Ny = G.Ny;
+// accessors of u(i + a, j + b), v(i + a, j + b) for |a|, |b| <= 3
+// global memory (INTERIOR: the block's x range and its +-3 halo lie inside
+// [0, Nx), so x offsets need no wrap)
+template <bool INTERIOR>
+struct GlobalAcc4 {
+  const double* ur[7];
+  const double* vr[7];
+  int ix[7];
+  __device__ __forceinline__ GlobalAcc4(const PerGrid& G, const double* __restrict__ u, const double* __restrict__ v,
+                                        int i, int j) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const size_t row = (size_t)G.Nx * wrap(j + k - 3, G.Ny);
+      ur[k] = u + row;
+      vr[k] = v + row;
+      ix[k] = INTERIOR ? i + k - 3 : wrap(i + k - 3, G.Nx);
+    }
+  }
+  __device__ __forceinline__ double U(int a, int b) const { return ur[b + 3][ix[a + 3]]; }
+  __device__ __forceinline__ double V(int a, int b) const { return vr[b + 3][ix[a + 3]]; }
+};
+// rows j-3 .. j+3 staged in LDS (pointers already at this thread's column)
+struct RingAcc4 {
+  const double* ur[7];
+  const double* vr[7];
+  __device__ __forceinline__ double U(int a, int b) const { return ur[b + 3][a]; }
+  __device__ __forceinline__ double V(int a, int b) const { return vr[b + 3][a]; }
+};
+
+template <class ACC>
+__device__ __forceinline__ void rhs4_core(const PerGrid& G, const ACC& acc, double& fu, double& fv) {
   const double hx = G.hx, hy = G.hy;
   const double w1 = 9.0 / 16.0, w2 = -1.0 / 16.0;   // beta/2, (1-beta)/2
-  int ix[8], jy[8];   // offsets -3..+4
-#pragma unroll
-  for (int k = 0; k < 8; ++k) { ix[k] = wrap(i + k - 3, Nx); jy[k] = wrap(j + k - 3, Ny); }
-#define U(a, b) u[ix[(a) + 3] + (size_t)Nx * jy[(b) + 3]]
-#define V(a, b) v[ix[(a) + 3] + (size_t)Nx * jy[(b) + 3]]
+#define U(a, b) acc.U(a, b)
+#define V(a, b) acc.V(a, b)
   // ---------------- u-momentum ----------------
   // x: 1h faces f = i (between u_{i-1}, u_i) and i+1; 3h faces at i-3/2, i+3/2
   auto Ix1 = [&](int f) { return hy * (w2 * U(f - 2, 0) + w1 * U(f - 1, 0) + w1 * U(f, 0) + w2 * U(f + 1, 0)); };
@@ -155,6 +180,13 @@ __device__ __forceinline__ void rhs_per4(const PerGrid& G, const double* __restr
 #undef V
 }
 
+template <bool INTERIOR>
+__device__ __forceinline__ void rhs_per4(const PerGrid& G, const double* __restrict__ u, const double* __restrict__ v,
+                                         int i, int j, double& fu, double& fv) {
+  GlobalAcc4<INTERIOR> acc(G, u, v, i, j);
+  rhs4_core(G, acc, fu, fv);
+}
+
 template <int ORDER>
 __device__ __forceinline__ void grad_per(const PerGrid& G, const double* __restrict__ p, int i, int j, double& gx,
                                          double& gy) {
@@ -182,8 +214,13 @@ __global__ void __launch_bounds__(256) k_rhs_per(PerGrid G, const double* __rest
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= G.Nx || j >= G.Ny) return;
   double fu, fv;
-  if (ORDER == 2) rhs_per2(G, u, v, i, j, fu, fv);
-  else rhs_per4(G, u, v, i, j, fu, fv);
+  if (ORDER == 2) {
+    rhs_per2(G, u, v, i, j, fu, fv);
+  } else {
+    const int x0 = blockIdx.x * blockDim.x;
+    if (x0 >= 3 && x0 + (int)blockDim.x + 3 <= G.Nx) rhs_per4<true>(G, u, v, i, j, fu, fv);
+    else rhs_per4<false>(G, u, v, i, j, fu, fv);
+  }
   if (mode & RHS_WITH_P) {
     double gx, gy;
     grad_per<ORDER>(G, p, i, j, gx, gy);
@@ -203,6 +240,91 @@ __global__ void __launch_bounds__(256) k_rhs_per(PerGrid G, const double* __rest
     for (int q = 0; q < sc.n; ++q) { a += sc.a[q] * sc.k[q][idx]; b += sc.a[q] * sc.k[q][N + idx]; }
     vtemp[idx] = a;
     vtemp[N + idx] = b;
+  }
+}
+
+// fourth order, row-streaming: one wave per 64-wide column strip of kStripH
+// rows; the rows j-3 .. j+3 of u and v live in an LDS ring of 7 rows (+-3
+// halo in x), so every value is read from memory once per strip (plus the
+// 6-row start-up and the 6-column halo)
+constexpr int kStripW = 64, kStripH = 64;
+__global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __restrict__ u,
+                                                 const double* __restrict__ v, const double* __restrict__ p, int mode,
+                                                 double* __restrict__ out, StageCombine sc,
+                                                 double* __restrict__ vtemp) {
+  __shared__ double su[7][kStripW + 6];
+  __shared__ double sv[7][kStripW + 6];
+  const int tx = threadIdx.x;
+  const int Nx = G.Nx, Ny = G.Ny;
+  const int x0 = blockIdx.x * kStripW, j0 = blockIdx.y * kStripH;
+  const int i = x0 + tx;
+  // LDS column c holds x = x0 - 3 + c (periodic)
+  const int xa = ((x0 - 3 + tx) % Nx + Nx) % Nx;
+  const int xb = tx < 6 ? (x0 + kStripW - 3 + tx) % Nx : 0;
+  // a row is fetched into registers one iteration ahead (its memory latency
+  // overlaps the previous row's arithmetic) and stored to the ring after
+  double pu = 0.0, pv = 0.0, pu2 = 0.0, pv2 = 0.0;
+  auto fetch_row = [&](int y) {
+    const size_t r = (size_t)Nx * wrap(y, Ny);
+    pu = u[r + xa];
+    pv = v[r + xa];
+    if (tx < 6) {
+      pu2 = u[r + xb];
+      pv2 = v[r + xb];
+    }
+  };
+  auto store_row = [&](int slot) {
+    su[slot][tx] = pu;
+    sv[slot][tx] = pv;
+    if (tx < 6) {
+      su[slot][kStripW + tx] = pu2;
+      sv[slot][kStripW + tx] = pv2;
+    }
+  };
+  for (int k = 0; k < 6; ++k) {
+    fetch_row(j0 - 3 + k);
+    store_row(k);
+  }
+  const size_t N = (size_t)Nx * Ny;
+  const double om = 72.0 * G.hx * G.hy;
+  const int jend = min(j0 + kStripH, Ny);
+  fetch_row(j0 + 3);
+  int s0 = 0;   // ring slot of row j - 3
+  for (int j = j0; j < jend; ++j) {
+    store_row(s0 == 0 ? 6 : s0 - 1);   // row j + 3
+    __syncthreads();
+    if (j + 1 < jend) fetch_row(j + 4);
+    RingAcc4 acc;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int sl = s0 + k < 7 ? s0 + k : s0 + k - 7;
+      acc.ur[k] = &su[sl][tx + 3];
+      acc.vr[k] = &sv[sl][tx + 3];
+    }
+    if (i < Nx) {
+      double fu, fv;
+      rhs4_core(G, acc, fu, fv);
+      if (mode & RHS_WITH_P) {
+        double gx, gy;
+        grad_per<4>(G, p, i, j, gx, gy);
+        fu -= gx;
+        fv -= gy;
+      }
+      if (mode & RHS_SCALE_OM) {
+        fu /= om;
+        fv /= om;
+      }
+      const size_t idx = (size_t)i + (size_t)Nx * j;
+      if (out) { out[idx] = fu; out[N + idx] = fv; }
+      if (vtemp) {
+        double a = sc.aself * fu, b = sc.aself * fv;
+        for (int q = 0; q < sc.n; ++q) { a += sc.a[q] * sc.k[q][idx]; b += sc.a[q] * sc.k[q][N + idx]; }
+        vtemp[idx] = a;
+        vtemp[N + idx] = b;
+      }
+    }
+    __syncthreads();
+    s0 = s0 == 6 ? 0 : s0 + 1;
   }
 }
 
@@ -730,7 +852,8 @@ void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double
   if (order == 2)
     hipLaunchKernelGGL(k_rhs_per<2>, grid2(Nx, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
   else
-    hipLaunchKernelGGL(k_rhs_per<4>, grid2(Nx, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
+    hipLaunchKernelGGL(k_rhs_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + kStripH - 1) / kStripH), dim3(kStripW), 0,
+                       st, G, u, v, p, mode, out, sc, vt);
 }
 
 void launch_div_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* A, const double* B,
