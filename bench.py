@@ -264,6 +264,17 @@ def main():
     value = cells * stages * a.steps / el / 1e6
 
     progress(f'timed region {el:.3f} s')
+    # the same stepping with the reference's per-step diagnostics
+    # (process_iteration.m:4,22: maxres of an extra F, maxdiv, momentum, energy)
+    nd = max(5, a.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(nd):
+        solver.step(t, dt, 1)
+        t += dt
+        solver.diagnostics(t)
+    el_diag = time.perf_counter() - t0
+    value_diag = cells * stages * nd / el_diag / 1e6
+    progress(f'with per-step diagnostics: {1e3 * el_diag / nd:.3f} ms/step')
     # per-kernel timing with HIP events on the library's stream (separate pass)
     nprof = max(2, min(a.steps, 50))
     solver.profile(True)
@@ -314,6 +325,7 @@ def main():
                      'algorithmic_bytes_per_launch': bytes_per_stage_cell * cells,
                      'kernel': 'momentum RHS + stage combination',
                      'bytes_per_cell': bytes_per_stage_cell},
+        'with_diagnostics': {'value': value_diag, 'ms_per_step': 1e3 * el_diag / nd, 'steps': nd},
         'kernels': prof,
         'diagnostics': {'maxres': diag[0], 'maxdiv': diag[1], 'k': diag[4]},
     }

@@ -69,6 +69,10 @@ typedef struct {
   int cg_maxit;              /* CG_maxit                                        */
   int p_add_solve;           /* 1: additional pressure solve per step           */
   int kernels;               /* 0: auto, 1: force generic kron-factored kernels */
+  int cg_precond;            /* PCG modes: 0 Jacobi; 1 IC(0) as the reference
+                                (make_cholinc.m, cg_matlab.m:135-198), one
+                                workgroup with wavefront triangular solves,
+                                up to 2^20 cells                                */
 } ins2d_case_desc;
 
 /* operator construction: replaces operators.m -> operator_mesh/averaging/

@@ -19,7 +19,8 @@ class CaseDesc(ctypes.Structure):
                 ("bc_u", c_int * 4), ("bc_v", c_int * 4), ("Re", c_double), ("order4", c_int),
                 ("alfa", c_double), ("beta", c_double), ("rk_stages", c_int), ("rk_A", P_double),
                 ("rk_b", P_double), ("rk_c", P_double), ("poisson", c_int), ("fft_symbol", c_int),
-                ("cg_tol", c_double), ("cg_maxit", c_int), ("p_add_solve", c_int), ("kernels", c_int)]
+                ("cg_tol", c_double), ("cg_maxit", c_int), ("p_add_solve", c_int), ("kernels", c_int),
+                ("cg_precond", c_int)]
 
 
 # every entry point declared in include/ins2d.h: name -> (restype, argtypes)

@@ -59,6 +59,9 @@ struct ins2d_handle {
   bool has_fft = false;
   // fast-diagonalization direct solve (poisson = direct)
   FDMPlan fdm{};
+  // IC(0) preconditioner of the reference (cg_precond = 1)
+  bool ic0 = false;
+  DIC dic{};
   // accelerated PCG: 1 learned warm start, 2 two-level preconditioner
   int accel = 0;
   ins2d_nn* nn = nullptr;
@@ -339,6 +342,19 @@ void poisson(ins2d_handle* h, const double* f, double* dp) {
   const double tol2 = h->tol * h->tol;
   if (h->accel) {
     accel_poisson(h, f, dp);
+    return;
+  }
+  if (h->ic0) {
+    launch_cg_ic0(h->st, h->lap, h->dic, f, dp, h->cr, h->cz, h->cp, h->cq, tol2, h->maxit, 1, h->d_it, h->d_rel);
+    int it = 0;
+    double rel = 0;
+    HIPCHK(hipMemcpyAsync(&it, h->d_it, sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(&rel, h->d_rel, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->last_it = it;
+    h->last_rel = std::sqrt(rel);
+    h->total_it += it;
+    if (it >= h->maxit && rel > tol2) throw NoConv("CG did not converge in maxit iterations");
     return;
   }
   if (h->Np <= 32768) {
@@ -666,6 +682,21 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
       h->cg.flag = dalloc<int>(h.get(), 4);
       h->cg.part = dalloc<double>(h.get(), 2 * (size_t)h->cg.nblk);
       h->cg.maxit = h->maxit;
+      if (d->cg_precond == 1 && (h->poisson == INS2D_POISSON_CG || h->poisson == INS2D_POISSON_CG_IC)) {
+        if (h->Np > (1 << 20)) throw Unsupported("IC(0) preconditioner: at most 2^20 pressure cells");
+        IC0Host I;
+        build_ic0_host(*h->plan, I);
+        h->dic.nx = h->Nx;
+        h->dic.ny = h->Ny;
+        h->dic.D = dupload(h.get(), I.D);
+        h->dic.cw = dupload(h.get(), I.cw);
+        h->dic.cs = dupload(h.get(), I.cs);
+        h->dic.cwx = I.cwx.empty() ? nullptr : dupload(h.get(), I.cwx);
+        h->dic.cwy = I.cwy.empty() ? nullptr : dupload(h.get(), I.cwy);
+        h->ic0 = true;
+      } else if (d->cg_precond != 0 && d->cg_precond != 1) {
+        throw ArgError("cg_precond must be 0 (Jacobi) or 1 (IC(0))");
+      }
       if (h->poisson == INS2D_POISSON_DIRECT) {
         FDMHost F;
         build_fdm_host(*h->plan, F);

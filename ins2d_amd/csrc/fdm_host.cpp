@@ -257,4 +257,56 @@ void build_fdm_host(const Plan& P, FDMHost& F) {
   F.VyT = transpose(F.Vy, F.ny);
 }
 
+void build_ic0_host(const Plan& P, IC0Host& H) {
+  const int nx = P.Npx, ny = P.Npy;
+  const std::vector<double> Lx = dense_of(P.L1.X), Ly = dense_of(P.L2.Y);
+  auto lx = [&](int a, int b) { return Lx[(size_t)a + (size_t)nx * b]; };
+  auto ly = [&](int a, int b) { return Ly[(size_t)a + (size_t)ny * b]; };
+  if (nx < 3 || ny < 3) throw std::runtime_error("IC(0) needs at least 3 cells per direction");
+  for (int a = 0; a < nx; ++a)
+    for (int b = 0; b < nx; ++b) {
+      const int d = std::abs(a - b);
+      if (d > 1 && d != nx - 1 && lx(a, b) != 0.0) throw std::runtime_error("IC(0): unexpected pressure stencil");
+    }
+  for (int a = 0; a < ny; ++a)
+    for (int b = 0; b < ny; ++b) {
+      const int d = std::abs(a - b);
+      if (d > 1 && d != ny - 1 && ly(a, b) != 0.0) throw std::runtime_error("IC(0): unexpected pressure stencil");
+    }
+  const size_t n = (size_t)nx * ny;
+  H.D.assign(n, 0.0);
+  H.cw.assign(n, 0.0);
+  H.cs.assign(n, 0.0);
+  const bool wx = lx(nx - 1, 0) != 0.0, wy = ly(ny - 1, 0) != 0.0;
+  H.cwx.assign(wx ? ny : 0, 0.0);
+  H.cwy.assign(wy ? nx : 0, 0.0);
+  // A((i,j),(i',j')) = -hy_j Lx(i,i') delta_jj' - hx_i Ly(j,j') delta_ii'
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const size_t r = (size_t)i + (size_t)nx * j;
+      double d = -(P.hy[j] * lx(i, i) + P.hx[i] * ly(j, j));
+      if (i > 0) {
+        const double a = -P.hy[j] * lx(i, i - 1);
+        H.cw[r] = a;
+        d -= a * a / H.D[r - 1];
+      }
+      if (j > 0) {
+        const double a = -P.hx[i] * ly(j, j - 1);
+        H.cs[r] = a;
+        d -= a * a / H.D[r - nx];
+      }
+      if (wx && i == nx - 1) {
+        const double a = -P.hy[j] * lx(nx - 1, 0);
+        H.cwx[j] = a;
+        d -= a * a / H.D[(size_t)nx * j];
+      }
+      if (wy && j == ny - 1) {
+        const double a = -P.hx[i] * ly(ny - 1, 0);
+        H.cwy[i] = a;
+        d -= a * a / H.D[i];
+      }
+      H.D[r] = d;
+    }
+}
+
 }  // namespace ins2d

@@ -70,6 +70,14 @@ struct TLPlan {
   int maxit;
   int coarse;
 };
+// IC(0)-preconditioned CG in one workgroup, triangular solves by
+// anti-diagonal wavefronts (pcg2.hip); factor from build_ic0_host
+struct DIC {
+  const double *D, *cw, *cs, *cwx, *cwy;   // cwx / cwy null when not periodic
+  int nx, ny;
+};
+void launch_cg_ic0(hipStream_t st, const DLap& L, const DIC& P, const double* f, double* x, double* r, double* z,
+                   double* p, double* q, double tol2, int maxit, int x0_zero, int* out_it, double* out_rel);
 void launch_tl_init(hipStream_t st, const TLPlan& P, const double* f, const double* x, double* r, double* pdir,
                     int x0_zero, double tol2);
 void launch_tl_iter(hipStream_t st, const TLPlan& P, double* x, double* r, double* pdir, double* q);

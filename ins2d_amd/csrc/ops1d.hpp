@@ -167,6 +167,17 @@ void build_fdm_host(const Plan& P, FDMHost& F);
 // VxT/VyT are left empty
 void build_coarse_host(const Plan& P, int agg, FDMHost& C);
 
+// IC(0) factor of the pressure matrix in the reference's form
+// (make_cholinc.m:15-36, cg.F:394-430): M = (D + L) D^-1 (D + L^T), L the
+// strict lower triangle of A, D_i = a_ii - sum_{c < i} a_ic^2 / D_c.  Lower
+// entries of cell (i, j): west (cw), south (cs), and for periodic directions
+// the wrap of the last column to the first (cwx[j]) and of the last row to the
+// first (cwy[i]); empty vectors when absent.
+struct IC0Host {
+  std::vector<double> D, cw, cs, cwx, cwy;
+};
+void build_ic0_host(const Plan& P, IC0Host& H);
+
 // sampled boundary values (set_bc_vectors.m:53-79), host copies
 struct BCSamples {
   std::vector<double> s[S_NSRC];

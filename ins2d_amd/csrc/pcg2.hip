@@ -262,6 +262,119 @@ __global__ void k_tl_true(TL T, int nblk) {
   if (threadIdx.x == 0) T.s[7] = ff > 0.0 ? rr / ff : 0.0;
 }
 
+// z = M^-1 r for M = (D + L) D^-1 (D + L^T) (cg_matlab.m:135-198): forward
+// (D + L) q = r, then (I + D^-1 L^T) z = q in place; cell (i, j) depends on
+// its west/south (forward) or east/north (backward) neighbours and on the
+// periodic wrap cells, all on earlier anti-diagonals i + j in sweep order
+__device__ void ic0_apply(const DIC& P, const double* __restrict__ r, double* __restrict__ z) {
+  const int nx = P.nx, ny = P.ny, T = blockDim.x;
+  const int ns = nx + ny - 1;
+  for (int s = 0; s < ns; ++s) {
+    const int ilo = max(0, s - (ny - 1)), ihi = min(s, nx - 1);
+    for (int i = ilo + (int)threadIdx.x; i <= ihi; i += T) {
+      const int j = s - i;
+      const size_t c = (size_t)i + (size_t)nx * j;
+      double acc = r[c];
+      if (i > 0) acc -= P.cw[c] * z[c - 1];
+      if (j > 0) acc -= P.cs[c] * z[c - nx];
+      if (P.cwx && i == nx - 1) acc -= P.cwx[j] * z[(size_t)nx * j];
+      if (P.cwy && j == ny - 1) acc -= P.cwy[i] * z[i];
+      z[c] = acc / P.D[c];
+    }
+    __syncthreads();
+  }
+  for (int s = ns - 1; s >= 0; --s) {
+    const int ilo = max(0, s - (ny - 1)), ihi = min(s, nx - 1);
+    for (int i = ilo + (int)threadIdx.x; i <= ihi; i += T) {
+      const int j = s - i;
+      const size_t c = (size_t)i + (size_t)nx * j;
+      double acc = 0.0;
+      if (i < nx - 1) acc += P.cw[c + 1] * z[c + 1];
+      if (j < ny - 1) acc += P.cs[c + nx] * z[c + nx];
+      if (P.cwx && i == 0) acc += P.cwx[j] * z[c + nx - 1];
+      if (P.cwy && j == 0) acc += P.cwy[i] * z[(size_t)i + (size_t)nx * (ny - 1)];
+      z[c] -= acc / P.D[c];
+    }
+    __syncthreads();
+  }
+}
+
+// IC(0)-PCG in one workgroup, the reference's iteration (cg_matlab.m:52-100):
+// stop when ||r||^2 / ||f||^2 <= tol^2, checked before the preconditioner
+__global__ void __launch_bounds__(1024) k_cg_ic0(DLap L, DIC P, const double* __restrict__ f, double* __restrict__ x,
+                                                double* __restrict__ r, double* __restrict__ z, double* __restrict__ p,
+                                                double* __restrict__ q, double tol2, int maxit, int x0_zero,
+                                                int* __restrict__ out_it, double* __restrict__ out_rel) {
+  __shared__ double red[16];
+  __shared__ double bc[2];
+  const int n = L.nx * L.ny;
+  const int T = blockDim.x;
+  double ff = 0.0;
+  for (int idx = threadIdx.x; idx < n; idx += T) {
+    const int i = idx % L.nx, j = idx / L.nx;
+    r[idx] = x0_zero ? f[idx] : f[idx] - lap_at(L, x, i, j);
+    if (x0_zero) x[idx] = 0.0;
+    ff += f[idx] * f[idx];
+  }
+  ff = block_sum(ff, red);
+  if (threadIdx.x == 0) bc[0] = ff;
+  __syncthreads();
+  ff = bc[0];
+  if (ff <= tol2) {
+    if (threadIdx.x == 0) { *out_it = 0; *out_rel = 0.0; }
+    return;
+  }
+  ic0_apply(P, r, z);
+  double rz = 0.0;
+  for (int idx = threadIdx.x; idx < n; idx += T) {
+    p[idx] = z[idx];
+    rz += r[idx] * z[idx];
+  }
+  rz = block_sum(rz, red);
+  if (threadIdx.x == 0) bc[0] = rz;
+  __syncthreads();
+  rz = bc[0];
+  int it = 0;
+  double rel = 1.0;
+  while (true) {
+    double pq = 0.0;
+    for (int idx = threadIdx.x; idx < n; idx += T) {
+      const int i = idx % L.nx, j = idx / L.nx;
+      const double a = lap_at(L, p, i, j);
+      q[idx] = a;
+      pq += p[idx] * a;
+    }
+    pq = block_sum(pq, red);
+    if (threadIdx.x == 0) bc[0] = pq;
+    __syncthreads();
+    const double alpha = rz / bc[0];
+    double rr = 0.0;
+    for (int idx = threadIdx.x; idx < n; idx += T) {
+      x[idx] += alpha * p[idx];
+      const double rn = r[idx] - alpha * q[idx];
+      r[idx] = rn;
+      rr += rn * rn;
+    }
+    rr = block_sum(rr, red);
+    if (threadIdx.x == 0) bc[0] = rr;
+    __syncthreads();
+    ++it;
+    rel = bc[0] / ff;
+    if (rel <= tol2 || it >= maxit) break;
+    ic0_apply(P, r, z);
+    double rzn = 0.0;
+    for (int idx = threadIdx.x; idx < n; idx += T) rzn += r[idx] * z[idx];
+    rzn = block_sum(rzn, red);
+    if (threadIdx.x == 0) bc[1] = rzn;
+    __syncthreads();
+    const double beta = bc[1] / rz;
+    rz = bc[1];
+    for (int idx = threadIdx.x; idx < n; idx += T) p[idx] = z[idx] + beta * p[idx];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { *out_it = it; *out_rel = rel; }
+}
+
 TL make(const TLPlan& P) {
   TL T;
   T.L = P.L;
@@ -278,6 +391,12 @@ dim3 grid_of(const TLPlan& P) { return dim3(P.C.ncx, P.C.ncy); }
 dim3 block_of(const TLPlan& P) { return dim3(P.C.agg, P.C.agg); }
 
 }  // namespace
+
+void launch_cg_ic0(hipStream_t st, const DLap& L, const DIC& P, const double* f, double* x, double* r, double* z,
+                   double* p, double* q, double tol2, int maxit, int x0_zero, int* out_it, double* out_rel) {
+  hipLaunchKernelGGL(k_cg_ic0, dim3(1), dim3(1024), 0, st, L, P, f, x, r, z, p, q, tol2, maxit, x0_zero, out_it,
+                     out_rel);
+}
 
 void launch_tl_init(hipStream_t st, const TLPlan& P, const double* f, const double* x, double* r, double* pdir,
                     int x0_zero, double tol2) {

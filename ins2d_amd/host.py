@@ -65,7 +65,7 @@ class Solver:
     """One libins2d_hip handle: operators, state and solver buffers on the GPU."""
 
     def __init__(self, x, y, bct, Re, RK='RK44', order4=0, poisson=POISSON_FFT, cg_tol=1e-8, cg_maxit=1000,
-                 p_add_solve=1, fft_symbol=0, kernels=0, alfa=81.0, beta=9 / 8):
+                 p_add_solve=1, fft_symbol=0, kernels=0, alfa=81.0, beta=9 / 8, cg_precond='jacobi'):
         lib = _lib.load()
         self.lib = lib
         _check_bc(bct)
@@ -92,6 +92,7 @@ class Solver:
         d.cg_maxit = int(cg_maxit)
         d.p_add_solve = int(p_add_solve)
         d.kernels = int(kernels)
+        d.cg_precond = {'jacobi': 0, 'ic0': 1}[cg_precond]
         h = ctypes.c_void_p()
         rc = lib.ins2d_create(ctypes.byref(d), ctypes.byref(h))
         if rc != 0:
@@ -386,7 +387,8 @@ def setup(case, **overrides):
                     poisson=prm.get('poisson', POISSON_FFT), cg_tol=prm.get('CG_acc', 1e-8),
                     cg_maxit=prm.get('CG_maxit', 1000), p_add_solve=prm.get('p_add_solve', 1),
                     fft_symbol=prm.get('fft_symbol', 0), kernels=prm.get('kernels', 0),
-                    alfa=prm.get('alfa', 81.0), beta=prm.get('beta', 9 / 8))
+                    alfa=prm.get('alfa', 81.0), beta=prm.get('beta', 9 / 8),
+                    cg_precond=prm.get('CG_precond', 'jacobi'))
     sizes = [solver.Nux_in, solver.Nuy_in, solver.Nvx_in, solver.Nvy_in, solver.Npx, solver.Npy]
     V0, p0, samples = _finish(case, opts, sizes, solver.xin, solver.yin)
     solver.set_bc(samples, opts.t_start)

@@ -5,7 +5,7 @@
 //
 // stdin: Nx Ny | x[0..Nx] | y[0..Ny] | 8 BC kinds | Re | 12 samples (n v...)
 // With the argument --fdm it also prints the fast-diagonalization factors of
-// the pressure matrix (fdm_host.cpp).
+// the pressure matrix and its IC(0) factor (fdm_host.cpp).
 #include <cstdio>
 #include <iostream>
 #include <string>
@@ -144,6 +144,15 @@ int main(int argc, char** argv) {
     printf(",\"ly\":");
     vec(F.ly);
     printf("}");
+    IC0Host I;
+    try {
+      build_ic0_host(P, I);
+      printf(",\"ic0\":{\"D\":");
+      vec(I.D);
+      printf("}");
+    } catch (const std::exception& e) {
+      printf(",\"ic0_error\":\"%s\"", e.what());
+    }
   }
   printf("}\n");
   return 0;

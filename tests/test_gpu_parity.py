@@ -253,3 +253,39 @@ def test_fp64_matrix_core_result_layout():
     expect = np.array([[ref[(l // 16) + 4 * r, l % 16] for r in range(4)] for l in range(64)])
     _say('fp64 mfma layout as assumed:', np.array_equal(mapped, expect))
     assert np.array_equal(mapped, expect)
+
+
+@pytest.mark.parametrize('name', ['ldc', 'outflow', 'channel', 'bfs'])
+def test_ic0_pcg_matches_cg_matlab(name):
+    """IC(0)-PCG on the GPU (wavefront triangular solves) against the oracle's
+    cg_matlab: the same iteration count (within one) and the same solution."""
+    _say('ic0 pcg', name)
+    kw = dict(poisson=4, CG_acc=1e-12, CG_maxit=5000, CG_precond='ic0')
+    if name == 'ldc':
+        case = _wall_case('ldc', 40, 30, sx=1.03, **kw)
+    elif name == 'outflow':
+        case = _wall_case('outflow', 48, 20, sx=1.03, **kw)
+    elif name == 'channel':
+        case = _wall_case('channel', 36, 20, sy=1.04, **kw)
+    else:
+        case = load('BFS_unsteady', Nx=128, Ny=32, **kw)
+    opts = O.build(case)
+    solver, _, _, _ = host.setup(case)
+    rng = np.random.default_rng(8)
+    f = rng.standard_normal(opts.grid.Np)
+    if name in ('ldc', 'channel'):
+        f -= f.mean()
+    dp, it, rel = solver.poisson(f)
+    xo, ito, _ = O.cg_matlab(opts.discretization.A, f, 1e-12, np.zeros(opts.grid.Np), 5000)
+    _say(name, 'iterations gpu', it, 'oracle', ito, 'rel diff', _rel(dp, xo))
+    assert abs(it - ito) <= 1
+    assert _rel(dp, xo) < 1e-9
+    solver.close()
+
+
+def test_ldc_run_ic0_matches_oracle():
+    """LDC 20x20 fixture configuration with the reference's cg_matlab
+    preconditioner (poisson = 4, IC(0)) on both sides."""
+    _say('ldc 20x20 run ic0')
+    case = load('LDC_unsteady', RK='RK33P2', poisson=4, CG_acc=1e-13, CG_maxit=2000, CG_precond='ic0')
+    _compare_runs(case, 30, True)

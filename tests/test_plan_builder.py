@@ -184,3 +184,15 @@ def test_fast_diagonalization_factors(dumper, kind, Nx, Ny, stretch):
         pref = sp.linalg.spsolve(A.tocsc(), f)
     assert np.abs(p - pref).max() <= 1e-11 * np.abs(pref).max()
     assert np.abs(A @ p - f).max() <= 1e-11 * np.abs(f).max()
+
+
+@pytest.mark.parametrize('kind,Nx,Ny,stretch', CASES)
+def test_ic0_factor_matches_make_cholinc(dumper, kind, Nx, Ny, stretch):
+    """The host IC(0) diagonal (fdm_host.cpp build_ic0_host) against the
+    oracle's restatement of make_cholinc.m on the assembled A."""
+    case = _case(kind, Nx, Ny, stretch)
+    opts = O.build(case)
+    d = _dump(dumper, opts, case, fdm=True)
+    assert 'ic0' in d, d.get('ic0_error')
+    Dref = O.make_cholinc_diag(opts.discretization.A)
+    np.testing.assert_allclose(d['ic0']['D'], Dref, rtol=1e-12, atol=0)
