@@ -205,6 +205,10 @@ CASES = {
     'LDC_unsteady': ('LDC_unsteady', lambda N: dict(Nx=N or 64, Ny=N or 64, Re=1000.0, dt=1e-3, RK='RK44', poisson=3,
                                                      CG_acc=1e-10, CG_maxit=20000),
                      'lid-driven cavity {Nx}x{Ny}, order 2, RK44, dt=1e-3, Jacobi-PCG 1e-10, Re=1000'),
+    'LDC_unsteady_ic0': ('LDC_unsteady', lambda N: dict(Nx=N or 64, Ny=N or 64, Re=1000.0, dt=1e-3, RK='RK44', poisson=4,
+                                                         CG_acc=1e-10, CG_maxit=20000, CG_precond='ic0'),
+                         'lid-driven cavity {Nx}x{Ny}, order 2, RK44, dt=1e-3, IC(0)-PCG 1e-10 (the reference preconditioner), '
+                         'Re=1000'),
     'LDC_unsteady_direct': ('LDC_unsteady', lambda N: dict(Nx=N or 64, Ny=N or 64, Re=1000.0, dt=1e-3, RK='RK44',
                                                             poisson=1),
                             'lid-driven cavity {Nx}x{Ny}, order 2, RK44, dt=1e-3, direct pressure solve '
