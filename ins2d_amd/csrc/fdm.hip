@@ -34,10 +34,13 @@ namespace ins2d {
 namespace {
 
 // C = A B (column-major; M x K times K x N) on the FP64 matrix cores
-// (v_mfma_f64_16x16x4f64): 32 x 32 tile per workgroup (enough workgroups to
-// fill the chip at the solve's sizes), 4 waves of one 16 x 16 MFMA tile, K
-// staged through LDS 32 at a time.  EPI 1 scales C_ij by -1 / (lx_i + ly_j) and
-// zeroes the null mode (nx0, ny0).
+// (v_mfma_f64_16x16x4f64).  A workgroup owns a 32 x 32 tile of C; its four
+// waves split K (wave w takes the k-steps w, w + 4, ...), each keeping four
+// independent 16 x 16 accumulators (2 x 2 tiles) so that consecutive matrix
+// instructions do not wait on each other; operands come straight from
+// memory (L2) one k-step ahead, and the four partial tiles are summed through
+// LDS.  EPI 1 scales C_ij by -1 / (lx_i + ly_j) and zeroes the null mode
+// (nx0, ny0).
 // Operand layout of the 16x16x4 instruction: lane l holds A[l % 16][l / 16]
 // and B[l / 16][l % 16]; the result lane l, register r holds
 // D[l / 16 + 4 r][l % 16] (pinned by a GPU test through ins2d_fdm_mfma_probe).
@@ -48,40 +51,48 @@ __global__ void __launch_bounds__(256) k_dgemm(int M, int N, int K, const double
                                                const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc,
                                                const double* __restrict__ lx, const double* __restrict__ ly, int nx0,
                                                int ny0) {
-  __shared__ double As[32][33];   // [k][m]
-  __shared__ double Bs[32][33];   // [k][n]
+  __shared__ double part[4][32][33];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
-  const int wm = (wave & 1) * 16, wn = (wave >> 1) * 16;
-  double4_t acc = double4_t{0.0, 0.0, 0.0, 0.0};
   const int lr = lane & 15, lk = lane >> 4;
-  for (int k0 = 0; k0 < K; k0 += 32) {
+  const int ma = m0 + lr, mb = m0 + 16 + lr, na = n0 + lr, nb = n0 + 16 + lr;
+  double4_t acc[2][2];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int e = threadIdx.x + 256 * r;
-      const int mm = e & 31, kk = e >> 5;      // A: consecutive threads along m (contiguous)
-      const int gm = m0 + mm, gk = k0 + kk;
-      As[kk][mm] = (gm < M && gk < K) ? A[(size_t)gm + (size_t)lda * gk] : 0.0;
-      const int kb = e & 31, nn = e >> 5;      // B: consecutive threads along k (contiguous)
-      const int gk2 = k0 + kb, gn = n0 + nn;
-      Bs[kb][nn] = (gk2 < K && gn < N) ? B[(size_t)gk2 + (size_t)ldb * gn] : 0.0;
-    }
-    __syncthreads();
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int ks = 0; ks < 32; ks += 4) {
-      const double a = As[ks + lk][wm + lr];
-      const double b = Bs[ks + lk][wn + lr];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-    }
-    __syncthreads();
+    for (int b = 0; b < 2; ++b) acc[a][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+  auto ld = [&](int k, double& a0, double& a1, double& b0, double& b1) {
+    const bool kin = k < K;
+    a0 = (kin && ma < M) ? A[(size_t)ma + (size_t)lda * k] : 0.0;
+    a1 = (kin && mb < M) ? A[(size_t)mb + (size_t)lda * k] : 0.0;
+    b0 = (kin && na < N) ? B[(size_t)k + (size_t)ldb * na] : 0.0;
+    b1 = (kin && nb < N) ? B[(size_t)k + (size_t)ldb * nb] : 0.0;
+  };
+  int k = 4 * wave + lk;
+  double a0, a1, b0, b1;
+  ld(k, a0, a1, b0, b1);
+  for (int k0 = 4 * wave; k0 < K; k0 += 16) {
+    double c0, c1, d0, d1;
+    ld(k + 16, c0, c1, d0, d1);   // next k-step of this wave
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    a0 = c0; a1 = c1; b0 = d0; b1 = d1;
+    k += 16;
   }
-  const int gn = n0 + wn + lr;
-  if (gn >= N) return;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int gm = m0 + wm + lk + 4 * r;
-    if (gm >= M) continue;
-    double v = acc[r];
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) part[wave][16 * a + lk + 4 * r][16 * b + lr] = acc[a][b][r];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+    const int mm = e & 31, nn = e >> 5;
+    const int gm = m0 + mm, gn = n0 + nn;
+    if (gm >= M || gn >= N) continue;
+    double v = part[0][mm][nn] + part[1][mm][nn] + part[2][mm][nn] + part[3][mm][nn];
     if (EPI == 1) v = (gm == nx0 && gn == ny0) ? 0.0 : -v / (lx[gm] + ly[gn]);
     C[(size_t)gm + (size_t)ldc * gn] = v;
   }
