@@ -23,10 +23,10 @@ CHANNEL = {'BC_unsteady': 0, 'u': dict(left='dir', right='pres', low='dir', up='
            'v': dict(left='dir', right='sym', low='dir', up='dir')}
 
 
-def member(kind, Nx, Ny, aspect, Re, U, CFL=0.5, RK='RK44', cg_tol=1e-10, cg_maxit=20000):
+def member(kind, Nx, Ny, aspect, Re, U, CFL=0.5, RK='RK44', cg_tol=1e-10, cg_maxit=20000, precond='jacobi'):
     prm = dict(Re=float(Re), Nx=int(Nx), Ny=int(Ny), x1=0.0, x2=float(aspect), y1=0.0, y2=1.0, sx=1.0, sy=1.0,
-               order4=0, RK=RK, poisson=3, CG_acc=cg_tol, CG_maxit=cg_maxit, p_initial=1, p_add_solve=1,
-               t_start=0.0, nsteps=200, speed=float(U), kind=kind)
+               order4=0, RK=RK, poisson=3, CG_acc=cg_tol, CG_maxit=cg_maxit, CG_precond=precond, p_initial=1,
+               p_add_solve=1, t_start=0.0, nsteps=200, speed=float(U), kind=kind)
     h = min(aspect / Nx, 1.0 / Ny)
     prm['dt'] = min(CFL * h / U, 0.25 * h * h * Re)
 

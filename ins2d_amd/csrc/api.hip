@@ -694,8 +694,8 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
         h->dic.cwx = I.cwx.empty() ? nullptr : dupload(h.get(), I.cwx);
         h->dic.cwy = I.cwy.empty() ? nullptr : dupload(h.get(), I.cwy);
         h->ic0 = true;
-      } else if (d->cg_precond != 0 && d->cg_precond != 1) {
-        throw ArgError("cg_precond must be 0 (Jacobi) or 1 (IC(0))");
+      } else if (d->cg_precond != 0 && d->cg_precond != 1 && d->cg_precond != 2) {
+        throw ArgError("cg_precond must be 0 (Jacobi), 1 (IC(0)) or 2 (two-level)");
       }
       if (h->poisson == INS2D_POISSON_DIRECT) {
         FDMHost F;
@@ -728,7 +728,16 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
     h->st = nullptr;
     return code;
   }
-  *out = h.release();
+  ins2d_handle* hp = h.release();
+  if (d->cg_precond == 2 && (hp->poisson == INS2D_POISSON_CG || hp->poisson == INS2D_POISSON_CG_IC)) {
+    const int rc = ins2d_set_poisson_accel(hp, nullptr, 2, 0);
+    if (rc != INS2D_OK) {
+      g_create_err = hp->err;
+      ins2d_destroy(hp);
+      return rc;
+    }
+  }
+  *out = hp;
   return INS2D_OK;
 }
 

@@ -29,12 +29,20 @@ struct BInst {
   long voff, poff;
   int nu;
   int pt0, pt1;   // range of pressure tiles of this instance
+  // two-level preconditioner: aggregate tiles [at0, at1) of kAgg x kAgg cells,
+  // coarse grid ncx x ncy at offset coff of rc / ec, fast-diagonalization
+  // factors of the Galerkin coarse operator
+  int at0, at1, ncx, ncy, nx0, ny0;
+  long coff;
+  const double *cVx, *cVy, *clx, *cly;
 };
 struct Tile {
   int inst, comp, i0, j0;
 };
 
 constexpr int TX = 64, TY = 4;
+constexpr int kAgg = 16;            // aggregate side of the two-level preconditioner
+constexpr int kBMaxCoarse = 2048;   // coarse unknowns per instance
 
 // ------------------------------------------------------------------ kernels
 __global__ void __launch_bounds__(256) k_b_rhs(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
@@ -248,8 +256,214 @@ __global__ void __launch_bounds__(256) k_b_cg_dir(const BInst* __restrict__ I, c
   p[k] = z[k] + S.s[8 * t.inst + 3] * p[k];
 }
 
+// ---- batched two-level PCG: z = D^-1 r + P Ac^-1 P^T r per instance (P the
+// aggregation over kAgg x kAgg tiles, Ac = P^T A P solved by fast
+// diagonalization in one workgroup per instance); partials 3 per tile
+// {p.q or f.f, r.r, r.D^-1 r}; same per-instance scalars as the Jacobi PCG
+struct BTL {
+  double* part;
+  double* rc;
+  double* ec;
+};
+
+__device__ __forceinline__ int agg_index(const BInst& B, const Tile& t) {
+  return (t.i0 / kAgg) + B.ncx * (t.j0 / kAgg);
+}
+
+__global__ void __launch_bounds__(256) k_b2_init(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
+                                                BTL T, const double* __restrict__ f, double* __restrict__ x,
+                                                double* __restrict__ r) {
+  __shared__ double red[4];
+  const Tile t = tiles[blockIdx.x];
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  double fi = 0.0, di = 0.0;
+  if (i < B.npx && j < B.npy) {
+    const long k = B.poff + (long)i + (long)B.npx * j;
+    fi = f[k];
+    x[k] = 0.0;
+    r[k] = fi;
+    di = B.lap.dinv[(long)i + (long)B.npx * j];
+  }
+  const double ff = block_sum(fi * fi, red);
+  const double rd = block_sum(fi * di * fi, red);
+  const double rs = block_sum(fi, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    T.part[3 * blockIdx.x] = ff;
+    T.part[3 * blockIdx.x + 1] = ff;
+    T.part[3 * blockIdx.x + 2] = rd;
+    T.rc[B.coff + agg_index(B, t)] = rs;
+  }
+  (void)S;
+}
+
+__global__ void __launch_bounds__(256) k_b2_apply(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
+                                                 BTL T, const double* __restrict__ p, double* __restrict__ q) {
+  __shared__ double red[4];
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  double acc = 0.0;
+  if (i < B.npx && j < B.npy) {
+    const long k = B.poff + (long)i + (long)B.npx * j;
+    const double a = lap_at(B.lap, p + B.poff, i, j);
+    q[k] = a;
+    acc = p[k] * a;
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) T.part[3 * blockIdx.x] = s;
+}
+
+__global__ void k_b2_alpha(const BInst* __restrict__ I, BCG S, BTL T) {
+  __shared__ double red[4];
+  const int k = blockIdx.x;
+  if (S.f[4 * k]) return;
+  const BInst& B = I[k];
+  double a = 0.0;
+  for (int t = B.at0 + threadIdx.x; t < B.at1; t += blockDim.x) a += T.part[3 * t];
+  const double pq = block_sum(a, red);
+  if (threadIdx.x == 0) S.s[8 * k + 2] = S.s[8 * k + 0] / pq;
+}
+
+__global__ void __launch_bounds__(256) k_b2_update(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
+                                                  BTL T, const double* __restrict__ p, const double* __restrict__ q,
+                                                  double* __restrict__ x, double* __restrict__ r) {
+  __shared__ double red[4];
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  const double alpha = S.s[8 * t.inst + 2];
+  double ri = 0.0, di = 0.0;
+  if (i < B.npx && j < B.npy) {
+    const long k = B.poff + (long)i + (long)B.npx * j;
+    x[k] += alpha * p[k];
+    ri = r[k] - alpha * q[k];
+    r[k] = ri;
+    di = B.lap.dinv[(long)i + (long)B.npx * j];
+  }
+  const double rr = block_sum(ri * ri, red);
+  const double rd = block_sum(ri * di * ri, red);
+  const double rs = block_sum(ri, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    T.part[3 * blockIdx.x + 1] = rr;
+    T.part[3 * blockIdx.x + 2] = rd;
+    T.rc[B.coff + agg_index(B, t)] = rs;
+  }
+}
+
+// one workgroup per instance: convergence test (cg.F:247), coarse solve,
+// r.z and beta; init: after k_b2_init (ff, beta = 0)
+__global__ void __launch_bounds__(256) k_b2_coarse(const BInst* __restrict__ I, BCG S, BTL T, int init,
+                                                  double tol2) {
+  __shared__ double red[4];
+  __shared__ double Ab[kBMaxCoarse];
+  __shared__ double Bb[kBMaxCoarse];
+  __shared__ double bc[2];
+  __shared__ int done;
+  const int k = blockIdx.x;
+  if (S.f[4 * k]) return;
+  const BInst& B = I[k];
+  double rr = 0.0, rd = 0.0;
+  for (int t = B.at0 + threadIdx.x; t < B.at1; t += blockDim.x) {
+    rr += T.part[3 * t + 1];
+    rd += T.part[3 * t + 2];
+  }
+  rr = block_sum(rr, red);
+  rd = block_sum(rd, red);
+  if (threadIdx.x == 0) {
+    int d = 0;
+    if (init) {
+      S.s[8 * k + 1] = rr;   // ||f||^2 (r = f)
+      S.s[8 * k + 4] = 1.0;
+      S.s[8 * k + 5] = tol2;
+      S.f[4 * k + 1] = 0;
+      S.f[4 * k + 2] = 0;
+      d = (rr <= tol2) ? 1 : 0;   // cg.F:222
+    } else {
+      const int it = S.f[4 * k + 1] + 1;
+      S.f[4 * k + 1] = it;
+      const double rel = rr / S.s[8 * k + 1];
+      S.s[8 * k + 4] = rel;
+      if (rel <= S.s[8 * k + 5]) {
+        d = 1;
+      } else if (it >= S.maxit) {
+        d = 1;
+        S.f[4 * k + 2] = 1;
+      }
+    }
+    if (d) {
+      S.f[4 * k + 0] = 1;
+      atomicSub(S.nactive, 1);
+    }
+    done = d;
+    bc[0] = rd;
+  }
+  __syncthreads();
+  if (done) return;
+  double rz = bc[0];
+  const int m = B.ncx, n = B.ncy, mn = m * n;
+  const double* rc = T.rc + B.coff;
+  double* ec = T.ec + B.coff;
+  for (int e = threadIdx.x; e < mn; e += blockDim.x) Ab[e] = rc[e];
+  __syncthreads();
+  for (int e = threadIdx.x; e < mn; e += blockDim.x) {
+    const int a = e % m, b2 = e / m;
+    double s = 0.0;
+    for (int q = 0; q < m; ++q) s += B.cVx[q + (long)m * a] * Ab[q + m * b2];
+    Bb[e] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < mn; e += blockDim.x) {
+    const int a = e % m, b2 = e / m;
+    double s = 0.0;
+    for (int q = 0; q < n; ++q) s += Bb[a + m * q] * B.cVy[q + (long)n * b2];
+    Ab[e] = (a == B.nx0 && b2 == B.ny0) ? 0.0 : -s / (B.clx[a] + B.cly[b2]);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < mn; e += blockDim.x) {
+    const int a = e % m, b2 = e / m;
+    double s = 0.0;
+    for (int q = 0; q < m; ++q) s += B.cVx[a + (long)m * q] * Ab[q + m * b2];
+    Bb[e] = s;
+  }
+  __syncthreads();
+  double dot = 0.0;
+  for (int e = threadIdx.x; e < mn; e += blockDim.x) {
+    const int a = e % m, b2 = e / m;
+    double s = 0.0;
+    for (int q = 0; q < n; ++q) s += Bb[a + m * q] * B.cVy[b2 + (long)n * q];
+    ec[e] = s;
+    dot += rc[e] * s;
+  }
+  dot = block_sum(dot, red);
+  if (threadIdx.x == 0) {
+    rz += dot;
+    S.s[8 * k + 3] = init ? 0.0 : rz / S.s[8 * k + 0];
+    S.s[8 * k + 0] = rz;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_b2_dir(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
+                                               BTL T, const double* __restrict__ r, double* __restrict__ p) {
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  if (i >= B.npx || j >= B.npy) return;
+  const long k = B.poff + (long)i + (long)B.npx * j;
+  const double z = B.lap.dinv[(long)i + (long)B.npx * j] * r[k] + T.ec[B.coff + agg_index(B, t)];
+  p[k] = z + S.s[8 * t.inst + 3] * p[k];
+}
+
 __global__ void k_b_set_int(int* x, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *x = v;
+}
+
+__global__ void k_b_clear_flags(int* f, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) f[4 * k] = 0;
 }
 
 }  // namespace ins2d
@@ -328,6 +542,11 @@ struct ins2d_batch {
   int p_add = 1;
   hipGraphExec_t cg_chunk = nullptr;
   hipGraph_t cg_graph = nullptr;
+  // two-level PCG (cg_precond = 2)
+  bool two_level = false;
+  std::vector<Tile> at;
+  Tile* dat = nullptr;
+  BTL tl{};
   int chunk = 16;
   long long total_it = 0;
   int max_it = 0;
@@ -429,6 +648,19 @@ void build_ops(ins2d_batch* b, Arena& A) {
     I.lap.dinv = A.push(dinv);
     I.lap.nx = P.Npx;
     I.lap.ny = P.Npy;
+    if (b->two_level) {
+      FDMHost C;
+      build_coarse_host(P, kAgg, C);
+      if ((long)C.nx * C.ny > kBMaxCoarse) throw BErr(INS2D_E_UNSUPPORTED, "instance too large for the coarse solve");
+      I.ncx = C.nx;
+      I.ncy = C.ny;
+      I.nx0 = C.nx0;
+      I.ny0 = C.ny0;
+      I.cVx = A.push(C.Vx);
+      I.cVy = A.push(C.Vy);
+      I.clx = A.push(C.lx);
+      I.cly = A.push(C.ly);
+    }
   }
 }
 
@@ -449,6 +681,16 @@ void launch_tiles_proj(ins2d_batch* b, const double* Vn, const double* Vt, const
 }
 
 void cg_iter_launches(ins2d_batch* b) {
+  if (b->two_level) {
+    const unsigned na = (unsigned)b->at.size();
+    const dim3 tb(kAgg, kAgg);
+    hipLaunchKernelGGL(k_b2_apply, dim3(na), tb, 0, b->st, b->dinst, b->dat, b->cg, b->tl, b->pp, b->q);
+    hipLaunchKernelGGL(k_b2_alpha, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tl);
+    hipLaunchKernelGGL(k_b2_update, dim3(na), tb, 0, b->st, b->dinst, b->dat, b->cg, b->tl, b->pp, b->q, b->x, b->r);
+    hipLaunchKernelGGL(k_b2_coarse, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tl, 0, 0.0);
+    hipLaunchKernelGGL(k_b2_dir, dim3(na), tb, 0, b->st, b->dinst, b->dat, b->cg, b->tl, b->r, b->pp);
+    return;
+  }
   const unsigned np = (unsigned)b->pt.size();
   hipLaunchKernelGGL(k_b_cg_apply, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q);
   hipLaunchKernelGGL(k_b_cg_alpha, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
@@ -462,9 +704,19 @@ void cg_iter_launches(ins2d_batch* b) {
 void b_poisson(ins2d_batch* b, const double* f) {
   const unsigned np = (unsigned)b->pt.size();
   hipLaunchKernelGGL(k_b_set_int, dim3(1), dim3(1), 0, b->st, b->cg.nactive, b->n);
-  hipLaunchKernelGGL(k_b_cg_init, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, f, b->x, b->r, b->z,
-                     b->pp);
-  hipLaunchKernelGGL(k_b_cg_init_fin, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tol * b->tol);
+  if (b->two_level) {
+    const unsigned na = (unsigned)b->at.size();
+    const dim3 tb(kAgg, kAgg);
+    // per-instance flags cleared first: k_b2_coarse skips finished instances
+    hipLaunchKernelGGL(k_b_clear_flags, dim3((b->n + 255) / 256), dim3(256), 0, b->st, b->cg.f, b->n);
+    hipLaunchKernelGGL(k_b2_init, dim3(na), tb, 0, b->st, b->dinst, b->dat, b->cg, b->tl, f, b->x, b->r);
+    hipLaunchKernelGGL(k_b2_coarse, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tl, 1, b->tol * b->tol);
+    hipLaunchKernelGGL(k_b2_dir, dim3(na), tb, 0, b->st, b->dinst, b->dat, b->cg, b->tl, b->r, b->pp);
+  } else {
+    hipLaunchKernelGGL(k_b_cg_init, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, f, b->x, b->r, b->z,
+                       b->pp);
+    hipLaunchKernelGGL(k_b_cg_init_fin, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tol * b->tol);
+  }
   if (!b->cg_chunk) {
     BHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
     for (int c = 0; c < b->chunk; ++c) cg_iter_launches(b);
@@ -569,6 +821,8 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
       b->plans.emplace_back(new Plan());
       build_plan(cd, *b->plans.back());
     }
+    if (d0.cg_precond == 2) b->two_level = true;
+    else if (d0.cg_precond != 0) throw BErr(INS2D_E_UNSUPPORTED, "the ensemble runs Jacobi or two-level PCG");
     BHIP(hipStreamCreate(&b->st));
     b->hcomp.assign(2 * (size_t)n, DComp{});
     b->hinst.assign(n, BInst{});
@@ -585,7 +839,7 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
     build_ops(b.get(), A);
     BHIP(hipMemcpy(A.dev, A.host.data(), A.cap, hipMemcpyHostToDevice));
     // layout and tiles
-    long voff = 0, poff = 0;
+    long voff = 0, poff = 0, coff = 0;
     for (int k = 0; k < n; ++k) {
       const Plan& P = *b->plans[k];
       BInst& I = b->hinst[k];
@@ -604,6 +858,14 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
       for (int j = 0; j < I.npy; j += TY)
         for (int i = 0; i < I.npx; i += TX) b->pt.push_back(Tile{k, 2, i, j});
       I.pt1 = (int)b->pt.size();
+      I.at0 = (int)b->at.size();
+      I.coff = coff;
+      if (b->two_level) {
+        for (int j = 0; j < I.npy; j += kAgg)
+          for (int i = 0; i < I.npx; i += kAgg) b->at.push_back(Tile{k, 2, i, j});
+        coff += (long)I.ncx * I.ncy;
+      }
+      I.at1 = (int)b->at.size();
       // samples default to zero
       BCSamples& S = b->samp[k];
       for (int s = 0; s < S_NSRC; ++s) S.s[s].assign(sample_len(P, s), 0.0);
@@ -622,6 +884,13 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
     BHIP(hipMemcpy(b->duv, b->uvt.data(), b->uvt.size() * sizeof(Tile), hipMemcpyHostToDevice));
     b->dpt = balloc<Tile>(b.get(), b->pt.size());
     BHIP(hipMemcpy(b->dpt, b->pt.data(), b->pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    if (b->two_level) {
+      b->dat = balloc<Tile>(b.get(), b->at.size());
+      BHIP(hipMemcpy(b->dat, b->at.data(), b->at.size() * sizeof(Tile), hipMemcpyHostToDevice));
+      b->tl.part = balloc<double>(b.get(), 3 * b->at.size());
+      b->tl.rc = balloc<double>(b.get(), (size_t)coff);
+      b->tl.ec = balloc<double>(b.get(), (size_t)coff);
+    }
     // tableau (time_ERK.m:29,32)
     const int s = b->s;
     b->At.assign((size_t)s * s, 0.0);
