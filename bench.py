@@ -208,6 +208,10 @@ CASES = {
     'LDC_unsteady': ('LDC_unsteady', lambda N: dict(Nx=N or 64, Ny=N or 64, Re=1000.0, dt=1e-3, RK='RK44', poisson=3,
                                                      CG_acc=1e-10, CG_maxit=20000),
                      'lid-driven cavity {Nx}x{Ny}, order 2, RK44, dt=1e-3, Jacobi-PCG 1e-10, Re=1000'),
+    'LDC_unsteady_2level': ('LDC_unsteady', lambda N: dict(Nx=N or 64, Ny=N or 64, Re=1000.0, dt=1e-3, RK='RK44',
+                                                            poisson=3, CG_acc=1e-10, CG_maxit=20000,
+                                                            CG_precond='two_level'),
+                            'lid-driven cavity {Nx}x{Ny}, order 2, RK44, dt=1e-3, two-level PCG 1e-10, Re=1000'),
     'LDC_unsteady_ic0': ('LDC_unsteady', lambda N: dict(Nx=N or 64, Ny=N or 64, Re=1000.0, dt=1e-3, RK='RK44', poisson=4,
                                                          CG_acc=1e-10, CG_maxit=20000, CG_precond='ic0'),
                          'lid-driven cavity {Nx}x{Ny}, order 2, RK44, dt=1e-3, IC(0)-PCG 1e-10 (the reference preconditioner), '
@@ -221,6 +225,9 @@ CASES = {
                      '(fast diagonalization), Re=800'),
     'BFS_unsteady_pcg': ('BFS_unsteady', lambda N: dict(Nx=N or 1024, Ny=(N // 4 if N else 256), poisson=3),
                          'backward-facing step {Nx}x{Ny} tanh-stretched, order 2, RK44, Jacobi-PCG 1e-10, Re=800'),
+    'BFS_unsteady_2level': ('BFS_unsteady', lambda N: dict(Nx=N or 1024, Ny=(N // 4 if N else 256), poisson=3,
+                                                           CG_precond='two_level'),
+                            'backward-facing step {Nx}x{Ny} tanh-stretched, order 2, RK44, two-level PCG 1e-10, Re=800'),
 }
 
 

@@ -67,6 +67,12 @@ struct ins2d_handle {
   ins2d_nn* nn = nullptr;
   TLPlan tl{};
   long long fallbacks = 0;
+  // chunks of PCG iterations captured once into a graph and replayed (the
+  // iteration kernels test the device-side done flag, so a chunk may run
+  // past convergence harmlessly); keyed by the solution array
+  hipGraphExec_t cg_exec = nullptr, tl_exec = nullptr;
+  hipGraph_t cg_graph = nullptr, tl_graph = nullptr;
+  const double *cg_x = nullptr, *tl_x = nullptr;
   // state and work buffers
   double *V = nullptr, *Vb = nullptr, *p = nullptr, *Vt = nullptr, *f = nullptr, *dp = nullptr, *tV = nullptr;
   std::vector<double*> kv;
@@ -310,8 +316,16 @@ void accel_poisson(ins2d_handle* h, const double* f, double* dp) {
   int flag[3] = {0, 0, 0};
   HIPCHK(hipMemcpyAsync(flag, h->tl.flag, 3 * sizeof(int), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
-  while (!flag[0]) {
+  if (!flag[0] && (!h->tl_exec || h->tl_x != dp)) {
+    if (h->tl_exec) { HIPCHK(hipGraphExecDestroy(h->tl_exec)); HIPCHK(hipGraphDestroy(h->tl_graph)); }
+    HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < 16; ++k) launch_tl_iter(h->st, h->tl, dp, h->cr, h->cp, h->cq);
+    HIPCHK(hipStreamEndCapture(h->st, &h->tl_graph));
+    HIPCHK(hipGraphInstantiate(&h->tl_exec, h->tl_graph, nullptr, nullptr, 0));
+    h->tl_x = dp;
+  }
+  while (!flag[0]) {
+    HIPCHK(hipGraphLaunch(h->tl_exec, h->st));
     HIPCHK(hipMemcpyAsync(flag, h->tl.flag, 3 * sizeof(int), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
   }
@@ -373,8 +387,16 @@ void poisson(ins2d_handle* h, const double* f, double* dp) {
   HIPCHK(hipMemsetAsync(dp, 0, sizeof(double) * h->Np, h->st));
   launch_cg_init(h->st, h->lap, h->cg, f, dp, h->cr, h->cz, h->cp, 1, tol2);
   int flag[3] = {0, 0, 0};
-  while (true) {
+  if (!h->cg_exec || h->cg_x != dp) {
+    if (h->cg_exec) { HIPCHK(hipGraphExecDestroy(h->cg_exec)); HIPCHK(hipGraphDestroy(h->cg_graph)); }
+    HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < 16; ++k) launch_cg_iter(h->st, h->lap, h->cg, dp, h->cr, h->cz, h->cp, h->cq);
+    HIPCHK(hipStreamEndCapture(h->st, &h->cg_graph));
+    HIPCHK(hipGraphInstantiate(&h->cg_exec, h->cg_graph, nullptr, nullptr, 0));
+    h->cg_x = dp;
+  }
+  while (true) {
+    HIPCHK(hipGraphLaunch(h->cg_exec, h->st));
     HIPCHK(hipMemcpyAsync(flag, h->cg.flag, 3 * sizeof(int), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     if (flag[0]) break;
@@ -744,6 +766,8 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
 void ins2d_destroy(ins2d_handle* h) {
   if (!h) return;
   if (h->st) (void)hipStreamSynchronize(h->st);
+  if (h->cg_exec) { (void)hipGraphExecDestroy(h->cg_exec); (void)hipGraphDestroy(h->cg_graph); }
+  if (h->tl_exec) { (void)hipGraphExecDestroy(h->tl_exec); (void)hipGraphDestroy(h->tl_graph); }
   for (auto& pe : h->pend) { h->ev_free.push_back(pe.a); h->ev_free.push_back(pe.b); }
   for (auto e : h->ev_free) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
@@ -953,6 +977,12 @@ int ins2d_set_poisson_accel(ins2d_handle* h, ins2d_nn* nn, int flags, int agg) {
   HIPCHK(hipMemset(T.s, 0, 8 * sizeof(double)));
   HIPCHK(hipMemset(T.flag, 0, 4 * sizeof(int)));
   h->tl = T;
+  if (h->tl_exec) {
+    HIPCHK(hipGraphExecDestroy(h->tl_exec));
+    HIPCHK(hipGraphDestroy(h->tl_graph));
+    h->tl_exec = nullptr;
+    h->tl_graph = nullptr;
+  }
   h->accel = flags;
   h->nn = (flags & 1) ? nn : nullptr;
   return INS2D_OK;
