@@ -136,6 +136,8 @@ def bench_ensemble(a):
     t0 = time.perf_counter()
     B = host.Batch(members)
     B.input_check(1)
+    tile = B.autotune()
+    progress('stage tile shape', tile)
     t_setup = time.perf_counter() - t0
     B.step(a.warmup)
     it0, _, cells = B.stats()
@@ -149,7 +151,7 @@ def bench_ensemble(a):
            'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
            'config': {'workload': f'ensemble of {a.members} cavity/channel members, RK44, '
                                   f'{"Jacobi" if precond == "jacobi" else "two-level (Jacobi + coarse grid)"}-PCG 1e-10',
-                      'cells': cells, 'setup_s': t_setup,
+                      'cells': cells, 'setup_s': t_setup, 'stage_tile': list(tile),
                       'pcg_iterations_per_member_solve': (it1 - it0) / (a.members * 4 * a.steps)}}
     B.close()
     print(json.dumps(out))

@@ -165,6 +165,10 @@ int ins2d_batch_get_state(ins2d_batch* b, int k, double* V, double* p);
 int ins2d_batch_input_check(ins2d_batch* b, int p_initial);
 /* nsteps of time_ERK.m for every instance, instance k with time step dt[k] */
 int ins2d_batch_step(ins2d_batch* b, const double* dt, int nsteps);
+/* pick the workgroup shape of the stage kernels (velocity tiles) by timing
+ * the momentum right-hand side over all instances for a few candidates on the
+ * resident state; returns the chosen shape (no reference counterpart) */
+int ins2d_batch_autotune(ins2d_batch* b, int* tx, int* ty);
 /* accumulated PCG iterations over all instances and solves, the largest
  * per-solve iteration count, total pressure cells */
 int ins2d_batch_stats(const ins2d_batch* b, long long* total_iters, int* max_iters, long long* cells);

@@ -542,6 +542,8 @@ struct ins2d_batch {
   int p_add = 1;
   hipGraphExec_t cg_chunk = nullptr;
   hipGraph_t cg_graph = nullptr;
+  // stage-kernel tile shape (velocity tiles), chosen by ins2d_batch_autotune
+  int tx = TX, ty = TY;
   // two-level PCG (cg_precond = 2)
   bool two_level = false;
   std::vector<Tile> at;
@@ -664,9 +666,17 @@ void build_ops(ins2d_batch* b, Arena& A) {
   }
 }
 
+void append_velocity_tiles(const BInst& I, int k, int tx, int ty, std::vector<Tile>& out) {
+  for (int c = 0; c < 2; ++c) {
+    const int nx = c == 0 ? I.nux : I.nvx, ny = c == 0 ? I.nuy : I.nvy;
+    for (int j = 0; j < ny; j += ty)
+      for (int i = 0; i < nx; i += tx) out.push_back(Tile{k, c, i, j});
+  }
+}
+
 void launch_tiles_rhs(ins2d_batch* b, const double* V, const double* P, int mode, double* out, const StageCombine& sc,
                       double* vt) {
-  hipLaunchKernelGGL(k_b_rhs, dim3((unsigned)b->uvt.size()), dim3(TX, TY), 0, b->st, b->dinst, b->duv, V, P, mode,
+  hipLaunchKernelGGL(k_b_rhs, dim3((unsigned)b->uvt.size()), dim3(b->tx, b->ty), 0, b->st, b->dinst, b->duv, V, P, mode,
                      out, sc, vt);
 }
 void launch_tiles_div(ins2d_batch* b, const double* A, const double* B, const double* dt, double ca, double cb,
@@ -676,7 +686,7 @@ void launch_tiles_div(ins2d_batch* b, const double* A, const double* B, const do
 }
 void launch_tiles_proj(ins2d_batch* b, const double* Vn, const double* Vt, const double* dp, const double* dt,
                        double c, double* V) {
-  hipLaunchKernelGGL(k_b_proj, dim3((unsigned)b->uvt.size()), dim3(TX, TY), 0, b->st, b->dinst, b->duv, Vn, Vt, dp,
+  hipLaunchKernelGGL(k_b_proj, dim3((unsigned)b->uvt.size()), dim3(b->tx, b->ty), 0, b->st, b->dinst, b->duv, Vn, Vt, dp,
                      dt, c, V);
 }
 
@@ -849,11 +859,7 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
       I.poff = poff;
       voff += P.NV;
       poff += P.Np;
-      for (int c = 0; c < 2; ++c) {
-        const int nx = c == 0 ? I.nux : I.nvx, ny = c == 0 ? I.nuy : I.nvy;
-        for (int j = 0; j < ny; j += TY)
-          for (int i = 0; i < nx; i += TX) b->uvt.push_back(Tile{k, c, i, j});
-      }
+      append_velocity_tiles(I, k, TX, TY, b->uvt);
       I.pt0 = (int)b->pt.size();
       for (int j = 0; j < I.npy; j += TY)
         for (int i = 0; i < I.npx; i += TX) b->pt.push_back(Tile{k, 2, i, j});
@@ -948,6 +954,58 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
   }
   *out = b.release();
   return INS2D_OK;
+}
+
+// time the stage kernel (the momentum right-hand side over every velocity
+// tile) for a few workgroup shapes on the resident state and keep the fastest
+int ins2d_batch_autotune(ins2d_batch* b, int* tx, int* ty) {
+  if (!b) return INS2D_E_ARG;
+  B_TRY
+  // 256 threads each (the kernels are compiled for at most 256)
+  static const int cand[][2] = {{64, 4}, {32, 8}, {128, 2}, {16, 16}, {256, 1}};
+  hipEvent_t e0, e1;
+  BHIP(hipEventCreate(&e0));
+  BHIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bx = b->tx, by = b->ty;
+  std::vector<Tile> best_list = b->uvt;
+  const StageCombine none{0, 0.0, {0}, {nullptr}};
+  for (const auto& c : cand) {
+    std::vector<Tile> list;
+    for (int k = 0; k < b->n; ++k) append_velocity_tiles(b->hinst[k], k, c[0], c[1], list);
+    Tile* d = nullptr;
+    BHIP(hipMalloc(&d, list.size() * sizeof(Tile)));
+    BHIP(hipMemcpy(d, list.data(), list.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    float tot = 0.f;
+    for (int rep = 0; rep < 4; ++rep) {
+      BHIP(hipEventRecord(e0, b->st));
+      hipLaunchKernelGGL(k_b_rhs, dim3((unsigned)list.size()), dim3(c[0], c[1]), 0, b->st, b->dinst, d, b->V, b->P,
+                         RHS_SCALE_OM, b->tV, none, nullptr);
+      BHIP(hipEventRecord(e1, b->st));
+      BHIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      BHIP(hipEventElapsedTime(&ms, e0, e1));
+      if (rep > 0) tot += ms;   // the first launch warms caches
+    }
+    BHIP(hipFree(d));
+    if (tot < best) {
+      best = tot;
+      bx = c[0];
+      by = c[1];
+      best_list.swap(list);
+    }
+  }
+  BHIP(hipEventDestroy(e0));
+  BHIP(hipEventDestroy(e1));
+  if (best_list.size() > b->uvt.size()) b->duv = balloc<Tile>(b, best_list.size());
+  BHIP(hipMemcpy(b->duv, best_list.data(), best_list.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  b->uvt.swap(best_list);
+  b->tx = bx;
+  b->ty = by;
+  if (tx) *tx = bx;
+  if (ty) *ty = by;
+  return INS2D_OK;
+  B_CATCH(b)
 }
 
 void ins2d_batch_destroy(ins2d_batch* b) {

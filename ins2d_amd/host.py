@@ -358,6 +358,12 @@ class Batch:
         self._call(self.lib.ins2d_batch_get_state, ctypes.c_int(k), _dp(V), _dp(p))
         return V, p
 
+    def autotune(self):
+        """Time the stage kernel for a few workgroup shapes; keep the fastest."""
+        tx, ty = ctypes.c_int(), ctypes.c_int()
+        self._call(self.lib.ins2d_batch_autotune, ctypes.byref(tx), ctypes.byref(ty))
+        return tx.value, ty.value
+
     def stats(self):
         tot = ctypes.c_longlong()
         mx = ctypes.c_int()

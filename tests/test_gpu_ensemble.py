@@ -29,6 +29,9 @@ def test_ensemble_matches_single_runs_and_oracle(precond):
     nsteps = 5
     B = host.Batch(members)
     B.input_check(1)
+    tile = B.autotune()
+    _say('stage tile', tile)
+    assert tile[0] * tile[1] == 256
     B.step(nsteps)
     for k, m in enumerate(members):
         Vb, pb = B.get_state(k)
