@@ -383,7 +383,9 @@ __device__ __forceinline__ double2 conjd(double2 a) { return make_double2(a.x, -
 // Stockham autosort radix-4 passes (plus one radix-2 pass when log2 N is odd);
 // each thread owns at most 4 radix-4 (8 radix-2) butterflies per pass, held
 // in registers between the LDS read and the LDS write (N <= 16 * blockDim).
-template <bool INV>
+// C: radix-4 butterflies per thread and pass (2 C in the radix-2 tail);
+// the launch must give N <= 4 C blockDim
+template <bool INV, int C>
 __device__ void lds_fft(double2* z, int N, int log2N, const double2* __restrict__ tw) {
   const int T = blockDim.x;
   const int tid = threadIdx.x;
@@ -392,9 +394,9 @@ __device__ void lds_fft(double2* z, int N, int log2N, const double2* __restrict_
   for (int pass = 0; pass < passes4; ++pass) {
     const int Q = N / 4;
     const int step = N / (Ns * 4);
-    double2 o0[4], o1[4], o2[4], o3[4];
+    double2 o0[C], o1[C], o2[C], o3[C];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < C; ++c) {
       const int jq = tid + c * T;
       if (jq < Q) {
         const int k = jq % Ns;
@@ -413,7 +415,7 @@ __device__ void lds_fft(double2* z, int N, int log2N, const double2* __restrict_
     }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < C; ++c) {
       const int jq = tid + c * T;
       if (jq < Q) {
         const int k = jq % Ns;
@@ -430,9 +432,9 @@ __device__ void lds_fft(double2* z, int N, int log2N, const double2* __restrict_
   if (log2N & 1) {
     const int Q = N / 2;
     const int step = N / (Ns * 2);
-    double2 o0[8], o1[8];
+    double2 o0[2 * C], o1[2 * C];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < 2 * C; ++c) {
       const int jq = tid + c * T;
       if (jq < Q) {
         const int k = jq % Ns;
@@ -446,7 +448,7 @@ __device__ void lds_fft(double2* z, int N, int log2N, const double2* __restrict_
     }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < 2 * C; ++c) {
       const int jq = tid + c * T;
       if (jq < Q) {
         const int k = jq % Ns;
@@ -479,15 +481,16 @@ __device__ void lds_dft(double2* z, double2* out, int N, const double2* __restri
   __syncthreads();
 }
 
-template <bool INV>
+template <bool INV, int C>
 __device__ __forceinline__ void lds_transform(double2* z, double2* scratch, int N, bool pow2, int log2N,
                                               const double2* __restrict__ tw) {
-  if (pow2) lds_fft<INV>(z, N, log2N, tw);
+  if (pow2) lds_fft<INV, C>(z, N, log2N, tw);
   else lds_dft<INV>(z, scratch, N, tw);
 }
 
 // rows y0 = 2b and y1 = 2b+1 packed as one complex row; half spectra out
-__global__ void __launch_bounds__(256) k_fft_r2c_rows(int Nx, int Ny, int Nh, bool pow2, int log2N,
+template <int C>
+__global__ void __launch_bounds__(C == 1 ? 1024 : 256) k_fft_r2c_rows(int Nx, int Ny, int Nh, bool pow2, int log2N,
                                                      const double2* __restrict__ tw, const double* __restrict__ f,
                                                      double2* __restrict__ H) {
   extern __shared__ double2 sm[];
@@ -498,7 +501,7 @@ __global__ void __launch_bounds__(256) k_fft_r2c_rows(int Nx, int Ny, int Nh, bo
   for (int n = threadIdx.x; n < Nx; n += blockDim.x)
     z[n] = make_double2(f[(size_t)y0 * Nx + n], has1 ? f[(size_t)y1 * Nx + n] : 0.0);
   __syncthreads();
-  lds_transform<false>(z, scratch, Nx, pow2, log2N, tw);
+  lds_transform<false, C>(z, scratch, Nx, pow2, log2N, tw);
   for (int k = threadIdx.x; k < Nh; k += blockDim.x) {
     const double2 a = z[k], b = conjd(z[(Nx - k) % Nx]);
     H[(size_t)y0 * Nh + k] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
@@ -507,7 +510,8 @@ __global__ void __launch_bounds__(256) k_fft_r2c_rows(int Nx, int Ny, int Nh, bo
   }
 }
 
-__global__ void __launch_bounds__(256) k_fft_c2r_rows(int Nx, int Ny, int Nh, bool pow2, int log2N,
+template <int C>
+__global__ void __launch_bounds__(C == 1 ? 1024 : 256) k_fft_c2r_rows(int Nx, int Ny, int Nh, bool pow2, int log2N,
                                                      const double2* __restrict__ tw, const double2* __restrict__ H,
                                                      double* __restrict__ out) {
   extern __shared__ double2 sm[];
@@ -529,7 +533,7 @@ __global__ void __launch_bounds__(256) k_fft_c2r_rows(int Nx, int Ny, int Nh, bo
     z[k] = make_double2(a.x - b.y, a.y + b.x);
   }
   __syncthreads();
-  lds_transform<true>(z, scratch, Nx, pow2, log2N, tw);
+  lds_transform<true, C>(z, scratch, Nx, pow2, log2N, tw);
   for (int n = threadIdx.x; n < Nx; n += blockDim.x) {
     out[(size_t)y0 * Nx + n] = z[n].x;
     if (has1) out[(size_t)y1 * Nx + n] = z[n].y;
@@ -552,7 +556,8 @@ __global__ void k_transpose_c(const double2* __restrict__ in, double2* __restric
 }
 
 // one block per kx: forward FFT along y, divide by -Ahat, inverse FFT along y
-__global__ void __launch_bounds__(256) k_fft_cols_solve(int Nx, int Ny, bool pow2, int log2N,
+template <int C>
+__global__ void __launch_bounds__(C == 1 ? 1024 : 256) k_fft_cols_solve(int Nx, int Ny, bool pow2, int log2N,
                                                        const double2* __restrict__ tw, const double* __restrict__ ax,
                                                        const double* __restrict__ ay, double c,
                                                        double2* __restrict__ HT) {
@@ -563,7 +568,7 @@ __global__ void __launch_bounds__(256) k_fft_cols_solve(int Nx, int Ny, bool pow
   double2* row = HT + (size_t)kx * Ny;
   for (int n = threadIdx.x; n < Ny; n += blockDim.x) z[n] = row[n];
   __syncthreads();
-  lds_transform<false>(z, scratch, Ny, pow2, log2N, tw);
+  lds_transform<false, C>(z, scratch, Ny, pow2, log2N, tw);
   const double scale = 1.0 / ((double)Nx * (double)Ny);
   for (int ky = threadIdx.x; ky < Ny; ky += blockDim.x) {
     double Ah = c * (ax[kx] + ay[ky]);
@@ -572,7 +577,7 @@ __global__ void __launch_bounds__(256) k_fft_cols_solve(int Nx, int Ny, bool pow
     z[ky] = make_double2(z[ky].x * s, z[ky].y * s);
   }
   __syncthreads();
-  lds_transform<true>(z, scratch, Ny, pow2, log2N, tw);
+  lds_transform<true, C>(z, scratch, Ny, pow2, log2N, tw);
   for (int n = threadIdx.x; n < Ny; n += blockDim.x) row[n] = z[n];
 }
 
@@ -882,21 +887,25 @@ static int ilog2(int n) {
   return l;
 }
 
+// threads per transform: one radix-4 butterfly per thread and pass for the
+// power-of-two lengths (N / 4 threads, at least 64); 256 for the direct DFT
+static int fft_threads(int N, bool pow2) { return pow2 ? std::max(64, std::min(1024, N / 4)) : 256; }
+
 void launch_fft_poisson(hipStream_t st, const FFTPlan& P, const double* f, double* dp) {
   const int Nx = P.Nx, Ny = P.Ny, Nh = P.Nh;
-  const int T = 256;
+  const int Tx = fft_threads(Nx, P.pow2x), Ty = fft_threads(Ny, P.pow2y);
   size_t shx = (size_t)(P.pow2x ? Nx : 2 * Nx) * sizeof(double2);
   size_t shy = (size_t)(P.pow2y ? Ny : 2 * Ny) * sizeof(double2);
   double2* H = (double2*)P.H;
   double2* HT = (double2*)P.HT;
-  hipLaunchKernelGGL(k_fft_r2c_rows, dim3((Ny + 1) / 2), dim3(T), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
+  hipLaunchKernelGGL(k_fft_r2c_rows<1>, dim3((Ny + 1) / 2), dim3(Tx), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
                      (const double2*)P.twx, f, H);
   dim3 tb(32, 8);
   hipLaunchKernelGGL(k_transpose_c, dim3((Nh + 31) / 32, (Ny + 31) / 32), tb, 0, st, H, HT, Ny, Nh);
-  hipLaunchKernelGGL(k_fft_cols_solve, dim3(Nh), dim3(T), shy, st, Nx, Ny, P.pow2y, ilog2(Ny), (const double2*)P.twy,
-                     P.ax, P.ay, P.c, HT);
+  hipLaunchKernelGGL(k_fft_cols_solve<1>, dim3(Nh), dim3(Ty), shy, st, Nx, Ny, P.pow2y, ilog2(Ny),
+                     (const double2*)P.twy, P.ax, P.ay, P.c, HT);
   hipLaunchKernelGGL(k_transpose_c, dim3((Ny + 31) / 32, (Nh + 31) / 32), tb, 0, st, HT, H, Nh, Ny);
-  hipLaunchKernelGGL(k_fft_c2r_rows, dim3((Ny + 1) / 2), dim3(T), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
+  hipLaunchKernelGGL(k_fft_c2r_rows<1>, dim3((Ny + 1) / 2), dim3(Tx), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
                      (const double2*)P.twx, H, dp);
 }
 

@@ -289,3 +289,27 @@ def test_ldc_run_ic0_matches_oracle():
     _say('ldc 20x20 run ic0')
     case = load('LDC_unsteady', RK='RK33P2', poisson=4, CG_acc=1e-13, CG_maxit=2000, CG_precond='ic0')
     _compare_runs(case, 30, True)
+
+
+@pytest.mark.parametrize('name', ['ldc', 'bfs'])
+def test_pressure_rhs_and_residual_match_oracle(name):
+    """ins2d_pressure_rhs: f = M Om^-1 F(V, 0) (pressure_additional_solve.m:19-22
+    with p = 0, ydM = 0 for steady boundary conditions) on the resident
+    velocity; ins2d_residual: ||f - A x|| / ||f||."""
+    _say('pressure rhs', name)
+    case = (_wall_case('ldc', 24, 20, sx=1.04, poisson=1) if name == 'ldc'
+            else load('BFS_unsteady', Nx=64, Ny=32, poisson=1))
+    opts = O.build(case)
+    solver, _, V0, p0 = host.setup(case)
+    rng = np.random.default_rng(12)
+    V = V0 + 0.1 * rng.standard_normal(V0.shape)
+    solver.set_state(V, p0)
+    f = solver.pressure_rhs(0.0)
+    g, d = opts.grid, opts.discretization
+    _, R = O.F(V, V, np.zeros(g.Np), 0.0, opts)
+    fo = d.M @ (g.Om_inv * R) + d.ydM
+    assert _rel(f, fo) < 1e-12
+    x = rng.standard_normal(g.Np)
+    ro = np.linalg.norm(f - d.A @ x) / np.linalg.norm(f)
+    assert abs(solver.residual(f, x) - ro) <= 1e-12 * ro
+    solver.close()
