@@ -351,6 +351,46 @@ __global__ void __launch_bounds__(256) k_div_per(PerGrid G, const double* __rest
   f[(size_t)i + r0] = oscale * s;
 }
 
+// fourth-order divergence, row-streaming like k_rhs_per4s: u of row j with
+// its x halo (-1 .. +2) staged in LDS; the v column values of rows j-1 .. j+2
+// of this thread's column carried in registers from row to row
+__global__ void __launch_bounds__(64) k_div_per4s(PerGrid G, const double* __restrict__ A,
+                                                 const double* __restrict__ B, double ca, double cb, double oscale,
+                                                 double* __restrict__ f) {
+  __shared__ double su[kStripW + 3];
+  const int tx = threadIdx.x;
+  const int Nx = G.Nx, Ny = G.Ny;
+  const size_t N = (size_t)Nx * Ny;
+  const int x0 = blockIdx.x * kStripW, j0 = blockIdx.y * kStripH;
+  const int i = x0 + tx;
+  // LDS column c holds x = x0 - 1 + c (periodic); c = tx and, for tx < 3, 64 + tx
+  const int xa = ((x0 - 1 + tx) % Nx + Nx) % Nx;
+  const int xb = tx < 3 ? (x0 + kStripW - 1 + tx) % Nx : 0;
+  const int iv = i < Nx ? i : 0;
+  auto W = [&](size_t k) { return B ? ca * A[k] + cb * B[k] : ca * A[k]; };
+  auto wv = [&](int y) { return W(N + (size_t)iv + (size_t)Nx * wrap(y, Ny)); };
+  double vm1 = wv(j0 - 1), v0 = wv(j0), vp1 = wv(j0 + 1), vp2 = wv(j0 + 2);
+  const int jend = min(j0 + kStripH, Ny);
+  for (int j = j0; j < jend; ++j) {
+    const size_t r = (size_t)Nx * j;
+    su[tx] = W(r + xa);
+    if (tx < 3) su[kStripW + tx] = W(r + xb);
+    const double vnext = (j + 1 < jend) ? wv(j + 3) : 0.0;
+    __syncthreads();
+    if (i < Nx) {
+      const double um1 = su[tx], u0 = su[tx + 1], up1 = su[tx + 2], up2 = su[tx + 3];
+      const double sdiv = 81.0 * G.hy * (up1 - u0) - 3.0 * G.hy * (up2 - um1) + 81.0 * G.hx * (vp1 - v0) -
+                          3.0 * G.hx * (vp2 - vm1);
+      f[(size_t)i + r] = oscale * sdiv;
+    }
+    __syncthreads();
+    vm1 = v0;
+    v0 = vp1;
+    vp1 = vp2;
+    vp2 = vnext;
+  }
+}
+
 template <int ORDER>
 __global__ void __launch_bounds__(256) k_proj_per(PerGrid G, const double* __restrict__ Vn, const double* __restrict__ Vt,
                                                  const double* __restrict__ dp, double dt, double c,
@@ -868,7 +908,8 @@ void launch_div_per(hipStream_t st, int order, int Nx, int Ny, double hx, double
   if (order == 2)
     hipLaunchKernelGGL(k_div_per<2>, grid2(Nx, Ny, b), b, 0, st, G, A, B, ca, cb, oscale, f);
   else
-    hipLaunchKernelGGL(k_div_per<4>, grid2(Nx, Ny, b), b, 0, st, G, A, B, ca, cb, oscale, f);
+    hipLaunchKernelGGL(k_div_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + kStripH - 1) / kStripH), dim3(kStripW),
+                       0, st, G, A, B, ca, cb, oscale, f);
 }
 
 void launch_proj_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* Vn,
