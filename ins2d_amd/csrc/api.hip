@@ -207,16 +207,27 @@ void push_comps(ins2d_handle* h) {
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-// timing with HIP events on the handle's stream
+// timing with HIP events on the handle's stream.  KERNEL scopes wrap one
+// launcher of the stencil kernels: its kernels record their own start and end
+// timestamps into the two events (launch_timed), which measures the kernels
+// alone; other scopes record the events on the stream around the work.
 struct ProfScope {
   ins2d_handle* h;
   int kind;
+  bool kernel;
   hipEvent_t a{}, b{};
-  ProfScope(ins2d_handle* h_, int k) : h(h_), kind(k) {
+  KernelTimer kt;
+  ProfScope(ins2d_handle* h_, int k, bool kernel_ts = false) : h(h_), kind(k), kernel(kernel_ts) {
     if (!h->prof) return;
     a = get_ev();
     b = get_ev();
-    HIPCHK(hipEventRecord(a, h->st));
+    if (kernel) {
+      kt.start = a;
+      kt.stop = b;
+      g_ktimer = &kt;
+    } else {
+      HIPCHK(hipEventRecord(a, h->st));
+    }
   }
   hipEvent_t get_ev() {
     if (!h->ev_free.empty()) {
@@ -230,6 +241,16 @@ struct ProfScope {
   }
   ~ProfScope() {
     if (!h->prof) return;
+    if (kernel) {
+      g_ktimer = nullptr;
+      if (kt.launches > 0) {
+        h->pend.push_back({kind, a, b});
+      } else {
+        h->ev_free.push_back(a);
+        h->ev_free.push_back(b);
+      }
+      return;
+    }
     if (hipEventRecord(b, h->st) == hipSuccess) h->pend.push_back({kind, a, b});
   }
 };
@@ -251,7 +272,7 @@ void prof_collect(ins2d_handle* h) {
 const StageCombine kNoCombine{0, 0.0, {0}, {nullptr}};
 
 void rhs(ins2d_handle* h, const double* V, const double* p, int mode, double* out, const StageCombine& sc, double* vt) {
-  ProfScope ps(h, 0);
+  ProfScope ps(h, 0, true);
   if (h->path) {
     launch_rhs_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, h->invRe, V, V + h->Nu, p, mode, out, sc, vt);
   } else {
@@ -265,7 +286,7 @@ void rhs(ins2d_handle* h, const double* V, const double* p, int mode, double* ou
 // f = oscale * (M (ca A + cb B) + ycoef yM)
 void div(ins2d_handle* h, const double* A, const double* B, double ca, double cb, double ycoef, double oscale,
          double* f) {
-  ProfScope ps(h, 1);
+  ProfScope ps(h, 1, true);
   if (h->path)
     launch_div_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, A, B, ca, cb, oscale, f);
   else
@@ -273,7 +294,7 @@ void div(ins2d_handle* h, const double* A, const double* B, double ca, double cb
 }
 
 void proj(ins2d_handle* h, const double* Vn, const double* Vt, const double* dp, double dt, double c, double* V) {
-  ProfScope ps(h, 3);
+  ProfScope ps(h, 3, true);
   if (h->path)
     launch_proj_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, Vn, Vt, dp, dt, c, V);
   else

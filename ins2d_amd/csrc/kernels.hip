@@ -59,9 +59,13 @@ __global__ void __launch_bounds__(256) k_proj_generic(const DComp* __restrict__ 
 // ===========================================================================
 // periodic uniform stencils (second and fourth order)
 // ===========================================================================
+// rhx = 1 / hx, rhy = 1 / hy, rom = 1 / Omega (hx hy, 72 hx hy at fourth
+// order): the kernels multiply by these instead of dividing (an FP64
+// division costs about ten instructions, one of them quarter rate)
 struct PerGrid {
   int Nx, Ny;
   double hx, hy, invRe;
+  double rhx, rhy, rom;
 };
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -86,14 +90,14 @@ __device__ __forceinline__ void rhs_per2(const PerGrid& G, const double* __restr
   const double du2dx = (hy * Ue) * Ue - (hy * Uw) * Uw;
   const double Vn = 0.5 * hx * (v_wn + vn), Vs = 0.5 * hx * (vw + v0);
   const double duvdy = Vn * (0.5 * (u0 + un)) - Vs * (0.5 * (us + u0));
-  const double d2u = hy * ((ue - u0) / hx - (u0 - uw) / hx) + hx * ((un - u0) / hy - (u0 - us) / hy);
+  const double d2u = hy * ((ue - u0) * G.rhx - (u0 - uw) * G.rhx) + hx * ((un - u0) * G.rhy - (u0 - us) * G.rhy);
   fu = -(du2dx + duvdy) + G.invRe * d2u;
   // v-momentum
   const double Ue2 = 0.5 * hy * (u_se + ue), Uw2 = 0.5 * hy * (us + u0);
   const double duvdx = Ue2 * (0.5 * (v0 + ve)) - Uw2 * (0.5 * (vw + v0));
   const double Vn2 = 0.5 * (v0 + vn), Vs2 = 0.5 * (vs + v0);
   const double dv2dy = (hx * Vn2) * Vn2 - (hx * Vs2) * Vs2;
-  const double d2v = hy * ((ve - v0) / hx - (v0 - vw) / hx) + hx * ((vn - v0) / hy - (v0 - vs) / hy);
+  const double d2v = hy * ((ve - v0) * G.rhx - (v0 - vw) * G.rhx) + hx * ((vn - v0) * G.rhy - (v0 - vs) * G.rhy);
   fv = -(duvdx + dv2dy) + G.invRe * d2v;
 }
 
@@ -132,6 +136,7 @@ template <class ACC>
 __device__ __forceinline__ void rhs4_core(const PerGrid& G, const ACC& acc, double& fu, double& fv) {
   const double hx = G.hx, hy = G.hy;
   const double w1 = 9.0 / 16.0, w2 = -1.0 / 16.0;   // beta/2, (1-beta)/2
+  const double r72x = G.rhx * (1.0 / 72.0), r72y = G.rhy * (1.0 / 72.0);
 #define U(a, b) acc.U(a, b)
 #define V(a, b) acc.V(a, b)
   // ---------------- u-momentum ----------------
@@ -152,8 +157,8 @@ __device__ __forceinline__ void rhs4_core(const PerGrid& G, const ACC& acc, doub
   const double duvdy3 = Vf3(3) * Ay3(3) - Vf3(0) * Ay3(0);
   const double convu = 81.0 * (du2dx1 + duvdy1) - (du2dx3 + duvdy3);
   // diffusion: Phi_{k+1/2} = [81 (u_{k+1}-u_k) - 3 (u_{k+2}-u_{k-1})] / (72 h)
-  auto Px = [&](int k) { return (81.0 * (U(k + 1, 0) - U(k, 0)) - 3.0 * (U(k + 2, 0) - U(k - 1, 0))) / (72.0 * hx); };
-  auto Py = [&](int k) { return (81.0 * (U(0, k + 1) - U(0, k)) - 3.0 * (U(0, k + 2) - U(0, k - 1))) / (72.0 * hy); };
+  auto Px = [&](int k) { return (81.0 * (U(k + 1, 0) - U(k, 0)) - 3.0 * (U(k + 2, 0) - U(k - 1, 0))) * r72x; };
+  auto Py = [&](int k) { return (81.0 * (U(0, k + 1) - U(0, k)) - 3.0 * (U(0, k + 2) - U(0, k - 1))) * r72y; };
   const double d2u = hy * (81.0 * (Px(0) - Px(-1)) - 3.0 * (Px(1) - Px(-2))) +
                      hx * (81.0 * (Py(0) - Py(-1)) - 3.0 * (Py(1) - Py(-2)));
   fu = -convu + G.invRe * d2u;
@@ -171,8 +176,8 @@ __device__ __forceinline__ void rhs4_core(const PerGrid& G, const ACC& acc, doub
   auto By3 = [&](int g) { return 0.5 * (V(0, g - 3) + V(0, g)); };
   const double dv2dy3 = Iy3(3) * By3(3) - Iy3(0) * By3(0);
   const double convv = 81.0 * (duvdx1 + dv2dy1) - (duvdx3 + dv2dy3);
-  auto Qx = [&](int k) { return (81.0 * (V(k + 1, 0) - V(k, 0)) - 3.0 * (V(k + 2, 0) - V(k - 1, 0))) / (72.0 * hx); };
-  auto Qy = [&](int k) { return (81.0 * (V(0, k + 1) - V(0, k)) - 3.0 * (V(0, k + 2) - V(0, k - 1))) / (72.0 * hy); };
+  auto Qx = [&](int k) { return (81.0 * (V(k + 1, 0) - V(k, 0)) - 3.0 * (V(k + 2, 0) - V(k - 1, 0))) * r72x; };
+  auto Qy = [&](int k) { return (81.0 * (V(0, k + 1) - V(0, k)) - 3.0 * (V(0, k + 2) - V(0, k - 1))) * r72y; };
   const double d2v = hy * (81.0 * (Qx(0) - Qx(-1)) - 3.0 * (Qx(1) - Qx(-2))) +
                      hx * (81.0 * (Qy(0) - Qy(-1)) - 3.0 * (Qy(1) - Qy(-2)));
   fv = -convv + G.invRe * d2v;
@@ -228,9 +233,8 @@ __global__ void __launch_bounds__(256) k_rhs_per(PerGrid G, const double* __rest
     fv -= gy;
   }
   if (mode & RHS_SCALE_OM) {
-    const double om = (ORDER == 2 ? 1.0 : 72.0) * G.hx * G.hy;
-    fu /= om;
-    fv /= om;
+    fu *= G.rom;
+    fv *= G.rom;
   }
   const size_t N = (size_t)G.Nx * G.Ny;
   const size_t idx = (size_t)i + (size_t)G.Nx * j;
@@ -286,7 +290,6 @@ __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __res
     store_row(k);
   }
   const size_t N = (size_t)Nx * Ny;
-  const double om = 72.0 * G.hx * G.hy;
   const int jend = min(j0 + kStripH, Ny);
   fetch_row(j0 + 3);
   int s0 = 0;   // ring slot of row j - 3
@@ -311,8 +314,8 @@ __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __res
         fv -= gy;
       }
       if (mode & RHS_SCALE_OM) {
-        fu /= om;
-        fv /= om;
+        fu *= G.rom;
+        fv *= G.rom;
       }
       const size_t idx = (size_t)i + (size_t)Nx * j;
       if (out) { out[idx] = fu; out[N + idx] = fv; }
@@ -402,10 +405,9 @@ __global__ void __launch_bounds__(256) k_proj_per(PerGrid G, const double* __res
   const size_t idx = (size_t)i + (size_t)G.Nx * j;
   double gx, gy;
   grad_per<ORDER>(G, dp, i, j, gx, gy);
-  const double om = (ORDER == 2 ? 1.0 : 72.0) * G.hx * G.hy;
   const double tu = Vt ? Vt[idx] : 0.0, tv = Vt ? Vt[N + idx] : 0.0;
-  V[idx] = Vn[idx] + dt * (tu - c * (gx / om));
-  V[N + idx] = Vn[N + idx] + dt * (tv - c * (gy / om));
+  V[idx] = Vn[idx] + dt * (tu - c * (gx * G.rom));
+  V[N + idx] = Vn[N + idx] + dt * (tv - c * (gy * G.rom));
 }
 
 // ===========================================================================
@@ -866,18 +868,24 @@ __global__ void __launch_bounds__(256) k_reduce_stats(const double* __restrict__
 // ===========================================================================
 static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
 
+thread_local KernelTimer* g_ktimer = nullptr;
+
+static PerGrid per_grid(int order, int Nx, int Ny, double hx, double hy, double invRe) {
+  return PerGrid{Nx, Ny, hx, hy, invRe, 1.0 / hx, 1.0 / hy, 1.0 / ((order == 2 ? 1.0 : 72.0) * hx * hy)};
+}
+
 void launch_rhs_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,
                         const double* u, const double* v, const double* p, int mode, double* out_u, double* out_v,
                         const StageCombine& scu, const StageCombine& scv, double* vt_u, double* vt_v) {
   dim3 b(64, 4);
-  hipLaunchKernelGGL(k_rhs_generic<0>, grid2(nux, nuy, b), b, 0, st, dCu, u, v, p, mode, out_u, scu, vt_u);
-  hipLaunchKernelGGL(k_rhs_generic<1>, grid2(nvx, nvy, b), b, 0, st, dCv, u, v, p, mode, out_v, scv, vt_v);
+  launch_timed(k_rhs_generic<0>, grid2(nux, nuy, b), b, 0, st, dCu, u, v, p, mode, out_u, scu, vt_u);
+  launch_timed(k_rhs_generic<1>, grid2(nvx, nvy, b), b, 0, st, dCv, u, v, p, mode, out_v, scv, vt_v);
 }
 
 void launch_div_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int npx, int npy, const double* A,
                         const double* B, double ca, double cb, double ycoef, double oscale, int nu, double* f) {
   dim3 b(64, 4);
-  hipLaunchKernelGGL(k_div_generic, grid2(npx, npy, b), b, 0, st, dCu, dCv, npx, npy, A, B, ca, cb, ycoef, oscale,
+  launch_timed(k_div_generic, grid2(npx, npy, b), b, 0, st, dCu, dCv, npx, npy, A, B, ca, cb, ycoef, oscale,
                      nu, f);
 }
 
@@ -885,41 +893,41 @@ void launch_proj_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int
                          int nu, const double* Vn, const double* Vt, const double* dp, double dt, double c,
                          double* V) {
   dim3 b(64, 4);
-  hipLaunchKernelGGL(k_proj_generic<0>, grid2(nux, nuy, b), b, 0, st, dCu, Vn, Vt, dp, dt, c, V);
-  hipLaunchKernelGGL(k_proj_generic<1>, grid2(nvx, nvy, b), b, 0, st, dCv, Vn + nu, Vt ? Vt + nu : nullptr, dp, dt,
+  launch_timed(k_proj_generic<0>, grid2(nux, nuy, b), b, 0, st, dCu, Vn, Vt, dp, dt, c, V);
+  launch_timed(k_proj_generic<1>, grid2(nvx, nvy, b), b, 0, st, dCv, Vn + nu, Vt ? Vt + nu : nullptr, dp, dt,
                      c, V + nu);
 }
 
 void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, double invRe, const double* u,
                     const double* v, const double* p, int mode, double* out, const StageCombine& sc, double* vt) {
-  PerGrid G{Nx, Ny, hx, hy, invRe};
+  const PerGrid G = per_grid(order, Nx, Ny, hx, hy, invRe);
   dim3 b(64, 4);
   if (order == 2)
-    hipLaunchKernelGGL(k_rhs_per<2>, grid2(Nx, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
+    launch_timed(k_rhs_per<2>, grid2(Nx, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
   else
-    hipLaunchKernelGGL(k_rhs_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + kStripH - 1) / kStripH), dim3(kStripW), 0,
+    launch_timed(k_rhs_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + kStripH - 1) / kStripH), dim3(kStripW), 0,
                        st, G, u, v, p, mode, out, sc, vt);
 }
 
 void launch_div_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* A, const double* B,
                     double ca, double cb, double oscale, double* f) {
-  PerGrid G{Nx, Ny, hx, hy, 0.0};
+  const PerGrid G = per_grid(order, Nx, Ny, hx, hy, 0.0);
   dim3 b(64, 4);
   if (order == 2)
-    hipLaunchKernelGGL(k_div_per<2>, grid2(Nx, Ny, b), b, 0, st, G, A, B, ca, cb, oscale, f);
+    launch_timed(k_div_per<2>, grid2(Nx, Ny, b), b, 0, st, G, A, B, ca, cb, oscale, f);
   else
-    hipLaunchKernelGGL(k_div_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + kStripH - 1) / kStripH), dim3(kStripW),
+    launch_timed(k_div_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + kStripH - 1) / kStripH), dim3(kStripW),
                        0, st, G, A, B, ca, cb, oscale, f);
 }
 
 void launch_proj_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* Vn,
                      const double* Vt, const double* dp, double dt, double c, double* V) {
-  PerGrid G{Nx, Ny, hx, hy, 0.0};
+  const PerGrid G = per_grid(order, Nx, Ny, hx, hy, 0.0);
   dim3 b(64, 4);
   if (order == 2)
-    hipLaunchKernelGGL(k_proj_per<2>, grid2(Nx, Ny, b), b, 0, st, G, Vn, Vt, dp, dt, c, V);
+    launch_timed(k_proj_per<2>, grid2(Nx, Ny, b), b, 0, st, G, Vn, Vt, dp, dt, c, V);
   else
-    hipLaunchKernelGGL(k_proj_per<4>, grid2(Nx, Ny, b), b, 0, st, G, Vn, Vt, dp, dt, c, V);
+    launch_timed(k_proj_per<4>, grid2(Nx, Ny, b), b, 0, st, G, Vn, Vt, dp, dt, c, V);
 }
 
 static int ilog2(int n) {

@@ -1,6 +1,7 @@
 // Host launch wrappers of the kernels in kernels.hip.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,6 +14,28 @@ namespace ins2d {
 
 // right-hand-side modes
 enum : int { RHS_WITH_P = 1, RHS_SCALE_OM = 2 };
+
+// Kernel timestamps for profiling: while g_ktimer is set (on the calling
+// thread), the kernels of a launcher go out through launch_timed, which
+// records the first kernel's own start timestamp into `start` and each
+// kernel's end timestamp into `stop` (hipExtLaunchKernelGGL), so that the
+// elapsed time between the two events is the launcher's kernel time without
+// the cost of separate event commands on the stream.
+struct KernelTimer {
+  hipEvent_t start = nullptr, stop = nullptr;
+  int launches = 0;
+};
+extern thread_local KernelTimer* g_ktimer;
+
+template <typename F, typename... Args>
+inline void launch_timed(F kernel, dim3 grid, dim3 block, unsigned shmem, hipStream_t st, Args... args) {
+  if (KernelTimer* t = g_ktimer) {
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, st, t->launches == 0 ? t->start : nullptr, t->stop, 0u, args...);
+    ++t->launches;
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, shmem, st, args...);
+  }
+}
 
 struct CGStateH {
   double* s;
