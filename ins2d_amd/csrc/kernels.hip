@@ -247,20 +247,20 @@ __global__ void __launch_bounds__(256) k_rhs_per(PerGrid G, const double* __rest
   }
 }
 
-// fourth order, row-streaming: one wave per 64-wide column strip of kStripH
+// fourth order, row-streaming: one wave per 64-wide column strip of H
 // rows; the rows j-3 .. j+3 of u and v live in an LDS ring of 7 rows (+-3
 // halo in x), so every value is read from memory once per strip (plus the
 // 6-row start-up and the 6-column halo)
-constexpr int kStripW = 64, kStripH = 64;
+constexpr int kStripW = 64;
 __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __restrict__ u,
                                                  const double* __restrict__ v, const double* __restrict__ p, int mode,
                                                  double* __restrict__ out, StageCombine sc,
-                                                 double* __restrict__ vtemp) {
+                                                 double* __restrict__ vtemp, int H) {
   __shared__ double su[7][kStripW + 6];
   __shared__ double sv[7][kStripW + 6];
   const int tx = threadIdx.x;
   const int Nx = G.Nx, Ny = G.Ny;
-  const int x0 = blockIdx.x * kStripW, j0 = blockIdx.y * kStripH;
+  const int x0 = blockIdx.x * kStripW, j0 = blockIdx.y * H;
   const int i = x0 + tx;
   // LDS column c holds x = x0 - 3 + c (periodic)
   const int xa = ((x0 - 3 + tx) % Nx + Nx) % Nx;
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __res
     store_row(k);
   }
   const size_t N = (size_t)Nx * Ny;
-  const int jend = min(j0 + kStripH, Ny);
+  const int jend = min(j0 + H, Ny);
   fetch_row(j0 + 3);
   int s0 = 0;   // ring slot of row j - 3
   for (int j = j0; j < jend; ++j) {
@@ -354,43 +354,63 @@ __global__ void __launch_bounds__(256) k_div_per(PerGrid G, const double* __rest
   f[(size_t)i + r0] = oscale * s;
 }
 
-// fourth-order divergence, row-streaming like k_rhs_per4s: u of row j with
-// its x halo (-1 .. +2) staged in LDS; the v column values of rows j-1 .. j+2
-// of this thread's column carried in registers from row to row
+// fourth-order divergence, row-streaming like k_rhs_per4s (strips of H
+// rows): u of row j with its x halo (-1 .. +2) staged in LDS (two row
+// buffers, so one barrier per row); the v column values of rows j-1 .. j+2 of
+// this thread's column carried in registers from row to row.  The raw A and B
+// values of the next row are loaded one row ahead and combined when stored.
 __global__ void __launch_bounds__(64) k_div_per4s(PerGrid G, const double* __restrict__ A,
                                                  const double* __restrict__ B, double ca, double cb, double oscale,
-                                                 double* __restrict__ f) {
-  __shared__ double su[kStripW + 3];
+                                                 int H, double* __restrict__ f) {
+  __shared__ double su[2][kStripW + 3];
   const int tx = threadIdx.x;
   const int Nx = G.Nx, Ny = G.Ny;
   const size_t N = (size_t)Nx * Ny;
-  const int x0 = blockIdx.x * kStripW, j0 = blockIdx.y * kStripH;
+  const int x0 = blockIdx.x * kStripW, j0 = blockIdx.y * H;
   const int i = x0 + tx;
   // LDS column c holds x = x0 - 1 + c (periodic); c = tx and, for tx < 3, 64 + tx
   const int xa = ((x0 - 1 + tx) % Nx + Nx) % Nx;
-  const int xb = tx < 3 ? (x0 + kStripW - 1 + tx) % Nx : 0;
+  const int xb = tx < 3 ? (x0 + kStripW - 1 + tx) % Nx : xa;
   const int iv = i < Nx ? i : 0;
-  auto W = [&](size_t k) { return B ? ca * A[k] + cb * B[k] : ca * A[k]; };
-  auto wv = [&](int y) { return W(N + (size_t)iv + (size_t)Nx * wrap(y, Ny)); };
-  double vm1 = wv(j0 - 1), v0 = wv(j0), vp1 = wv(j0 + 1), vp2 = wv(j0 + 2);
-  const int jend = min(j0 + kStripH, Ny);
+  const bool hasB = B != nullptr;
+  auto vrow = [&](int y) { return N + (size_t)iv + (size_t)Nx * wrap(y, Ny); };
+  auto W = [&](size_t k) { return hasB ? ca * A[k] + cb * B[k] : ca * A[k]; };
+  double vm1 = W(vrow(j0 - 1)), v0 = W(vrow(j0)), vp1 = W(vrow(j0 + 1)), vp2 = W(vrow(j0 + 2));
+  size_t r = (size_t)Nx * j0;
+  size_t kv = vrow(j0 + 3);
+  double ua = A[r + xa], uc = A[r + xb], va = A[kv];
+  double ub = hasB ? B[r + xa] : 0.0, ud = hasB ? B[r + xb] : 0.0, vb = hasB ? B[kv] : 0.0;
+  const int jend = min(j0 + H, Ny);
+  int s = 0;
   for (int j = j0; j < jend; ++j) {
-    const size_t r = (size_t)Nx * j;
-    su[tx] = W(r + xa);
-    if (tx < 3) su[kStripW + tx] = W(r + xb);
-    const double vnext = (j + 1 < jend) ? wv(j + 3) : 0.0;
+    su[s][tx] = hasB ? ca * ua + cb * ub : ca * ua;
+    if (tx < 3) su[s][kStripW + tx] = hasB ? ca * uc + cb * ud : ca * uc;
+    const double vnext = hasB ? ca * va + cb * vb : ca * va;
+    if (j + 1 < jend) {
+      const size_t rn = r + Nx;
+      kv = vrow(j + 4);
+      ua = A[rn + xa];
+      uc = A[rn + xb];
+      va = A[kv];
+      if (hasB) {
+        ub = B[rn + xa];
+        ud = B[rn + xb];
+        vb = B[kv];
+      }
+    }
     __syncthreads();
     if (i < Nx) {
-      const double um1 = su[tx], u0 = su[tx + 1], up1 = su[tx + 2], up2 = su[tx + 3];
+      const double um1 = su[s][tx], u0 = su[s][tx + 1], up1 = su[s][tx + 2], up2 = su[s][tx + 3];
       const double sdiv = 81.0 * G.hy * (up1 - u0) - 3.0 * G.hy * (up2 - um1) + 81.0 * G.hx * (vp1 - v0) -
                           3.0 * G.hx * (vp2 - vm1);
       f[(size_t)i + r] = oscale * sdiv;
     }
-    __syncthreads();
     vm1 = v0;
     v0 = vp1;
     vp1 = vp2;
     vp2 = vnext;
+    r += Nx;
+    s ^= 1;
   }
 }
 
@@ -528,6 +548,261 @@ __device__ __forceinline__ void lds_transform(double2* z, double2* scratch, int 
                                               const double2* __restrict__ tw) {
   if (pow2) lds_fft<INV, C>(z, N, log2N, tw);
   else lds_dft<INV>(z, scratch, N, tw);
+}
+
+// ---------------------------------------------------------------------------
+// Power-of-two FFTs (16 <= N <= 4096) with register butterflies.  A transform
+// runs on T = max(64, N / 16) threads; each Stockham pass of radix R (16, and
+// one pass of 2, 4 or 8 for the rest of log2 N) gives every thread 16 / R
+// butterflies held in registers.  The first pass takes its inputs straight
+// from the caller (global memory or registers) and the last pass hands its
+// outputs straight back, so only the passes in between go through LDS.  The
+// LDS index is swizzled, n ^ ((n >> 4) & 15), which keeps the stride-R writes
+// of the early passes and the unit-stride reads free of bank conflicts.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int swz(int n) { return n ^ ((n >> 4) & 15); }
+
+// x * exp(-+2 pi i m / 16) for a compile-time m in 0 .. 9 (sign + for INV)
+template <bool INV>
+__device__ __forceinline__ double2 mul_w16(double2 x, int m) {
+  constexpr double c1 = 0.92387953251128675613;   // cos(pi / 8)
+  constexpr double s1 = 0.38268343236508977173;   // sin(pi / 8)
+  constexpr double r2 = 0.70710678118654752440;   // cos(pi / 4)
+  double wr, wi;
+  switch (m) {
+    case 0: return x;
+    case 4: return INV ? make_double2(-x.y, x.x) : make_double2(x.y, -x.x);
+    case 8: return make_double2(-x.x, -x.y);
+    case 1: wr = c1; wi = -s1; break;
+    case 2: wr = r2; wi = -r2; break;
+    case 3: wr = s1; wi = -c1; break;
+    case 5: wr = -s1; wi = -c1; break;
+    case 6: wr = -r2; wi = -r2; break;
+    case 7: wr = -c1; wi = -s1; break;
+    default: wr = -c1; wi = s1; break;   // 9
+  }
+  if (INV) wi = -wi;
+  return make_double2(x.x * wr - x.y * wi, x.x * wi + x.y * wr);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(double2& a0, double2& a1, double2& a2, double2& a3) {
+  const double2 s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = csub(a1, a3);
+  const double2 jd13 = INV ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);   // -+ i d13
+  a0 = cadd(s02, s13);
+  a1 = cadd(d02, jd13);
+  a2 = csub(s02, s13);
+  a3 = csub(d02, jd13);
+}
+
+// R-point DFT of x[0 .. R) in place, natural order in and out
+template <int R, bool INV>
+__device__ __forceinline__ void dft_reg(double2* x) {
+  if (R == 2) {
+    const double2 a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+  } else if (R == 4) {
+    dft4<INV>(x[0], x[1], x[2], x[3]);
+  } else if (R == 8) {
+    // n = 2 n1 + n2, k = k1 + 4 k2
+    double2 e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6], o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    o1 = mul_w16<INV>(o1, 2);
+    o2 = mul_w16<INV>(o2, 4);
+    o3 = mul_w16<INV>(o3, 6);
+    x[0] = cadd(e0, o0); x[4] = csub(e0, o0);
+    x[1] = cadd(e1, o1); x[5] = csub(e1, o1);
+    x[2] = cadd(e2, o2); x[6] = csub(e2, o2);
+    x[3] = cadd(e3, o3); x[7] = csub(e3, o3);
+  } else {
+    // n = 4 n1 + n2, k = k1 + 4 k2: DFT4 over n1, twiddle W16^(n2 k1), DFT4 over n2
+    double2 y[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      double2 a0 = x[n2], a1 = x[4 + n2], a2 = x[8 + n2], a3 = x[12 + n2];
+      dft4<INV>(a0, a1, a2, a3);
+      y[4 * n2 + 0] = a0;
+      y[4 * n2 + 1] = mul_w16<INV>(a1, n2);
+      y[4 * n2 + 2] = mul_w16<INV>(a2, 2 * n2);
+      y[4 * n2 + 3] = mul_w16<INV>(a3, 3 * n2);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      double2 b0 = y[k1], b1 = y[4 + k1], b2 = y[8 + k1], b3 = y[12 + k1];
+      dft4<INV>(b0, b1, b2, b3);
+      x[k1] = b0;
+      x[k1 + 4] = b1;
+      x[k1 + 8] = b2;
+      x[k1 + 12] = b3;
+    }
+  }
+}
+
+// One radix-R Stockham pass of an N-point transform (Ns: product of the
+// earlier radices).  Thread tid owns the butterflies jq = tid + c T
+// (c < 16 / R, jq < N / R); ld(s, n) gives input element n for register slot
+// s = c R + m, st(s, n, x) takes output element n.  TO_LDS: st writes LDS, so
+// the pass waits for every thread's loads before storing and after.
+template <int E, int R, bool INV, bool TO_LDS, class LD, class ST>
+__device__ __forceinline__ void fft_pass(int N, int Ns, const double2* __restrict__ tw, LD& ld, ST& st) {
+  const int T = blockDim.x, tid = threadIdx.x;
+  const int Q = N / R, step = N / (Ns * R);
+  double2 a[E];
+#pragma unroll
+  for (int c = 0; c < E / R; ++c) {
+    const int jq = tid + c * T;
+    if (jq < Q) {
+#pragma unroll
+      for (int m = 0; m < R; ++m) a[c * R + m] = ld(c * R + m, jq + m * Q);
+      if (Ns > 1) {
+        // twiddles W^(m ks), m < R: W^ks, W^2ks, W^4ks, W^8ks from the
+        // table, the others as products of these (at most three deep)
+        const int ks = (jq & (Ns - 1)) * step;
+        double2 t[R];
+#pragma unroll
+        for (int b = 1; b < R; b *= 2) {
+          t[b] = tw[b * ks];
+          if (INV) t[b] = conjd(t[b]);
+        }
+#pragma unroll
+        for (int m = 3; m < R; ++m)
+          if (m & (m - 1)) t[m] = cmul(t[m & (m - 1)], t[m & -m]);
+#pragma unroll
+        for (int m = 1; m < R; ++m) a[c * R + m] = cmul(a[c * R + m], t[m]);
+      }
+      dft_reg<R, INV>(a + c * R);
+    }
+  }
+  if (TO_LDS) __syncthreads();
+#pragma unroll
+  for (int c = 0; c < E / R; ++c) {
+    const int jq = tid + c * T;
+    if (jq < Q) {
+      const int k = jq & (Ns - 1);
+      const int o = (jq - k) * R + k;   // (jq / Ns) Ns R + k
+#pragma unroll
+      for (int m = 0; m < R; ++m) st(c * R + m, o + m * Ns, a[c * R + m]);
+    }
+  }
+  if (TO_LDS) __syncthreads();
+}
+
+template <int E, bool INV, bool TO_LDS, class LD, class ST>
+__device__ __forceinline__ void fft_pass_r(int Rr, int N, int Ns, const double2* __restrict__ tw, LD& ld, ST& st) {
+  if (Rr == 2) fft_pass<E, 2, INV, TO_LDS>(N, Ns, tw, ld, st);
+  else if (Rr == 4) fft_pass<E, 4, INV, TO_LDS>(N, Ns, tw, ld, st);
+  if constexpr (E >= 8)
+    if (Rr == 8) fft_pass<E, 8, INV, TO_LDS>(N, Ns, tw, ld, st);
+  if constexpr (E >= 16)
+    if (Rr == 16) fft_pass<E, 16, INV, TO_LDS>(N, Ns, tw, ld, st);
+}
+
+// N-point transform, N = 2^log2N in [E, 4096], z: N-element LDS work array,
+// max(64, N / E) threads.  Pass radices: the remainder 2^(log2N mod log2 E)
+// first, then radix E (forward); reversed for INV, so that the forward
+// transform's last pass and the inverse transform's first pass use the same
+// register slots for the same elements.  OUT_LDS: the caller's st writes LDS.
+template <int E, bool INV, bool OUT_LDS, class LD, class ST>
+__device__ __forceinline__ void fft_reg(double2* z, int N, int log2N, const double2* __restrict__ tw, LD& ld,
+                                        ST& st) {
+  auto ldz = [&](int, int n) { return z[swz(n)]; };
+  auto stz = [&](int, int n, double2 x) { z[swz(n)] = x; };
+  constexpr int lgE = E == 16 ? 4 : (E == 8 ? 3 : 2);
+  const int rem = log2N % lgE;
+  const int P = log2N / lgE + (rem ? 1 : 0);
+  int Ns = 1;
+  for (int p = 0; p < P; ++p) {
+    const bool remp = rem && (INV ? p == P - 1 : p == 0);
+    const int R = remp ? (1 << rem) : E;
+    const bool first = p == 0, last = p == P - 1;
+    if (first && last) fft_pass_r<E, INV, OUT_LDS>(R, N, Ns, tw, ld, st);
+    else if (first) fft_pass_r<E, INV, true>(R, N, Ns, tw, ld, stz);
+    else if (last) fft_pass_r<E, INV, OUT_LDS>(R, N, Ns, tw, ldz, st);
+    else fft_pass_r<E, INV, true>(R, N, Ns, tw, ldz, stz);
+    Ns *= R;
+  }
+}
+
+template <int E>
+constexpr int fft_block(int) { return E == 16 ? 256 : (E == 8 ? 512 : 1024); }
+
+// rows y0 = 2b and y1 = 2b+1 packed as one complex row, half spectra out
+// (power-of-two Nx in [E, 4096], max(64, Nx / E) threads)
+template <int E>
+__global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_r2c_rows(int Nx, int Ny, int Nh, int log2N,
+                                                       const double2* __restrict__ tw, const double* __restrict__ f,
+                                                       double2* __restrict__ H) {
+  extern __shared__ double2 z[];
+  const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
+  const bool has1 = y1 < Ny;
+  const double* f0 = f + (size_t)y0 * Nx;
+  const double* f1 = f + (size_t)(has1 ? y1 : y0) * Nx;
+  auto ld = [&](int, int n) { return make_double2(f0[n], has1 ? f1[n] : 0.0); };
+  auto st = [&](int, int n, double2 x) { z[swz(n)] = x; };
+  fft_reg<E, false, true>(z, Nx, log2N, tw, ld, st);
+  for (int k = threadIdx.x; k < Nh; k += blockDim.x) {
+    const double2 a = z[swz(k)], b = conjd(z[swz((Nx - k) & (Nx - 1))]);
+    H[(size_t)y0 * Nh + k] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+    // (a - b) / (2i) = -i/2 (a - b)
+    if (has1) H[(size_t)y1 * Nh + k] = make_double2(0.5 * (a.y - b.y), -0.5 * (a.x - b.x));
+  }
+}
+
+// inverse of k_fftr_r2c_rows: the two half spectra as one complex row
+// a + i b, transformed and split into the real rows y0 and y1
+template <int E>
+__global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_c2r_rows(int Nx, int Ny, int Nh, int log2N,
+                                                       const double2* __restrict__ tw, const double2* __restrict__ H,
+                                                       double* __restrict__ out) {
+  extern __shared__ double2 z[];
+  const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
+  const bool has1 = y1 < Ny;
+  const double2* H0 = H + (size_t)y0 * Nh;
+  const double2* H1 = H + (size_t)(has1 ? y1 : y0) * Nh;
+  double* o0 = out + (size_t)y0 * Nx;
+  double* o1 = out + (size_t)(has1 ? y1 : y0) * Nx;
+  // the full row a + i b, staged in LDS (the half spectra read once, coalesced)
+  for (int k = threadIdx.x; k < Nh; k += blockDim.x) {
+    double2 a = H0[k], b = has1 ? H1[k] : make_double2(0.0, 0.0);
+    if (k == 0 || 2 * k == Nx) { a.y = 0.0; b.y = 0.0; }
+    z[swz(k)] = make_double2(a.x - b.y, a.y + b.x);
+    if (k > 0 && 2 * k < Nx) z[swz(Nx - k)] = make_double2(a.x + b.y, b.x - a.y);   // conj(a) + i conj(b)
+  }
+  __syncthreads();
+  auto ld = [&](int, int n) { return z[swz(n)]; };
+  auto st = [&](int, int n, double2 x) {
+    o0[n] = x.x;
+    if (has1) o1[n] = x.y;
+  };
+  fft_reg<E, true, false>(z, Nx, log2N, tw, ld, st);
+}
+
+// one block per kx: forward FFT along y, division by -Ahat in registers,
+// inverse FFT along y (power-of-two Ny in [E, 4096], max(64, Ny / E) threads)
+template <int E>
+__global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_cols_solve(int Nx, int Ny, int log2N, const double2* __restrict__ tw,
+                                                         const double* __restrict__ ax, const double* __restrict__ ay,
+                                                         double c, double2* __restrict__ HT) {
+  extern __shared__ double2 z[];
+  const int kx = blockIdx.x;
+  double2* row = HT + (size_t)kx * Ny;
+  const double scale = 1.0 / ((double)Nx * (double)Ny);
+  const double axk = ax[kx];
+  double2 v[E];
+  auto ldg = [&](int, int n) { return row[n]; };
+  auto stv = [&](int s, int ky, double2 x) {
+    double Ah = c * (axk + ay[ky]);
+    if (kx == 0 && ky == 0) Ah = 1.0;
+    const double sc = -scale / Ah;
+    v[s] = make_double2(x.x * sc, x.y * sc);
+  };
+  fft_reg<E, false, false>(z, Ny, log2N, tw, ldg, stv);
+  __syncthreads();
+  auto ldv = [&](int s, int) { return v[s]; };
+  auto stg = [&](int, int n, double2 x) { row[n] = x; };
+  fft_reg<E, true, false>(z, Ny, log2N, tw, ldv, stg);
 }
 
 // rows y0 = 2b and y1 = 2b+1 packed as one complex row; half spectra out
@@ -870,6 +1145,15 @@ static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / 
 
 thread_local KernelTimer* g_ktimer = nullptr;
 
+// rows per strip of the row-streaming kernels: 64, halved (down to 8) while
+// the grid would give fewer than 4096 strips (one wave each, 16 per CU)
+static int strip_height(int Nx, int Ny) {
+  const long sx = (Nx + kStripW - 1) / kStripW;
+  int H = 64;
+  while (H > 8 && sx * ((Ny + H - 1) / H) < 4096) H /= 2;
+  return H;
+}
+
 static PerGrid per_grid(int order, int Nx, int Ny, double hx, double hy, double invRe) {
   return PerGrid{Nx, Ny, hx, hy, invRe, 1.0 / hx, 1.0 / hy, 1.0 / ((order == 2 ? 1.0 : 72.0) * hx * hy)};
 }
@@ -901,23 +1185,25 @@ void launch_proj_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int
 void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, double invRe, const double* u,
                     const double* v, const double* p, int mode, double* out, const StageCombine& sc, double* vt) {
   const PerGrid G = per_grid(order, Nx, Ny, hx, hy, invRe);
+  const int H = strip_height(Nx, Ny);
   dim3 b(64, 4);
   if (order == 2)
     launch_timed(k_rhs_per<2>, grid2(Nx, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
   else
-    launch_timed(k_rhs_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + kStripH - 1) / kStripH), dim3(kStripW), 0,
-                       st, G, u, v, p, mode, out, sc, vt);
+    launch_timed(k_rhs_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + H - 1) / H), dim3(kStripW), 0, st, G, u, v, p,
+                 mode, out, sc, vt, H);
 }
 
 void launch_div_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* A, const double* B,
                     double ca, double cb, double oscale, double* f) {
   const PerGrid G = per_grid(order, Nx, Ny, hx, hy, 0.0);
+  const int H = strip_height(Nx, Ny);
   dim3 b(64, 4);
   if (order == 2)
     launch_timed(k_div_per<2>, grid2(Nx, Ny, b), b, 0, st, G, A, B, ca, cb, oscale, f);
   else
-    launch_timed(k_div_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + kStripH - 1) / kStripH), dim3(kStripW),
-                       0, st, G, A, B, ca, cb, oscale, f);
+    launch_timed(k_div_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + H - 1) / H), dim3(kStripW), 0, st, G, A, B,
+                 ca, cb, oscale, H, f);
 }
 
 void launch_proj_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* Vn,
@@ -939,23 +1225,71 @@ static int ilog2(int n) {
 // threads per transform: one radix-4 butterfly per thread and pass for the
 // power-of-two lengths (N / 4 threads, at least 64); 256 for the direct DFT
 static int fft_threads(int N, bool pow2) { return pow2 ? std::max(64, std::min(1024, N / 4)) : 256; }
+// register-butterfly transforms: power-of-two lengths 16 .. 4096.  Elements
+// per thread E (the radix of the main passes) by length and kernel, from
+// timings on MI355X: more elements per thread mean fewer passes through LDS
+// but more registers and fewer waves; at 1024 (half-spectrum 513 x 1024) the
+// wave count decides, and the column solve at 4096 is bound by its 64 KB LDS
+// per workgroup, where 1024 light threads hide latency best.
+//   rows (r2c, c2r):  E = 16 at 4096, 4 at 1024, 8 otherwise
+//   column solve:     E = 4 at 1024 and 4096, 8 otherwise
+static bool fftr_ok(int N, bool pow2) { return pow2 && N >= 16 && N <= 4096; }
+static int fft_elems(int N, bool cols) {
+  if (N == 1024) return 4;
+  if (N == 4096) return cols ? 4 : 16;
+  return 8;
+}
+
+template <int E>
+static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, double* dp, int which) {
+  const int Nx = P.Nx, Ny = P.Ny, Nh = P.Nh;
+  double2* H = (double2*)P.H;
+  double2* HT = (double2*)P.HT;
+  if (which == 0)
+    hipLaunchKernelGGL(k_fftr_r2c_rows<E>, dim3((Ny + 1) / 2), dim3(std::max(64, Nx / E)),
+                       (size_t)Nx * sizeof(double2), st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx, f, H);
+  else if (which == 1)
+    hipLaunchKernelGGL(k_fftr_cols_solve<E>, dim3(Nh), dim3(std::max(64, Ny / E)), (size_t)Ny * sizeof(double2), st,
+                       Nx, Ny, ilog2(Ny), (const double2*)P.twy, P.ax, P.ay, P.c, HT);
+  else
+    hipLaunchKernelGGL(k_fftr_c2r_rows<E>, dim3((Ny + 1) / 2), dim3(std::max(64, Nx / E)),
+                       (size_t)Nx * sizeof(double2), st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx, H, dp);
+}
+
+// which: 0 row transforms, 1 column solve, 2 inverse row transforms
+static void fftr(hipStream_t st, const FFTPlan& P, const double* f, double* dp, int which) {
+  const int E = which == 1 ? fft_elems(P.Ny, true) : fft_elems(P.Nx, false);
+  if (E == 4) fftr_launch<4>(st, P, f, dp, which);
+  else if (E == 8) fftr_launch<8>(st, P, f, dp, which);
+  else fftr_launch<16>(st, P, f, dp, which);
+}
 
 void launch_fft_poisson(hipStream_t st, const FFTPlan& P, const double* f, double* dp) {
   const int Nx = P.Nx, Ny = P.Ny, Nh = P.Nh;
   const int Tx = fft_threads(Nx, P.pow2x), Ty = fft_threads(Ny, P.pow2y);
+  const bool fx = fftr_ok(Nx, P.pow2x), fy = fftr_ok(Ny, P.pow2y);
   size_t shx = (size_t)(P.pow2x ? Nx : 2 * Nx) * sizeof(double2);
   size_t shy = (size_t)(P.pow2y ? Ny : 2 * Ny) * sizeof(double2);
   double2* H = (double2*)P.H;
   double2* HT = (double2*)P.HT;
-  hipLaunchKernelGGL(k_fft_r2c_rows<1>, dim3((Ny + 1) / 2), dim3(Tx), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
-                     (const double2*)P.twx, f, H);
+  if (fx)
+    fftr(st, P, f, dp, 0);
+  else
+    hipLaunchKernelGGL(k_fft_r2c_rows<1>, dim3((Ny + 1) / 2), dim3(Tx), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
+                       (const double2*)P.twx, f, H);
   dim3 tb(32, 8);
   hipLaunchKernelGGL(k_transpose_c, dim3((Nh + 31) / 32, (Ny + 31) / 32), tb, 0, st, H, HT, Ny, Nh);
-  hipLaunchKernelGGL(k_fft_cols_solve<1>, dim3(Nh), dim3(Ty), shy, st, Nx, Ny, P.pow2y, ilog2(Ny),
-                     (const double2*)P.twy, P.ax, P.ay, P.c, HT);
+  if (fy)
+    fftr(st, P, f, dp, 1);
+  else
+    hipLaunchKernelGGL(k_fft_cols_solve<1>, dim3(Nh), dim3(Ty), shy, st, Nx, Ny, P.pow2y, ilog2(Ny),
+                       (const double2*)P.twy, P.ax, P.ay, P.c, HT);
   hipLaunchKernelGGL(k_transpose_c, dim3((Ny + 31) / 32, (Nh + 31) / 32), tb, 0, st, HT, H, Nh, Ny);
-  hipLaunchKernelGGL(k_fft_c2r_rows<1>, dim3((Ny + 1) / 2), dim3(Tx), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
-                     (const double2*)P.twx, H, dp);
+  if (fx)
+    fftr(st, P, f, dp, 2);
+  else
+    hipLaunchKernelGGL(k_fft_c2r_rows<1>, dim3((Ny + 1) / 2), dim3(Tx), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
+                       (const double2*)P.twx, H, dp);
 }
 
 void launch_lap(hipStream_t st, const DLap& L, const double* p, double* q) {

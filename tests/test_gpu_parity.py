@@ -67,6 +67,30 @@ def test_rhs_matches_oracle(name, kernels):
     solver.close()
 
 
+@pytest.mark.parametrize('Nx,Ny', [(1024, 1024), (4096, 128), (128, 4096), (2048, 32), (16, 16), (512, 256)])
+def test_fft_solve_register_transform_lengths(Nx, Ny):
+    """The power-of-two FFT kernels pick 4, 8 or 16 elements per thread by
+    length and kernel (row transforms: 16 at 4096, 4 at 1024, else 8; column
+    solve: 4 at 1024 and 4096, else 8); each choice and each remainder pass
+    (log2 N mod 2, 3, 4) against the oracle's numpy FFT solve.  The solve's
+    condition number grows like N^2, so the large grids are compared at
+    1e-11 (max norm, relative) and the residual at 1e-10 max|f|."""
+    _say('fft lengths', Nx, Ny)
+    case = load('taylor_green', Nx=Nx, Ny=Ny, Re=500.0, noise=1e-2, seed=3)
+    opts = O.build(case)
+    solver, _, _, _ = host.setup(case)
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(opts.grid.Np)
+    f -= f.mean()
+    dp, _, _ = solver.poisson(f)
+    dpo = O.pressure_poisson(f, 0.0, opts)
+    r = opts.discretization.A @ dp - f
+    _say('fft lengths', Nx, Ny, 'rel', _rel(dp, dpo), 'residual', np.abs(r).max() / np.abs(f).max())
+    assert _rel(dp, dpo) < (1e-12 if Nx * Ny <= 1 << 17 else 1e-11)
+    assert np.abs(r).max() < 1e-10 * np.abs(f).max()
+    solver.close()
+
+
 @pytest.mark.parametrize('name', ['fft2', 'fft4_ref', 'fft4_exact', 'fft_nonpow2', 'cg_ldc', 'cg_outflow', 'cg_large',
                                   'direct_ldc', 'direct_outflow', 'direct_channel', 'direct_bfs'])
 def test_poisson_solves_assembled_system(name):
