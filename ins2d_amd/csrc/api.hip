@@ -137,6 +137,7 @@ DOp1 up_op(ins2d_handle* h, const HOp1& o) {
   d.nrows = o.nrows;
   d.ncols = o.ncols;
   d.K = o.K;
+  d.kmax = o.kmax;
   d.col = dupload(h, o.col);
   d.val = dupload(h, o.val);
   return d;

@@ -505,7 +505,7 @@ struct Arena {
 
 DOp1 a_op(Arena& A, const HOp1& o) {
   DOp1 d{};
-  d.nrows = o.nrows; d.ncols = o.ncols; d.K = o.K;
+  d.nrows = o.nrows; d.ncols = o.ncols; d.K = o.K; d.kmax = o.kmax;
   d.col = A.push(o.col);
   d.val = A.push(o.val);
   return d;

@@ -9,8 +9,10 @@
 namespace ins2d {
 
 // 1-D operator, ELL layout: row r has K (col, val) pairs
+// K: entries stored per row (the row stride); kmax: the most nonzero entries
+// of any row (K >= kmax, the rest zero-padded)
 struct DOp1 {
-  int nrows, ncols, K, pad;
+  int nrows, ncols, K, kmax;
   const int* col;
   const double* val;
 };

@@ -241,11 +241,12 @@ Closure bc_general_stag(int Nt, int Nin, int Nb, int bc1, int bc2, double h1, do
   return make_closure(Nt, Nin, Nb, off, Bb, Bbc, y1, y2);
 }
 
-HOp1 to_ell(const Sp& A) {
+HOp1 to_ell(const Sp& A, int min_k) {
   HOp1 o;
   o.nrows = A.m;
   o.ncols = A.n;
-  o.K = std::max(1, A.max_row_nnz());
+  o.kmax = std::max(1, A.max_row_nnz());
+  o.K = std::max(min_k, o.kmax);
   o.col.assign((size_t)o.nrows * o.K, 0);
   o.val.assign((size_t)o.nrows * o.K, 0.0);
   for (int i = 0; i < A.m; ++i) {
@@ -261,11 +262,33 @@ HOp1 to_ell(const Sp& A) {
   return o;
 }
 
-static HKron kron_of(const Sp& Y, const Sp& X) {
+// min_k 2 for the operators of the stencil kernels, which read exactly two
+// (zero-padded) entries of every factor
+static HKron kron_of(const Sp& Y, const Sp& X, int min_k = 2) {
   HKron K;
-  K.X = to_ell(X);
-  K.Y = to_ell(Y);
+  K.X = to_ell(X, min_k);
+  K.Y = to_ell(Y, min_k);
   return K;
+}
+
+// the momentum, divergence and gradient factors of a component must have
+// exactly two entries per row (the form the generic stencil kernels read)
+static void require_two_per_row(const HComp& C, const char* name) {
+  auto chk = [&](const HKron& k) {
+    if (k.X.K != 2 || k.Y.K != 2)
+      throw std::runtime_error(std::string("operator of ") + name + " with more than two entries per row");
+  };
+  for (auto& t : C.conv) {
+    chk(t.D);
+    chk(t.I);
+    chk(t.A);
+  }
+  for (auto& t : C.diff) {
+    chk(t.D[0]);
+    chk(t.S[0]);
+  }
+  chk(C.G);
+  chk(C.M);
 }
 
 static std::vector<double> vmul(const std::vector<double>& a, const std::vector<double>& b) {
@@ -643,11 +666,13 @@ void build_plan(const CaseDesc& cd, Plan& P) {
   std::vector<double> mhy(Ny), mhx(Nx);
   for (int j = 0; j < Ny; ++j) mhy[j] = -hy[j];
   for (int i = 0; i < Nx; ++i) mhx[i] = -hx[i];
-  P.L1 = kron_of(diag(mhy), Lx);
-  P.L2 = kron_of(Ly, diag(mhx));
+  P.L1 = kron_of(diag(mhy), Lx, 1);
+  P.L2 = kron_of(Ly, diag(mhx), 1);
   P.Adiag.resize(P.Np);
   for (int j = 0; j < Ny; ++j)
     for (int i = 0; i < Nx; ++i) P.Adiag[i + (size_t)Nx * j] = -(hy[j] * Lx.get(i, i) + hx[i] * Ly.get(j, j));
+  require_two_per_row(P.cu, "u");
+  require_two_per_row(P.cv, "v");
 }
 
 }  // namespace ins2d

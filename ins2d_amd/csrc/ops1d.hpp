@@ -68,11 +68,12 @@ Closure bc_general_stag(int Nt, int Nin, int Nb, int bc1, int bc2, double h1, do
 // host description of the kron-factored operators
 // ---------------------------------------------------------------------------
 struct HOp1 {            // 1-D operator in ELL form
-  int nrows = 0, ncols = 0, K = 0;
+  int nrows = 0, ncols = 0, K = 0;   // K: stored entries per row (row stride)
+  int kmax = 0;                      // most nonzero entries of any row
   std::vector<int> col;
   std::vector<double> val;
 };
-HOp1 to_ell(const Sp& A);
+HOp1 to_ell(const Sp& A, int min_k = 1);
 
 struct HKron {           // kron(Y, X)
   HOp1 X, Y;

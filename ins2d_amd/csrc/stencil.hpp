@@ -25,38 +25,92 @@ __device__ __forceinline__ double kron_at(const DKron& K, const double* __restri
   return acc;
 }
 
-// kron applied to (ca * A + cb * B) without forming the sum
-__device__ __forceinline__ double kron_at2(const DKron& K, const double* __restrict__ A, double ca,
-                                           const double* __restrict__ B, double cb, int f, int g) {
-  const int KY = K.Y.K, KX = K.X.K, ld = K.X.ncols;
-  const int* cy = K.Y.col + (size_t)g * KY;
-  const double* wy = K.Y.val + (size_t)g * KY;
-  const int* cx = K.X.col + (size_t)f * KX;
-  const double* wx = K.X.val + (size_t)f * KX;
-  double acc = 0.0;
-  for (int a = 0; a < KY; ++a) {
-    const size_t ro = (size_t)cy[a] * ld;
-    double s = 0.0;
-    if (B) {
-      for (int b = 0; b < KX; ++b) s += wx[b] * (ca * A[ro + cx[b]] + cb * B[ro + cx[b]]);
-    } else {
-      for (int b = 0; b < KX; ++b) s += wx[b] * (ca * A[ro + cx[b]]);
-    }
-    acc += wy[a] * s;
+__device__ __forceinline__ double yv_at(const DYV& Y, int f, int g) {
+  double s = 0.0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    if (t < Y.nt) s += Y.ax[t][f] * Y.ay[t][g];
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Second-order operators of the generic path: every 1-D factor is stored with
+// exactly two (column, weight) entries per row (zero-padded by the host), so
+// the index and data loads below are unrolled and independent of each other
+// and issue together instead of as a chain of loop-carried loads.  A factor
+// whose rows hold one entry (kmax 1: identities, diagonals) skips the second
+// entry on a uniform branch.
+// ---------------------------------------------------------------------------
+struct Row2 {
+  int2 c;
+  double2 w;
+};
+__device__ __forceinline__ Row2 row2(const DOp1& O, int r) {
+  return Row2{reinterpret_cast<const int2*>(O.col)[r], reinterpret_cast<const double2*>(O.val)[r]};
+}
+
+// (kron(Y, X) in)(f, g)
+__device__ __forceinline__ double kron2_at(const DKron& K, const double* __restrict__ in, int f, int g) {
+  const Row2 y = row2(K.Y, g), x = row2(K.X, f);
+  const size_t ld = K.X.ncols;
+  const bool x2 = K.X.kmax > 1;
+  const double* r0 = in + (size_t)y.c.x * ld;
+  double s0 = x.w.x * r0[x.c.x];
+  if (x2) s0 += x.w.y * r0[x.c.y];
+  double acc = y.w.x * s0;
+  if (K.Y.kmax > 1) {
+    const double* r1 = in + (size_t)y.c.y * ld;
+    double s1 = x.w.x * r1[x.c.x];
+    if (x2) s1 += x.w.y * r1[x.c.y];
+    acc += y.w.y * s1;
   }
   return acc;
 }
 
-__device__ __forceinline__ double yv_at(const DYV& Y, int f, int g) {
-  double s = 0.0;
-  for (int t = 0; t < Y.nt; ++t) s += Y.ax[t][f] * Y.ay[t][g];
-  return s;
+// kron applied to (ca * A + cb * B) without forming the sum
+__device__ __forceinline__ double kron2_at2(const DKron& K, const double* __restrict__ A, double ca,
+                                            const double* __restrict__ B, double cb, int f, int g) {
+  const Row2 y = row2(K.Y, g), x = row2(K.X, f);
+  const size_t ld = K.X.ncols;
+  const bool x2 = K.X.kmax > 1;
+  auto w = [&](size_t o) { return B ? ca * A[o] + cb * B[o] : ca * A[o]; };
+  const size_t r0 = (size_t)y.c.x * ld;
+  double s0 = x.w.x * w(r0 + x.c.x);
+  if (x2) s0 += x.w.y * w(r0 + x.c.y);
+  double acc = y.w.x * s0;
+  if (K.Y.kmax > 1) {
+    const size_t r1 = (size_t)y.c.y * ld;
+    double s1 = x.w.x * w(r1 + x.c.x);
+    if (x2) s1 += x.w.y * w(r1 + x.c.y);
+    acc += y.w.y * s1;
+  }
+  return acc;
+}
+
+// sum over the (up to 2 x 2) entries of D's rows i (x) and j (y) of
+// D(i, f) D(j, g) face(f, g)
+template <class FACE>
+__device__ __forceinline__ double apply_d2(const DKron& D, int i, int j, FACE&& face) {
+  const Row2 dy = row2(D.Y, j), dx = row2(D.X, i);
+  const bool x2 = D.X.kmax > 1;
+  double s0 = dx.w.x * face(dx.c.x, dy.c.x);
+  if (x2) s0 += dx.w.y * face(dx.c.y, dy.c.x);
+  double acc = dy.w.x * s0;
+  if (D.Y.kmax > 1) {
+    double s1 = dx.w.x * face(dx.c.x, dy.c.y);
+    if (x2) s1 += dx.w.y * face(dx.c.y, dy.c.y);
+    acc += dy.w.y * s1;
+  }
+  return acc;
 }
 
 // momentum right-hand side at (i, j) of component COMP
 //   out = [ -sum conv + sum diff  (- G p if WITH_P) ] (/ volume if SCALE_OM)
 // and, for an RK stage, vtemp = aself * out + sum_j a_j k_j at the same point.
 // ``koff`` offsets the k_j pointers (batched layouts).
+// conv term:  D [ (I c + yI) .* (A own + yA) ]   (convection.m:198-212)
+// diff term:  D [ sscale S own + yS ]            (diffusion.m:29-30 with
+//             Diffu = Dux (Su_ux / Re) + ..., operator_convection_diffusion.m:460)
 template <int COMP>
 __device__ __forceinline__ void rhs_point(const DComp& C, const double* __restrict__ u, const double* __restrict__ v,
                                           const double* __restrict__ p, int mode, double* __restrict__ out,
@@ -64,48 +118,26 @@ __device__ __forceinline__ void rhs_point(const DComp& C, const double* __restri
                                           int j) {
   const double* own = COMP == 0 ? u : v;
   double acc = 0.0;
-  for (int t = 0; t < C.nconv; ++t) {
-    const DConv& T = C.conv[t];
-    const double* cf = T.icomp == 0 ? u : v;
-    const int KDY = T.D.Y.K, KDX = T.D.X.K;
-    double s = 0.0;
-    for (int a = 0; a < KDY; ++a) {
-      const int g = T.D.Y.col[(size_t)j * KDY + a];
-      const double dy = T.D.Y.val[(size_t)j * KDY + a];
-      if (dy == 0.0) continue;
-      double sx = 0.0;
-      for (int b = 0; b < KDX; ++b) {
-        const int f = T.D.X.col[(size_t)i * KDX + b];
-        const double dx = T.D.X.val[(size_t)i * KDX + b];
-        if (dx == 0.0) continue;
-        const double fi = kron_at(T.I, cf, f, g) + yv_at(T.yI, f, g);
-        const double fa = kron_at(T.A, own, f, g) + yv_at(T.yA, f, g);
-        sx += dx * (fi * fa);
-      }
-      s += dy * sx;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    if (t < C.nconv) {
+      const DConv& T = C.conv[t];
+      const double* cf = T.icomp == 0 ? u : v;
+      acc -= T.scale * apply_d2(T.D, i, j, [&](int f, int g) {
+        const double fi = kron2_at(T.I, cf, f, g) + yv_at(T.yI, f, g);
+        const double fa = kron2_at(T.A, own, f, g) + yv_at(T.yA, f, g);
+        return fi * fa;
+      });
     }
-    acc -= T.scale * s;
   }
-  for (int t = 0; t < C.ndiff; ++t) {
-    const DDiff& T = C.diff[t];
-    const int KDY = T.D.Y.K, KDX = T.D.X.K;
-    double s = 0.0;
-    for (int a = 0; a < KDY; ++a) {
-      const int g = T.D.Y.col[(size_t)j * KDY + a];
-      const double dy = T.D.Y.val[(size_t)j * KDY + a];
-      if (dy == 0.0) continue;
-      double sx = 0.0;
-      for (int b = 0; b < KDX; ++b) {
-        const int f = T.D.X.col[(size_t)i * KDX + b];
-        const double dx = T.D.X.val[(size_t)i * KDX + b];
-        if (dx == 0.0) continue;
-        sx += dx * (T.sscale * kron_at(T.S, own, f, g) + yv_at(T.yS, f, g));
-      }
-      s += dy * sx;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    if (t < C.ndiff) {
+      const DDiff& T = C.diff[t];
+      acc += apply_d2(T.D, i, j, [&](int f, int g) { return T.sscale * kron2_at(T.S, own, f, g) + yv_at(T.yS, f, g); });
     }
-    acc += s;
   }
-  if (mode & RHS_WITH_P) acc -= kron_at(C.G, p, i, j);
+  if (mode & RHS_WITH_P) acc -= kron2_at(C.G, p, i, j);
   if (mode & RHS_SCALE_OM) acc = acc / (C.ox[i] * C.oy[j]);
   const size_t idx = (size_t)i + (size_t)C.nx * j;
   if (out) out[idx] = acc;
@@ -120,8 +152,8 @@ __device__ __forceinline__ void rhs_point(const DComp& C, const double* __restri
 __device__ __forceinline__ double div_point(const DComp& Cu, const DComp& Cv, const double* __restrict__ A,
                                             const double* __restrict__ B, double ca, double cb, double ycoef, int nu,
                                             int i, int j) {
-  double s = kron_at2(Cu.M, A, ca, B, cb, i, j);
-  s += kron_at2(Cv.M, A + nu, ca, B ? B + nu : nullptr, cb, i, j);
+  double s = kron2_at2(Cu.M, A, ca, B, cb, i, j);
+  s += kron2_at2(Cv.M, A + nu, ca, B ? B + nu : nullptr, cb, i, j);
   if (ycoef != 0.0) s += ycoef * (yv_at(Cu.yM, i, j) + yv_at(Cv.yM, i, j));
   return s;
 }
@@ -131,7 +163,7 @@ __device__ __forceinline__ double proj_point(const DComp& C, const double* __res
                                              const double* __restrict__ Vt, const double* __restrict__ dp, double dt,
                                              double c, int i, int j) {
   const size_t idx = (size_t)i + (size_t)C.nx * j;
-  const double g = kron_at(C.G, dp, i, j) / (C.ox[i] * C.oy[j]);
+  const double g = kron2_at(C.G, dp, i, j) / (C.ox[i] * C.oy[j]);
   const double vt = Vt ? Vt[idx] : 0.0;
   return Vn[idx] + dt * (vt - c * g);
 }
