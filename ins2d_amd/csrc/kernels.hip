@@ -250,7 +250,12 @@ __global__ void __launch_bounds__(256) k_rhs_per(PerGrid G, const double* __rest
 // fourth order, row-streaming: one wave per 64-wide column strip of H
 // rows; the rows j-3 .. j+3 of u and v live in an LDS ring of 7 rows (+-3
 // halo in x), so every value is read from memory once per strip (plus the
-// 6-row start-up and the 6-column halo)
+// 6-row start-up and the 6-column halo).  The y-direction face fluxes are
+// streamed: each row computes only the fluxes of its newest faces (1h faces
+// j+1, 3h faces j+3, diffusive faces j+3/2) and takes the older ones from
+// registers, where the rows above left them; the x-direction terms are
+// computed per row as in rhs4_core.  Every flux is the same expression as in
+// rhs4_core.
 constexpr int kStripW = 64;
 __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __restrict__ u,
                                                  const double* __restrict__ v, const double* __restrict__ p, int mode,
@@ -285,28 +290,103 @@ __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __res
       sv[slot][kStripW + tx] = pv2;
     }
   };
-  for (int k = 0; k < 6; ++k) {
-    fetch_row(j0 - 3 + k);
-    store_row(k);
-  }
-  const size_t N = (size_t)Nx * Ny;
-  const int jend = min(j0 + H, Ny);
-  fetch_row(j0 + 3);
-  int s0 = 0;   // ring slot of row j - 3
-  for (int j = j0; j < jend; ++j) {
-    store_row(s0 == 0 ? 6 : s0 - 1);   // row j + 3
-    __syncthreads();
-    if (j + 1 < jend) fetch_row(j + 4);
-    RingAcc4 acc;
+  const double hx = G.hx, hy = G.hy;
+  const double w1 = 9.0 / 16.0, w2 = -1.0 / 16.0;   // beta/2, (1-beta)/2
+  const double r72x = G.rhx * (1.0 / 72.0), r72y = G.rhy * (1.0 / 72.0);
+  RingAcc4 acc;
+  // rows j-3 .. j+3 of the ring (s0: slot of row j - 3)
+  auto set_ring = [&](int s0) {
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
       const int sl = s0 + k < 7 ? s0 + k : s0 + k - 7;
       acc.ur[k] = &su[sl][tx + 3];
       acc.vr[k] = &sv[sl][tx + 3];
     }
+  };
+#define U(a, b) acc.U(a, b)
+#define V(a, b) acc.V(a, b)
+  // y-direction fluxes at the face j + d of the current row j
+  auto fy1u = [&](int d) {   // 1h u-flux v * u on grid line y_{j+d}
+    return (hx * (w2 * V(-2, d) + w1 * V(-1, d) + w1 * V(0, d) + w2 * V(1, d))) * (0.5 * (U(0, d - 1) + U(0, d)));
+  };
+  auto fy3u = [&](int d) {   // 3h u-flux on grid line y_{j+d-1}
+    return (3.0 * hx * (w2 * V(-2, d - 1) + w1 * V(-1, d - 1) + w1 * V(0, d - 1) + w2 * V(1, d - 1))) *
+           (0.5 * (U(0, d - 3) + U(0, d)));
+  };
+  auto pya = [&](int d) {    // du/dy between rows j+d and j+d+1
+    return (81.0 * (U(0, d + 1) - U(0, d)) - 3.0 * (U(0, d + 2) - U(0, d - 1))) * r72y;
+  };
+  auto gy1v = [&](int d) {   // 1h v-flux v * v
+    return (hx * (w2 * V(0, d - 2) + w1 * V(0, d - 1) + w1 * V(0, d) + w2 * V(0, d + 1))) *
+           (0.5 * (V(0, d - 1) + V(0, d)));
+  };
+  auto gy3v = [&](int d) {   // 3h v-flux
+    return (3.0 * hx * (w2 * V(0, d - 3) + w1 * V(0, d - 2) + w1 * V(0, d - 1) + w2 * V(0, d))) *
+           (0.5 * (V(0, d - 3) + V(0, d)));
+  };
+  auto qya = [&](int d) {    // dv/dy between rows j+d and j+d+1
+    return (81.0 * (V(0, d + 1) - V(0, d)) - 3.0 * (V(0, d + 2) - V(0, d - 1))) * r72y;
+  };
+  for (int k = 0; k < 6; ++k) {
+    fetch_row(j0 - 3 + k);
+    store_row(k);
+  }
+  fetch_row(j0 + 3);
+  __syncthreads();
+  // the fluxes of the faces at and above row j0 (rows j0-3 .. j0+2 of the ring)
+  set_ring(0);
+  double F1 = fy1u(0);                                 // face j
+  double F30 = fy3u(0), F31 = fy3u(1), F32 = fy3u(2);   // faces j, j+1, j+2
+  double Pm2 = pya(-2), Pm1 = pya(-1), P0 = pya(0);      // j-3/2 .. j+1/2
+  double G1 = gy1v(0);
+  double G30 = gy3v(0), G31 = gy3v(1), G32 = gy3v(2);
+  double Qm2 = qya(-2), Qm1 = qya(-1), Q0 = qya(0);
+  const size_t N = (size_t)Nx * Ny;
+  const int jend = min(j0 + H, Ny);
+  int s0 = 0;   // ring slot of row j - 3
+  for (int j = j0; j < jend; ++j) {
+    store_row(s0 == 0 ? 6 : s0 - 1);   // row j + 3
+    __syncthreads();
+    if (j + 1 < jend) fetch_row(j + 4);
+    set_ring(s0);
+    // ---------------- u-momentum ----------------
+    auto Ix1 = [&](int f) { return hy * (w2 * U(f - 2, 0) + w1 * U(f - 1, 0) + w1 * U(f, 0) + w2 * U(f + 1, 0)); };
+    auto Ax1 = [&](int f) { return 0.5 * (U(f - 1, 0) + U(f, 0)); };
+    const double du2dx1 = Ix1(1) * Ax1(1) - Ix1(0) * Ax1(0);
+    auto Ix3 = [&](int r) { return 3.0 * hy * (w2 * U(r - 3, 0) + w1 * U(r - 2, 0) + w1 * U(r - 1, 0) + w2 * U(r, 0)); };
+    auto Ax3 = [&](int r) { return 0.5 * (U(r - 3, 0) + U(r, 0)); };
+    const double du2dx3 = Ix3(3) * Ax3(3) - Ix3(0) * Ax3(0);
+    const double F1n = fy1u(1), F3n = fy3u(3), Pn = pya(1);
+    const double duvdy1 = F1n - F1;
+    const double duvdy3 = F3n - F30;
+    const double convu = 81.0 * (du2dx1 + duvdy1) - (du2dx3 + duvdy3);
+    auto Px = [&](int k) { return (81.0 * (U(k + 1, 0) - U(k, 0)) - 3.0 * (U(k + 2, 0) - U(k - 1, 0))) * r72x; };
+    const double d2u = hy * (81.0 * (Px(0) - Px(-1)) - 3.0 * (Px(1) - Px(-2))) +
+                       hx * (81.0 * (P0 - Pm1) - 3.0 * (Pn - Pm2));
+    double fu = -convu + G.invRe * d2u;
+    // ---------------- v-momentum ----------------
+    auto Uf1 = [&](int f) { return hy * (w2 * U(f, -2) + w1 * U(f, -1) + w1 * U(f, 0) + w2 * U(f, 1)); };
+    auto Bx1 = [&](int f) { return 0.5 * (V(f - 1, 0) + V(f, 0)); };
+    const double duvdx1 = Uf1(1) * Bx1(1) - Uf1(0) * Bx1(0);
+    auto Uf3 = [&](int f) { return 3.0 * hy * (w2 * U(f - 1, -2) + w1 * U(f - 1, -1) + w1 * U(f - 1, 0) + w2 * U(f - 1, 1)); };
+    auto Bx3 = [&](int f) { return 0.5 * (V(f - 3, 0) + V(f, 0)); };
+    const double duvdx3 = Uf3(3) * Bx3(3) - Uf3(0) * Bx3(0);
+    const double G1n = gy1v(1), G3n = gy3v(3), Qn = qya(1);
+    const double dv2dy1 = G1n - G1;
+    const double dv2dy3 = G3n - G30;
+    const double convv = 81.0 * (duvdx1 + dv2dy1) - (duvdx3 + dv2dy3);
+    auto Qx = [&](int k) { return (81.0 * (V(k + 1, 0) - V(k, 0)) - 3.0 * (V(k + 2, 0) - V(k - 1, 0))) * r72x; };
+    const double d2v = hy * (81.0 * (Qx(0) - Qx(-1)) - 3.0 * (Qx(1) - Qx(-2))) +
+                       hx * (81.0 * (Q0 - Qm1) - 3.0 * (Qn - Qm2));
+    double fv = -convv + G.invRe * d2v;
+    // advance the flux queues to the next row
+    F1 = F1n;
+    F30 = F31; F31 = F32; F32 = F3n;
+    Pm2 = Pm1; Pm1 = P0; P0 = Pn;
+    G1 = G1n;
+    G30 = G31; G31 = G32; G32 = G3n;
+    Qm2 = Qm1; Qm1 = Q0; Q0 = Qn;
     if (i < Nx) {
-      double fu, fv;
-      rhs4_core(G, acc, fu, fv);
       if (mode & RHS_WITH_P) {
         double gx, gy;
         grad_per<4>(G, p, i, j, gx, gy);
@@ -329,6 +409,8 @@ __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __res
     __syncthreads();
     s0 = s0 == 6 ? 0 : s0 + 1;
   }
+#undef U
+#undef V
 }
 
 template <int ORDER>
