@@ -5,8 +5,8 @@
 //    combination fused in its epilogue, divergence, projection, Laplacian;
 //  * periodic uniform stencils (second and fourth order) with the boundary
 //    closures resolved at compile time;
-//  * hand-written FFT Poisson solve (radix-4/2 Stockham in LDS, direct DFT for
-//    lengths that are not powers of two);
+//  * hand-written FFT Poisson solve (register-butterfly Stockham passes for
+//    power-of-two lengths, direct DFT for other lengths);
 //  * Jacobi-preconditioned CG (multi-kernel, device-resident scalars, and a
 //    one-workgroup-per-system form for small systems);
 //  * reductions for the per-step diagnostics.
