@@ -127,11 +127,13 @@ def measured_traffic(key):
 def bench_ensemble(a):
     """Batched ensemble: 512 cavity/channel members of uneven size (48..512
     cells on the long side, aspect 1..4), Re in [100, 5000], speed in
-    [0.5, 1.5], RK4, PCG pressure to 1e-10 (two-level preconditioner; the
-    case ensemble_jacobi uses the plain Jacobi preconditioner)."""
+    [0.5, 1.5], RK4, PCG pressure to 1e-10 with the fast-diagonalization
+    inverse of each member's pressure matrix as preconditioner (the cases
+    ensemble_2level and ensemble_jacobi use the two-level and the plain
+    Jacobi preconditioner)."""
     from ins2d_amd import host
     from ins2d_amd.cases import ensemble
-    precond = 'jacobi' if a.case == 'ensemble_jacobi' else 'two_level'
+    precond = {'ensemble_jacobi': 'jacobi', 'ensemble_2level': 'two_level'}.get(a.case, 'fdm')
     members = ensemble.members(a.members, precond=precond)
     t0 = time.perf_counter()
     B = host.Batch(members)
@@ -150,7 +152,8 @@ def bench_ensemble(a):
            'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': 1e3 * el / a.steps, 'higher_is_better': True,
            'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
            'config': {'workload': f'ensemble of {a.members} cavity/channel members, RK44, '
-                                  f'{"Jacobi" if precond == "jacobi" else "two-level (Jacobi + coarse grid)"}-PCG 1e-10',
+                                  + {'jacobi': 'Jacobi', 'two_level': 'two-level (Jacobi + coarse grid)',
+                                     'fdm': 'fast-diagonalization'}[precond] + '-PCG 1e-10',
                       'cells': cells, 'setup_s': t_setup, 'stage_tile': list(tile),
                       'pcg_iterations_per_member_solve': (it1 - it0) / (a.members * 4 * a.steps)}}
     B.close()
@@ -235,7 +238,7 @@ CASES = {
 
 def main():
     a = parse()
-    if a.case in ('ensemble', 'ensemble_jacobi'):
+    if a.case in ('ensemble', 'ensemble_jacobi', 'ensemble_2level'):
         return bench_ensemble(a)
     if a.case == 'poisson_learned':
         return bench_poisson_learned(a)

@@ -74,7 +74,9 @@ typedef struct {
                                 workgroup with wavefront triangular solves,
                                 up to 2^20 cells; 2 (ensemble only) Jacobi plus
                                 an aggregation coarse-grid correction, as
-                                ins2d_set_poisson_accel flag 2 (single case) */
+                                ins2d_set_poisson_accel flag 2 (single case);
+                                3 (ensemble only) the fast-diagonalization
+                                inverse of each instance's pressure matrix */
 } ins2d_case_desc;
 
 /* operator construction: replaces operators.m -> operator_mesh/averaging/
@@ -149,8 +151,8 @@ int ins2d_sync(ins2d_handle* h);
  * of uneven size advanced together; second order, PCG pressure).  All
  * instances share the Butcher tableau, cg_tol, cg_maxit, cg_precond (0 Jacobi,
  * 2 two-level with 16 x 16 aggregates, at most 2048 coarse cells per
- * instance) and p_add_solve of descs[0]; each has its own grid, boundary kinds
- * and Re. */
+ * instance, 3 fast-diagonalization) and p_add_solve of descs[0]; each has its
+ * own grid, boundary kinds and Re. */
 typedef struct ins2d_batch ins2d_batch;
 int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out);
 void ins2d_batch_destroy(ins2d_batch* b);

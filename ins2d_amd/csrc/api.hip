@@ -739,7 +739,7 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
         h->dic.cwy = I.cwy.empty() ? nullptr : dupload(h.get(), I.cwy);
         h->ic0 = true;
       } else if (d->cg_precond != 0 && d->cg_precond != 1 && d->cg_precond != 2) {
-        throw ArgError("cg_precond must be 0 (Jacobi), 1 (IC(0)) or 2 (two-level)");
+        throw ArgError("cg_precond must be 0 (Jacobi), 1 (IC(0)) or 2 (two-level); 3 is for the ensemble");
       }
       if (h->poisson == INS2D_POISSON_DIRECT) {
         FDMHost F;

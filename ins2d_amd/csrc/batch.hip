@@ -1,9 +1,11 @@
 // Batched ensemble: many independent cavity/channel instances of uneven size
 // and Reynolds number advanced together.  Every phase of an RK stage is one
 // launch over a work list of 64x4 tiles that spans all instances; the
-// pressure solve is a batched Jacobi-PCG with per-instance scalars (converged
+// pressure solve is a batched PCG with per-instance scalars (converged
 // instances skip their tiles), run in chunks of iterations that are captured
-// once into a hipGraph and replayed.
+// once into a hipGraph and replayed.  Preconditioners: Jacobi, Jacobi plus an
+// aggregation coarse grid, or the fast-diagonalization inverse of each
+// instance's pressure matrix (batched FP64 GEMMs on the matrix cores).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -11,10 +13,12 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ins2d.h"
 #include "device.hpp"
+#include "dgemm.hpp"
 #include "launch.hpp"
 #include "ops1d.hpp"
 #include "stencil.hpp"
@@ -35,6 +39,10 @@ struct BInst {
   int at0, at1, ncx, ncy, nx0, ny0;
   long coff;
   const double *cVx, *cVy, *clx, *cly;
+  // fast-diagonalization preconditioner: the factors of this instance's
+  // pressure matrix (Vx, Vx^T, Vy, Vy^T, eigenvalues, null mode)
+  const double *fVx, *fVxT, *fVy, *fVyT, *flx, *fly;
+  int fnx0, fny0;
 };
 struct Tile {
   int inst, comp, i0, j0;
@@ -457,6 +465,103 @@ __global__ void __launch_bounds__(256) k_b2_dir(const BInst* __restrict__ I, con
   p[k] = z + S.s[8 * t.inst + 3] * p[k];
 }
 
+// ---- batched PCG with the fast-diagonalization preconditioner: z = A^-1 r
+// by the four GEMMs of the direct solve (fdm.hip), per instance, over a work
+// list of 32 x 32 tiles of the Npx x Npy pressure arrays; the per-instance
+// scalars and the convergence test (cg.F:247) are the Jacobi PCG's, so the
+// preconditioner (the exact inverse on these tensor-product grids, up to
+// rounding) changes only how many iterations reach the tolerance.
+// PH 0: dst = Vx^T src; 1: dst = -(src Vy) ./ (lx + ly); 2: dst = Vx src;
+// 3: dst = src Vy^T
+template <int PH>
+__global__ void __launch_bounds__(64 * kGemmWaves) k_bf_gemm(const BInst* __restrict__ I,
+                                                            const Tile* __restrict__ tiles, BCG S,
+                                                            const double* __restrict__ src,
+                                                            double* __restrict__ dst) {
+  __shared__ GemmPart part;
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int m = B.npx, n = B.npy;
+  const double* s = src + B.poff;
+  double* d = dst + B.poff;
+  if (PH == 0) dgemm_tile<0>(m, n, m, B.fVxT, m, s, m, d, m, nullptr, nullptr, -1, -1, t.i0, t.j0, part);
+  if (PH == 1) dgemm_tile<1>(m, n, n, s, m, B.fVy, n, d, m, B.flx, B.fly, B.fnx0, B.fny0, t.i0, t.j0, part);
+  if (PH == 2) dgemm_tile<0>(m, n, m, B.fVx, m, s, m, d, m, nullptr, nullptr, -1, -1, t.i0, t.j0, part);
+  if (PH == 3) dgemm_tile<0>(m, n, n, s, m, B.fVyT, n, d, m, nullptr, nullptr, -1, -1, t.i0, t.j0, part);
+}
+
+// x = 0, r = f; partial f.f per pressure tile
+__global__ void __launch_bounds__(256) k_bf_init(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
+                                                const double* __restrict__ f, double* __restrict__ x,
+                                                double* __restrict__ r) {
+  __shared__ double red[4];
+  const Tile t = tiles[blockIdx.x];
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  double ff = 0.0;
+  if (i < B.npx && j < B.npy) {
+    const long k = B.poff + (long)i + (long)B.npx * j;
+    const double rn = f[k];
+    x[k] = 0.0;
+    r[k] = rn;
+    ff = rn * rn;
+  }
+  const double a = block_sum(ff, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) S.part[2 * blockIdx.x] = a;
+}
+
+// partial r.z per pressure tile (active instances)
+__global__ void __launch_bounds__(256) k_bf_rz(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
+                                              const double* __restrict__ r, const double* __restrict__ z) {
+  __shared__ double red[4];
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  double rz = 0.0;
+  if (i < B.npx && j < B.npy) {
+    const long k = B.poff + (long)i + (long)B.npx * j;
+    rz = r[k] * z[k];
+  }
+  const double a = block_sum(rz, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) S.part[2 * blockIdx.x + 1] = a;
+}
+
+// x += alpha p, r -= alpha q; partial r.r per pressure tile
+__global__ void __launch_bounds__(256) k_bf_update(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
+                                                  const double* __restrict__ p, const double* __restrict__ q,
+                                                  double* __restrict__ x, double* __restrict__ r) {
+  __shared__ double red[4];
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  const double alpha = S.s[8 * t.inst + 2];
+  double rr = 0.0;
+  if (i < B.npx && j < B.npy) {
+    const long k = B.poff + (long)i + (long)B.npx * j;
+    x[k] += alpha * p[k];
+    const double rn = r[k] - alpha * q[k];
+    r[k] = rn;
+    rr = rn * rn;
+  }
+  const double a = block_sum(rr, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) S.part[2 * blockIdx.x] = a;
+}
+
+// first direction p = z (active instances)
+__global__ void __launch_bounds__(256) k_bf_dir0(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
+                                                const double* __restrict__ z, double* __restrict__ p) {
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  if (i >= B.npx || j >= B.npy) return;
+  const long k = B.poff + (long)i + (long)B.npx * j;
+  p[k] = z[k];
+}
+
 __global__ void k_b_set_int(int* x, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *x = v;
 }
@@ -546,6 +651,13 @@ struct ins2d_batch {
   int tx = TX, ty = TY;
   // two-level PCG (cg_precond = 2)
   bool two_level = false;
+  // fast-diagonalization PCG (cg_precond = 3): per-instance factors, the
+  // 32 x 32 GEMM tile list, two work arrays
+  bool fdm = false;
+  std::vector<FDMHost> fh;
+  std::vector<Tile> gt;
+  Tile* dgt = nullptr;
+  double *gw1 = nullptr, *gw2 = nullptr;
   std::vector<Tile> at;
   Tile* dat = nullptr;
   BTL tl{};
@@ -663,7 +775,46 @@ void build_ops(ins2d_batch* b, Arena& A) {
       I.clx = A.push(C.lx);
       I.cly = A.push(C.ly);
     }
+    if (b->fdm) {
+      const FDMHost& F = b->fh[k];
+      I.fVx = A.push(F.Vx);
+      I.fVxT = A.push(F.VxT);
+      I.fVy = A.push(F.Vy);
+      I.fVyT = A.push(F.VyT);
+      I.flx = A.push(F.lx);
+      I.fly = A.push(F.ly);
+      I.fnx0 = F.nx0;
+      I.fny0 = F.ny0;
+    }
   }
+}
+
+// the fast-diagonalization factors of every instance, on host threads
+void build_fdm_factors(ins2d_batch* b) {
+  b->fh.assign(b->n, FDMHost{});
+  const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  std::vector<std::string> errs(nt);
+  for (int w = 0; w < nt; ++w)
+    pool.emplace_back([b, w, nt, &errs] {
+      try {
+        for (int k = w; k < b->n; k += nt) build_fdm_host(*b->plans[k], b->fh[k]);
+      } catch (const std::exception& e) {
+        errs[w] = e.what();
+      }
+    });
+  for (auto& th : pool) th.join();
+  for (auto& e : errs)
+    if (!e.empty()) throw std::runtime_error("fast-diagonalization factors: " + e);
+}
+
+// z = A^-1 r for every active instance (four batched GEMM phases)
+void fdm_apply(ins2d_batch* b, const double* r, double* z) {
+  const dim3 g((unsigned)b->gt.size()), t(64 * kGemmWaves);
+  hipLaunchKernelGGL(k_bf_gemm<0>, g, t, 0, b->st, b->dinst, b->dgt, b->cg, r, b->gw1);
+  hipLaunchKernelGGL(k_bf_gemm<1>, g, t, 0, b->st, b->dinst, b->dgt, b->cg, b->gw1, b->gw2);
+  hipLaunchKernelGGL(k_bf_gemm<2>, g, t, 0, b->st, b->dinst, b->dgt, b->cg, b->gw2, b->gw1);
+  hipLaunchKernelGGL(k_bf_gemm<3>, g, t, 0, b->st, b->dinst, b->dgt, b->cg, b->gw1, z);
 }
 
 void append_velocity_tiles(const BInst& I, int k, int tx, int ty, std::vector<Tile>& out) {
@@ -691,6 +842,18 @@ void launch_tiles_proj(ins2d_batch* b, const double* Vn, const double* Vt, const
 }
 
 void cg_iter_launches(ins2d_batch* b) {
+  if (b->fdm) {
+    const unsigned np = (unsigned)b->pt.size();
+    hipLaunchKernelGGL(k_b_cg_apply, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q);
+    hipLaunchKernelGGL(k_b_cg_alpha, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
+    hipLaunchKernelGGL(k_bf_update, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q, b->x,
+                       b->r);
+    fdm_apply(b, b->r, b->z);
+    hipLaunchKernelGGL(k_bf_rz, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->r, b->z);
+    hipLaunchKernelGGL(k_b_cg_beta, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
+    hipLaunchKernelGGL(k_b_cg_dir, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->z, b->pp);
+    return;
+  }
   if (b->two_level) {
     const unsigned na = (unsigned)b->at.size();
     const dim3 tb(kAgg, kAgg);
@@ -714,7 +877,15 @@ void cg_iter_launches(ins2d_batch* b) {
 void b_poisson(ins2d_batch* b, const double* f) {
   const unsigned np = (unsigned)b->pt.size();
   hipLaunchKernelGGL(k_b_set_int, dim3(1), dim3(1), 0, b->st, b->cg.nactive, b->n);
-  if (b->two_level) {
+  if (b->fdm) {
+    // flags cleared first: the GEMM phases skip finished instances
+    hipLaunchKernelGGL(k_b_clear_flags, dim3((b->n + 255) / 256), dim3(256), 0, b->st, b->cg.f, b->n);
+    hipLaunchKernelGGL(k_bf_init, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, f, b->x, b->r);
+    fdm_apply(b, b->r, b->z);
+    hipLaunchKernelGGL(k_bf_rz, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->r, b->z);
+    hipLaunchKernelGGL(k_b_cg_init_fin, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tol * b->tol);
+    hipLaunchKernelGGL(k_bf_dir0, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->z, b->pp);
+  } else if (b->two_level) {
     const unsigned na = (unsigned)b->at.size();
     const dim3 tb(kAgg, kAgg);
     // per-instance flags cleared first: k_b2_coarse skips finished instances
@@ -832,7 +1003,15 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
       build_plan(cd, *b->plans.back());
     }
     if (d0.cg_precond == 2) b->two_level = true;
-    else if (d0.cg_precond != 0) throw BErr(INS2D_E_UNSUPPORTED, "the ensemble runs Jacobi or two-level PCG");
+    else if (d0.cg_precond == 3) b->fdm = true;
+    else if (d0.cg_precond != 0)
+      throw BErr(INS2D_E_UNSUPPORTED, "the ensemble runs Jacobi, two-level or fast-diagonalization PCG");
+    // with the exact inverse as preconditioner a solve takes one or two
+    // iterations: replay short chunks
+    if (b->fdm) {
+      b->chunk = 2;
+      build_fdm_factors(b.get());
+    }
     BHIP(hipStreamCreate(&b->st));
     b->hcomp.assign(2 * (size_t)n, DComp{});
     b->hinst.assign(n, BInst{});
@@ -872,6 +1051,9 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
         coff += (long)I.ncx * I.ncy;
       }
       I.at1 = (int)b->at.size();
+      if (b->fdm)
+        for (int j = 0; j < I.npy; j += 32)
+          for (int i = 0; i < I.npx; i += 32) b->gt.push_back(Tile{k, 2, i, j});
       // samples default to zero
       BCSamples& S = b->samp[k];
       for (int s = 0; s < S_NSRC; ++s) S.s[s].assign(sample_len(P, s), 0.0);
@@ -896,6 +1078,12 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
       b->tl.part = balloc<double>(b.get(), 3 * b->at.size());
       b->tl.rc = balloc<double>(b.get(), (size_t)coff);
       b->tl.ec = balloc<double>(b.get(), (size_t)coff);
+    }
+    if (b->fdm) {
+      b->dgt = balloc<Tile>(b.get(), b->gt.size());
+      BHIP(hipMemcpy(b->dgt, b->gt.data(), b->gt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+      b->gw1 = balloc<double>(b.get(), b->Npt);
+      b->gw2 = balloc<double>(b.get(), b->Npt);
     }
     // tableau (time_ERK.m:29,32)
     const int s = b->s;

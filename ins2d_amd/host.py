@@ -92,7 +92,7 @@ class Solver:
         d.cg_maxit = int(cg_maxit)
         d.p_add_solve = int(p_add_solve)
         d.kernels = int(kernels)
-        d.cg_precond = {'jacobi': 0, 'ic0': 1, 'two_level': 2}[cg_precond]
+        d.cg_precond = {'jacobi': 0, 'ic0': 1, 'two_level': 2, 'fdm': 3}[cg_precond]
         h = ctypes.c_void_p()
         rc = lib.ins2d_create(ctypes.byref(d), ctypes.byref(h))
         if rc != 0:
@@ -309,7 +309,7 @@ class Batch:
             d.cg_tol = float(prm.get('CG_acc', 1e-10))
             d.cg_maxit = int(prm.get('CG_maxit', 1000))
             d.p_add_solve = int(prm.get('p_add_solve', 1))
-            d.cg_precond = {'jacobi': 0, 'ic0': 1, 'two_level': 2}[prm.get('CG_precond', 'jacobi')]
+            d.cg_precond = {'jacobi': 0, 'ic0': 1, 'two_level': 2, 'fdm': 3}[prm.get('CG_precond', 'jacobi')]
             descs.append(d)
             self.opts.append(opts)
         self._keep = keep

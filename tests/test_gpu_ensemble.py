@@ -16,7 +16,7 @@ def _say(*a):
     print('[gpu-test]', *a, flush=True)
 
 
-@pytest.mark.parametrize('precond', ['jacobi', 'two_level'])
+@pytest.mark.parametrize('precond', ['jacobi', 'two_level', 'fdm'])
 def test_ensemble_matches_single_runs_and_oracle(precond):
     _say('ensemble 6 members', precond)
     kw = dict(cg_tol=1e-13, precond=precond)
@@ -72,4 +72,30 @@ def test_two_level_ensemble_cuts_iterations():
     _say('total PCG iterations: jacobi', res['jacobi'][0], 'two-level', res['two_level'][0])
     assert res['two_level'][0] < 0.5 * res['jacobi'][0]
     for (Va, pa), (Vb, pb) in zip(res['jacobi'][1], res['two_level'][1]):
+        assert np.abs(Va - Vb).max() <= 1e-8 * np.abs(Va).max()
+
+
+def test_fdm_ensemble_converges_in_two_iterations():
+    """With the fast-diagonalization inverse of each member's pressure
+    matrix as preconditioner (exact on these tensor-product grids up to
+    rounding), every batched PCG solve meets the plain stopping rule
+    ||r|| <= tol ||f|| (cg.F:247) within two iterations, and the velocities
+    agree with the Jacobi-PCG ensemble to the solver tolerance."""
+    _say('ensemble fast-diagonalization PCG')
+    def make(precond):
+        return [ensemble.member('cavity', 128, 96, 128 / 96, 800.0, 1.0, precond=precond),
+                ensemble.member('channel', 192, 64, 3.0, 1500.0, 0.8, precond=precond),
+                ensemble.member('cavity', 60, 60, 1.0, 300.0, 1.2, precond=precond),
+                ensemble.member('channel', 37, 29, 37 / 29, 250.0, 1.0, precond=precond)]
+    res = {}
+    for pc in ('jacobi', 'fdm'):
+        B = host.Batch(make(pc))
+        B.input_check(1)
+        B.step(3)
+        tot, mx, _ = B.stats()
+        res[pc] = (tot, mx, [B.get_state(k) for k in range(4)])
+        B.close()
+    _say('PCG iterations: jacobi total', res['jacobi'][0], 'fdm total', res['fdm'][0], 'fdm max per solve', res['fdm'][1])
+    assert res['fdm'][1] <= 2
+    for (Va, pa), (Vb, pb) in zip(res['jacobi'][2], res['fdm'][2]):
         assert np.abs(Va - Vb).max() <= 1e-8 * np.abs(Va).max()
