@@ -225,6 +225,10 @@ CASES = {
                                                             poisson=1),
                             'lid-driven cavity {Nx}x{Ny}, order 2, RK44, dt=1e-3, direct pressure solve '
                             '(fast diagonalization), Re=1000'),
+    'LDC_unsteady_fdm': ('LDC_unsteady', lambda N: dict(Nx=N or 64, Ny=N or 64, Re=1000.0, dt=1e-3, RK='RK44', poisson=3,
+                                                      CG_acc=1e-10, CG_maxit=20000, CG_precond='fdm'),
+                         'lid-driven cavity {Nx}x{Ny}, order 2, RK44, dt=1e-3, PCG 1e-10 with the fast-diagonalization '
+                         'preconditioner, Re=1000'),
     'BFS_unsteady': ('BFS_unsteady', lambda N: dict(Nx=N or 1024, Ny=(N // 4 if N else 256)),
                      'backward-facing step {Nx}x{Ny} tanh-stretched, order 2, RK44, direct pressure solve '
                      '(fast diagonalization), Re=800'),

@@ -57,8 +57,11 @@ struct ins2d_handle {
   // FFT
   FFTPlan fft{};
   bool has_fft = false;
-  // fast-diagonalization direct solve (poisson = direct)
+  // fast-diagonalization direct solve (poisson = direct), or the
+  // preconditioner of PCG (cg_precond = 3) with its two work arrays
   FDMPlan fdm{};
+  bool fdm_pcg = false;
+  double *fw1 = nullptr, *fw2 = nullptr;
   // IC(0) preconditioner of the reference (cg_precond = 1)
   bool ic0 = false;
   DIC dic{};
@@ -391,6 +394,34 @@ void poisson(ins2d_handle* h, const double* f, double* dp) {
     h->last_rel = std::sqrt(rel);
     h->total_it += it;
     if (it >= h->maxit && rel > tol2) throw NoConv("CG did not converge in maxit iterations");
+    return;
+  }
+  if (h->fdm_pcg) {
+    // PCG preconditioned by the fast-diagonalization inverse (exact on this
+    // tensor-product grid up to rounding): the plain stopping rule, normally
+    // met after one iteration; the convergence test precedes each further
+    // preconditioner application
+    HIPCHK(hipMemsetAsync(dp, 0, sizeof(double) * h->Np, h->st));
+    launch_cgx(h->st, h->lap, h->cg, 0, f, h->cr, nullptr, nullptr, tol2);
+    launch_fdm_poisson(h->st, h->fdm, h->cr, h->cz, h->fw1, h->fw2);
+    launch_cgx(h->st, h->lap, h->cg, 1, h->cr, h->cz, h->cp, nullptr, tol2);
+    // one iteration between host checks; its kernels after the convergence
+    // test (the GEMMs included) do nothing once the done flag is set
+    int flag[3] = {0, 0, 0};
+    double rel = 0;
+    while (true) {
+      launch_cgx(h->st, h->lap, h->cg, 2, h->cp, h->cq, dp, h->cr, tol2);
+      launch_fdm_poisson(h->st, h->fdm, h->cr, h->cz, h->fw1, h->fw2, h->cg.flag);
+      launch_cgx(h->st, h->lap, h->cg, 3, h->cr, h->cz, h->cp, nullptr, tol2);
+      HIPCHK(hipMemcpyAsync(flag, h->cg.flag, 3 * sizeof(int), hipMemcpyDeviceToHost, h->st));
+      HIPCHK(hipMemcpyAsync(&rel, h->cg.s + 4, sizeof(double), hipMemcpyDeviceToHost, h->st));
+      HIPCHK(hipStreamSynchronize(h->st));
+      if (flag[0]) break;
+    }
+    h->last_it = flag[1];
+    h->last_rel = std::sqrt(rel);
+    h->total_it += flag[1];
+    if (flag[2]) throw NoConv("CG did not converge in maxit iterations");
     return;
   }
   if (h->Np <= 32768) {
@@ -738,10 +769,14 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
         h->dic.cwx = I.cwx.empty() ? nullptr : dupload(h.get(), I.cwx);
         h->dic.cwy = I.cwy.empty() ? nullptr : dupload(h.get(), I.cwy);
         h->ic0 = true;
-      } else if (d->cg_precond != 0 && d->cg_precond != 1 && d->cg_precond != 2) {
-        throw ArgError("cg_precond must be 0 (Jacobi), 1 (IC(0)) or 2 (two-level); 3 is for the ensemble");
+      } else if (d->cg_precond == 3 && (h->poisson == INS2D_POISSON_CG || h->poisson == INS2D_POISSON_CG_IC)) {
+        h->fdm_pcg = true;
+        h->fw1 = dalloc<double>(h.get(), h->Np);
+        h->fw2 = dalloc<double>(h.get(), h->Np);
+      } else if (d->cg_precond < 0 || d->cg_precond > 3) {
+        throw ArgError("cg_precond must be 0 (Jacobi), 1 (IC(0)), 2 (two-level) or 3 (fast diagonalization)");
       }
-      if (h->poisson == INS2D_POISSON_DIRECT) {
+      if (h->poisson == INS2D_POISSON_DIRECT || h->fdm_pcg) {
         FDMHost F;
         build_fdm_host(*h->plan, F);
         h->fdm.nx = F.nx;

@@ -34,21 +34,25 @@ namespace ins2d {
 
 namespace {
 
+// skip: optional device flag (a converged PCG); the launch does nothing
+// while it is set
 template <int EPI>
 __global__ void __launch_bounds__(64 * kGemmWaves) k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
                                                           const double* __restrict__ B, int ldb,
                                                           double* __restrict__ C, int ldc,
                                                           const double* __restrict__ lx,
-                                                          const double* __restrict__ ly, int nx0, int ny0) {
+                                                          const double* __restrict__ ly, int nx0, int ny0,
+                                                          const int* __restrict__ skip) {
   __shared__ GemmPart part;
+  if (skip && *skip) return;
   dgemm_tile<EPI>(M, N, K, A, lda, B, ldb, C, ldc, lx, ly, nx0, ny0, blockIdx.x * 32, blockIdx.y * 32, part);
 }
 
 template <int EPI>
-void gemm(hipStream_t st, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
-          const double* lx = nullptr, const double* ly = nullptr, int nx0 = -1, int ny0 = -1) {
+void gemm(hipStream_t st, const int* skip, int M, int N, int K, const double* A, int lda, const double* B, int ldb,
+          double* C, int ldc, const double* lx = nullptr, const double* ly = nullptr, int nx0 = -1, int ny0 = -1) {
   hipLaunchKernelGGL(k_dgemm<EPI>, dim3((M + 31) / 32, (N + 31) / 32), dim3(64 * kGemmWaves), 0, st, M, N, K, A, lda, B,
-                     ldb, C, ldc, lx, ly, nx0, ny0);
+                     ldb, C, ldc, lx, ly, nx0, ny0, skip);
 }
 
 // one v_mfma_f64_16x16x4f64 with the operand layout the GEMM uses; the raw
@@ -77,13 +81,14 @@ int fdm_mfma_probe(const double* A, const double* B, double* Draw) {
   return e == hipSuccess ? 0 : 2;
 }
 
-// dp = A^-1 f; w1, w2: Npx*Npy work arrays
-void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2) {
+// dp = A^-1 f; w1, w2: Npx*Npy work arrays; nothing while *skip is set
+void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2,
+                        const int* skip) {
   const int nx = F.nx, ny = F.ny;
-  gemm<0>(st, nx, ny, nx, F.VxT, nx, f, nx, w1, nx);                           // Vx^T F
-  gemm<1>(st, nx, ny, ny, w1, nx, F.Vy, ny, w2, nx, F.lx, F.ly, F.nx0, F.ny0);   // (.) Vy, scaled
-  gemm<0>(st, nx, ny, nx, F.Vx, nx, w2, nx, w1, nx);                            // Vx (.)
-  gemm<0>(st, nx, ny, ny, w1, nx, F.VyT, ny, dp, nx);                           // (.) Vy^T
+  gemm<0>(st, skip, nx, ny, nx, F.VxT, nx, f, nx, w1, nx);                           // Vx^T F
+  gemm<1>(st, skip, nx, ny, ny, w1, nx, F.Vy, ny, w2, nx, F.lx, F.ly, F.nx0, F.ny0);   // (.) Vy, scaled
+  gemm<0>(st, skip, nx, ny, nx, F.Vx, nx, w2, nx, w1, nx);                            // Vx (.)
+  gemm<0>(st, skip, nx, ny, ny, w1, nx, F.VyT, ny, dp, nx);                           // (.) Vy^T
 }
 
 }  // namespace ins2d

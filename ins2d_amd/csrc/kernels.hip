@@ -1119,6 +1119,92 @@ __global__ void k_cg_init_fin(CGState S, double tol2) {
   }
 }
 
+// ---- PCG with a preconditioner applied outside these kernels (the
+// fast-diagonalization solve): same scalars and stopping rule, z supplied
+// ---------------------------------------------------------------------------
+// r = f (x = 0 set by the caller); partials of f.f; clears the done flag of
+// the previous solve, which k_cgx_rz reads before k_cg_init_fin sets it anew
+__global__ void __launch_bounds__(256) k_cgx_init(CGState S, size_t n, const double* __restrict__ f,
+                                                 double* __restrict__ r) {
+  __shared__ double red[4];
+  if (blockIdx.x == 0 && threadIdx.x == 0) S.flag[0] = 0;
+  double ff = 0.0;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const double fn = f[idx];
+    r[idx] = fn;
+    ff += fn * fn;
+  }
+  const double a = block_sum(ff, red);
+  if (threadIdx.x == 0) S.part[2 * blockIdx.x] = a;
+}
+
+// partials of r.z; on the first call also p = z
+__global__ void __launch_bounds__(256) k_cgx_rz(CGState S, size_t n, const double* __restrict__ r,
+                                               const double* __restrict__ z, double* __restrict__ p, int first) {
+  __shared__ double red[4];
+  if (S.flag[0]) return;
+  double rz = 0.0;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const double zn = z[idx];
+    rz += r[idx] * zn;
+    if (first) p[idx] = zn;
+  }
+  const double b = block_sum(rz, red);
+  if (threadIdx.x == 0) S.part[2 * blockIdx.x + 1] = b;
+}
+
+// x += alpha p, r -= alpha q; partials of r.r
+__global__ void __launch_bounds__(256) k_cgx_update(CGState S, size_t n, const double* __restrict__ p,
+                                                   const double* __restrict__ q, double* __restrict__ x,
+                                                   double* __restrict__ r) {
+  __shared__ double red[4];
+  if (S.flag[0]) return;
+  const double alpha = S.s[2];
+  double rr = 0.0;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    x[idx] += alpha * p[idx];
+    const double rn = r[idx] - alpha * q[idx];
+    r[idx] = rn;
+    rr += rn * rn;
+  }
+  const double a = block_sum(rr, red);
+  if (threadIdx.x == 0) S.part[2 * blockIdx.x] = a;
+}
+
+// iteration count and the convergence test of cg.F:247 on ||r||^2
+__global__ void k_cgx_test(CGState S) {
+  __shared__ double red[4];
+  if (S.flag[0]) return;
+  double a = 0.0;
+  for (int k = threadIdx.x; k < S.nblk; k += blockDim.x) a += S.part[2 * k];
+  const double rr = block_sum(a, red);
+  if (threadIdx.x == 0) {
+    const int it = S.flag[1] + 1;
+    S.flag[1] = it;
+    const double rel = rr / S.s[1];
+    S.s[4] = rel;
+    if (rel <= S.s[5]) {
+      S.flag[0] = 1;
+    } else if (it >= S.maxit) {
+      S.flag[0] = 1;
+      S.flag[2] = 1;
+    }
+  }
+}
+
+// beta = (r.z)_new / (r.z)_old
+__global__ void k_cgx_beta(CGState S) {
+  __shared__ double red[4];
+  if (S.flag[0]) return;
+  double b = 0.0;
+  for (int k = threadIdx.x; k < S.nblk; k += blockDim.x) b += S.part[2 * k + 1];
+  const double rz = block_sum(b, red);
+  if (threadIdx.x == 0) {
+    S.s[3] = rz / S.s[0];
+    S.s[0] = rz;
+  }
+}
+
 // one workgroup per system: the whole PCG loop in one launch (small systems;
 // vectors in global memory, reductions in LDS)
 __global__ void __launch_bounds__(1024) k_cg_single(DLap L, const double* __restrict__ f, double* __restrict__ x,
@@ -1401,6 +1487,33 @@ void launch_cg_iter(hipStream_t st, const DLap& L, const CGStateH& S, double* x,
   hipLaunchKernelGGL(k_cg_update, dim3(S.nblk), dim3(256), 0, st, L, D, p, q, x, r, z);
   hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k_cg_dir, dim3(S.nblk), dim3(256), 0, st, D, n, z, p);
+}
+
+void launch_cgx(hipStream_t st, const DLap& L, const CGStateH& S, int phase, const double* a, double* b, double* c,
+                double* d, double tol2) {
+  CGState D{S.s, S.flag, S.part, S.nblk, S.maxit};
+  const size_t n = (size_t)L.nx * L.ny;
+  const dim3 g(S.nblk), t(256);
+  switch (phase) {
+    case 0:   // r (b) = f (a), f.f
+      hipLaunchKernelGGL(k_cgx_init, g, t, 0, st, D, n, a, b);
+      break;
+    case 1:   // first r.z (r a, z b), p (c) = z, then the initial scalars
+      hipLaunchKernelGGL(k_cgx_rz, g, t, 0, st, D, n, a, b, c, 1);
+      hipLaunchKernelGGL(k_cg_init_fin, dim3(1), t, 0, st, D, tol2);
+      break;
+    case 2:   // q (b) = A p (a), alpha, x (c) and r (d) updated, convergence test
+      hipLaunchKernelGGL(k_cg_apply, g, t, 0, st, L, D, a, b);
+      hipLaunchKernelGGL(k_cg_alpha, dim3(1), t, 0, st, D);
+      hipLaunchKernelGGL(k_cgx_update, g, t, 0, st, D, n, a, b, c, d);
+      hipLaunchKernelGGL(k_cgx_test, dim3(1), t, 0, st, D);
+      break;
+    default:  // r.z (r a, z b), beta, p (c) = z + beta p
+      hipLaunchKernelGGL(k_cgx_rz, g, t, 0, st, D, n, a, b, c, 0);
+      hipLaunchKernelGGL(k_cgx_beta, dim3(1), t, 0, st, D);
+      hipLaunchKernelGGL(k_cg_dir, g, t, 0, st, D, n, b, c);
+      break;
+  }
 }
 
 void launch_axpy(hipStream_t st, size_t n, double a, const double* x, double* y) {

@@ -64,6 +64,12 @@ void launch_cg_single(hipStream_t st, const DLap& L, const double* f, double* x,
                       double* q, double tol2, int maxit, int x0_zero, int* out_it, double* out_rel);
 void launch_cg_init(hipStream_t st, const DLap& L, const CGStateH& S, const double* f, const double* x, double* r,
                     double* z, double* p, int x0_zero, double tol2);
+// PCG with a preconditioner applied by the caller between the phases:
+// 0 r = f (+ f.f); [z = M^-1 r]; 1 r.z, p = z, initial scalars; per
+// iteration: 2 q = A p, alpha, x and r, convergence test; [z = M^-1 r];
+// 3 r.z, beta, p = z + beta p
+void launch_cgx(hipStream_t st, const DLap& L, const CGStateH& S, int phase, const double* a, double* b, double* c,
+                double* d, double tol2);
 void launch_cg_iter(hipStream_t st, const DLap& L, const CGStateH& S, double* x, double* r, double* z, double* p,
                     double* q);
 void launch_axpy(hipStream_t st, size_t n, double a, const double* x, double* y);
@@ -74,7 +80,10 @@ struct FDMPlan {
   int nx, ny, nx0, ny0;
   const double *Vx, *VxT, *Vy, *VyT, *lx, *ly;
 };
-void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2);
+// dp = A^-1 f by fast diagonalization; the launches do nothing while *skip
+// (optional device flag) is set
+void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2,
+                        const int* skip = nullptr);
 
 // accelerated PCG (pcg2.hip): optional warm start and two-level preconditioner
 constexpr int kTLMaxCoarse = 2048;   // coarse unknowns solved in one workgroup

@@ -307,6 +307,48 @@ def test_ic0_pcg_matches_cg_matlab(name):
     solver.close()
 
 
+@pytest.mark.parametrize('name', ['ldc', 'outflow', 'channel', 'bfs'])
+def test_fdm_pcg_meets_the_plain_stopping_rule(name):
+    """PCG with the fast-diagonalization inverse as preconditioner
+    (cg_precond 3): the stopping rule of cg.F:247 at 1e-10 within two
+    iterations, the true residual within 1.5x of it, and the solution of the
+    oracle's sparse LU (modulo the constant where A is singular)."""
+    _say('fdm pcg', name)
+    def make(**kw):
+        if name == 'ldc':
+            return _wall_case('ldc', 40, 30, sx=1.03, **kw)
+        if name == 'outflow':
+            return _wall_case('outflow', 48, 20, sx=1.03, **kw)
+        if name == 'channel':
+            return _wall_case('channel', 36, 20, sy=1.04, **kw)
+        return load('BFS_unsteady', Nx=256, Ny=64, **kw)
+    case = make(poisson=3, CG_acc=1e-10, CG_maxit=5000, CG_precond='fdm')
+    opts = O.build(case)
+    solver, _, _, _ = host.setup(case)
+    rng = np.random.default_rng(9)
+    f = rng.standard_normal(opts.grid.Np)
+    singular = name in ('ldc', 'channel')
+    if singular:
+        f -= f.mean()
+    dp, it, rel = solver.poisson(f)
+    r_true = np.linalg.norm(opts.discretization.A @ dp - f) / np.linalg.norm(f)
+    dpo = O.pressure_poisson(f, 0.0, O.build(make(poisson=1)))
+    if singular:
+        dp, dpo = dp - dp.mean(), dpo - dpo.mean()
+    _say(name, 'iterations', it, 'relres', rel, 'true', r_true, 'vs LU', _rel(dp, dpo))
+    assert 1 <= it <= 2 and rel <= 1e-10 and r_true <= 1.5e-10
+    assert _rel(dp, dpo) < 1e-9
+    solver.close()
+
+
+def test_ldc_run_fdm_pcg_matches_oracle():
+    """LDC 20x20 fixture configuration, PCG with the fast-diagonalization
+    preconditioner on the GPU against the oracle's direct solve."""
+    _say('ldc 20x20 run fdm pcg')
+    case = load('LDC_unsteady', RK='RK33P2', poisson=3, CG_acc=1e-13, CG_maxit=2000, CG_precond='fdm')
+    _compare_runs(case, 30, True)
+
+
 def test_ldc_run_ic0_matches_oracle():
     """LDC 20x20 fixture configuration with the reference's cg_matlab
     preconditioner (poisson = 4, IC(0)) on both sides."""
