@@ -550,6 +550,50 @@ __global__ void __launch_bounds__(256) k_bf_update(const BInst* __restrict__ I, 
   if (threadIdx.x == 0 && threadIdx.y == 0) S.part[2 * blockIdx.x] = a;
 }
 
+// per instance: iteration count and the convergence test of cg.F:247 on
+// ||r||^2 (partials 2t), before the preconditioner is applied again
+__global__ void k_bf_test(const BInst* __restrict__ I, BCG S) {
+  __shared__ double red[4];
+  const int k = blockIdx.x;
+  if (S.f[4 * k]) return;
+  const BInst& B = I[k];
+  double a = 0.0;
+  for (int t = B.pt0 + threadIdx.x; t < B.pt1; t += blockDim.x) a += S.part[2 * t];
+  const double rr = block_sum(a, red);
+  if (threadIdx.x == 0) {
+    const int it = S.f[4 * k + 1] + 1;
+    S.f[4 * k + 1] = it;
+    const double rel = rr / S.s[8 * k + 1];
+    S.s[8 * k + 4] = rel;
+    int done = 0;
+    if (rel <= S.s[8 * k + 5]) {
+      done = 1;
+    } else if (it >= S.maxit) {
+      done = 1;
+      S.f[4 * k + 2] = 1;
+    }
+    if (done) {
+      S.f[4 * k + 0] = 1;
+      atomicSub(S.nactive, 1);
+    }
+  }
+}
+
+// per instance: beta = (r.z)_new / (r.z)_old (partials 2t + 1)
+__global__ void k_bf_beta(const BInst* __restrict__ I, BCG S) {
+  __shared__ double red[4];
+  const int k = blockIdx.x;
+  if (S.f[4 * k]) return;
+  const BInst& B = I[k];
+  double b = 0.0;
+  for (int t = B.pt0 + threadIdx.x; t < B.pt1; t += blockDim.x) b += S.part[2 * t + 1];
+  const double rz = block_sum(b, red);
+  if (threadIdx.x == 0) {
+    S.s[8 * k + 3] = rz / S.s[8 * k + 0];
+    S.s[8 * k + 0] = rz;
+  }
+}
+
 // first direction p = z (active instances)
 __global__ void __launch_bounds__(256) k_bf_dir0(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
                                                 const double* __restrict__ z, double* __restrict__ p) {
@@ -848,9 +892,12 @@ void cg_iter_launches(ins2d_batch* b) {
     hipLaunchKernelGGL(k_b_cg_alpha, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
     hipLaunchKernelGGL(k_bf_update, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q, b->x,
                        b->r);
+    // converged instances stop here: the preconditioner is applied only to
+    // those still iterating
+    hipLaunchKernelGGL(k_bf_test, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
     fdm_apply(b, b->r, b->z);
     hipLaunchKernelGGL(k_bf_rz, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->r, b->z);
-    hipLaunchKernelGGL(k_b_cg_beta, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
+    hipLaunchKernelGGL(k_bf_beta, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
     hipLaunchKernelGGL(k_b_cg_dir, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->z, b->pp);
     return;
   }
