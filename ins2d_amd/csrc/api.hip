@@ -160,8 +160,13 @@ void up_yv(ins2d_handle* h, const HYVec& Y, DYV& d) {
   for (int t = 0; t < d.nt; ++t) {
     d.ax[t] = dupload(h, ax[t], &h->yv_allocs);
     d.ay[t] = dupload(h, ay[t], &h->yv_allocs);
+    nonzero_range(ax[t].data(), ax[t].size(), d.x0[t], d.x1[t]);
+    nonzero_range(ay[t].data(), ay[t].size(), d.y0[t], d.y1[t]);
   }
-  for (int t = d.nt; t < 4; ++t) d.ax[t] = d.ay[t] = nullptr;
+  for (int t = d.nt; t < 4; ++t) {
+    d.ax[t] = d.ay[t] = nullptr;
+    d.x0[t] = d.x1[t] = d.y0[t] = d.y1[t] = 0;
+  }
 }
 
 void upload_comp_ops(ins2d_handle* h, const HComp& C, DComp& D) {

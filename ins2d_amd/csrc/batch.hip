@@ -756,10 +756,16 @@ void b_upload_bc(ins2d_batch* b) {
   size_t off = 0;
   for (auto& pe : pending) {
     DYV& dy = *pe.first;
-    for (int t = 0; t < 4; ++t) { dy.ax[t] = nullptr; dy.ay[t] = nullptr; }
+    for (int t = 0; t < 4; ++t) {
+      dy.ax[t] = nullptr;
+      dy.ay[t] = nullptr;
+      dy.x0[t] = dy.x1[t] = dy.y0[t] = dy.y1[t] = 0;
+    }
     for (int t = 0; t < dy.nt; ++t) {
       const auto& ax = pe.second[2 * t];
       const auto& ay = pe.second[2 * t + 1];
+      nonzero_range(ax.data(), ax.size(), dy.x0[t], dy.x1[t]);
+      nonzero_range(ay.data(), ay.size(), dy.y0[t], dy.y1[t]);
       std::copy(ax.begin(), ax.end(), hbuf.begin() + off);
       dy.ax[t] = dbuf + off;
       off += ax.size();

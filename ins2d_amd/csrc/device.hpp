@@ -20,12 +20,28 @@ struct DOp1 {
 struct DKron {
   DOp1 X, Y;
 };
-// y(f, g) = sum_t ax[t][f] * ay[t][g]
+// y(f, g) = sum_t ax[t][f] * ay[t][g]; term t is nonzero only on the box
+// [x0[t], x1[t]) x [y0[t], y1[t]) (the nonzero ranges of ax[t] and ay[t],
+// set when the vectors are uploaded), so the kernels skip its loads outside
 struct DYV {
   int nt, pad;
   const double* ax[4];
   const double* ay[4];
+  int x0[4], x1[4], y0[4], y1[4];
 };
+
+// [lo, hi): the range of the nonzero entries of v (empty: lo = hi = 0)
+inline void nonzero_range(const double* v, size_t n, int& lo, int& hi) {
+  lo = 0;
+  hi = 0;
+  size_t a = 0, b = n;
+  while (a < n && v[a] == 0.0) ++a;
+  while (b > a && v[b - 1] == 0.0) --b;
+  if (a < b) {
+    lo = (int)a;
+    hi = (int)b;
+  }
+}
 // D * [(I c + yI) .* (A own + yA)] * scale
 struct DConv {
   DKron D, I, A;

@@ -29,7 +29,7 @@ __device__ __forceinline__ double yv_at(const DYV& Y, int f, int g) {
   double s = 0.0;
 #pragma unroll
   for (int t = 0; t < 4; ++t)
-    if (t < Y.nt) s += Y.ax[t][f] * Y.ay[t][g];
+    if (t < Y.nt && f >= Y.x0[t] && f < Y.x1[t] && g >= Y.y0[t] && g < Y.y1[t]) s += Y.ax[t][f] * Y.ay[t][g];
   return s;
 }
 
