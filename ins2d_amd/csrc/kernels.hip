@@ -101,96 +101,15 @@ __device__ __forceinline__ void rhs_per2(const PerGrid& G, const double* __restr
   fv = -(duvdx + dv2dy) + G.invRe * d2v;
 }
 
-// fourth order (alfa = 81, beta = 9/8): alfa * (1h operators) - (3h operators)
-// (convection.m:38-43, operator_convection_diffusion.m:463-470)
-// accessors of u(i + a, j + b), v(i + a, j + b) for |a|, |b| <= 3
-// global memory (INTERIOR: the block's x range and its +-3 halo lie inside
-// [0, Nx), so x offsets need no wrap)
-template <bool INTERIOR>
-struct GlobalAcc4 {
-  const double* ur[7];
-  const double* vr[7];
-  int ix[7];
-  __device__ __forceinline__ GlobalAcc4(const PerGrid& G, const double* __restrict__ u, const double* __restrict__ v,
-                                        int i, int j) {
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-      const size_t row = (size_t)G.Nx * wrap(j + k - 3, G.Ny);
-      ur[k] = u + row;
-      vr[k] = v + row;
-      ix[k] = INTERIOR ? i + k - 3 : wrap(i + k - 3, G.Nx);
-    }
-  }
-  __device__ __forceinline__ double U(int a, int b) const { return ur[b + 3][ix[a + 3]]; }
-  __device__ __forceinline__ double V(int a, int b) const { return vr[b + 3][ix[a + 3]]; }
-};
-// rows j-3 .. j+3 staged in LDS (pointers already at this thread's column)
+// fourth order: accessor of u(i + a, j + b), v(i + a, j + b), |a|, |b| <= 3,
+// from the rows j-3 .. j+3 staged in LDS (pointers already at this thread's
+// column)
 struct RingAcc4 {
   const double* ur[7];
   const double* vr[7];
   __device__ __forceinline__ double U(int a, int b) const { return ur[b + 3][a]; }
   __device__ __forceinline__ double V(int a, int b) const { return vr[b + 3][a]; }
 };
-
-template <class ACC>
-__device__ __forceinline__ void rhs4_core(const PerGrid& G, const ACC& acc, double& fu, double& fv) {
-  const double hx = G.hx, hy = G.hy;
-  const double w1 = 9.0 / 16.0, w2 = -1.0 / 16.0;   // beta/2, (1-beta)/2
-  const double r72x = G.rhx * (1.0 / 72.0), r72y = G.rhy * (1.0 / 72.0);
-#define U(a, b) acc.U(a, b)
-#define V(a, b) acc.V(a, b)
-  // ---------------- u-momentum ----------------
-  // x: 1h faces f = i (between u_{i-1}, u_i) and i+1; 3h faces at i-3/2, i+3/2
-  auto Ix1 = [&](int f) { return hy * (w2 * U(f - 2, 0) + w1 * U(f - 1, 0) + w1 * U(f, 0) + w2 * U(f + 1, 0)); };
-  auto Ax1 = [&](int f) { return 0.5 * (U(f - 1, 0) + U(f, 0)); };
-  const double du2dx1 = Ix1(1) * Ax1(1) - Ix1(0) * Ax1(0);
-  auto Ix3 = [&](int r) { return 3.0 * hy * (w2 * U(r - 3, 0) + w1 * U(r - 2, 0) + w1 * U(r - 1, 0) + w2 * U(r, 0)); };
-  auto Ax3 = [&](int r) { return 0.5 * (U(r - 3, 0) + U(r, 0)); };
-  const double du2dx3 = Ix3(3) * Ax3(3) - Ix3(0) * Ax3(0);
-  // y: 1h faces g = j, j+1 (grid lines y_j, y_{j+1}); v interpolated in x
-  auto Vf1 = [&](int g) { return hx * (w2 * V(-2, g) + w1 * V(-1, g) + w1 * V(0, g) + w2 * V(1, g)); };
-  auto Ay1 = [&](int g) { return 0.5 * (U(0, g - 1) + U(0, g)); };
-  const double duvdy1 = Vf1(1) * Ay1(1) - Vf1(0) * Ay1(0);
-  // 3h faces g = j, j+3 on grid lines y_{g-1}
-  auto Vf3 = [&](int g) { return 3.0 * hx * (w2 * V(-2, g - 1) + w1 * V(-1, g - 1) + w1 * V(0, g - 1) + w2 * V(1, g - 1)); };
-  auto Ay3 = [&](int g) { return 0.5 * (U(0, g - 3) + U(0, g)); };
-  const double duvdy3 = Vf3(3) * Ay3(3) - Vf3(0) * Ay3(0);
-  const double convu = 81.0 * (du2dx1 + duvdy1) - (du2dx3 + duvdy3);
-  // diffusion: Phi_{k+1/2} = [81 (u_{k+1}-u_k) - 3 (u_{k+2}-u_{k-1})] / (72 h)
-  auto Px = [&](int k) { return (81.0 * (U(k + 1, 0) - U(k, 0)) - 3.0 * (U(k + 2, 0) - U(k - 1, 0))) * r72x; };
-  auto Py = [&](int k) { return (81.0 * (U(0, k + 1) - U(0, k)) - 3.0 * (U(0, k + 2) - U(0, k - 1))) * r72y; };
-  const double d2u = hy * (81.0 * (Px(0) - Px(-1)) - 3.0 * (Px(1) - Px(-2))) +
-                     hx * (81.0 * (Py(0) - Py(-1)) - 3.0 * (Py(1) - Py(-2)));
-  fu = -convu + G.invRe * d2u;
-  // ---------------- v-momentum ----------------
-  auto Uf1 = [&](int f) { return hy * (w2 * U(f, -2) + w1 * U(f, -1) + w1 * U(f, 0) + w2 * U(f, 1)); };
-  auto Bx1 = [&](int f) { return 0.5 * (V(f - 1, 0) + V(f, 0)); };
-  const double duvdx1 = Uf1(1) * Bx1(1) - Uf1(0) * Bx1(0);
-  auto Uf3 = [&](int f) { return 3.0 * hy * (w2 * U(f - 1, -2) + w1 * U(f - 1, -1) + w1 * U(f - 1, 0) + w2 * U(f - 1, 1)); };
-  auto Bx3 = [&](int f) { return 0.5 * (V(f - 3, 0) + V(f, 0)); };
-  const double duvdx3 = Uf3(3) * Bx3(3) - Uf3(0) * Bx3(0);
-  auto Iy1 = [&](int g) { return hx * (w2 * V(0, g - 2) + w1 * V(0, g - 1) + w1 * V(0, g) + w2 * V(0, g + 1)); };
-  auto By1 = [&](int g) { return 0.5 * (V(0, g - 1) + V(0, g)); };
-  const double dv2dy1 = Iy1(1) * By1(1) - Iy1(0) * By1(0);
-  auto Iy3 = [&](int g) { return 3.0 * hx * (w2 * V(0, g - 3) + w1 * V(0, g - 2) + w1 * V(0, g - 1) + w2 * V(0, g)); };
-  auto By3 = [&](int g) { return 0.5 * (V(0, g - 3) + V(0, g)); };
-  const double dv2dy3 = Iy3(3) * By3(3) - Iy3(0) * By3(0);
-  const double convv = 81.0 * (duvdx1 + dv2dy1) - (duvdx3 + dv2dy3);
-  auto Qx = [&](int k) { return (81.0 * (V(k + 1, 0) - V(k, 0)) - 3.0 * (V(k + 2, 0) - V(k - 1, 0))) * r72x; };
-  auto Qy = [&](int k) { return (81.0 * (V(0, k + 1) - V(0, k)) - 3.0 * (V(0, k + 2) - V(0, k - 1))) * r72y; };
-  const double d2v = hy * (81.0 * (Qx(0) - Qx(-1)) - 3.0 * (Qx(1) - Qx(-2))) +
-                     hx * (81.0 * (Qy(0) - Qy(-1)) - 3.0 * (Qy(1) - Qy(-2)));
-  fv = -convv + G.invRe * d2v;
-#undef U
-#undef V
-}
-
-template <bool INTERIOR>
-__device__ __forceinline__ void rhs_per4(const PerGrid& G, const double* __restrict__ u, const double* __restrict__ v,
-                                         int i, int j, double& fu, double& fv) {
-  GlobalAcc4<INTERIOR> acc(G, u, v, i, j);
-  rhs4_core(G, acc, fu, fv);
-}
 
 template <int ORDER>
 __device__ __forceinline__ void grad_per(const PerGrid& G, const double* __restrict__ p, int i, int j, double& gx,
@@ -211,21 +130,17 @@ __device__ __forceinline__ void grad_per(const PerGrid& G, const double* __restr
   }
 }
 
+// second order, one thread per cell (fourth order: k_rhs_per4s)
 template <int ORDER>
 __global__ void __launch_bounds__(256) k_rhs_per(PerGrid G, const double* __restrict__ u, const double* __restrict__ v,
                                                 const double* __restrict__ p, int mode, double* __restrict__ out,
                                                 StageCombine sc, double* __restrict__ vtemp) {
+  static_assert(ORDER == 2, "the fourth-order right-hand side is k_rhs_per4s");
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= G.Nx || j >= G.Ny) return;
   double fu, fv;
-  if (ORDER == 2) {
-    rhs_per2(G, u, v, i, j, fu, fv);
-  } else {
-    const int x0 = blockIdx.x * blockDim.x;
-    if (x0 >= 3 && x0 + (int)blockDim.x + 3 <= G.Nx) rhs_per4<true>(G, u, v, i, j, fu, fv);
-    else rhs_per4<false>(G, u, v, i, j, fu, fv);
-  }
+  rhs_per2(G, u, v, i, j, fu, fv);
   if (mode & RHS_WITH_P) {
     double gx, gy;
     grad_per<ORDER>(G, p, i, j, gx, gy);
@@ -247,15 +162,15 @@ __global__ void __launch_bounds__(256) k_rhs_per(PerGrid G, const double* __rest
   }
 }
 
-// fourth order, row-streaming: one wave per 64-wide column strip of H
-// rows; the rows j-3 .. j+3 of u and v live in an LDS ring of 7 rows (+-3
-// halo in x), so every value is read from memory once per strip (plus the
-// 6-row start-up and the 6-column halo).  The y-direction face fluxes are
-// streamed: each row computes only the fluxes of its newest faces (1h faces
-// j+1, 3h faces j+3, diffusive faces j+3/2) and takes the older ones from
-// registers, where the rows above left them; the x-direction terms are
-// computed per row as in rhs4_core.  Every flux is the same expression as in
-// rhs4_core.
+// fourth order (alfa = 81, beta = 9/8: alfa * (1h operators) - (3h
+// operators), convection.m:38-43, operator_convection_diffusion.m:463-470),
+// row-streaming: one wave per 64-wide column strip of H rows; the rows
+// j-3 .. j+3 of u and v live in an LDS ring of 7 rows (+-3 halo in x), so
+// every value is read from memory once per strip (plus the 6-row start-up and
+// the 6-column halo).  The y-direction face fluxes are streamed: each row
+// computes only the fluxes of its newest faces (1h faces j+1, 3h faces j+3,
+// diffusive faces j+3/2) and takes the older ones from registers, where the
+// rows above left them; the x-direction terms are computed per row.
 constexpr int kStripW = 64;
 __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __restrict__ u,
                                                  const double* __restrict__ v, const double* __restrict__ p, int mode,
@@ -413,9 +328,11 @@ __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __res
 #undef V
 }
 
+// second order (fourth order: k_div_per4s)
 template <int ORDER>
 __global__ void __launch_bounds__(256) k_div_per(PerGrid G, const double* __restrict__ A, const double* __restrict__ B,
                                                 double ca, double cb, double oscale, double* __restrict__ f) {
+  static_assert(ORDER == 2, "the fourth-order divergence is k_div_per4s");
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= G.Nx || j >= G.Ny) return;
@@ -424,15 +341,7 @@ __global__ void __launch_bounds__(256) k_div_per(PerGrid G, const double* __rest
   auto W = [&](size_t k) { return B ? ca * A[k] + cb * B[k] : ca * A[k]; };
   const int ip = wrap(i + 1, Nx), jp = wrap(j + 1, Ny);
   const size_t r0 = (size_t)Nx * j;
-  double s;
-  if (ORDER == 2) {
-    s = G.hy * (W(ip + r0) - W(i + r0)) + G.hx * (W(N + i + (size_t)Nx * jp) - W(N + i + r0));
-  } else {
-    const int ipp = wrap(i + 2, Nx), im = wrap(i - 1, Nx), jpp = wrap(j + 2, Ny), jm = wrap(j - 1, Ny);
-    s = 81.0 * G.hy * (W(ip + r0) - W(i + r0)) - 3.0 * G.hy * (W(ipp + r0) - W(im + r0)) +
-        81.0 * G.hx * (W(N + i + (size_t)Nx * jp) - W(N + i + r0)) -
-        3.0 * G.hx * (W(N + i + (size_t)Nx * jpp) - W(N + i + (size_t)Nx * jm));
-  }
+  const double s = G.hy * (W(ip + r0) - W(i + r0)) + G.hx * (W(N + i + (size_t)Nx * jp) - W(N + i + r0));
   f[(size_t)i + r0] = oscale * s;
 }
 
