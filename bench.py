@@ -86,10 +86,18 @@ def cpu_child(case_json, steps):
     t_setup = time.perf_counter() - t0
     progress(f'cpu oracle setup {t_setup:.1f} s')
     dt = opts.time.dt
+    # at least `steps` steps and, for fast cases, on until about 10 s of CPU
+    # work (a sample long enough to time)
     t0 = time.perf_counter()
-    for n in range(steps):
+    t_print = t0
+    n = 0
+    while n < steps or (time.perf_counter() - t0 < 10.0 and n < 200 * steps):
         V, p = O.time_ERK(V, p, n * dt, dt, opts)
-        progress(f'cpu oracle step {n + 1}/{steps}')
+        n += 1
+        if time.perf_counter() - t_print >= 1.0 or n == steps:
+            progress(f'cpu oracle step {n}')
+            t_print = time.perf_counter()
+    steps = n
     el = time.perf_counter() - t0
     stages = len(O.get_rk_method(opts.time.RK)[1])
     cells = opts.grid.Np
