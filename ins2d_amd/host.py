@@ -402,33 +402,107 @@ def setup(case, **overrides):
     return solver, opts, V0, p0
 
 
-def main(case_name, nsteps=None, diagnostics=True, **overrides):
+def main(case_name, nsteps=None, diagnostics=True, path_results=None, restart_n=0, snapshots=False,
+         restart_from=None, **overrides):
     """main.m + solver_unsteady.m: returns (V, p, rows) where rows are the
-    convergence.txt records (n, dt, t, maxres, maxdiv, umom, vmom, k)."""
+    convergence.txt records (n, dt, t, maxres, maxdiv, umom, vmom, k).
+
+    Output as the reference writes it (options.output, restart.*), when
+    path_results is given: convergence.txt (create_files.m:54-56,
+    solver_unsteady.m:4-8, process_iteration.m:112-119), a restart file every
+    restart_n steps (process_iteration.m:56-57, write_restart.m) and, with
+    snapshots, snapshots/Vp_n=<n>.mat every step (process_iteration.m:78-86,
+    save_eco = 1).  restart_from: a restart file to continue from (its V, p,
+    n, t and dt replace the initial condition, as main.m with restart.load =
+    1; input_check is not repeated)."""
+    import os
     solver, opts, V0, p0 = setup(case_name, **overrides)
     prm = opts.prm
     t = opts.t_start
     dt = opts.dt
+    n0 = 1
+    if restart_from is not None:
+        r = read_restart(restart_from)
+        V0 = np.concatenate([r['uh'], r['vh']])
+        p0 = r['p']
+        n0, t, dt = int(r['n']), float(r['t']), float(r['dt'])
     solver.set_state(V0, p0)
-    solver.input_check(t, int(prm.get('p_initial', 1)))
-    rows = [(1, dt, t) + solver.diagnostics(t)] if diagnostics else []
+    if restart_from is None:
+        solver.input_check(t, int(prm.get('p_initial', 1)))
+    rows = [(n0, dt, t) + solver.diagnostics(t)] if diagnostics else []
     nt = opts.nt if nsteps is None else nsteps
-    for n in range(2, nt + 2):
+    name = case_name if isinstance(case_name, str) else getattr(case_name, 'name', 'case')
+    nu = solver.Nu
+    for n in range(n0 + 1, n0 + nt + 1):
         solver.step(t, dt, 1)
         t = t + dt
         if diagnostics:
             rows.append((n, dt, t) + solver.diagnostics(t))
+        if path_results and (restart_n or snapshots):
+            V, p = solver.get_state()
+            if restart_n and n % restart_n == 0:
+                write_restart(path_results, n, t, dt, V, p, nu)
+            if snapshots:
+                write_snapshot(path_results, n, V, p)
     V, p = solver.get_state()
     solver.close()
+    if path_results and diagnostics:
+        os.makedirs(path_results, exist_ok=True)
+        g = opts.grid
+        write_convergence(os.path.join(path_results, 'convergence.txt'), name, g.Nx, g.Ny, float(prm['Re']), rows)
     return V, p, rows
 
 
 def write_convergence(path, case_name, Nx, Ny, Re, rows):
-    """convergence.txt in the layout of create_files.m and
-    solver_unsteady.m:4-8 / process_iteration.m:112-119."""
+    """convergence.txt in the layout of create_files.m:54-56 (case, Nx, Ny,
+    num2str(Re, '%10.3e')), the column header and first record of
+    solver_unsteady.m:4-8 (no trailing blank) and the records of
+    process_iteration.m:112-119 (a trailing blank)."""
     with open(path, 'w') as fh:
-        fh.write(f'{case_name} {Nx} {Ny} {Re:.3e}\n')
+        # num2str(Re, '%10.3e') drops the padding blanks of the format
+        fh.write(f'{case_name} {Nx} {Ny} {("%10.3e" % Re).strip()}\n')
         fh.write('residual and conservation information\n')
         fh.write('n            dt               t                res              maxdiv           umom             vmom             k\n')
-        for r in rows:
-            fh.write('%-10i %16.8e %16.8e %16.8e %16.8e %16.8e %16.8e %16.8e \n' % tuple(r))
+        for q, r in enumerate(rows):
+            line = '%-10i %16.8e %16.8e %16.8e %16.8e %16.8e %16.8e %16.8e' % tuple(r)
+            fh.write(line + ('\n' if q == 0 else ' \n'))
+
+
+def _numbered(n, filelen=8):
+    # process_iteration.m:103-104: n left-padded with zeros to filelen digits
+    s = str(int(n))
+    return '0' * max(0, filelen - len(s)) + s
+
+
+def write_restart(path_results, n, t, dt, V, p, nu, filelen=8):
+    """restart_<n>.mat as write_restart.m: uh, vh, p (columns), n, t, dt
+    (MAT-file version 5, readable by MATLAB's load)."""
+    import os
+    from scipy.io import savemat
+    os.makedirs(path_results, exist_ok=True)
+    path = os.path.join(path_results, f'restart_{_numbered(n, filelen)}.mat')
+    savemat(path, {'uh': np.asarray(V[:nu], dtype=np.float64), 'vh': np.asarray(V[nu:], dtype=np.float64),
+                   'p': np.asarray(p, dtype=np.float64), 'n': float(n), 't': float(t), 'dt': float(dt)},
+            oned_as='column')
+    return path
+
+
+def read_restart(path):
+    """The variables of a restart file (write_restart), flattened."""
+    from scipy.io import loadmat
+    m = loadmat(path)
+    out = {k: np.asarray(m[k], dtype=np.float64).ravel() for k in ('uh', 'vh', 'p')}
+    for k in ('n', 't', 'dt'):
+        out[k] = float(np.asarray(m[k]).ravel()[0])
+    return out
+
+
+def write_snapshot(path_results, n, V, p):
+    """snapshots/Vp_n=<n>.mat with V and p (process_iteration.m:84-86)."""
+    import os
+    from scipy.io import savemat
+    d = os.path.join(path_results, 'snapshots')
+    os.makedirs(d, exist_ok=True)
+    path = os.path.join(d, f'Vp_n={int(n)}.mat')
+    savemat(path, {'V': np.asarray(V, dtype=np.float64), 'p': np.asarray(p, dtype=np.float64)}, oned_as='column')
+    return path

@@ -349,6 +349,26 @@ def test_ldc_run_fdm_pcg_matches_oracle():
     _compare_runs(case, 30, True)
 
 
+def test_ldc_convergence_file_and_restart(tmp_path):
+    """host.main writes convergence.txt in the reference's layout: against
+    the LDC 20x20 fixture (the test_outputs comparison), and a run resumed
+    from a restart file (restart_n, restart_from) ends bit-identical to the
+    run that went straight through."""
+    from test_outputs import compare_with_fixture
+    _say('ldc 20x20 convergence.txt and restart')
+    case = load('LDC_unsteady', RK='RK33P2', poisson=1)
+    d = str(tmp_path / 'run')
+    host.main(case, nsteps=100, path_results=d)
+    compare_with_fixture(os.path.join(d, 'convergence.txt'), slack=True)
+    d2 = str(tmp_path / 'restart')
+    V1, p1, _ = host.main(case, nsteps=9, path_results=d2, restart_n=5, snapshots=True)
+    assert os.path.exists(os.path.join(d2, 'restart_00000005.mat'))
+    assert os.path.exists(os.path.join(d2, 'snapshots', 'Vp_n=10.mat'))
+    V2, p2, rows = host.main(case, nsteps=5, restart_from=os.path.join(d2, 'restart_00000005.mat'))
+    assert rows[-1][0] == 10
+    assert np.array_equal(V1, V2) and np.array_equal(p1, p2)
+
+
 def test_ldc_run_ic0_matches_oracle():
     """LDC 20x20 fixture configuration with the reference's cg_matlab
     preconditioner (poisson = 4, IC(0)) on both sides."""
