@@ -67,7 +67,11 @@ typedef struct {
                                 assembled A = M Om^-1 G                         */
   double cg_tol;             /* relative residual ||r||/||f|| (CG_acc)          */
   int cg_maxit;              /* CG_maxit                                        */
-  int p_add_solve;           /* 1: additional pressure solve per step           */
+  int p_add_solve;           /* time-dependent boundary values: 1 additional
+                                pressure solve per step, 0 the last stage
+                                pressure (time_ERK.m:145-158); steady boundary
+                                values always take the additional solve
+                                (time_ERK.m:160-163)                            */
   int kernels;               /* 0: auto, 1: force generic kron-factored kernels */
   int cg_precond;            /* PCG modes: 0 Jacobi; 1 IC(0) as the reference
                                 (make_cholinc.m, cg_matlab.m:135-198), one
@@ -94,7 +98,9 @@ int ins2d_grid_info(const ins2d_handle* h, int* out, double* xin, double* yin);
 /* boundary samples at time t, in the order set_bc_vectors.m:53-71 takes them:
  * uLo, uUp (x, len Nx+1), uLo_i, uUp_i (xin), uLe_i, uRi_i (yp, len Ny),
  * vLe, vRi (y, len Ny+1), vLo_i, vUp_i (xp, len Nx), vLe_i, vRi_i (yin).
- * Replaces set_bc_vectors.m (steady boundary conditions). */
+ * Replaces set_bc_vectors.m; with time-dependent boundary values
+ * (BC_unsteady = 1) the caller sets them at the time it then evaluates:
+ * ins2d_rhs, ins2d_input_check and ins2d_diagnostics use the last set. */
 int ins2d_set_bc(ins2d_handle* h, double t, const double* const* samples);
 
 /* state transfer (host arrays, reference layout) */
@@ -113,8 +119,32 @@ int ins2d_poisson(ins2d_handle* h, const double* f, double* dp, int* iters, doub
 int ins2d_input_check(ins2d_handle* h, double t, int p_initial);
 
 /* nsteps of time_ERK.m:1-173 (with pressure_additional_solve.m at the end of
- * each step) on the resident state, t = tn, tn + dt, ... */
+ * each step) on the resident state, t = tn, tn + dt, ... (steady boundary
+ * values: those of the last ins2d_set_bc) */
 int ins2d_step(ins2d_handle* h, double tn, double dt, int nsteps);
+
+/* time derivatives of the boundary values at the time of the last
+ * ins2d_set_bc, in the order set_bc_vectors.m:73-78 takes them: dudtLe_i,
+ * dudtRi_i (yp, len Ny), dvdtLo_i, dvdtUp_i (xp, len Nx).  They make ydM,
+ * the boundary term of the additional pressure solve of ins2d_input_check
+ * (pressure_additional_solve.m:8-21, set_bc_vectors.m:120-146); NULL clears
+ * them (ydM = 0, as for steady boundary values). */
+int ins2d_set_bc_rate(ins2d_handle* h, const double* const* rates);
+
+/* nsteps of time_ERK.m:1-173 with time-dependent boundary values
+ * (BC_unsteady = 1) on the resident state, step n from tn + n dt.  samples:
+ * per step s + 1 sample sets, each the 12 arrays of ins2d_set_bc concatenated
+ * in that order, at tn + n dt (time_ERK.m:62-64) and at the stage times
+ * tn + (n + c_i) dt, i = 2..s, then tn + (n + 1) dt (time_ERK.m:106-109; c_i
+ * from rk_c).  rates: per step the 4 arrays of ins2d_set_bc_rate at
+ * tn + (n + 1) dt.  Each stage evaluates F with the boundary values at its
+ * start (F.m:27-29) and takes a pressure solve (time_ERK.m:121-123); each step
+ * ends with the additional solve with ydM (p_add_solve = 1) or the last stage
+ * pressure (time_ERK.m:145-158).  Returns with the boundary state of
+ * tn + nsteps dt set as by ins2d_set_bc and ins2d_set_bc_rate.  Needs a case
+ * with boundaries (INS2D_E_UNSUPPORTED on the fully periodic kernels). */
+int ins2d_step_unsteady(ins2d_handle* h, double tn, double dt, int nsteps, const double* samples,
+                        const double* rates);
 
 /* check_conservation.m:1-76 + the extra F of process_iteration.m:22 on the
  * resident state: out = {maxres, maxdiv, umom, vmom, k} */

@@ -36,6 +36,8 @@ SIGNATURES = {
     "ins2d_poisson": (c_int, [c_void_p, P_double, P_double, P_int, P_double]),
     "ins2d_input_check": (c_int, [c_void_p, c_double, c_int]),
     "ins2d_step": (c_int, [c_void_p, c_double, c_double, c_int]),
+    "ins2d_set_bc_rate": (c_int, [c_void_p, ctypes.POINTER(P_double)]),
+    "ins2d_step_unsteady": (c_int, [c_void_p, c_double, c_double, c_int, P_double, P_double]),
     "ins2d_diagnostics": (c_int, [c_void_p, c_double, P_double]),
     "ins2d_solver_stats": (c_int, [c_void_p, P_int, P_double, ctypes.POINTER(ctypes.c_longlong)]),
     "ins2d_profile": (c_int, [c_void_p, c_int]),

@@ -10,6 +10,8 @@ initialize.m:6; mesh_generation.m:7):
 ``IC(t, options)``          -> (u, v, p) 2-D arrays on the u/v/p points
 ``uBC(x, y, t, options)``   -> u boundary values, length len(x)*len(y)
 ``vBC(x, y, t, options)``   -> v boundary values, length len(x)*len(y)
+``dudtBC``, ``dvdtBC``      -> their time derivatives (cases with
+                               BC_unsteady = 1; set_bc_vectors.m:73-79)
 
 ``load(name, **overrides)`` returns a case object whose ``parameters()``
 includes the overrides.
@@ -28,4 +30,5 @@ def load(name, **overrides):
         return p
 
     return SimpleNamespace(name=name, parameters=parameters, BCtype=mod.BCtype, mesh=mod.mesh,
-                           IC=mod.IC, uBC=mod.uBC, vBC=mod.vBC, module=mod)
+                           IC=mod.IC, uBC=mod.uBC, vBC=mod.vBC, dudtBC=getattr(mod, 'dudtBC', None),
+                           dvdtBC=getattr(mod, 'dvdtBC', None), module=mod)

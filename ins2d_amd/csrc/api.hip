@@ -102,6 +102,23 @@ struct ins2d_handle {
   // boundary samples (for the momentum/energy boundary terms)
   BCSamples samp;
   bool have_bc = false;
+  // time derivatives of the boundary samples (dudtLe_i, dudtRi_i, dvdtLo_i,
+  // dvdtUp_i) and the operator records whose yM holds ydM, built from them
+  std::vector<double> rate[4];
+  bool have_rate = false;
+  DComp *dCu_rate = nullptr, *dCv_rate = nullptr;
+  std::vector<void*> rate_allocs;
+  // boundary vectors of time-dependent boundary values, one version per
+  // stage time of a chunk of steps: all arrays of all versions in one device
+  // block (filled from pinned staging by one copy), and per version a (u, v)
+  // pair of operator records pointing into it; a stage selects its version by
+  // pointing dCu/dCv at that pair, so the step allocates nothing
+  size_t ver_len = 0;
+  int ver_cap = 0;
+  double* ver_dev = nullptr;
+  double* ver_pin = nullptr;
+  DComp* ver_comp = nullptr;
+  hipEvent_t ver_ev = nullptr;
   // stats
   int last_it = 0;
   double last_rel = 0.0;
@@ -275,6 +292,133 @@ void prof_collect(ins2d_handle* h) {
     h->ev_free.push_back(pe.b);
   }
   h->pend.clear();
+}
+
+// the samples with the four divergence-side sources replaced by their time
+// derivatives and all others zero: the yM of these samples is ydM
+// (set_bc_vectors.m:73-79,120-146)
+BCSamples rate_samples(const ins2d_handle* h, const double* const* rate) {
+  BCSamples S;
+  for (int q = 0; q < S_NSRC; ++q) S.s[q].assign(h->samp.s[q].size(), 0.0);
+  const int src[4] = {S_uLe_i, S_uRi_i, S_vLo_i, S_vUp_i};
+  for (int q = 0; q < 4; ++q) std::copy(rate[q], rate[q] + S.s[src[q]].size(), S.s[src[q]].begin());
+  return S;
+}
+
+// operator records whose yM is ydM of the stored rates (regular allocation)
+void upload_rate(ins2d_handle* h) {
+  for (void* p : h->rate_allocs) (void)hipFree(p);
+  h->rate_allocs.clear();
+  if (!h->plan) return;
+  if (!h->dCu_rate) {
+    h->dCu_rate = dalloc<DComp>(h, 1);
+    h->dCv_rate = dalloc<DComp>(h, 1);
+  }
+  const double* r[4] = {h->rate[0].data(), h->rate[1].data(), h->rate[2].data(), h->rate[3].data()};
+  const BCSamples S = rate_samples(h, r);
+  DComp cu = h->hCu, cv = h->hCv;
+  for (int c = 0; c < 2; ++c) {
+    std::vector<std::vector<double>> ax, ay;
+    eval_yvec(c ? h->plan->cv.yM : h->plan->cu.yM, S, ax, ay);
+    DYV& d = c ? cv.yM : cu.yM;
+    for (int t = 0; t < d.nt; ++t) {
+      d.ax[t] = dupload(h, ax[t], &h->rate_allocs);
+      d.ay[t] = dupload(h, ay[t], &h->rate_allocs);
+      nonzero_range(ax[t].data(), ax[t].size(), d.x0[t], d.x1[t]);
+      nonzero_range(ay[t].data(), ay[t].size(), d.y0[t], d.y1[t]);
+    }
+  }
+  HIPCHK(hipMemcpy(h->dCu_rate, &cu, sizeof(DComp), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->dCv_rate, &cv, sizeof(DComp), hipMemcpyHostToDevice));
+}
+
+// writes the arrays of one boundary vector at hb + off (when hb is given) and
+// points d at the same offsets from db; the boxes span the whole arrays, as
+// the values change from version to version
+size_t put_yv(const HYVec& Y, const BCSamples& S, double* hb, double* db, size_t off, DYV& d) {
+  std::vector<std::vector<double>> ax, ay;
+  eval_yvec(Y, S, ax, ay);
+  if (ax.size() > 4) throw std::runtime_error("too many boundary terms");
+  d.nt = (int)ax.size();
+  for (int t = 0; t < 4; ++t) {
+    if (t >= d.nt) {
+      d.ax[t] = d.ay[t] = nullptr;
+      d.x0[t] = d.x1[t] = d.y0[t] = d.y1[t] = 0;
+      continue;
+    }
+    if (hb) std::copy(ax[t].begin(), ax[t].end(), hb + off);
+    d.ax[t] = db + off;
+    d.x0[t] = 0;
+    d.x1[t] = (int)ax[t].size();
+    off += ax[t].size();
+    if (hb) std::copy(ay[t].begin(), ay[t].end(), hb + off);
+    d.ay[t] = db + off;
+    d.y0[t] = 0;
+    d.y1[t] = (int)ay[t].size();
+    off += ay[t].size();
+  }
+  return off;
+}
+
+size_t put_comp(const HComp& C, const BCSamples& S, double* hb, double* db, size_t off, DComp& D) {
+  for (int t = 0; t < D.nconv; ++t) {
+    off = put_yv(C.conv[t].yI, S, hb, db, off, D.conv[t].yI);
+    off = put_yv(C.conv[t].yA, S, hb, db, off, D.conv[t].yA);
+  }
+  for (int t = 0; t < D.ndiff; ++t) off = put_yv(C.diff[t].yS, S, hb, db, off, D.diff[t].yS);
+  return put_yv(C.yM, S, hb, db, off, D.yM);
+}
+
+// room for n versions; the operator records of every slot are written once
+// here (they depend on the slot, not on the values)
+void ver_reserve(ins2d_handle* h, int n) {
+  if (n <= h->ver_cap) return;
+  HIPCHK(hipStreamSynchronize(h->st));
+  const Plan& P = *h->plan;
+  if (!h->ver_len) {
+    DComp cu = h->hCu, cv = h->hCv;
+    h->ver_len = put_comp(P.cv, h->samp, nullptr, nullptr, put_comp(P.cu, h->samp, nullptr, nullptr, 0, cu), cv);
+    HIPCHK(hipEventCreateWithFlags(&h->ver_ev, hipEventDisableTiming));
+  }
+  if (h->ver_dev) {
+    (void)hipFree(h->ver_dev);
+    (void)hipFree(h->ver_comp);
+    (void)hipHostFree(h->ver_pin);
+    h->ver_dev = h->ver_pin = nullptr;
+    h->ver_comp = nullptr;
+  }
+  const size_t len = std::max<size_t>(h->ver_len, 1);
+  HIPCHK(hipMalloc(&h->ver_dev, sizeof(double) * len * n));
+  HIPCHK(hipMalloc(&h->ver_comp, sizeof(DComp) * 2 * n));
+  HIPCHK(hipHostMalloc(&h->ver_pin, sizeof(double) * len * n, hipHostMallocDefault));
+  h->ver_cap = n;
+  std::vector<DComp> recs(2 * (size_t)n);
+  for (int v = 0; v < n; ++v) {
+    DComp cu = h->hCu, cv = h->hCv;
+    double* db = h->ver_dev + (size_t)v * len;
+    put_comp(P.cv, h->samp, nullptr, db, put_comp(P.cu, h->samp, nullptr, db, 0, cu), cv);
+    recs[2 * v] = cu;
+    recs[2 * v + 1] = cv;
+  }
+  HIPCHK(hipMemcpy(h->ver_comp, recs.data(), sizeof(DComp) * recs.size(), hipMemcpyHostToDevice));
+}
+
+// the values of version v from samples S (staged; copied by the caller)
+void ver_fill(ins2d_handle* h, int v, const BCSamples& S) {
+  const Plan& P = *h->plan;
+  const size_t len = std::max<size_t>(h->ver_len, 1);
+  double* hb = h->ver_pin + (size_t)v * len;
+  DComp cu = h->hCu, cv = h->hCv;
+  put_comp(P.cv, S, hb, hb, put_comp(P.cu, S, hb, hb, 0, cu), cv);
+}
+
+// one sample set of the C ABI's layout (the 12 arrays concatenated)
+void unpack_samples(const ins2d_handle* h, const double* src, BCSamples& S) {
+  for (int q = 0; q < S_NSRC; ++q) {
+    const size_t n = h->samp.s[q].size();
+    S.s[q].assign(src, src + n);
+    src += n;
+  }
 }
 
 // ------------------------------------------------------------------ kernels
@@ -467,19 +611,38 @@ void poisson(ins2d_handle* h, const double* f, double* dp) {
   if (flag[2]) throw NoConv("CG did not converge in maxit iterations");
 }
 
-// p <- p + A^-1 (M Om^-1 F(V, p) + ydM)   (pressure_additional_solve.m:19-25)
-void additional_solve(ins2d_handle* h, const double* V) {
+// p <- p + A^-1 (M Om^-1 F(V, p) + ydM)   (pressure_additional_solve.m:19-25);
+// rate_cu/rate_cv: operator records whose yM is ydM (null: ydM = 0)
+void additional_solve(ins2d_handle* h, const double* V, DComp* rate_cu = nullptr, DComp* rate_cv = nullptr) {
   rhs(h, V, h->p, RHS_WITH_P | RHS_SCALE_OM, h->tV, kNoCombine, nullptr);
-  div(h, h->tV, nullptr, 1.0, 0.0, 0.0, 1.0, h->f);
+  if (rate_cu && !h->path) {
+    DComp *cu = h->dCu, *cv = h->dCv;
+    h->dCu = rate_cu;
+    h->dCv = rate_cv;
+    div(h, h->tV, nullptr, 1.0, 0.0, 1.0, 1.0, h->f);
+    h->dCu = cu;
+    h->dCv = cv;
+  } else {
+    div(h, h->tV, nullptr, 1.0, 0.0, 0.0, 1.0, h->f);
+  }
   poisson(h, h->f, h->dp);
   launch_axpy(h->st, (size_t)h->Np, 1.0, h->dp, h->p);
 }
 
-// one step of time_ERK.m (steady boundary conditions)
-void erk_step(ins2d_handle* h, double tn, double dt) {
+// one step of time_ERK.m.  ver: for time-dependent boundary values, the
+// operator records of s + 2 consecutive versions (boundary values at tn, at
+// the s stage times tn + c_i dt, and the ydM record at tn + dt); null for
+// steady boundary values
+void erk_step(ins2d_handle* h, double tn, double dt, DComp* ver = nullptr) {
   (void)tn;
   double* Vn = h->V;
   double* Vs = h->Vb;   // stage velocity, V_{n+1} at the end
+  DComp *cu0 = h->dCu, *cv0 = h->dCv;
+  auto use = [&](int k) {
+    h->dCu = ver + 2 * k;
+    h->dCv = ver + 2 * k + 1;
+  };
+  if (ver) use(0);   // time_ERK.m:62-64
   for (int i = 0; i < h->s; ++i) {
     const double* Vsrc = (i == 0) ? Vn : Vs;
     StageCombine sc{};
@@ -498,7 +661,10 @@ void erk_step(ins2d_handle* h, double tn, double dt) {
     rhs(h, Vsrc, h->p, RHS_SCALE_OM, h->keep[i] ? h->kv[i] : nullptr, sc, h->Vt);
     const double c = h->ct[i];
     const double* dpp;
-    if (i > 0) {
+    // unsteady BC: yM at tn + c_i dt (time_ERK.m:106-109), and a pressure
+    // solve at every stage (time_ERK.m:121-123)
+    if (ver) use(i + 1);
+    if (ver || i > 0) {
       // f = (M (Vn/dt + Vtemp) + yM/dt) / c_i                 (time_ERK.m:117)
       div(h, Vn, h->Vt, 1.0 / dt, 1.0, 1.0 / dt, 1.0 / c, h->f);
       poisson(h, h->f, h->dp);
@@ -510,7 +676,18 @@ void erk_step(ins2d_handle* h, double tn, double dt) {
     proj(h, Vn, h->Vt, dpp, dt, c, Vs);
   }
   std::swap(h->V, h->Vb);
-  if (h->p_add) additional_solve(h, h->V);
+  if (!ver) {
+    // steady BC: always the additional solve (time_ERK.m:160-163)
+    additional_solve(h, h->V);
+  } else if (h->p_add) {
+    // at tn + dt with ydM (time_ERK.m:147-148)
+    additional_solve(h, h->V, ver + 2 * (h->s + 1), ver + 2 * (h->s + 1) + 1);
+  } else {
+    // p = kp(:, end) (time_ERK.m:162)
+    HIPCHK(hipMemcpyAsync(h->p, h->dp, sizeof(double) * h->Np, hipMemcpyDeviceToDevice, h->st));
+  }
+  h->dCu = cu0;
+  h->dCv = cv0;
 }
 
 void fill_samples_default(ins2d_handle* h) {
@@ -834,6 +1011,11 @@ void ins2d_destroy(ins2d_handle* h) {
   for (auto e : h->ev_free) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
   for (void* p : h->yv_allocs) (void)hipFree(p);
+  for (void* p : h->rate_allocs) (void)hipFree(p);
+  if (h->ver_dev) (void)hipFree(h->ver_dev);
+  if (h->ver_comp) (void)hipFree(h->ver_comp);
+  if (h->ver_pin) (void)hipHostFree(h->ver_pin);
+  if (h->ver_ev) (void)hipEventDestroy(h->ver_ev);
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
 }
@@ -865,7 +1047,29 @@ int ins2d_set_bc(ins2d_handle* h, double t, const double* const* samples) {
     upload_comp_bc(h, h->plan->cu, h->hCu);
     upload_comp_bc(h, h->plan->cv, h->hCv);
     push_comps(h);
+    if (h->have_rate) upload_rate(h);
   }
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_set_bc_rate(ins2d_handle* h, const double* const* rates) {
+  if (!h) return INS2D_E_ARG;
+  API_TRY
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (!rates) {
+    h->have_rate = false;
+    for (void* p : h->rate_allocs) (void)hipFree(p);
+    h->rate_allocs.clear();
+    return INS2D_OK;
+  }
+  const int src[4] = {S_uLe_i, S_uRi_i, S_vLo_i, S_vUp_i};
+  for (int q = 0; q < 4; ++q) {
+    if (!rates[q]) throw ArgError("missing boundary rate sample");
+    h->rate[q].assign(rates[q], rates[q] + h->samp.s[src[q]].size());
+  }
+  h->have_rate = true;
+  upload_rate(h);
   return INS2D_OK;
   API_CATCH(h)
 }
@@ -936,7 +1140,10 @@ int ins2d_input_check(ins2d_handle* h, double t, int p_initial) {
     HIPCHK(hipMemcpyAsync(h->Vb, h->V, sizeof(double) * h->NV, hipMemcpyDeviceToDevice, h->st));
     proj(h, h->Vb, nullptr, h->dp, 1.0, 1.0, h->V);
   }
-  if (p_initial) additional_solve(h, h->V);
+  if (p_initial) {
+    if (h->have_rate) additional_solve(h, h->V, h->dCu_rate, h->dCv_rate);
+    else additional_solve(h, h->V);
+  }
   HIPCHK(hipStreamSynchronize(h->st));
   return INS2D_OK;
   API_CATCH(h)
@@ -946,6 +1153,77 @@ int ins2d_step(ins2d_handle* h, double tn, double dt, int nsteps) {
   if (!h || nsteps < 0 || !(dt > 0)) return INS2D_E_ARG;
   API_TRY
   for (int n = 0; n < nsteps; ++n) erk_step(h, tn + n * dt, dt);
+  if (h->prof) prof_collect(h);
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_step_unsteady(ins2d_handle* h, double tn, double dt, int nsteps, const double* samples,
+                        const double* rates) {
+  if (!h || nsteps < 0 || !(dt > 0) || (nsteps > 0 && (!samples || !rates))) return INS2D_E_ARG;
+  API_TRY
+  if (h->path || !h->plan)
+    throw Unsupported("time-dependent boundary values need the kernels with boundary terms (this handle runs the periodic ones)");
+  if (nsteps == 0) return INS2D_OK;
+  const int s = h->s, nv = s + 2;
+  size_t set_len = 0, rate_len = 0;
+  for (int q = 0; q < S_NSRC; ++q) set_len += h->samp.s[q].size();
+  const int src[4] = {S_uLe_i, S_uRi_i, S_vLo_i, S_vUp_i};
+  for (int q = 0; q < 4; ++q) rate_len += h->samp.s[src[q]].size();
+  // versions are staged for up to kChunk steps at a time
+  constexpr int kChunk = 64;
+  ver_reserve(h, nv * std::min(nsteps, kChunk));
+  const size_t len = std::max<size_t>(h->ver_len, 1);
+  BCSamples S;
+  for (int n0 = 0; n0 < nsteps; n0 += kChunk) {
+    const int m = std::min(kChunk, nsteps - n0);
+    // the staging buffer is free once the previous chunk's copy has ended
+    HIPCHK(hipEventSynchronize(h->ver_ev));
+    for (int n = 0; n < m; ++n) {
+      const double* smp = samples + (size_t)(n0 + n) * (s + 1) * set_len;
+      for (int k = 0; k <= s; ++k) {
+        unpack_samples(h, smp + k * set_len, S);
+        ver_fill(h, n * nv + k, S);
+      }
+      const double* r = rates + (size_t)(n0 + n) * rate_len;
+      const double* rq[4];
+      for (int q = 0; q < 4; ++q) {
+        rq[q] = r;
+        r += h->samp.s[src[q]].size();
+      }
+      ver_fill(h, n * nv + s + 1, rate_samples(h, rq));
+    }
+    HIPCHK(hipMemcpyAsync(h->ver_dev, h->ver_pin, sizeof(double) * len * nv * m, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipEventRecord(h->ver_ev, h->st));
+    for (int n = 0; n < m; ++n) erk_step(h, tn + (n0 + n) * dt, dt, h->ver_comp + 2 * (size_t)n * nv);
+    if (n0 + m < nsteps) {
+      // the next chunk rewrites the device versions: this chunk must be done
+      HIPCHK(hipStreamSynchronize(h->st));
+    }
+  }
+  // the handle's own boundary state becomes that of tn + nsteps dt
+  const double* last = samples + ((size_t)nsteps * (s + 1) - 1) * set_len;
+  std::vector<const double*> sp(S_NSRC);
+  for (int q = 0; q < S_NSRC; ++q) {
+    sp[q] = last;
+    last += h->samp.s[q].size();
+  }
+  const double* r = rates + (size_t)(nsteps - 1) * rate_len;
+  const double* rq[4];
+  for (int q = 0; q < 4; ++q) {
+    rq[q] = r;
+    r += h->samp.s[src[q]].size();
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (int q = 0; q < S_NSRC; ++q) std::copy(sp[q], sp[q] + h->samp.s[q].size(), h->samp.s[q].begin());
+  for (void* p : h->yv_allocs) (void)hipFree(p);
+  h->yv_allocs.clear();
+  upload_comp_bc(h, h->plan->cu, h->hCu);
+  upload_comp_bc(h, h->plan->cv, h->hCv);
+  push_comps(h);
+  for (int q = 0; q < 4; ++q) h->rate[q].assign(rq[q], rq[q] + h->samp.s[src[q]].size());
+  h->have_rate = true;
+  upload_rate(h);
   if (h->prof) prof_collect(h);
   return INS2D_OK;
   API_CATCH(h)

@@ -688,7 +688,6 @@ struct ins2d_batch {
   int s = 0;
   std::vector<double> At, ct;
   std::vector<bool> keep;
-  int p_add = 1;
   hipGraphExec_t cg_chunk = nullptr;
   hipGraph_t cg_graph = nullptr;
   // stage-kernel tile shape (velocity tiles), chosen by ins2d_batch_autotune
@@ -1004,7 +1003,8 @@ void b_step(ins2d_batch* b) {
     launch_tiles_proj(b, Vn, b->Vt, dpp, b->dt, c, Vs);
   }
   std::swap(b->V, b->Vb);
-  if (b->p_add) b_additional(b, b->V);
+  // steady boundary values: always the additional solve (time_ERK.m:160-163)
+  b_additional(b, b->V);
 }
 
 int bfail(ins2d_batch* b, const std::exception& e, int code) {
@@ -1037,7 +1037,6 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
     b->s = d0.rk_stages;
     b->tol = d0.cg_tol;
     b->maxit = d0.cg_maxit;
-    b->p_add = d0.p_add_solve;
     for (int k = 0; k < n; ++k) {
       const ins2d_case_desc& d = descs[k];
       if (d.order4) throw BErr(INS2D_E_UNSUPPORTED, "the ensemble runs the second-order operators");

@@ -61,6 +61,32 @@ def bc_samples(case, opts, t):
     return [np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel()) for a in s]
 
 
+def bc_rates(case, opts, t):
+    """Time derivatives of the boundary values on the point sets of
+    set_bc_vectors.m:73-78 (dudtLe_i, dudtRi_i, dvdtLo_i, dvdtUp_i)."""
+    g = opts.grid
+    s = [case.dudtBC(g.x[:1], g.yp, t, opts), case.dudtBC(g.x[-1:], g.yp, t, opts),
+         case.dvdtBC(g.xp, g.y[:1], t, opts), case.dvdtBC(g.xp, g.y[-1:], t, opts)]
+    return [np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel()) for a in s]
+
+
+def unsteady_samples(case, opts, c, tn, dt, nsteps):
+    """Boundary samples for nsteps RK steps from tn with time-dependent
+    boundary values, in the layout of ins2d_step_unsteady: per step the sets
+    at tn and at the stage times tn + c_i dt (i = 2..s) and tn + dt
+    (time_ERK.m:62-64,106-109), and the rates at tn + dt
+    (pressure_additional_solve.m:8-15).  The step start times accumulate as
+    solver_unsteady.m does (t = t + dt)."""
+    sets, rates = [], []
+    t = tn
+    for _ in range(nsteps):
+        for ti in [t] + [t + ci * dt for ci in c[1:]] + [t + dt]:
+            sets.extend(bc_samples(case, opts, ti))
+        t = t + dt
+        rates.extend(bc_rates(case, opts, t))
+    return np.concatenate(sets), np.concatenate(rates)
+
+
 class Solver:
     """One libins2d_hip handle: operators, state and solver buffers on the GPU."""
 
@@ -174,6 +200,24 @@ class Solver:
     def step(self, tn, dt, nsteps=1):
         self._call(self.lib.ins2d_step, ctypes.c_double(tn), ctypes.c_double(dt), ctypes.c_int(nsteps))
 
+    def set_bc_rate(self, rates):
+        """Time derivatives of the boundary values (bc_rates) at the time of
+        the last set_bc; None clears them."""
+        if rates is None:
+            self._call(self.lib.ins2d_set_bc_rate, None)
+            return
+        arrs = (ctypes.POINTER(ctypes.c_double) * 4)(*[_dp(r) for r in rates])
+        self._keep_rates = rates
+        self._call(self.lib.ins2d_set_bc_rate, arrs)
+
+    def step_unsteady(self, tn, dt, nsteps, samples, rates):
+        """nsteps with time-dependent boundary values; samples and rates as
+        unsteady_samples returns them."""
+        samples = np.ascontiguousarray(samples, dtype=np.float64)
+        rates = np.ascontiguousarray(rates, dtype=np.float64)
+        self._call(self.lib.ins2d_step_unsteady, ctypes.c_double(tn), ctypes.c_double(dt), ctypes.c_int(nsteps),
+                   _dp(samples), _dp(rates))
+
     def diagnostics(self, t=0.0):
         out = np.zeros(5)
         self._call(self.lib.ins2d_diagnostics, ctypes.c_double(t), _dp(out))
@@ -248,8 +292,9 @@ def _prepare(case):
     g.hx, g.hy = np.diff(x), np.diff(y)
     g.Nx, g.Ny = len(g.hx), len(g.hy)
     bct = case.BCtype()
-    if int(bct.get('BC_unsteady', 0)):
-        raise NotImplementedError('unsteady boundary conditions are not on this path')
+    opts.unsteady = bool(int(bct.get('BC_unsteady', 0)))
+    if opts.unsteady and (case.dudtBC is None or case.dvdtBC is None):
+        raise ValueError('BC_unsteady = 1 needs dudtBC and dvdtBC (set_bc_vectors.m:73-78)')
     _check_bc(bct)
     return prm, opts, bct
 
@@ -399,6 +444,9 @@ def setup(case, **overrides):
     sizes = [solver.Nux_in, solver.Nuy_in, solver.Nvx_in, solver.Nvy_in, solver.Npx, solver.Npy]
     V0, p0, samples = _finish(case, opts, sizes, solver.xin, solver.yin)
     solver.set_bc(samples, opts.t_start)
+    if opts.unsteady:
+        solver.set_bc_rate(bc_rates(case, opts, opts.t_start))
+    opts.case_obj = case
     return solver, opts, V0, p0
 
 
@@ -426,6 +474,9 @@ def main(case_name, nsteps=None, diagnostics=True, path_results=None, restart_n=
         V0 = np.concatenate([r['uh'], r['vh']])
         p0 = r['p']
         n0, t, dt = int(r['n']), float(r['t']), float(r['dt'])
+        if opts.unsteady:
+            solver.set_bc(bc_samples(opts.case_obj, opts, t), t)
+            solver.set_bc_rate(bc_rates(opts.case_obj, opts, t))
     solver.set_state(V0, p0)
     if restart_from is None:
         solver.input_check(t, int(prm.get('p_initial', 1)))
@@ -434,7 +485,10 @@ def main(case_name, nsteps=None, diagnostics=True, path_results=None, restart_n=
     name = case_name if isinstance(case_name, str) else getattr(case_name, 'name', 'case')
     nu = solver.Nu
     for n in range(n0 + 1, n0 + nt + 1):
-        solver.step(t, dt, 1)
+        if opts.unsteady:
+            solver.step_unsteady(t, dt, 1, *unsteady_samples(opts.case_obj, opts, solver._c, t, dt, 1))
+        else:
+            solver.step(t, dt, 1)
         t = t + dt
         if diagnostics:
             rows.append((n, dt, t) + solver.diagnostics(t))

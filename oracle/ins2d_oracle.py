@@ -939,7 +939,23 @@ def set_bc_vectors(t, opts):
         b = d.My_BC3
         yMy = alfa * yMy - b.Bbc @ (k(b.ybc1, vLo_i) + k(b.ybc2, vUp_i))
     d.yM = yMx + yMy
-    d.ydM = np.zeros(g.Np)   # steady BC (unsteady BC: :122-146)
+    # time derivative of the divergence's boundary vector (:120-146): the same
+    # construction on the dudtBC / dvdtBC samples (unsteady BC only)
+    if BC.BC_unsteady:
+        dudtLe_i = case.dudtBC(x[0:1], yp, t, opts); dudtRi_i = case.dudtBC(x[-1:], yp, t, opts)
+        dvdtLo_i = case.dvdtBC(xp, y[0:1], t, opts); dvdtUp_i = case.dvdtBC(xp, y[-1:], t, opts)
+        b = d.Mx_BC
+        ydMx = b.Bbc @ (k(dudtLe_i, b.ybc1) + k(dudtRi_i, b.ybc2))
+        b = d.My_BC
+        ydMy = b.Bbc @ (k(b.ybc1, dvdtLo_i) + k(b.ybc2, dvdtUp_i))
+        if d.order4:
+            b = d.Mx_BC3
+            ydMx = alfa * ydMx - b.Bbc @ (k(dudtLe_i, b.ybc1) + k(dudtRi_i, b.ybc2))
+            b = d.My_BC3
+            ydMy = alfa * ydMy - b.Bbc @ (k(b.ybc1, dvdtLo_i) + k(b.ybc2, dvdtUp_i))
+        d.ydM = ydMx + ydMy
+    else:
+        d.ydM = np.zeros(g.Np)
 
     # pressure (:156-180), p_inf = 0 (boundary_conditions.m:107)
     pLe = pRi = pLo = pUp = opts.BC.p_inf
@@ -1063,7 +1079,10 @@ def diffusion(V, t, opts):
 
 
 def F(V, C, p, t, opts, nopressure=False):
-    """spatial/F.m:1-117 -> (maxres, F)."""
+    """spatial/F.m:1-117 -> (maxres, F); unsteady BC: the boundary vectors at
+    t first (F.m:27-29)."""
+    if opts.BC.BC_unsteady:
+        set_bc_vectors(t, opts)
     d = opts.discretization
     convu, convv = convection(V, C, t, opts)
     d2u, d2v = diffusion(V, t, opts)
@@ -1183,16 +1202,22 @@ def pressure_poisson(f, t, opts):
 
 
 def pressure_additional_solve(V, p, t, opts):
-    """solvers/pressure/pressure_additional_solve.m:1-26 (steady BC)."""
+    """solvers/pressure/pressure_additional_solve.m:1-26 (ydM at t for
+    unsteady BC, :8-14)."""
     d = opts.discretization; g = opts.grid
+    if opts.BC.BC_unsteady:
+        set_bc_vectors(t, opts)
     _, R = F(V, V, p, t, opts)
     f = d.M @ (g.Om_inv * R) + d.ydM
     return p + pressure_poisson(f, t, opts)
 
 
 def check_conservation(V, t, opts):
-    """spatial/check_conservation.m:1-76."""
+    """spatial/check_conservation.m:1-76 (boundary vectors at t for unsteady
+    BC, :12-14)."""
     g = opts.grid; d = opts.discretization; BC = opts.BC; case = opts.case_obj
+    if BC.BC_unsteady:
+        set_bc_vectors(t, opts)
     uh, vh = V[:g.Nu], V[g.Nu:]
     uLe_i = case.uBC(g.x[0:1], g.yp, t, opts); uRi_i = case.uBC(g.x[-1:], g.yp, t, opts)
     vLo_i = case.vBC(g.xp, g.y[0:1], t, opts); vUp_i = case.vBC(g.xp, g.y[-1:], t, opts)
@@ -1267,28 +1292,42 @@ def get_rk_method(name):
 
 
 def time_ERK(Vn, pn, tn, dt, opts):
-    """unsteady/time_ERK.m:1-173 (steady boundary conditions)."""
+    """unsteady/time_ERK.m:1-173.  Unsteady BC: boundary vectors at tn (:62-64),
+    F at each stage time (F.m:27-29), yM at the stage's end time for the
+    divergence (:106-109), a pressure solve at every stage (:121-127), and at
+    the end the additional solve at tn + dt or the last stage pressure
+    (:145-164)."""
     g = opts.grid; d = opts.discretization
+    unsteady = opts.BC.BC_unsteady
     A_RK, b_RK, c_RK = get_rk_method(opts.time.RK)
     s = len(b_RK)
     A_RK = np.vstack([A_RK[1:, :], b_RK[None, :]])
     c_RK = np.r_[c_RK[1:], 1.0]
     V = Vn.copy(); p = pn.copy()
     kV = np.zeros((g.NV, s))
+    if unsteady:
+        set_bc_vectors(tn, opts)
     yM = d.yM
+    dp = pn
     for i in range(s):
         ti = tn + (0 if i == 0 else c_RK[i - 1] * dt)
         _, Frhs = F(V, V, p, ti, opts)
         kV[:, i] = g.Om_inv * (Frhs + d.G @ p)
         Vtemp = kV @ A_RK[i, :]
         ti = tn + c_RK[i] * dt
+        if unsteady:
+            set_bc_vectors(ti, opts)
+            yM = d.yM
         f = (d.M @ (Vn / dt + Vtemp) + yM / dt) / c_RK[i]
-        if i > 0:
+        if unsteady or i > 0:
             dp = pressure_poisson(f, ti, opts)
         else:
             dp = pn
         V = Vn + dt * (Vtemp - c_RK[i] * g.Om_inv * (d.G @ dp))
-    p = pressure_additional_solve(V, p, tn + dt, opts)
+    if unsteady and not opts.solversettings.p_add_solve:
+        p = dp   # kp(:, end)
+    else:
+        p = pressure_additional_solve(V, p, tn + dt, opts)
     return V, p
 
 
@@ -1322,8 +1361,6 @@ def build(case, overrides=None):
     mesh_generation(opts, x, y)
     bct = case.BCtype()
     opts.BC = _ns(u=_ns(**bct['u']), v=_ns(**bct['v']), BC_unsteady=int(bct.get('BC_unsteady', 0)), p_inf=0.0)
-    if opts.BC.BC_unsteady:
-        raise NotImplementedError('unsteady boundary conditions are outside this oracle')
     operators(opts)
     t = opts.time.t_start
     u0, v0, p0 = case.IC(t, opts)

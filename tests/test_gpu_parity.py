@@ -399,3 +399,59 @@ def test_pressure_rhs_and_residual_match_oracle(name):
     ro = np.linalg.norm(f - d.A @ x) / np.linalg.norm(f)
     assert abs(solver.residual(f, x) - ro) <= 1e-12 * ro
     solver.close()
+
+
+@pytest.mark.parametrize('RK,poisson,p_add', [('RK44', 1, 1), ('RK33P2', 3, 1), ('RK44', 1, 0)])
+def test_tg_temporal_unsteady_bc_matches_oracle(RK, poisson, p_add):
+    """case_files/TG_temporal: Dirichlet values that change in time
+    (BC_unsteady = 1), 20x20, dt = 0.01 to t = 1 through host.main (one
+    ins2d_step_unsteady call per step): the oracle's run step by step, and
+    with RK44 the reference-held solution at t = 1 (dt = 1e-3, within the
+    RK44 time error of dt = 0.01).  p_add = 0 takes the last stage pressure
+    (time_ERK.m:162)."""
+    from scipy.io import loadmat
+    _say('tg_temporal unsteady bc', RK, poisson, p_add)
+    case = load('TG_temporal', RK=RK, poisson=poisson, dt=0.01, CG_acc=1e-13, CG_maxit=2000, p_add_solve=p_add)
+    _compare_runs(case, 100, True)
+    if RK == 'RK44' and p_add:
+        V, p, _ = host.main(case, diagnostics=False)
+        m = loadmat(os.path.join(GOLDEN, 'TG_temporal', 'TG_Re100_N20_dt1e-3_Dir.mat'))
+        pf = m['p'].ravel()
+        assert np.max(np.abs(V - m['V'].ravel())) < 1e-9
+        assert np.max(np.abs((p - p.mean()) - (pf - pf.mean()))) < 1e-9
+
+
+def test_unsteady_step_chunks_match_single_steps():
+    """One ins2d_step_unsteady call over 150 steps (three staging chunks of
+    at most 64 steps) ends bit-identical to 150 calls of one step, and leaves
+    the handle's boundary state at the final time (diagnostics agree)."""
+    _say('tg_temporal unsteady chunks')
+    case = load('TG_temporal', dt=0.005, poisson=1)
+    out = []
+    for chunked in (False, True):
+        solver, opts, V0, p0 = host.setup(case)
+        solver.set_state(V0, p0)
+        solver.input_check(0.0, 1)
+        t, dt = 0.0, opts.dt
+        if chunked:
+            solver.step_unsteady(t, dt, 150, *host.unsteady_samples(case, opts, solver._c, t, dt, 150))
+            for _ in range(150):
+                t = t + dt
+        else:
+            for _ in range(150):
+                solver.step_unsteady(t, dt, 1, *host.unsteady_samples(case, opts, solver._c, t, dt, 1))
+                t = t + dt
+        V, p = solver.get_state()
+        out.append((V, p, solver.diagnostics(t)))
+        solver.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
+
+
+def test_unsteady_step_rejects_periodic_kernels():
+    _say('unsteady step on periodic kernels')
+    solver, opts, V0, p0 = host.setup(_case_periodic_noise(16))
+    with pytest.raises(host.Ins2dError) as e:
+        solver.step_unsteady(0.0, 0.01, 1, np.zeros(1000), np.zeros(100))
+    assert e.value.code == 4
+    solver.close()
