@@ -758,7 +758,11 @@ void build_fft(ins2d_handle* h, int order) {
   }
   F.ax = dupload(h, ax);
   F.ay = dupload(h, ay);
-  F.H = dalloc<double>(h, 2 * (size_t)F.Nh * h->Ny);
+  // the row-major spectra H are needed only where a direct DFT (length not a
+  // power of two in [16, 4096]) sits on one side of the transposes
+  // the row-major spectra H are needed only where a direct DFT (length not a
+  // power of two in [16, 4096]) sits on one side of the transposes
+  F.H = fft_register_lengths(F.Nx, F.Ny) ? nullptr : dalloc<double>(h, 2 * (size_t)F.Nh * h->Ny);
   F.HT = dalloc<double>(h, 2 * (size_t)F.Nh * h->Ny);
   h->has_fft = true;
 }

@@ -6,7 +6,9 @@
 //  * periodic uniform stencils (second and fourth order) with the boundary
 //    closures resolved at compile time;
 //  * hand-written FFT Poisson solve (register-butterfly Stockham passes for
-//    power-of-two lengths, direct DFT for other lengths);
+//    power-of-two lengths, whose row transforms store and load the half
+//    spectra transposed for the column solve, so three kernels per solve;
+//    direct DFT plus explicit transposes for other lengths);
 //  * Jacobi-preconditioned CG (multi-kernel, device-resident scalars, and a
 //    one-workgroup-per-system form for small systems);
 //  * reductions for the per-step diagnostics.
@@ -719,14 +721,25 @@ __device__ __forceinline__ void fft_reg(double2* z, int N, int log2N, const doub
 template <int E>
 constexpr int fft_block(int) { return E == 16 ? 256 : (E == 8 ? 512 : 1024); }
 
+// Work item of block b out of nb.  Blocks go to the 8 XCDs round-robin
+// (b mod 8); XCD x gets the contiguous range of nb / 8 items (one more for
+// x < nb mod 8) starting at x (nb / 8) + min(x, nb mod 8), so that the
+// transposed stores (loads) of neighbouring items, which share cache lines,
+// meet in the same L2.  A bijection of [0, nb).
+__device__ __forceinline__ int xcd_row_pair(int b, int nb) {
+  const int x = b & 7, q = nb >> 3, r = nb & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // rows y0 = 2b and y1 = 2b+1 packed as one complex row, half spectra out
-// (power-of-two Nx in [E, 4096], max(64, Nx / E) threads)
-template <int E>
+// (power-of-two Nx in [E, 4096], max(64, Nx / E) threads).  TR: the spectra
+// are stored transposed, H[k Ny + y] (Nh x Ny), ready for the column solve.
+template <int E, bool TR>
 __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_r2c_rows(int Nx, int Ny, int Nh, int log2N,
                                                        const double2* __restrict__ tw, const double* __restrict__ f,
                                                        double2* __restrict__ H) {
   extern __shared__ double2 z[];
-  const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
+  const int y0 = 2 * (TR ? xcd_row_pair(blockIdx.x, gridDim.x) : (int)blockIdx.x), y1 = y0 + 1;
   const bool has1 = y1 < Ny;
   const double* f0 = f + (size_t)y0 * Nx;
   const double* f1 = f + (size_t)(has1 ? y1 : y0) * Nx;
@@ -735,28 +748,38 @@ __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_r2c_rows(int Nx, int N
   fft_reg<E, false, true>(z, Nx, log2N, tw, ld, st);
   for (int k = threadIdx.x; k < Nh; k += blockDim.x) {
     const double2 a = z[swz(k)], b = conjd(z[swz((Nx - k) & (Nx - 1))]);
-    H[(size_t)y0 * Nh + k] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+    const double2 h0 = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
     // (a - b) / (2i) = -i/2 (a - b)
-    if (has1) H[(size_t)y1 * Nh + k] = make_double2(0.5 * (a.y - b.y), -0.5 * (a.x - b.x));
+    const double2 h1 = make_double2(0.5 * (a.y - b.y), -0.5 * (a.x - b.x));
+    if (TR) {
+      double2* o = H + (size_t)k * Ny + y0;
+      o[0] = h0;
+      if (has1) o[1] = h1;
+    } else {
+      H[(size_t)y0 * Nh + k] = h0;
+      if (has1) H[(size_t)y1 * Nh + k] = h1;
+    }
   }
 }
 
 // inverse of k_fftr_r2c_rows: the two half spectra as one complex row
 // a + i b, transformed and split into the real rows y0 and y1
-template <int E>
+// (TR: the spectra are read transposed, H[k Ny + y])
+template <int E, bool TR>
 __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_c2r_rows(int Nx, int Ny, int Nh, int log2N,
                                                        const double2* __restrict__ tw, const double2* __restrict__ H,
                                                        double* __restrict__ out) {
   extern __shared__ double2 z[];
-  const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
+  const int y0 = 2 * (TR ? xcd_row_pair(blockIdx.x, gridDim.x) : (int)blockIdx.x), y1 = y0 + 1;
   const bool has1 = y1 < Ny;
-  const double2* H0 = H + (size_t)y0 * Nh;
-  const double2* H1 = H + (size_t)(has1 ? y1 : y0) * Nh;
+  const size_t ks = TR ? (size_t)Ny : 1;   // stride of k
+  const double2* H0 = H + (TR ? (size_t)y0 : (size_t)y0 * Nh);
+  const double2* H1 = H + (TR ? (size_t)(has1 ? y1 : y0) : (size_t)(has1 ? y1 : y0) * Nh);
   double* o0 = out + (size_t)y0 * Nx;
   double* o1 = out + (size_t)(has1 ? y1 : y0) * Nx;
-  // the full row a + i b, staged in LDS (the half spectra read once, coalesced)
+  // the full row a + i b, staged in LDS (the half spectra read once)
   for (int k = threadIdx.x; k < Nh; k += blockDim.x) {
-    double2 a = H0[k], b = has1 ? H1[k] : make_double2(0.0, 0.0);
+    double2 a = H0[k * ks], b = has1 ? H1[k * ks] : make_double2(0.0, 0.0);
     if (k == 0 || 2 * k == Nx) { a.y = 0.0; b.y = 0.0; }
     z[swz(k)] = make_double2(a.x - b.y, a.y + b.x);
     if (k > 0 && 2 * k < Nx) z[swz(Nx - k)] = make_double2(a.x + b.y, b.x - a.y);   // conj(a) + i conj(b)
@@ -771,7 +794,8 @@ __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_c2r_rows(int Nx, int N
 }
 
 // one block per kx: forward FFT along y, division by -Ahat in registers,
-// inverse FFT along y (power-of-two Ny in [E, 4096], max(64, Ny / E) threads)
+// inverse FFT along y, in place on HT (Nh x Ny) (power-of-two Ny in
+// [E, 4096], max(64, Ny / E) threads)
 template <int E>
 __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_cols_solve(int Nx, int Ny, int log2N, const double2* __restrict__ tw,
                                                          const double* __restrict__ ax, const double* __restrict__ ay,
@@ -1317,28 +1341,34 @@ static int fft_elems(int N, bool cols) {
   return 8;
 }
 
-template <int E>
+// TR: the row transforms store (load) the spectra transposed
+template <int E, bool TR>
 static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, double* dp, int which) {
   const int Nx = P.Nx, Ny = P.Ny, Nh = P.Nh;
-  double2* H = (double2*)P.H;
+  double2* H = (double2*)(TR ? P.HT : P.H);
   double2* HT = (double2*)P.HT;
   if (which == 0)
-    hipLaunchKernelGGL(k_fftr_r2c_rows<E>, dim3((Ny + 1) / 2), dim3(std::max(64, Nx / E)),
+    hipLaunchKernelGGL((k_fftr_r2c_rows<E, TR>), dim3((Ny + 1) / 2), dim3(std::max(64, Nx / E)),
                        (size_t)Nx * sizeof(double2), st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx, f, H);
   else if (which == 1)
     hipLaunchKernelGGL(k_fftr_cols_solve<E>, dim3(Nh), dim3(std::max(64, Ny / E)), (size_t)Ny * sizeof(double2), st,
                        Nx, Ny, ilog2(Ny), (const double2*)P.twy, P.ax, P.ay, P.c, HT);
   else
-    hipLaunchKernelGGL(k_fftr_c2r_rows<E>, dim3((Ny + 1) / 2), dim3(std::max(64, Nx / E)),
+    hipLaunchKernelGGL((k_fftr_c2r_rows<E, TR>), dim3((Ny + 1) / 2), dim3(std::max(64, Nx / E)),
                        (size_t)Nx * sizeof(double2), st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx, H, dp);
 }
 
-// which: 0 row transforms, 1 column solve, 2 inverse row transforms
+template <bool TR>
 static void fftr(hipStream_t st, const FFTPlan& P, const double* f, double* dp, int which) {
   const int E = which == 1 ? fft_elems(P.Ny, true) : fft_elems(P.Nx, false);
-  if (E == 4) fftr_launch<4>(st, P, f, dp, which);
-  else if (E == 8) fftr_launch<8>(st, P, f, dp, which);
-  else fftr_launch<16>(st, P, f, dp, which);
+  if (E == 4) fftr_launch<4, TR>(st, P, f, dp, which);
+  else if (E == 8) fftr_launch<8, TR>(st, P, f, dp, which);
+  else fftr_launch<16, TR>(st, P, f, dp, which);
+}
+
+bool fft_register_lengths(int Nx, int Ny) {
+  auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
+  return fftr_ok(Nx, pow2(Nx)) && fftr_ok(Ny, pow2(Ny));
 }
 
 void launch_fft_poisson(hipStream_t st, const FFTPlan& P, const double* f, double* dp) {
@@ -1349,21 +1379,29 @@ void launch_fft_poisson(hipStream_t st, const FFTPlan& P, const double* f, doubl
   size_t shy = (size_t)(P.pow2y ? Ny : 2 * Ny) * sizeof(double2);
   double2* H = (double2*)P.H;
   double2* HT = (double2*)P.HT;
+  if (fx && fy) {
+    // register transforms in both directions: the row transforms store and
+    // load the spectra transposed, no separate transposes
+    fftr<true>(st, P, f, dp, 0);
+    fftr<true>(st, P, f, dp, 1);
+    fftr<true>(st, P, f, dp, 2);
+    return;
+  }
   if (fx)
-    fftr(st, P, f, dp, 0);
+    fftr<false>(st, P, f, dp, 0);
   else
     hipLaunchKernelGGL(k_fft_r2c_rows<1>, dim3((Ny + 1) / 2), dim3(Tx), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
                        (const double2*)P.twx, f, H);
   dim3 tb(32, 8);
   hipLaunchKernelGGL(k_transpose_c, dim3((Nh + 31) / 32, (Ny + 31) / 32), tb, 0, st, H, HT, Ny, Nh);
   if (fy)
-    fftr(st, P, f, dp, 1);
+    fftr<false>(st, P, f, dp, 1);   // in place on HT
   else
     hipLaunchKernelGGL(k_fft_cols_solve<1>, dim3(Nh), dim3(Ty), shy, st, Nx, Ny, P.pow2y, ilog2(Ny),
                        (const double2*)P.twy, P.ax, P.ay, P.c, HT);
   hipLaunchKernelGGL(k_transpose_c, dim3((Ny + 31) / 32, (Nh + 31) / 32), tb, 0, st, HT, H, Nh, Ny);
   if (fx)
-    fftr(st, P, f, dp, 2);
+    fftr<false>(st, P, f, dp, 2);
   else
     hipLaunchKernelGGL(k_fft_c2r_rows<1>, dim3((Ny + 1) / 2), dim3(Tx), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
                        (const double2*)P.twx, H, dp);

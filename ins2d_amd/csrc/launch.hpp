@@ -59,6 +59,8 @@ void launch_div_per(hipStream_t st, int order, int Nx, int Ny, double hx, double
 void launch_proj_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* Vn,
                      const double* Vt, const double* dp, double dt, double c, double* V);
 void launch_fft_poisson(hipStream_t st, const FFTPlan& P, const double* f, double* dp);
+// both lengths take the register transforms (no row-major spectra buffer)
+bool fft_register_lengths(int Nx, int Ny);
 void launch_lap(hipStream_t st, const DLap& L, const double* p, double* q);
 void launch_cg_single(hipStream_t st, const DLap& L, const double* f, double* x, double* r, double* z, double* p,
                       double* q, double tol2, int maxit, int x0_zero, int* out_it, double* out_rel);
