@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/ins2d.h"
+#include "bcmask.hpp"
 #include "device.hpp"
 #include "launch.hpp"
 #include "ops1d.hpp"
@@ -119,6 +120,7 @@ struct ins2d_handle {
   double* ver_pin = nullptr;
   DComp* ver_comp = nullptr;
   hipEvent_t ver_ev = nullptr;
+  unsigned* ver_ones = nullptr;
   // stats
   int last_it = 0;
   double last_rel = 0.0;
@@ -220,6 +222,10 @@ void upload_comp_bc(ins2d_handle* h, const HComp& C, DComp& D) {
   }
   for (int t = 0; t < D.ndiff; ++t) up_yv(h, C.diff[t].yS, D.diff[t].yS);
   up_yv(h, C.yM, D.yM);
+  std::vector<unsigned> mx, my;
+  bc_masks(C, D, mx, my);
+  D.bmx = dupload(h, mx, &h->yv_allocs);
+  D.bmy = dupload(h, my, &h->yv_allocs);
 }
 
 void push_comps(ins2d_handle* h) {
@@ -390,6 +396,12 @@ void ver_reserve(ins2d_handle* h, int n) {
   const size_t len = std::max<size_t>(h->ver_len, 1);
   HIPCHK(hipMalloc(&h->ver_dev, sizeof(double) * len * n));
   HIPCHK(hipMalloc(&h->ver_comp, sizeof(DComp) * 2 * n));
+  if (!h->ver_ones) {
+    // versions mark every point as reading boundary terms (their boxes span
+    // the whole arrays)
+    std::vector<unsigned> ones((size_t)std::max(std::max(h->nux, h->nvx), std::max(h->nuy, h->nvy)), ~0u);
+    h->ver_ones = dupload(h, ones);
+  }
   HIPCHK(hipHostMalloc(&h->ver_pin, sizeof(double) * len * n, hipHostMallocDefault));
   h->ver_cap = n;
   std::vector<DComp> recs(2 * (size_t)n);
@@ -397,6 +409,7 @@ void ver_reserve(ins2d_handle* h, int n) {
     DComp cu = h->hCu, cv = h->hCv;
     double* db = h->ver_dev + (size_t)v * len;
     put_comp(P.cv, h->samp, nullptr, db, put_comp(P.cu, h->samp, nullptr, db, 0, cu), cv);
+    cu.bmx = cu.bmy = cv.bmx = cv.bmy = h->ver_ones;
     recs[2 * v] = cu;
     recs[2 * v + 1] = cv;
   }

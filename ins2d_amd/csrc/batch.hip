@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/ins2d.h"
+#include "bcmask.hpp"
 #include "device.hpp"
 #include "dgemm.hpp"
 #include "launch.hpp"
@@ -53,6 +54,8 @@ constexpr int kAgg = 16;            // aggregate side of the two-level precondit
 constexpr int kBMaxCoarse = 2048;   // coarse unknowns per instance
 
 // ------------------------------------------------------------------ kernels
+// WAVE_ROW: the tile is a multiple of 64 wide, so each wave's points share j
+template <bool WAVE_ROW>
 __global__ void __launch_bounds__(256) k_b_rhs(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
                                               const double* __restrict__ V, const double* __restrict__ P, int mode,
                                               double* __restrict__ out, StageCombine sc,
@@ -65,12 +68,12 @@ __global__ void __launch_bounds__(256) k_b_rhs(const BInst* __restrict__ I, cons
   const double* p = P + B.poff;
   if (t.comp == 0) {
     if (i >= B.nux || j >= B.nuy) return;
-    rhs_point<0>(*B.cu, u, v, p, mode, out ? out + B.voff : nullptr, sc, vtemp ? vtemp + B.voff : nullptr, B.voff,
-                 i, j);
+    rhs_point<0, WAVE_ROW>(*B.cu, u, v, p, mode, out ? out + B.voff : nullptr, sc,
+                           vtemp ? vtemp + B.voff : nullptr, B.voff, i, j);
   } else {
     if (i >= B.nvx || j >= B.nvy) return;
     const long o = B.voff + B.nu;
-    rhs_point<1>(*B.cv, u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
+    rhs_point<1, WAVE_ROW>(*B.cv, u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
   }
 }
 
@@ -774,6 +777,26 @@ void b_upload_bc(ins2d_batch* b) {
     }
   }
   BHIP(hipMemcpy(dbuf, hbuf.data(), std::max<size_t>(total, 1) * sizeof(double), hipMemcpyHostToDevice));
+  // boundary masks of every instance and component, in one buffer
+  std::vector<unsigned> masks;
+  std::vector<size_t> moff;
+  for (int k = 0; k < b->n; ++k)
+    for (int c = 0; c < 2; ++c) {
+      std::vector<unsigned> mx, my;
+      bc_masks(c == 0 ? b->plans[k]->cu : b->plans[k]->cv, b->hcomp[2 * k + c], mx, my);
+      moff.push_back(masks.size());
+      masks.insert(masks.end(), mx.begin(), mx.end());
+      moff.push_back(masks.size());
+      masks.insert(masks.end(), my.begin(), my.end());
+    }
+  unsigned* dmask = nullptr;
+  BHIP(hipMalloc(&dmask, std::max<size_t>(masks.size(), 1) * sizeof(unsigned)));
+  b->yv_allocs.push_back(dmask);
+  BHIP(hipMemcpy(dmask, masks.data(), masks.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+  for (size_t q = 0; q < b->hcomp.size(); ++q) {
+    b->hcomp[q].bmx = dmask + moff[2 * q];
+    b->hcomp[q].bmy = dmask + moff[2 * q + 1];
+  }
   BHIP(hipMemcpy(b->dcomp, b->hcomp.data(), b->hcomp.size() * sizeof(DComp), hipMemcpyHostToDevice));
 }
 
@@ -876,8 +899,12 @@ void append_velocity_tiles(const BInst& I, int k, int tx, int ty, std::vector<Ti
 
 void launch_tiles_rhs(ins2d_batch* b, const double* V, const double* P, int mode, double* out, const StageCombine& sc,
                       double* vt) {
-  hipLaunchKernelGGL(k_b_rhs, dim3((unsigned)b->uvt.size()), dim3(b->tx, b->ty), 0, b->st, b->dinst, b->duv, V, P, mode,
-                     out, sc, vt);
+  if (b->tx % 64 == 0)
+    hipLaunchKernelGGL(k_b_rhs<true>, dim3((unsigned)b->uvt.size()), dim3(b->tx, b->ty), 0, b->st, b->dinst, b->duv, V,
+                       P, mode, out, sc, vt);
+  else
+    hipLaunchKernelGGL(k_b_rhs<false>, dim3((unsigned)b->uvt.size()), dim3(b->tx, b->ty), 0, b->st, b->dinst, b->duv, V,
+                       P, mode, out, sc, vt);
 }
 void launch_tiles_div(ins2d_batch* b, const double* A, const double* B, const double* dt, double ca, double cb,
                       double yc, double os, double* f) {
@@ -1219,8 +1246,12 @@ int ins2d_batch_autotune(ins2d_batch* b, int* tx, int* ty) {
     float tot = 0.f;
     for (int rep = 0; rep < 4; ++rep) {
       BHIP(hipEventRecord(e0, b->st));
-      hipLaunchKernelGGL(k_b_rhs, dim3((unsigned)list.size()), dim3(c[0], c[1]), 0, b->st, b->dinst, d, b->V, b->P,
-                         RHS_SCALE_OM, b->tV, none, nullptr);
+      if (c[0] % 64 == 0)
+        hipLaunchKernelGGL(k_b_rhs<true>, dim3((unsigned)list.size()), dim3(c[0], c[1]), 0, b->st, b->dinst, d, b->V,
+                           b->P, RHS_SCALE_OM, b->tV, none, nullptr);
+      else
+        hipLaunchKernelGGL(k_b_rhs<false>, dim3((unsigned)list.size()), dim3(c[0], c[1]), 0, b->st, b->dinst, d, b->V,
+                           b->P, RHS_SCALE_OM, b->tV, none, nullptr);
       BHIP(hipEventRecord(e1, b->st));
       BHIP(hipEventSynchronize(e1));
       float ms = 0.f;

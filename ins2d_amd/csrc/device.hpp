@@ -61,6 +61,10 @@ struct DComp {
   DDiff diff[2];
   const double* ox;   // volume = ox[i] * oy[j]
   const double* oy;
+  // boundary masks: the point (i, j) reads a boundary term only where
+  // bmx[i] & bmy[j] != 0 (interior points skip them all)
+  const unsigned* bmx;
+  const unsigned* bmy;
   DKron G;            // gradient of p onto this component
   DKron M;            // divergence of this component onto p
   DYV yM;

@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(256) k_rhs_generic(const DComp* __restrict__ C
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= C.nx || j >= C.ny) return;
-  rhs_point<COMP>(C, u, v, p, mode, out, sc, vtemp, 0, i, j);
+  rhs_point<COMP, true>(C, u, v, p, mode, out, sc, vtemp, 0, i, j);
 }
 
 // f = oscale * ( Mu (ca A_u + cb B_u) + Mv (ca A_v + cb B_v) + ycoef yM )

@@ -9,6 +9,14 @@ namespace ins2d {
 // ===========================================================================
 // kron-factored helpers
 // ===========================================================================
+// Pointers read from the operator records are generic pointers to the
+// compiler, which then loads through them with flat instructions; every array
+// here lives in device global memory, so the loads below go through the
+// global address space instead.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* glb(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
 __device__ __forceinline__ double kron_at(const DKron& K, const double* __restrict__ in, int f, int g) {
   const int KY = K.Y.K, KX = K.X.K, ld = K.X.ncols;
   const int* cy = K.Y.col + (size_t)g * KY;
@@ -29,7 +37,7 @@ __device__ __forceinline__ double yv_at(const DYV& Y, int f, int g) {
   double s = 0.0;
 #pragma unroll
   for (int t = 0; t < 4; ++t)
-    if (t < Y.nt && f >= Y.x0[t] && f < Y.x1[t] && g >= Y.y0[t] && g < Y.y1[t]) s += Y.ax[t][f] * Y.ay[t][g];
+    if (t < Y.nt && f >= Y.x0[t] && f < Y.x1[t] && g >= Y.y0[t] && g < Y.y1[t]) s += glb(Y.ax[t])[f] * glb(Y.ay[t])[g];
   return s;
 }
 
@@ -46,22 +54,31 @@ struct Row2 {
   double2 w;
 };
 __device__ __forceinline__ Row2 row2(const DOp1& O, int r) {
-  return Row2{reinterpret_cast<const int2*>(O.col)[r], reinterpret_cast<const double2*>(O.val)[r]};
+  const long long c = glb(reinterpret_cast<const long long*>(O.col))[r];
+  const auto w = glb(O.val) + 2 * r;
+  Row2 q;
+  q.c.x = (int)c;
+  q.c.y = (int)(c >> 32);
+  q.w.x = w[0];
+  q.w.y = w[1];
+  return q;
 }
 
 // (kron(Y, X) in)(f, g)
 __device__ __forceinline__ double kron2_at(const DKron& K, const double* __restrict__ in, int f, int g) {
   const Row2 y = row2(K.Y, g), x = row2(K.X, f);
-  const size_t ld = K.X.ncols;
+  // 32-bit element offsets (every array here holds fewer than 2^31 values)
+  const unsigned ld = K.X.ncols;
   const bool x2 = K.X.kmax > 1;
-  const double* r0 = in + (size_t)y.c.x * ld;
-  double s0 = x.w.x * r0[x.c.x];
-  if (x2) s0 += x.w.y * r0[x.c.y];
+  const auto a = glb(in);
+  const unsigned r0 = (unsigned)y.c.x * ld;
+  double s0 = x.w.x * a[r0 + (unsigned)x.c.x];
+  if (x2) s0 += x.w.y * a[r0 + (unsigned)x.c.y];
   double acc = y.w.x * s0;
   if (K.Y.kmax > 1) {
-    const double* r1 = in + (size_t)y.c.y * ld;
-    double s1 = x.w.x * r1[x.c.x];
-    if (x2) s1 += x.w.y * r1[x.c.y];
+    const unsigned r1 = (unsigned)y.c.y * ld;
+    double s1 = x.w.x * a[r1 + (unsigned)x.c.x];
+    if (x2) s1 += x.w.y * a[r1 + (unsigned)x.c.y];
     acc += y.w.y * s1;
   }
   return acc;
@@ -73,7 +90,7 @@ __device__ __forceinline__ double kron2_at2(const DKron& K, const double* __rest
   const Row2 y = row2(K.Y, g), x = row2(K.X, f);
   const size_t ld = K.X.ncols;
   const bool x2 = K.X.kmax > 1;
-  auto w = [&](size_t o) { return B ? ca * A[o] + cb * B[o] : ca * A[o]; };
+  auto w = [&](size_t o) { return B ? ca * glb(A)[o] + cb * glb(B)[o] : ca * glb(A)[o]; };
   const size_t r0 = (size_t)y.c.x * ld;
   double s0 = x.w.x * w(r0 + x.c.x);
   if (x2) s0 += x.w.y * w(r0 + x.c.y);
@@ -111,12 +128,11 @@ __device__ __forceinline__ double apply_d2(const DKron& D, int i, int j, FACE&& 
 // conv term:  D [ (I c + yI) .* (A own + yA) ]   (convection.m:198-212)
 // diff term:  D [ sscale S own + yS ]            (diffusion.m:29-30 with
 //             Diffu = Dux (Su_ux / Re) + ..., operator_convection_diffusion.m:460)
-template <int COMP>
-__device__ __forceinline__ void rhs_point(const DComp& C, const double* __restrict__ u, const double* __restrict__ v,
-                                          const double* __restrict__ p, int mode, double* __restrict__ out,
-                                          const StageCombine& sc, double* __restrict__ vtemp, size_t koff, int i,
-                                          int j) {
-  const double* own = COMP == 0 ? u : v;
+// -sum conv + sum diff at (i, j); BND: with the boundary terms (yI, yA, yS)
+template <bool BND>
+__device__ __forceinline__ double rhs_terms(const DComp& C, const double* __restrict__ u,
+                                            const double* __restrict__ v, const double* __restrict__ own, int i,
+                                            int j) {
   double acc = 0.0;
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
@@ -124,8 +140,12 @@ __device__ __forceinline__ void rhs_point(const DComp& C, const double* __restri
       const DConv& T = C.conv[t];
       const double* cf = T.icomp == 0 ? u : v;
       acc -= T.scale * apply_d2(T.D, i, j, [&](int f, int g) {
-        const double fi = kron2_at(T.I, cf, f, g) + yv_at(T.yI, f, g);
-        const double fa = kron2_at(T.A, own, f, g) + yv_at(T.yA, f, g);
+        double fi = kron2_at(T.I, cf, f, g);
+        double fa = kron2_at(T.A, own, f, g);
+        if (BND) {
+          fi += yv_at(T.yI, f, g);
+          fa += yv_at(T.yA, f, g);
+        }
         return fi * fa;
       });
     }
@@ -134,16 +154,38 @@ __device__ __forceinline__ void rhs_point(const DComp& C, const double* __restri
   for (int t = 0; t < 2; ++t) {
     if (t < C.ndiff) {
       const DDiff& T = C.diff[t];
-      acc += apply_d2(T.D, i, j, [&](int f, int g) { return T.sscale * kron2_at(T.S, own, f, g) + yv_at(T.yS, f, g); });
+      acc += apply_d2(T.D, i, j, [&](int f, int g) {
+        const double s = T.sscale * kron2_at(T.S, own, f, g);
+        return BND ? s + yv_at(T.yS, f, g) : s;
+      });
     }
   }
+  return acc;
+}
+
+// WAVE_ROW: every wave's points share one j (blocks 64 wide or a multiple),
+// so j and everything read through the y factors is wave-uniform
+template <int COMP, bool WAVE_ROW = false>
+__device__ __forceinline__ void rhs_point(const DComp& C, const double* __restrict__ u, const double* __restrict__ v,
+                                          const double* __restrict__ p, int mode, double* __restrict__ out,
+                                          const StageCombine& sc, double* __restrict__ vtemp, size_t koff, int i,
+                                          int j) {
+  if (WAVE_ROW) j = __builtin_amdgcn_readfirstlane(j);
+  const double* own = COMP == 0 ? u : v;
+  // interior points (no boundary term reaches their faces) take the path
+  // without the boundary terms; whole waves of interior points skip the other
+  double acc;
+  if (glb(C.bmx)[i] & glb(C.bmy)[j])
+    acc = rhs_terms<true>(C, u, v, own, i, j);
+  else
+    acc = rhs_terms<false>(C, u, v, own, i, j);
   if (mode & RHS_WITH_P) acc -= kron2_at(C.G, p, i, j);
-  if (mode & RHS_SCALE_OM) acc = acc / (C.ox[i] * C.oy[j]);
+  if (mode & RHS_SCALE_OM) acc = acc / (glb(C.ox)[i] * glb(C.oy)[j]);
   const size_t idx = (size_t)i + (size_t)C.nx * j;
   if (out) out[idx] = acc;
   if (vtemp) {
     double vt = sc.aself * acc;
-    for (int q = 0; q < sc.n; ++q) vt += sc.a[q] * sc.k[q][koff + idx];
+    for (int q = 0; q < sc.n; ++q) vt += sc.a[q] * glb(sc.k[q])[koff + idx];
     vtemp[idx] = vt;
   }
 }
