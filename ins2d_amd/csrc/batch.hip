@@ -66,12 +66,14 @@ __global__ void __launch_bounds__(256) k_b_rhs(const BInst* __restrict__ I, cons
   const double* u = V + B.voff;
   const double* v = u + B.nu;
   const double* p = P + B.poff;
+  // WAVE_ROW: lanes past the row's end stay (they take part in the face
+  // sharing of their wave); rows past the end leave as whole waves
   if (t.comp == 0) {
-    if (i >= B.nux || j >= B.nuy) return;
+    if ((!WAVE_ROW && i >= B.nux) || j >= B.nuy) return;
     rhs_point<0, WAVE_ROW>(*B.cu, u, v, p, mode, out ? out + B.voff : nullptr, sc,
                            vtemp ? vtemp + B.voff : nullptr, B.voff, i, j);
   } else {
-    if (i >= B.nvx || j >= B.nvy) return;
+    if ((!WAVE_ROW && i >= B.nvx) || j >= B.nvy) return;
     const long o = B.voff + B.nu;
     rhs_point<1, WAVE_ROW>(*B.cv, u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
   }

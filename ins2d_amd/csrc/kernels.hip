@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(256) k_rhs_generic(const DComp* __restrict__ C
   const DComp& C = *Cp;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= C.nx || j >= C.ny) return;
+  if (j >= C.ny) return;   // whole waves (64-wide blocks); i >= nx lanes share faces
   rhs_point<COMP, true>(C, u, v, p, mode, out, sc, vtemp, 0, i, j);
 }
 

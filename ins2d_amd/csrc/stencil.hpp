@@ -128,8 +128,52 @@ __device__ __forceinline__ double apply_d2(const DKron& D, int i, int j, FACE&& 
 // conv term:  D [ (I c + yI) .* (A own + yA) ]   (convection.m:198-212)
 // diff term:  D [ sscale S own + yS ]            (diffusion.m:29-30 with
 //             Diffu = Dux (Su_ux / Re) + ..., operator_convection_diffusion.m:460)
+__device__ __forceinline__ int wave_min(int x) {
+  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ int wave_max(int x) {
+  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+// (D face)(i, j).  SHARE (the wave's 64 lanes are consecutive points of one
+// row, all of them calling): a D that differences in x (one y entry per row)
+// has each face of the wave computed once: lane L computes face fmin + L
+// (and fmin + 64 + L when the wave's faces span more than 64), and every point
+// takes its two faces by shuffles, instead of each face being computed by both
+// points that read it.  Other D shapes, and spans of 128 or more (periodic
+// wrap), take the direct form.
+template <bool SHARE, class FACE>
+__device__ __forceinline__ double apply_d2s(const DKron& D, int i, int j, FACE&& face) {
+  if (SHARE && D.Y.kmax == 1) {
+    const Row2 dx = row2(D.X, i);
+    const bool x2 = D.X.kmax > 1;
+    const int fmin = wave_min(x2 ? min(dx.c.x, dx.c.y) : dx.c.x);
+    const int fmax = wave_max(x2 ? max(dx.c.x, dx.c.y) : dx.c.x);
+    if (fmax - fmin < 128) {
+      const Row2 dy = row2(D.Y, j);
+      const bool two = fmax - fmin >= 64;
+      const int lane = __lane_id();
+      double Fa = 0.0, Fb = 0.0;
+      if (fmin + lane <= fmax) Fa = face(fmin + lane, dy.c.x);
+      if (two && fmin + 64 + lane <= fmax) Fb = face(fmin + 64 + lane, dy.c.x);
+      auto pick = [&](int k) {
+        const double a = __shfl(Fa, k & 63, 64);
+        if (!two) return a;
+        const double b = __shfl(Fb, k & 63, 64);
+        return k < 64 ? a : b;
+      };
+      double s = dx.w.x * pick(dx.c.x - fmin);
+      if (x2) s += dx.w.y * pick(dx.c.y - fmin);
+      return dy.w.x * s;
+    }
+  }
+  return apply_d2(D, i, j, face);
+}
+
 // -sum conv + sum diff at (i, j); BND: with the boundary terms (yI, yA, yS)
-template <bool BND>
+template <bool BND, bool SHARE>
 __device__ __forceinline__ double rhs_terms(const DComp& C, const double* __restrict__ u,
                                             const double* __restrict__ v, const double* __restrict__ own, int i,
                                             int j) {
@@ -139,7 +183,7 @@ __device__ __forceinline__ double rhs_terms(const DComp& C, const double* __rest
     if (t < C.nconv) {
       const DConv& T = C.conv[t];
       const double* cf = T.icomp == 0 ? u : v;
-      acc -= T.scale * apply_d2(T.D, i, j, [&](int f, int g) {
+      acc -= T.scale * apply_d2s<SHARE>(T.D, i, j, [&](int f, int g) {
         double fi = kron2_at(T.I, cf, f, g);
         double fa = kron2_at(T.A, own, f, g);
         if (BND) {
@@ -154,7 +198,7 @@ __device__ __forceinline__ double rhs_terms(const DComp& C, const double* __rest
   for (int t = 0; t < 2; ++t) {
     if (t < C.ndiff) {
       const DDiff& T = C.diff[t];
-      acc += apply_d2(T.D, i, j, [&](int f, int g) {
+      acc += apply_d2s<SHARE>(T.D, i, j, [&](int f, int g) {
         const double s = T.sscale * kron2_at(T.S, own, f, g);
         return BND ? s + yv_at(T.yS, f, g) : s;
       });
@@ -163,22 +207,32 @@ __device__ __forceinline__ double rhs_terms(const DComp& C, const double* __rest
   return acc;
 }
 
-// WAVE_ROW: every wave's points share one j (blocks 64 wide or a multiple),
-// so j and everything read through the y factors is wave-uniform
+// WAVE_ROW: every wave's points share one j (blocks 64 wide or a multiple)
+// and every lane of the wave calls, also those past the row's end (i >= nx,
+// which compute shared faces and store nothing): j and everything read
+// through the y factors is wave-uniform, and x-direction faces are shared
 template <int COMP, bool WAVE_ROW = false>
 __device__ __forceinline__ void rhs_point(const DComp& C, const double* __restrict__ u, const double* __restrict__ v,
                                           const double* __restrict__ p, int mode, double* __restrict__ out,
                                           const StageCombine& sc, double* __restrict__ vtemp, size_t koff, int i,
                                           int j) {
-  if (WAVE_ROW) j = __builtin_amdgcn_readfirstlane(j);
+  const bool ok = i < C.nx;
+  if (WAVE_ROW) {
+    j = __builtin_amdgcn_readfirstlane(j);
+    i = ok ? i : C.nx - 1;
+  }
   const double* own = COMP == 0 ? u : v;
   // interior points (no boundary term reaches their faces) take the path
-  // without the boundary terms; whole waves of interior points skip the other
+  // without the boundary terms; whole waves of interior points skip the
+  // other (with shared faces the wave decides together: a face outside every
+  // box reads zero there)
   double acc;
-  if (glb(C.bmx)[i] & glb(C.bmy)[j])
-    acc = rhs_terms<true>(C, u, v, own, i, j);
+  const bool bnd = (glb(C.bmx)[i] & glb(C.bmy)[j]) != 0;
+  if (WAVE_ROW ? __any(bnd) : bnd)
+    acc = rhs_terms<true, WAVE_ROW>(C, u, v, own, i, j);
   else
-    acc = rhs_terms<false>(C, u, v, own, i, j);
+    acc = rhs_terms<false, WAVE_ROW>(C, u, v, own, i, j);
+  if (WAVE_ROW && !ok) return;
   if (mode & RHS_WITH_P) acc -= kron2_at(C.G, p, i, j);
   if (mode & RHS_SCALE_OM) acc = acc / (glb(C.ox)[i] * glb(C.oy)[j]);
   const size_t idx = (size_t)i + (size_t)C.nx * j;
