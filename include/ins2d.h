@@ -201,6 +201,9 @@ int ins2d_batch_step(ins2d_batch* b, const double* dt, int nsteps);
  * the momentum right-hand side over all instances for a few candidates on the
  * resident state; returns the chosen shape (no reference counterpart) */
 int ins2d_batch_autotune(ins2d_batch* b, int* tx, int* ty);
+/* set that workgroup shape directly: tx * ty = 256 with tx in {16, 32, 64,
+ * 128, 256} (no reference counterpart) */
+int ins2d_batch_set_tile(ins2d_batch* b, int tx, int ty);
 /* accumulated PCG iterations over all instances and solves, the largest
  * per-solve iteration count, total pressure cells */
 int ins2d_batch_stats(const ins2d_batch* b, long long* total_iters, int* max_iters, long long* cells);

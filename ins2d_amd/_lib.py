@@ -54,6 +54,7 @@ SIGNATURES = {
     "ins2d_batch_step": (c_int, [c_void_p, P_double, c_int]),
     "ins2d_batch_stats": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong), P_int, ctypes.POINTER(ctypes.c_longlong)]),
     "ins2d_batch_autotune": (c_int, [c_void_p, P_int, P_int]),
+    "ins2d_batch_set_tile": (c_int, [c_void_p, c_int, c_int]),
     "ins2d_nn_create": (c_int, [c_int, c_int, ctypes.POINTER(c_void_p)]),
     "ins2d_nn_destroy": (None, [c_void_p]),
     "ins2d_nn_last_error": (c_char_p, [c_void_p]),

@@ -12,6 +12,7 @@
 
 #include "../../include/ins2d.h"
 #include "bcmask.hpp"
+#include "uniform.hpp"
 #include "device.hpp"
 #include "launch.hpp"
 #include "ops1d.hpp"
@@ -873,6 +874,15 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
       h->nux = P.Nux_in; h->nuy = P.Nuy_in; h->nvx = P.Nvx_in; h->nvy = P.Nvy_in;
       upload_comp_ops(h.get(), P.cu, h->hCu);
       upload_comp_ops(h.get(), P.cv, h->hCv);
+      {
+        // closed form at the interior points of uniform grids
+        UniForm g{};
+        std::vector<unsigned char> fu, fv;
+        uniform_maps(P, g, fu, fv);
+        h->hCu.uni = h->hCv.uni = g;
+        h->hCu.uni.fast = fu.empty() ? nullptr : dupload(h.get(), fu);
+        h->hCv.uni.fast = fv.empty() ? nullptr : dupload(h.get(), fv);
+      }
       fill_samples_default(h.get());
       upload_comp_bc(h.get(), P.cu, h->hCu);
       upload_comp_bc(h.get(), P.cv, h->hCv);

@@ -18,6 +18,7 @@
 
 #include "../../include/ins2d.h"
 #include "bcmask.hpp"
+#include "uniform.hpp"
 #include "device.hpp"
 #include "dgemm.hpp"
 #include "launch.hpp"
@@ -54,8 +55,9 @@ constexpr int kAgg = 16;            // aggregate side of the two-level precondit
 constexpr int kBMaxCoarse = 2048;   // coarse unknowns per instance
 
 // ------------------------------------------------------------------ kernels
-// WAVE_ROW: the tile is a multiple of 64 wide, so each wave's points share j
-template <bool WAVE_ROW>
+// SEG: 0, or the segment width of the face sharing (rhs_point): 64 for
+// tiles a multiple of 64 wide, else the tile width (16 or 32)
+template <int SEG>
 __global__ void __launch_bounds__(256) k_b_rhs(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
                                               const double* __restrict__ V, const double* __restrict__ P, int mode,
                                               double* __restrict__ out, StageCombine sc,
@@ -66,16 +68,16 @@ __global__ void __launch_bounds__(256) k_b_rhs(const BInst* __restrict__ I, cons
   const double* u = V + B.voff;
   const double* v = u + B.nu;
   const double* p = P + B.poff;
-  // WAVE_ROW: lanes past the row's end stay (they take part in the face
-  // sharing of their wave); rows past the end leave as whole waves
+  // with SEG, lanes past the row's end stay (they take part in the face
+  // sharing of their segment); rows past the end leave as whole segments
   if (t.comp == 0) {
-    if ((!WAVE_ROW && i >= B.nux) || j >= B.nuy) return;
-    rhs_point<0, WAVE_ROW>(*B.cu, u, v, p, mode, out ? out + B.voff : nullptr, sc,
+    if ((!SEG && i >= B.nux) || j >= B.nuy) return;
+    rhs_point<0, SEG>(*B.cu, u, v, p, mode, out ? out + B.voff : nullptr, sc,
                            vtemp ? vtemp + B.voff : nullptr, B.voff, i, j);
   } else {
-    if ((!WAVE_ROW && i >= B.nvx) || j >= B.nvy) return;
+    if ((!SEG && i >= B.nvx) || j >= B.nvy) return;
     const long o = B.voff + B.nu;
-    rhs_point<1, WAVE_ROW>(*B.cv, u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
+    rhs_point<1, SEG>(*B.cv, u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
   }
 }
 
@@ -703,6 +705,9 @@ struct ins2d_batch {
   // 32 x 32 GEMM tile list, two work arrays
   bool fdm = false;
   std::vector<FDMHost> fh;
+  // closed-form maps of the momentum kernels (uniform members), per member
+  std::vector<UniForm> ug;
+  std::vector<std::vector<unsigned char>> ufu, ufv;
   std::vector<Tile> gt;
   Tile* dgt = nullptr;
   double *gw1 = nullptr, *gw2 = nullptr;
@@ -827,6 +832,9 @@ void build_ops(ins2d_batch* b, Arena& A) {
       D.oy = A.push(H.oy);
       D.G = a_kron(A, H.G);
       D.M = a_kron(A, H.M);
+      const std::vector<unsigned char>& fm = c == 0 ? b->ufu[k] : b->ufv[k];
+      D.uni = b->ug[k];
+      D.uni.fast = fm.empty() ? nullptr : A.push(fm);
     }
     BInst& I = b->hinst[k];
     I.lap.L1 = a_kron(A, P.L1);
@@ -864,6 +872,27 @@ void build_ops(ins2d_batch* b, Arena& A) {
 }
 
 // the fast-diagonalization factors of every instance, on host threads
+// the closed-form maps of every member (uniform_maps), on host threads
+void build_uniform_maps(ins2d_batch* b) {
+  b->ug.assign(b->n, UniForm{});
+  b->ufu.assign(b->n, {});
+  b->ufv.assign(b->n, {});
+  const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  std::vector<std::string> errs(nt);
+  for (int w = 0; w < nt; ++w)
+    pool.emplace_back([b, w, nt, &errs] {
+      try {
+        for (int k = w; k < b->n; k += nt) uniform_maps(*b->plans[k], b->ug[k], b->ufu[k], b->ufv[k]);
+      } catch (const std::exception& e) {
+        errs[w] = e.what();
+      }
+    });
+  for (auto& th : pool) th.join();
+  for (auto& e : errs)
+    if (!e.empty()) throw std::runtime_error("closed-form maps: " + e);
+}
+
 void build_fdm_factors(ins2d_batch* b) {
   b->fh.assign(b->n, FDMHost{});
   const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -899,14 +928,24 @@ void append_velocity_tiles(const BInst& I, int k, int tx, int ty, std::vector<Ti
   }
 }
 
+// the momentum right-hand side over a tile list of tx x ty threads (face
+// sharing within rows when tx is 16, 32 or a multiple of 64)
+void rhs_tiles(ins2d_batch* b, const Tile* tiles, size_t ntiles, int tx, int ty, const double* V, const double* P,
+               int mode, double* out, const StageCombine& sc, double* vt) {
+  const dim3 g((unsigned)ntiles), t(tx, ty);
+  if (tx % 64 == 0)
+    hipLaunchKernelGGL(k_b_rhs<64>, g, t, 0, b->st, b->dinst, tiles, V, P, mode, out, sc, vt);
+  else if (tx == 32)
+    hipLaunchKernelGGL(k_b_rhs<32>, g, t, 0, b->st, b->dinst, tiles, V, P, mode, out, sc, vt);
+  else if (tx == 16)
+    hipLaunchKernelGGL(k_b_rhs<16>, g, t, 0, b->st, b->dinst, tiles, V, P, mode, out, sc, vt);
+  else
+    hipLaunchKernelGGL(k_b_rhs<0>, g, t, 0, b->st, b->dinst, tiles, V, P, mode, out, sc, vt);
+}
+
 void launch_tiles_rhs(ins2d_batch* b, const double* V, const double* P, int mode, double* out, const StageCombine& sc,
                       double* vt) {
-  if (b->tx % 64 == 0)
-    hipLaunchKernelGGL(k_b_rhs<true>, dim3((unsigned)b->uvt.size()), dim3(b->tx, b->ty), 0, b->st, b->dinst, b->duv, V,
-                       P, mode, out, sc, vt);
-  else
-    hipLaunchKernelGGL(k_b_rhs<false>, dim3((unsigned)b->uvt.size()), dim3(b->tx, b->ty), 0, b->st, b->dinst, b->duv, V,
-                       P, mode, out, sc, vt);
+  rhs_tiles(b, b->duv, b->uvt.size(), b->tx, b->ty, V, P, mode, out, sc, vt);
 }
 void launch_tiles_div(ins2d_batch* b, const double* A, const double* B, const double* dt, double ca, double cb,
                       double yc, double os, double* f) {
@@ -1093,6 +1132,7 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
       b->chunk = 2;
       build_fdm_factors(b.get());
     }
+    build_uniform_maps(b.get());
     BHIP(hipStreamCreate(&b->st));
     b->hcomp.assign(2 * (size_t)n, DComp{});
     b->hinst.assign(n, BInst{});
@@ -1248,12 +1288,7 @@ int ins2d_batch_autotune(ins2d_batch* b, int* tx, int* ty) {
     float tot = 0.f;
     for (int rep = 0; rep < 4; ++rep) {
       BHIP(hipEventRecord(e0, b->st));
-      if (c[0] % 64 == 0)
-        hipLaunchKernelGGL(k_b_rhs<true>, dim3((unsigned)list.size()), dim3(c[0], c[1]), 0, b->st, b->dinst, d, b->V,
-                           b->P, RHS_SCALE_OM, b->tV, none, nullptr);
-      else
-        hipLaunchKernelGGL(k_b_rhs<false>, dim3((unsigned)list.size()), dim3(c[0], c[1]), 0, b->st, b->dinst, d, b->V,
-                           b->P, RHS_SCALE_OM, b->tV, none, nullptr);
+      rhs_tiles(b, d, list.size(), c[0], c[1], b->V, b->P, RHS_SCALE_OM, b->tV, none, nullptr);
       BHIP(hipEventRecord(e1, b->st));
       BHIP(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -1277,6 +1312,21 @@ int ins2d_batch_autotune(ins2d_batch* b, int* tx, int* ty) {
   b->ty = by;
   if (tx) *tx = bx;
   if (ty) *ty = by;
+  return INS2D_OK;
+  B_CATCH(b)
+}
+
+int ins2d_batch_set_tile(ins2d_batch* b, int tx, int ty) {
+  if (!b || tx * ty != 256 || !(tx == 16 || tx == 32 || tx == 64 || tx == 128 || tx == 256)) return INS2D_E_ARG;
+  B_TRY
+  std::vector<Tile> list;
+  for (int k = 0; k < b->n; ++k) append_velocity_tiles(b->hinst[k], k, tx, ty, list);
+  BHIP(hipStreamSynchronize(b->st));
+  if (list.size() > b->uvt.size()) b->duv = balloc<Tile>(b, list.size());
+  BHIP(hipMemcpy(b->duv, list.data(), list.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  b->uvt.swap(list);
+  b->tx = tx;
+  b->ty = ty;
   return INS2D_OK;
   B_CATCH(b)
 }

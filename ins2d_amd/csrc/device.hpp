@@ -55,6 +55,15 @@ struct DDiff {
   DYV yS;
   double sscale;
 };
+// closed form of the generic path's interior points on uniform grids:
+// point (i, j) of a component takes it where fast[i + nx j] is set (and no
+// boundary term reaches it); u array column 0 is grid line ou, v array row 0
+// grid line ov
+struct UniForm {
+  const unsigned char* fast;
+  double hx, hy, rhx, rhy, invRe;
+  int ou, ov, nux, nuy, nvx, nvy;
+};
 struct DComp {
   int nx, ny, nconv, ndiff;
   DConv conv[2];
@@ -65,6 +74,7 @@ struct DComp {
   // bmx[i] & bmy[j] != 0 (interior points skip them all)
   const unsigned* bmx;
   const unsigned* bmy;
+  UniForm uni;   // uni.fast null: no closed form
   DKron G;            // gradient of p onto this component
   DKron M;            // divergence of this component onto p
   DYV yM;

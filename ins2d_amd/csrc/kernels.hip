@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(256) k_rhs_generic(const DComp* __restrict__ C
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (j >= C.ny) return;   // whole waves (64-wide blocks); i >= nx lanes share faces
-  rhs_point<COMP, true>(C, u, v, p, mode, out, sc, vtemp, 0, i, j);
+  rhs_point<COMP, 64>(C, u, v, p, mode, out, sc, vtemp, 0, i, j);
 }
 
 // f = oscale * ( Mu (ca A_u + cb B_u) + Mv (ca A_v + cb B_v) + ycoef yM )

@@ -3,6 +3,7 @@
 
 #include "device.hpp"
 #include "launch.hpp"
+#include "uniform.hpp"
 
 namespace ins2d {
 
@@ -128,41 +129,45 @@ __device__ __forceinline__ double apply_d2(const DKron& D, int i, int j, FACE&& 
 // conv term:  D [ (I c + yI) .* (A own + yA) ]   (convection.m:198-212)
 // diff term:  D [ sscale S own + yS ]            (diffusion.m:29-30 with
 //             Diffu = Dux (Su_ux / Re) + ..., operator_convection_diffusion.m:460)
-__device__ __forceinline__ int wave_min(int x) {
-  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
+template <int SEG>
+__device__ __forceinline__ int seg_min(int x) {
+  for (int o = SEG / 2; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, SEG));
   return x;
 }
-__device__ __forceinline__ int wave_max(int x) {
-  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+template <int SEG>
+__device__ __forceinline__ int seg_max(int x) {
+  for (int o = SEG / 2; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, SEG));
   return x;
 }
 
-// (D face)(i, j).  SHARE (the wave's 64 lanes are consecutive points of one
-// row, all of them calling): a D that differences in x (one y entry per row)
-// has each face of the wave computed once: lane L computes face fmin + L
-// (and fmin + 64 + L when the wave's faces span more than 64), and every point
-// takes its two faces by shuffles, instead of each face being computed by both
-// points that read it.  Other D shapes, and spans of 128 or more (periodic
-// wrap), take the direct form.
-template <bool SHARE, class FACE>
+// (D face)(i, j).  SEG > 0: the wave's lanes form segments of SEG lanes
+// (16, 32 or 64), each holding consecutive points of one row, every lane
+// calling.  A D that differences in x (one y entry per row) then has each
+// face of a segment computed once: lane L of the segment computes face
+// fmin + L (and fmin + SEG + L when the segment's faces span more than SEG),
+// and every point takes its two faces by shuffles, instead of each face being
+// computed by both points that read it.  Other D shapes, and spans of 2 SEG
+// or more (periodic wrap), take the direct form.
+template <int SEG, class FACE>
 __device__ __forceinline__ double apply_d2s(const DKron& D, int i, int j, FACE&& face) {
-  if (SHARE && D.Y.kmax == 1) {
+  if (SEG > 0 && D.Y.kmax == 1) {
     const Row2 dx = row2(D.X, i);
     const bool x2 = D.X.kmax > 1;
-    const int fmin = wave_min(x2 ? min(dx.c.x, dx.c.y) : dx.c.x);
-    const int fmax = wave_max(x2 ? max(dx.c.x, dx.c.y) : dx.c.x);
-    if (fmax - fmin < 128) {
+    const int fmin = seg_min<SEG>(x2 ? min(dx.c.x, dx.c.y) : dx.c.x);
+    const int fmax = seg_max<SEG>(x2 ? max(dx.c.x, dx.c.y) : dx.c.x);
+    // segments of one wave may differ here: the branch is per lane, but every
+    // lane of a segment takes the same side
+    if (fmax - fmin < 2 * SEG) {
       const Row2 dy = row2(D.Y, j);
-      const bool two = fmax - fmin >= 64;
-      const int lane = __lane_id();
+      const bool two = fmax - fmin >= SEG;
+      const int lane = __lane_id() & (SEG - 1);
       double Fa = 0.0, Fb = 0.0;
       if (fmin + lane <= fmax) Fa = face(fmin + lane, dy.c.x);
-      if (two && fmin + 64 + lane <= fmax) Fb = face(fmin + 64 + lane, dy.c.x);
+      if (two && fmin + SEG + lane <= fmax) Fb = face(fmin + SEG + lane, dy.c.x);
       auto pick = [&](int k) {
-        const double a = __shfl(Fa, k & 63, 64);
-        if (!two) return a;
-        const double b = __shfl(Fb, k & 63, 64);
-        return k < 64 ? a : b;
+        const double a = __shfl(Fa, k & (SEG - 1), SEG);
+        const double b = __shfl(Fb, k & (SEG - 1), SEG);
+        return k < SEG ? a : b;
       };
       double s = dx.w.x * pick(dx.c.x - fmin);
       if (x2) s += dx.w.y * pick(dx.c.y - fmin);
@@ -173,7 +178,7 @@ __device__ __forceinline__ double apply_d2s(const DKron& D, int i, int j, FACE&&
 }
 
 // -sum conv + sum diff at (i, j); BND: with the boundary terms (yI, yA, yS)
-template <bool BND, bool SHARE>
+template <bool BND, int SHARE>
 __device__ __forceinline__ double rhs_terms(const DComp& C, const double* __restrict__ u,
                                             const double* __restrict__ v, const double* __restrict__ own, int i,
                                             int j) {
@@ -207,20 +212,19 @@ __device__ __forceinline__ double rhs_terms(const DComp& C, const double* __rest
   return acc;
 }
 
-// WAVE_ROW: every wave's points share one j (blocks 64 wide or a multiple)
-// and every lane of the wave calls, also those past the row's end (i >= nx,
-// which compute shared faces and store nothing): j and everything read
-// through the y factors is wave-uniform, and x-direction faces are shared
-template <int COMP, bool WAVE_ROW = false>
+// SEG (16, 32 or 64): the block is SEG wide, so every SEG consecutive lanes
+// hold consecutive points of one row, and every lane calls, also those past
+// the row's end (i >= nx, which compute shared faces and store nothing):
+// x-direction faces are shared within segments; with SEG = 64, j and
+// everything read through the y factors is also wave-uniform
+template <int COMP, int SEG = 0>
 __device__ __forceinline__ void rhs_point(const DComp& C, const double* __restrict__ u, const double* __restrict__ v,
                                           const double* __restrict__ p, int mode, double* __restrict__ out,
                                           const StageCombine& sc, double* __restrict__ vtemp, size_t koff, int i,
                                           int j) {
   const bool ok = i < C.nx;
-  if (WAVE_ROW) {
-    j = __builtin_amdgcn_readfirstlane(j);
-    i = ok ? i : C.nx - 1;
-  }
+  if (SEG == 64) j = __builtin_amdgcn_readfirstlane(j);
+  if (SEG > 0) i = ok ? i : C.nx - 1;
   const double* own = COMP == 0 ? u : v;
   // interior points (no boundary term reaches their faces) take the path
   // without the boundary terms; whole waves of interior points skip the
@@ -228,11 +232,25 @@ __device__ __forceinline__ void rhs_point(const DComp& C, const double* __restri
   // box reads zero there)
   double acc;
   const bool bnd = (glb(C.bmx)[i] & glb(C.bmy)[j]) != 0;
-  if (WAVE_ROW ? __any(bnd) : bnd)
-    acc = rhs_terms<true, WAVE_ROW>(C, u, v, own, i, j);
-  else
-    acc = rhs_terms<false, WAVE_ROW>(C, u, v, own, i, j);
-  if (WAVE_ROW && !ok) return;
+  // interior points of uniform grids take the closed form (a segment that
+  // shares faces decides together: its lanes either all take it or none)
+  bool closed = false;
+  if (C.uni.fast) {
+    closed = glb(C.uni.fast)[(size_t)i + (size_t)C.nx * j] && !bnd;
+    if (SEG > 0) closed = seg_min<SEG>(closed ? 1 : 0) == 1;
+  }
+  if (closed) {
+    const UniForm& g = C.uni;
+    const auto gu = glb(u), gv = glb(v);
+    auto U = [&](int a, int b) { return gu[(unsigned)(a - g.ou) + (unsigned)g.nux * (unsigned)b]; };
+    auto V = [&](int a, int b) { return gv[(unsigned)a + (unsigned)g.nvx * (unsigned)(b - g.ov)]; };
+    acc = COMP == 0 ? uni2_u(g, U, V, i + g.ou, j) : uni2_v(g, U, V, i, j + g.ov);
+  } else if (SEG > 0 ? __any(bnd) : bnd) {
+    acc = rhs_terms<true, SEG>(C, u, v, own, i, j);
+  } else {
+    acc = rhs_terms<false, SEG>(C, u, v, own, i, j);
+  }
+  if (SEG > 0 && !ok) return;
   if (mode & RHS_WITH_P) acc -= kron2_at(C.G, p, i, j);
   if (mode & RHS_SCALE_OM) acc = acc / (glb(C.ox)[i] * glb(C.oy)[j]);
   const size_t idx = (size_t)i + (size_t)C.nx * j;

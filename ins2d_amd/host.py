@@ -409,6 +409,10 @@ class Batch:
         self._call(self.lib.ins2d_batch_autotune, ctypes.byref(tx), ctypes.byref(ty))
         return tx.value, ty.value
 
+    def set_tile(self, tx, ty):
+        """Set the stage kernel's workgroup shape (tx * ty = 256)."""
+        self._call(self.lib.ins2d_batch_set_tile, ctypes.c_int(tx), ctypes.c_int(ty))
+
     def stats(self):
         tot = ctypes.c_longlong()
         mx = ctypes.c_int()

@@ -99,3 +99,24 @@ def test_fdm_ensemble_converges_in_two_iterations():
     assert res['fdm'][1] <= 2
     for (Va, pa), (Vb, pb) in zip(res['jacobi'][2], res['fdm'][2]):
         assert np.abs(Va - Vb).max() <= 1e-8 * np.abs(Va).max()
+
+
+@pytest.mark.parametrize('tile', [(64, 4), (32, 8), (16, 16), (128, 2), (256, 1)])
+def test_ensemble_stage_tile_shapes_match_oracle(tile):
+    """Every stage-tile shape (the face sharing of the momentum right-hand
+    side runs in 64-, 32- and 16-lane row segments, and tiles wider than some
+    members leave lanes past the row's end) gives the oracle's result."""
+    _say('ensemble tile', tile)
+    kw = dict(cg_tol=1e-13, precond='fdm')
+    members = [ensemble.member('cavity', 24, 16, 1.5, 400.0, 1.0, **kw),
+               ensemble.member('channel', 40, 20, 2.0, 3000.0, 1.1, **kw),
+               ensemble.member('cavity', 70, 18, 70 / 18, 900.0, 0.8, **kw)]
+    B = host.Batch(members)
+    B.input_check(1)
+    B.set_tile(*tile)
+    B.step(3)
+    for k, m in enumerate(members):
+        Vb, pb = B.get_state(k)
+        Vo, po, _, _ = O.run(m, nsteps=3)
+        assert np.abs(Vb - Vo).max() / np.abs(Vo).max() < 1e-11
+    B.close()
