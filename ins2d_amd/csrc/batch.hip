@@ -49,12 +49,54 @@ struct BInst {
 struct Tile {
   int inst, comp, i0, j0;
 };
+// a w x h block of points (w h <= 256) for the momentum right-hand side of
+// members with a closed form: comp bit 0 the component, bit 1 set for the
+// interior rectangle (closed form), clear for the boundary ring (generic)
+struct RTile {
+  int inst, comp, i0, j0, w, h;
+};
 
 constexpr int TX = 64, TY = 4;
 constexpr int kAgg = 16;            // aggregate side of the two-level preconditioner
 constexpr int kBMaxCoarse = 2048;   // coarse unknowns per instance
 
 // ------------------------------------------------------------------ kernels
+// right-hand side over the split tiles: point q of the block is (i0 + q mod w,
+// j0 + q div w)
+__global__ void __launch_bounds__(256) k_b_rhs_split(const BInst* __restrict__ I, const RTile* __restrict__ tiles,
+                                                    const double* __restrict__ V, const double* __restrict__ P,
+                                                    int mode, double* __restrict__ out, StageCombine sc,
+                                                    double* __restrict__ vtemp) {
+  const RTile t = tiles[blockIdx.x];
+  const int q = threadIdx.x;
+  if (q >= t.w * t.h) return;
+  const int i = t.i0 + q % t.w, j = t.j0 + q / t.w;
+  const BInst& B = I[t.inst];
+  const double* u = V + B.voff;
+  const double* v = u + B.nu;
+  const double* p = P + B.poff;
+  const bool closed = (t.comp & 2) != 0;
+  if ((t.comp & 1) == 0) {
+    const DComp& C = *B.cu;
+    const long o = B.voff;
+    double* ou = out ? out + o : nullptr;
+    double* vu = vtemp ? vtemp + o : nullptr;
+    if (closed && !(glb(C.bmx)[i] & glb(C.bmy)[j]))
+      rhs_closed_point<0>(C, u, v, p, mode, ou, sc, vu, o, i, j);
+    else
+      rhs_point<0, 0>(C, u, v, p, mode, ou, sc, vu, o, i, j);
+  } else {
+    const DComp& C = *B.cv;
+    const long o = B.voff + B.nu;
+    double* ov = out ? out + o : nullptr;
+    double* vv = vtemp ? vtemp + o : nullptr;
+    if (closed && !(glb(C.bmx)[i] & glb(C.bmy)[j]))
+      rhs_closed_point<1>(C, u, v, p, mode, ov, sc, vv, o, i, j);
+    else
+      rhs_point<1, 0>(C, u, v, p, mode, ov, sc, vv, o, i, j);
+  }
+}
+
 // SEG: 0, or the segment width of the face sharing (rhs_point): 64 for
 // tiles a multiple of 64 wide, else the tile width (16 or 32)
 template <int SEG>
@@ -682,6 +724,13 @@ struct ins2d_batch {
   BInst* dinst = nullptr;
   std::vector<Tile> uvt, pt;
   Tile *duv = nullptr, *dpt = nullptr;
+  // momentum right-hand side: members with a closed form over split tiles
+  // (rs), the others over tiles of the tuned shape tx x ty (rg)
+  std::vector<RTile> rs;
+  RTile* drs = nullptr;
+  std::vector<Tile> rg;
+  Tile* drg = nullptr;
+  std::vector<char> has_closed;
   Arena ops;                  // operator arrays
   std::vector<void*> allocs, yv_allocs;
   long NVt = 0, Npt = 0;
@@ -920,6 +969,47 @@ void fdm_apply(ins2d_batch* b, const double* r, double* z) {
   hipLaunchKernelGGL(k_bf_gemm<3>, g, t, 0, b->st, b->dinst, b->dgt, b->cg, b->gw1, z);
 }
 
+// split tiles of member k: closed-form tiles over the rectangle of points
+// where the closed form applies, generic tiles over the ring around it; false
+// (nothing appended) when a component has no such rectangle
+bool append_split_tiles(const BInst& I, int k, const std::vector<unsigned char>& fu,
+                        const std::vector<unsigned char>& fv, std::vector<RTile>& out) {
+  std::vector<RTile> add;
+  for (int c = 0; c < 2; ++c) {
+    const std::vector<unsigned char>& f = c == 0 ? fu : fv;
+    const int nx = c == 0 ? I.nux : I.nvx, ny = c == 0 ? I.nuy : I.nvy;
+    if (f.size() != (size_t)nx * ny) return false;
+    int i0 = nx, i1 = 0, j0 = ny, j1 = 0;
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        if (f[(size_t)i + (size_t)nx * j]) {
+          i0 = std::min(i0, i); i1 = std::max(i1, i + 1);
+          j0 = std::min(j0, j); j1 = std::max(j1, j + 1);
+        }
+    if (i0 >= i1 || j0 >= j1) return false;
+    for (int j = j0; j < j1; ++j)
+      for (int i = i0; i < i1; ++i)
+        if (!f[(size_t)i + (size_t)nx * j]) return false;
+    auto cover = [&](int a0, int a1, int b0, int b1, int kind) {
+      if (a0 >= a1 || b0 >= b1) return;
+      const int w = std::min(a1 - a0, 64), h = std::max(1, std::min(b1 - b0, 256 / w));
+      for (int b = b0; b < b1; b += h)
+        for (int a = a0; a < a1; a += w) add.push_back(RTile{k, c | kind, a, b, std::min(w, a1 - a), std::min(h, b1 - b)});
+    };
+    cover(i0, i1, j0, j1, 2);
+    cover(0, i0, 0, ny, 0);
+    cover(i1, nx, 0, ny, 0);
+    cover(i0, i1, 0, j0, 0);
+    cover(i0, i1, j1, ny, 0);
+  }
+  out.insert(out.end(), add.begin(), add.end());
+  return true;
+}
+
+// the generic-path right-hand-side tiles (tx x ty) of the members without a
+// closed form
+void build_generic_rhs_tiles(ins2d_batch* b, int tx, int ty, std::vector<Tile>& list);
+
 void append_velocity_tiles(const BInst& I, int k, int tx, int ty, std::vector<Tile>& out) {
   for (int c = 0; c < 2; ++c) {
     const int nx = c == 0 ? I.nux : I.nvx, ny = c == 0 ? I.nuy : I.nvy;
@@ -943,9 +1033,18 @@ void rhs_tiles(ins2d_batch* b, const Tile* tiles, size_t ntiles, int tx, int ty,
     hipLaunchKernelGGL(k_b_rhs<0>, g, t, 0, b->st, b->dinst, tiles, V, P, mode, out, sc, vt);
 }
 
+void build_generic_rhs_tiles(ins2d_batch* b, int tx, int ty, std::vector<Tile>& list) {
+  list.clear();
+  for (int k = 0; k < b->n; ++k)
+    if (!b->has_closed[k]) append_velocity_tiles(b->hinst[k], k, tx, ty, list);
+}
+
 void launch_tiles_rhs(ins2d_batch* b, const double* V, const double* P, int mode, double* out, const StageCombine& sc,
                       double* vt) {
-  rhs_tiles(b, b->duv, b->uvt.size(), b->tx, b->ty, V, P, mode, out, sc, vt);
+  if (!b->rs.empty())
+    hipLaunchKernelGGL(k_b_rhs_split, dim3((unsigned)b->rs.size()), dim3(256), 0, b->st, b->dinst, b->drs, V, P, mode,
+                       out, sc, vt);
+  if (!b->rg.empty()) rhs_tiles(b, b->drg, b->rg.size(), b->tx, b->ty, V, P, mode, out, sc, vt);
 }
 void launch_tiles_div(ins2d_batch* b, const double* A, const double* B, const double* dt, double ca, double cb,
                       double yc, double os, double* f) {
@@ -954,7 +1053,7 @@ void launch_tiles_div(ins2d_batch* b, const double* A, const double* B, const do
 }
 void launch_tiles_proj(ins2d_batch* b, const double* Vn, const double* Vt, const double* dp, const double* dt,
                        double c, double* V) {
-  hipLaunchKernelGGL(k_b_proj, dim3((unsigned)b->uvt.size()), dim3(b->tx, b->ty), 0, b->st, b->dinst, b->duv, Vn, Vt, dp,
+  hipLaunchKernelGGL(k_b_proj, dim3((unsigned)b->uvt.size()), dim3(TX, TY), 0, b->st, b->dinst, b->duv, Vn, Vt, dp,
                      dt, c, V);
 }
 
@@ -1160,6 +1259,7 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
       voff += P.NV;
       poff += P.Np;
       append_velocity_tiles(I, k, TX, TY, b->uvt);
+      b->has_closed.push_back(append_split_tiles(I, k, b->ufu[k], b->ufv[k], b->rs) ? 1 : 0);
       I.pt0 = (int)b->pt.size();
       for (int j = 0; j < I.npy; j += TY)
         for (int i = 0; i < I.npx; i += TX) b->pt.push_back(Tile{k, 2, i, j});
@@ -1191,6 +1291,13 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
     BHIP(hipMemcpy(b->dinst, b->hinst.data(), n * sizeof(BInst), hipMemcpyHostToDevice));
     b->duv = balloc<Tile>(b.get(), b->uvt.size());
     BHIP(hipMemcpy(b->duv, b->uvt.data(), b->uvt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    b->drs = balloc<RTile>(b.get(), b->rs.size());
+    if (!b->rs.empty())
+      BHIP(hipMemcpy(b->drs, b->rs.data(), b->rs.size() * sizeof(RTile), hipMemcpyHostToDevice));
+    build_generic_rhs_tiles(b.get(), b->tx, b->ty, b->rg);
+    b->drg = balloc<Tile>(b.get(), b->rg.size());
+    if (!b->rg.empty())
+      BHIP(hipMemcpy(b->drg, b->rg.data(), b->rg.size() * sizeof(Tile), hipMemcpyHostToDevice));
     b->dpt = balloc<Tile>(b.get(), b->pt.size());
     BHIP(hipMemcpy(b->dpt, b->pt.data(), b->pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
     if (b->two_level) {
@@ -1277,11 +1384,14 @@ int ins2d_batch_autotune(ins2d_batch* b, int* tx, int* ty) {
   BHIP(hipEventCreate(&e1));
   float best = 1e30f;
   int bx = b->tx, by = b->ty;
-  std::vector<Tile> best_list = b->uvt;
+  std::vector<Tile> best_list = b->rg;
   const StageCombine none{0, 0.0, {0}, {nullptr}};
+  // members with a closed form run over split tiles of their own: only the
+  // others depend on the shape
   for (const auto& c : cand) {
+    if (b->rg.empty()) break;
     std::vector<Tile> list;
-    for (int k = 0; k < b->n; ++k) append_velocity_tiles(b->hinst[k], k, c[0], c[1], list);
+    build_generic_rhs_tiles(b, c[0], c[1], list);
     Tile* d = nullptr;
     BHIP(hipMalloc(&d, list.size() * sizeof(Tile)));
     BHIP(hipMemcpy(d, list.data(), list.size() * sizeof(Tile), hipMemcpyHostToDevice));
@@ -1305,9 +1415,10 @@ int ins2d_batch_autotune(ins2d_batch* b, int* tx, int* ty) {
   }
   BHIP(hipEventDestroy(e0));
   BHIP(hipEventDestroy(e1));
-  if (best_list.size() > b->uvt.size()) b->duv = balloc<Tile>(b, best_list.size());
-  BHIP(hipMemcpy(b->duv, best_list.data(), best_list.size() * sizeof(Tile), hipMemcpyHostToDevice));
-  b->uvt.swap(best_list);
+  if (best_list.size() > b->rg.size()) b->drg = balloc<Tile>(b, best_list.size());
+  if (!best_list.empty())
+    BHIP(hipMemcpy(b->drg, best_list.data(), best_list.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  b->rg.swap(best_list);
   b->tx = bx;
   b->ty = by;
   if (tx) *tx = bx;
@@ -1320,11 +1431,11 @@ int ins2d_batch_set_tile(ins2d_batch* b, int tx, int ty) {
   if (!b || tx * ty != 256 || !(tx == 16 || tx == 32 || tx == 64 || tx == 128 || tx == 256)) return INS2D_E_ARG;
   B_TRY
   std::vector<Tile> list;
-  for (int k = 0; k < b->n; ++k) append_velocity_tiles(b->hinst[k], k, tx, ty, list);
+  build_generic_rhs_tiles(b, tx, ty, list);
   BHIP(hipStreamSynchronize(b->st));
-  if (list.size() > b->uvt.size()) b->duv = balloc<Tile>(b, list.size());
-  BHIP(hipMemcpy(b->duv, list.data(), list.size() * sizeof(Tile), hipMemcpyHostToDevice));
-  b->uvt.swap(list);
+  if (list.size() > b->rg.size()) b->drg = balloc<Tile>(b, list.size());
+  if (!list.empty()) BHIP(hipMemcpy(b->drg, list.data(), list.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  b->rg.swap(list);
   b->tx = tx;
   b->ty = ty;
   return INS2D_OK;

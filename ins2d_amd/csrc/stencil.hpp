@@ -212,6 +212,22 @@ __device__ __forceinline__ double rhs_terms(const DComp& C, const double* __rest
   return acc;
 }
 
+// -G p, 1/volume, and the stores of the right-hand side and the stage
+// combination at (i, j)
+__device__ __forceinline__ void rhs_store(const DComp& C, const double* __restrict__ p, int mode,
+                                          double* __restrict__ out, const StageCombine& sc,
+                                          double* __restrict__ vtemp, size_t koff, int i, int j, double acc) {
+  if (mode & RHS_WITH_P) acc -= kron2_at(C.G, p, i, j);
+  if (mode & RHS_SCALE_OM) acc = acc / (glb(C.ox)[i] * glb(C.oy)[j]);
+  const size_t idx = (size_t)i + (size_t)C.nx * j;
+  if (out) out[idx] = acc;
+  if (vtemp) {
+    double vt = sc.aself * acc;
+    for (int q = 0; q < sc.n; ++q) vt += sc.a[q] * glb(sc.k[q])[koff + idx];
+    vtemp[idx] = vt;
+  }
+}
+
 // SEG (16, 32 or 64): the block is SEG wide, so every SEG consecutive lanes
 // hold consecutive points of one row, and every lane calls, also those past
 // the row's end (i >= nx, which compute shared faces and store nothing):
@@ -251,15 +267,22 @@ __device__ __forceinline__ void rhs_point(const DComp& C, const double* __restri
     acc = rhs_terms<false, SEG>(C, u, v, own, i, j);
   }
   if (SEG > 0 && !ok) return;
-  if (mode & RHS_WITH_P) acc -= kron2_at(C.G, p, i, j);
-  if (mode & RHS_SCALE_OM) acc = acc / (glb(C.ox)[i] * glb(C.oy)[j]);
-  const size_t idx = (size_t)i + (size_t)C.nx * j;
-  if (out) out[idx] = acc;
-  if (vtemp) {
-    double vt = sc.aself * acc;
-    for (int q = 0; q < sc.n; ++q) vt += sc.a[q] * glb(sc.k[q])[koff + idx];
-    vtemp[idx] = vt;
-  }
+  rhs_store(C, p, mode, out, sc, vtemp, koff, i, j, acc);
+}
+
+// a point of the interior of a uniform grid (uni.fast set, no boundary term):
+// the closed form only
+template <int COMP>
+__device__ __forceinline__ void rhs_closed_point(const DComp& C, const double* __restrict__ u,
+                                                 const double* __restrict__ v, const double* __restrict__ p, int mode,
+                                                 double* __restrict__ out, const StageCombine& sc,
+                                                 double* __restrict__ vtemp, size_t koff, int i, int j) {
+  const UniForm& g = C.uni;
+  const auto gu = glb(u), gv = glb(v);
+  auto U = [&](int a, int b) { return gu[(unsigned)(a - g.ou) + (unsigned)g.nux * (unsigned)b]; };
+  auto V = [&](int a, int b) { return gv[(unsigned)a + (unsigned)g.nvx * (unsigned)(b - g.ov)]; };
+  const double acc = COMP == 0 ? uni2_u(g, U, V, i + g.ou, j) : uni2_v(g, U, V, i, j + g.ov);
+  rhs_store(C, p, mode, out, sc, vtemp, koff, i, j, acc);
 }
 
 // divergence at pressure cell (i, j): Mu (ca A_u + cb B_u) + Mv (...) + ycoef yM

@@ -101,16 +101,31 @@ def test_fdm_ensemble_converges_in_two_iterations():
         assert np.abs(Va - Vb).max() <= 1e-8 * np.abs(Va).max()
 
 
+def _stretched(m, s):
+    """The member on a geometrically stretched grid (no closed form there)."""
+    base = m.parameters
+
+    def parameters():
+        p = base()
+        p.update(sx=s, sy=s)
+        return p
+    m.parameters = parameters
+    return m
+
+
 @pytest.mark.parametrize('tile', [(64, 4), (32, 8), (16, 16), (128, 2), (256, 1)])
 def test_ensemble_stage_tile_shapes_match_oracle(tile):
-    """Every stage-tile shape (the face sharing of the momentum right-hand
-    side runs in 64-, 32- and 16-lane row segments, and tiles wider than some
-    members leave lanes past the row's end) gives the oracle's result."""
+    """Uniform members run the closed form over their interior and the generic
+    operators over the boundary ring (split tiles); a stretched member runs the
+    generic path over tiles of every stage-tile shape (its face sharing in 64-,
+    32- and 16-lane row segments, and tiles wider than the member leave lanes
+    past the row's end).  All give the oracle's result."""
     _say('ensemble tile', tile)
     kw = dict(cg_tol=1e-13, precond='fdm')
     members = [ensemble.member('cavity', 24, 16, 1.5, 400.0, 1.0, **kw),
                ensemble.member('channel', 40, 20, 2.0, 3000.0, 1.1, **kw),
-               ensemble.member('cavity', 70, 18, 70 / 18, 900.0, 0.8, **kw)]
+               ensemble.member('cavity', 70, 18, 70 / 18, 900.0, 0.8, **kw),
+               _stretched(ensemble.member('channel', 36, 22, 1.8, 700.0, 0.9, **kw), 1.04)]
     B = host.Batch(members)
     B.input_check(1)
     B.set_tile(*tile)
