@@ -54,6 +54,10 @@ struct ins2d_handle {
   // operators (generic)
   DComp hCu{}, hCv{};
   DComp *dCu = nullptr, *dCv = nullptr;
+  // split tiles of the momentum right-hand side (closed form inside, generic
+  // ring); none when the grid has no closed-form interior
+  RTile* split = nullptr;
+  int nsplit = 0;
   DLap lap{};
   std::vector<void*> yv_allocs;   // boundary vectors (re-uploaded by set_bc)
   // FFT
@@ -445,8 +449,12 @@ void rhs(ins2d_handle* h, const double* V, const double* p, int mode, double* ou
   } else {
     StageCombine scv = sc;
     for (int q = 0; q < sc.n; ++q) scv.k[q] = sc.k[q] + h->Nu;
-    launch_rhs_generic(h->st, h->dCu, h->dCv, h->nux, h->nuy, h->nvx, h->nvy, V, V + h->Nu, p, mode, out,
+    if (h->nsplit)
+      launch_rhs_split(h->st, h->dCu, h->dCv, h->split, h->nsplit, V, V + h->Nu, p, mode, out,
                        out ? out + h->Nu : nullptr, sc, scv, vt, vt ? vt + h->Nu : nullptr);
+    else
+      launch_rhs_generic(h->st, h->dCu, h->dCv, h->nux, h->nuy, h->nvx, h->nvy, V, V + h->Nu, p, mode, out,
+                         out ? out + h->Nu : nullptr, sc, scv, vt, vt ? vt + h->Nu : nullptr);
   }
 }
 
@@ -877,11 +885,23 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
       {
         // closed form at the interior points of uniform grids
         UniForm g{};
+        UniArrays ua;
         std::vector<unsigned char> fu, fv;
-        uniform_maps(P, g, fu, fv);
+        uniform_maps(P, g, ua, fu, fv);
+        if (g.ahx) {   // stretched grid: the spacing arrays on the device
+          g.ahx = dupload(h.get(), ua.hx); g.ahy = dupload(h.get(), ua.hy);
+          g.agx = dupload(h.get(), ua.gx); g.agy = dupload(h.get(), ua.gy);
+          g.arhx = dupload(h.get(), ua.rhx); g.arhy = dupload(h.get(), ua.rhy);
+          g.argx = dupload(h.get(), ua.rgx); g.argy = dupload(h.get(), ua.rgy);
+        }
         h->hCu.uni = h->hCv.uni = g;
         h->hCu.uni.fast = fu.empty() ? nullptr : dupload(h.get(), fu);
         h->hCv.uni.fast = fv.empty() ? nullptr : dupload(h.get(), fv);
+        std::vector<RTile> tiles;
+        if (split_tiles(0, 0, P.Nux_in, P.Nuy_in, fu, 64, tiles) && split_tiles(0, 1, P.Nvx_in, P.Nvy_in, fv, 64, tiles)) {
+          h->split = dupload(h.get(), tiles);
+          h->nsplit = (int)tiles.size();
+        }
       }
       fill_samples_default(h.get());
       upload_comp_bc(h.get(), P.cu, h->hCu);

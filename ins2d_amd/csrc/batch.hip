@@ -49,20 +49,37 @@ struct BInst {
 struct Tile {
   int inst, comp, i0, j0;
 };
-// a w x h block of points (w h <= 256) for the momentum right-hand side of
-// members with a closed form: comp bit 0 the component, bit 1 set for the
-// interior rectangle (closed form), clear for the boundary ring (generic)
-struct RTile {
-  int inst, comp, i0, j0, w, h;
-};
+
 
 constexpr int TX = 64, TY = 4;
 constexpr int kAgg = 16;            // aggregate side of the two-level preconditioner
 constexpr int kBMaxCoarse = 2048;   // coarse unknowns per instance
 
 // ------------------------------------------------------------------ kernels
-// right-hand side over the split tiles: point q of the block is (i0 + q mod w,
-// j0 + q div w)
+// right-hand side over the split tiles, as two kernels: the closed form over
+// the interior tiles (k_b_rhs_closed, small) and the generic operators over
+// the ring (k_b_rhs_split); point q of the block is (i0 + q mod w, j0 + q div w)
+__global__ void __launch_bounds__(256) k_b_rhs_closed(const BInst* __restrict__ I, const RTile* __restrict__ tiles,
+                                                     const double* __restrict__ V, const double* __restrict__ P,
+                                                     int mode, double* __restrict__ out, StageCombine sc,
+                                                     double* __restrict__ vtemp) {
+  const RTile t = tiles[blockIdx.x];
+  const int q = threadIdx.x;
+  if (q >= t.w * t.h) return;
+  const int i = t.i0 + q % t.w, j = t.j0 + q / t.w;
+  const BInst& B = I[t.inst];
+  const double* u = V + B.voff;
+  const double* v = u + B.nu;
+  const double* p = P + B.poff;
+  if ((t.comp & 1) == 0) {
+    const long o = B.voff;
+    rhs_closed_point<0>(*B.cu, u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
+  } else {
+    const long o = B.voff + B.nu;
+    rhs_closed_point<1>(*B.cv, u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_b_rhs_split(const BInst* __restrict__ I, const RTile* __restrict__ tiles,
                                                     const double* __restrict__ V, const double* __restrict__ P,
                                                     int mode, double* __restrict__ out, StageCombine sc,
@@ -728,9 +745,14 @@ struct ins2d_batch {
   // (rs), the others over tiles of the tuned shape tx x ty (rg)
   std::vector<RTile> rs;
   RTile* drs = nullptr;
+  size_t rs_cap = 0, nclosed = 0;
+  // no closed-form point reads a boundary term (checked against the boundary
+  // masks whenever they change): the interior tiles may run the closed-form
+  // kernel, which reads no masks
+  bool closed_ok = false;
   std::vector<Tile> rg;
   Tile* drg = nullptr;
-  std::vector<char> has_closed;
+  size_t rg_cap = 0;
   Arena ops;                  // operator arrays
   std::vector<void*> allocs, yv_allocs;
   long NVt = 0, Npt = 0;
@@ -756,6 +778,7 @@ struct ins2d_batch {
   std::vector<FDMHost> fh;
   // closed-form maps of the momentum kernels (uniform members), per member
   std::vector<UniForm> ug;
+  std::vector<UniArrays> ua;
   std::vector<std::vector<unsigned char>> ufu, ufv;
   std::vector<Tile> gt;
   Tile* dgt = nullptr;
@@ -853,6 +876,23 @@ void b_upload_bc(ins2d_batch* b) {
     b->hcomp[q].bmx = dmask + moff[2 * q];
     b->hcomp[q].bmy = dmask + moff[2 * q + 1];
   }
+  // the closed-form kernel reads no boundary masks: allowed only while no
+  // point of a closed-form map reads a boundary term
+  b->closed_ok = true;
+  for (int k = 0; k < b->n && b->closed_ok; ++k)
+    for (int c = 0; c < 2 && b->closed_ok; ++c) {
+      const std::vector<unsigned char>& f = c == 0 ? b->ufu[k] : b->ufv[k];
+      if (f.empty()) continue;
+      const int nx = b->hcomp[2 * k + c].nx, ny = b->hcomp[2 * k + c].ny;
+      const unsigned* mx = masks.data() + moff[2 * (2 * k + c)];
+      const unsigned* my = masks.data() + moff[2 * (2 * k + c) + 1];
+      for (int j = 0; j < ny && b->closed_ok; ++j)
+        for (int i = 0; i < nx; ++i)
+          if (f[(size_t)i + (size_t)nx * j] && (mx[i] & my[j])) {
+            b->closed_ok = false;
+            break;
+          }
+    }
   BHIP(hipMemcpy(b->dcomp, b->hcomp.data(), b->hcomp.size() * sizeof(DComp), hipMemcpyHostToDevice));
 }
 
@@ -884,6 +924,11 @@ void build_ops(ins2d_batch* b, Arena& A) {
       const std::vector<unsigned char>& fm = c == 0 ? b->ufu[k] : b->ufv[k];
       D.uni = b->ug[k];
       D.uni.fast = fm.empty() ? nullptr : A.push(fm);
+      if (D.uni.ahx) {   // stretched member: the spacing arrays in the arena
+        const UniArrays& S = b->ua[k];
+        D.uni.ahx = A.push(S.hx); D.uni.ahy = A.push(S.hy); D.uni.agx = A.push(S.gx); D.uni.agy = A.push(S.gy);
+        D.uni.arhx = A.push(S.rhx); D.uni.arhy = A.push(S.rhy); D.uni.argx = A.push(S.rgx); D.uni.argy = A.push(S.rgy);
+      }
     }
     BInst& I = b->hinst[k];
     I.lap.L1 = a_kron(A, P.L1);
@@ -924,6 +969,7 @@ void build_ops(ins2d_batch* b, Arena& A) {
 // the closed-form maps of every member (uniform_maps), on host threads
 void build_uniform_maps(ins2d_batch* b) {
   b->ug.assign(b->n, UniForm{});
+  b->ua.assign(b->n, UniArrays{});
   b->ufu.assign(b->n, {});
   b->ufv.assign(b->n, {});
   const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -932,7 +978,7 @@ void build_uniform_maps(ins2d_batch* b) {
   for (int w = 0; w < nt; ++w)
     pool.emplace_back([b, w, nt, &errs] {
       try {
-        for (int k = w; k < b->n; k += nt) uniform_maps(*b->plans[k], b->ug[k], b->ufu[k], b->ufv[k]);
+        for (int k = w; k < b->n; k += nt) uniform_maps(*b->plans[k], b->ug[k], b->ua[k], b->ufu[k], b->ufv[k]);
       } catch (const std::exception& e) {
         errs[w] = e.what();
       }
@@ -969,46 +1015,20 @@ void fdm_apply(ins2d_batch* b, const double* r, double* z) {
   hipLaunchKernelGGL(k_bf_gemm<3>, g, t, 0, b->st, b->dinst, b->dgt, b->cg, b->gw1, z);
 }
 
-// split tiles of member k: closed-form tiles over the rectangle of points
-// where the closed form applies, generic tiles over the ring around it; false
-// (nothing appended) when a component has no such rectangle
+// split tiles of member k (split_tiles for both components); false, with
+// nothing appended, when either component has no closed-form rectangle
 bool append_split_tiles(const BInst& I, int k, const std::vector<unsigned char>& fu,
-                        const std::vector<unsigned char>& fv, std::vector<RTile>& out) {
+                        const std::vector<unsigned char>& fv, int tx, int ty, std::vector<RTile>& out) {
+  (void)ty;
   std::vector<RTile> add;
-  for (int c = 0; c < 2; ++c) {
-    const std::vector<unsigned char>& f = c == 0 ? fu : fv;
-    const int nx = c == 0 ? I.nux : I.nvx, ny = c == 0 ? I.nuy : I.nvy;
-    if (f.size() != (size_t)nx * ny) return false;
-    int i0 = nx, i1 = 0, j0 = ny, j1 = 0;
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i)
-        if (f[(size_t)i + (size_t)nx * j]) {
-          i0 = std::min(i0, i); i1 = std::max(i1, i + 1);
-          j0 = std::min(j0, j); j1 = std::max(j1, j + 1);
-        }
-    if (i0 >= i1 || j0 >= j1) return false;
-    for (int j = j0; j < j1; ++j)
-      for (int i = i0; i < i1; ++i)
-        if (!f[(size_t)i + (size_t)nx * j]) return false;
-    auto cover = [&](int a0, int a1, int b0, int b1, int kind) {
-      if (a0 >= a1 || b0 >= b1) return;
-      const int w = std::min(a1 - a0, 64), h = std::max(1, std::min(b1 - b0, 256 / w));
-      for (int b = b0; b < b1; b += h)
-        for (int a = a0; a < a1; a += w) add.push_back(RTile{k, c | kind, a, b, std::min(w, a1 - a), std::min(h, b1 - b)});
-    };
-    cover(i0, i1, j0, j1, 2);
-    cover(0, i0, 0, ny, 0);
-    cover(i1, nx, 0, ny, 0);
-    cover(i0, i1, 0, j0, 0);
-    cover(i0, i1, j1, ny, 0);
-  }
+  if (!split_tiles(k, 0, I.nux, I.nuy, fu, tx, add) || !split_tiles(k, 1, I.nvx, I.nvy, fv, tx, add)) return false;
   out.insert(out.end(), add.begin(), add.end());
   return true;
 }
 
-// the generic-path right-hand-side tiles (tx x ty) of the members without a
-// closed form
-void build_generic_rhs_tiles(ins2d_batch* b, int tx, int ty, std::vector<Tile>& list);
+// the right-hand-side tile lists for interior tiles of tx x ty points: split
+// tiles of the members with a closed form, tx x ty tiles of the others
+void build_rhs_lists(ins2d_batch* b, int tx, int ty, std::vector<RTile>& rs, std::vector<Tile>& rg);
 
 void append_velocity_tiles(const BInst& I, int k, int tx, int ty, std::vector<Tile>& out) {
   for (int c = 0; c < 2; ++c) {
@@ -1025,26 +1045,58 @@ void rhs_tiles(ins2d_batch* b, const Tile* tiles, size_t ntiles, int tx, int ty,
   const dim3 g((unsigned)ntiles), t(tx, ty);
   if (tx % 64 == 0)
     hipLaunchKernelGGL(k_b_rhs<64>, g, t, 0, b->st, b->dinst, tiles, V, P, mode, out, sc, vt);
-  else if (tx == 32)
-    hipLaunchKernelGGL(k_b_rhs<32>, g, t, 0, b->st, b->dinst, tiles, V, P, mode, out, sc, vt);
-  else if (tx == 16)
-    hipLaunchKernelGGL(k_b_rhs<16>, g, t, 0, b->st, b->dinst, tiles, V, P, mode, out, sc, vt);
   else
     hipLaunchKernelGGL(k_b_rhs<0>, g, t, 0, b->st, b->dinst, tiles, V, P, mode, out, sc, vt);
 }
 
-void build_generic_rhs_tiles(ins2d_batch* b, int tx, int ty, std::vector<Tile>& list) {
-  list.clear();
+void build_rhs_lists(ins2d_batch* b, int tx, int ty, std::vector<RTile>& rs, std::vector<Tile>& rg) {
+  rs.clear();
+  rg.clear();
   for (int k = 0; k < b->n; ++k)
-    if (!b->has_closed[k]) append_velocity_tiles(b->hinst[k], k, tx, ty, list);
+    if (!append_split_tiles(b->hinst[k], k, b->ufu[k], b->ufv[k], tx, ty, rs))
+      append_velocity_tiles(b->hinst[k], k, tx, ty, rg);
+  order_split_tiles(rs);
+}
+
+size_t closed_count(const std::vector<RTile>& rs) {
+  size_t n = 0;
+  for (const RTile& r : rs) n += (r.comp & 2) ? 1 : 0;
+  return n;
+}
+
+void rhs_lists_launch(ins2d_batch* b, const RTile* drs, size_t nclosed, size_t nrs, const Tile* drg, size_t nrg,
+                      int tx, int ty, const double* V, const double* P, int mode, double* out, const StageCombine& sc,
+                      double* vt) {
+  if (nclosed)
+    hipLaunchKernelGGL(k_b_rhs_closed, dim3((unsigned)nclosed), dim3(256), 0, b->st, b->dinst, drs, V, P, mode, out,
+                       sc, vt);
+  if (nrs > nclosed)
+    hipLaunchKernelGGL(k_b_rhs_split, dim3((unsigned)(nrs - nclosed)), dim3(256), 0, b->st, b->dinst, drs + nclosed, V,
+                       P, mode, out, sc, vt);
+  if (nrg) rhs_tiles(b, drg, nrg, tx, ty, V, P, mode, out, sc, vt);
+}
+
+// upload rs and rg into the device lists (grown as needed)
+void upload_rhs_lists(ins2d_batch* b, std::vector<RTile>& rs, std::vector<Tile>& rg) {
+  if (!b->drs || rs.size() > b->rs_cap) {
+    b->drs = balloc<RTile>(b, rs.size());
+    b->rs_cap = rs.size();
+  }
+  if (!b->drg || rg.size() > b->rg_cap) {
+    b->drg = balloc<Tile>(b, rg.size());
+    b->rg_cap = rg.size();
+  }
+  if (!rs.empty()) BHIP(hipMemcpy(b->drs, rs.data(), rs.size() * sizeof(RTile), hipMemcpyHostToDevice));
+  if (!rg.empty()) BHIP(hipMemcpy(b->drg, rg.data(), rg.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  b->rs.swap(rs);
+  b->rg.swap(rg);
+  b->nclosed = closed_count(b->rs);
 }
 
 void launch_tiles_rhs(ins2d_batch* b, const double* V, const double* P, int mode, double* out, const StageCombine& sc,
                       double* vt) {
-  if (!b->rs.empty())
-    hipLaunchKernelGGL(k_b_rhs_split, dim3((unsigned)b->rs.size()), dim3(256), 0, b->st, b->dinst, b->drs, V, P, mode,
-                       out, sc, vt);
-  if (!b->rg.empty()) rhs_tiles(b, b->drg, b->rg.size(), b->tx, b->ty, V, P, mode, out, sc, vt);
+  rhs_lists_launch(b, b->drs, b->closed_ok ? b->nclosed : 0, b->rs.size(), b->drg, b->rg.size(), b->tx, b->ty, V, P,
+                   mode, out, sc, vt);
 }
 void launch_tiles_div(ins2d_batch* b, const double* A, const double* B, const double* dt, double ca, double cb,
                       double yc, double os, double* f) {
@@ -1259,7 +1311,6 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
       voff += P.NV;
       poff += P.Np;
       append_velocity_tiles(I, k, TX, TY, b->uvt);
-      b->has_closed.push_back(append_split_tiles(I, k, b->ufu[k], b->ufv[k], b->rs) ? 1 : 0);
       I.pt0 = (int)b->pt.size();
       for (int j = 0; j < I.npy; j += TY)
         for (int i = 0; i < I.npx; i += TX) b->pt.push_back(Tile{k, 2, i, j});
@@ -1291,13 +1342,12 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
     BHIP(hipMemcpy(b->dinst, b->hinst.data(), n * sizeof(BInst), hipMemcpyHostToDevice));
     b->duv = balloc<Tile>(b.get(), b->uvt.size());
     BHIP(hipMemcpy(b->duv, b->uvt.data(), b->uvt.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    b->drs = balloc<RTile>(b.get(), b->rs.size());
-    if (!b->rs.empty())
-      BHIP(hipMemcpy(b->drs, b->rs.data(), b->rs.size() * sizeof(RTile), hipMemcpyHostToDevice));
-    build_generic_rhs_tiles(b.get(), b->tx, b->ty, b->rg);
-    b->drg = balloc<Tile>(b.get(), b->rg.size());
-    if (!b->rg.empty())
-      BHIP(hipMemcpy(b->drg, b->rg.data(), b->rg.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    {
+      std::vector<RTile> rs;
+      std::vector<Tile> rg;
+      build_rhs_lists(b.get(), b->tx, b->ty, rs, rg);
+      upload_rhs_lists(b.get(), rs, rg);
+    }
     b->dpt = balloc<Tile>(b.get(), b->pt.size());
     BHIP(hipMemcpy(b->dpt, b->pt.data(), b->pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
     if (b->two_level) {
@@ -1373,52 +1423,55 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
 }
 
 // time the stage kernel (the momentum right-hand side over every velocity
-// tile) for a few workgroup shapes on the resident state and keep the fastest
+// point) for a few interior tile shapes on the resident state and keep the
+// fastest (closed-form interior tiles of the members with a closed form,
+// generic tiles of the others)
 int ins2d_batch_autotune(ins2d_batch* b, int* tx, int* ty) {
   if (!b) return INS2D_E_ARG;
   B_TRY
-  // 256 threads each (the kernels are compiled for at most 256)
+  // 256 points each (the kernels are compiled for at most 256 threads)
   static const int cand[][2] = {{64, 4}, {32, 8}, {128, 2}, {16, 16}, {256, 1}};
   hipEvent_t e0, e1;
   BHIP(hipEventCreate(&e0));
   BHIP(hipEventCreate(&e1));
   float best = 1e30f;
   int bx = b->tx, by = b->ty;
-  std::vector<Tile> best_list = b->rg;
   const StageCombine none{0, 0.0, {0}, {nullptr}};
-  // members with a closed form run over split tiles of their own: only the
-  // others depend on the shape
   for (const auto& c : cand) {
-    if (b->rg.empty()) break;
-    std::vector<Tile> list;
-    build_generic_rhs_tiles(b, c[0], c[1], list);
-    Tile* d = nullptr;
-    BHIP(hipMalloc(&d, list.size() * sizeof(Tile)));
-    BHIP(hipMemcpy(d, list.data(), list.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    std::vector<RTile> rs;
+    std::vector<Tile> rg;
+    build_rhs_lists(b, c[0], c[1], rs, rg);
+    RTile* d1 = nullptr;
+    Tile* d2 = nullptr;
+    BHIP(hipMalloc(&d1, std::max<size_t>(rs.size(), 1) * sizeof(RTile)));
+    BHIP(hipMalloc(&d2, std::max<size_t>(rg.size(), 1) * sizeof(Tile)));
+    if (!rs.empty()) BHIP(hipMemcpy(d1, rs.data(), rs.size() * sizeof(RTile), hipMemcpyHostToDevice));
+    if (!rg.empty()) BHIP(hipMemcpy(d2, rg.data(), rg.size() * sizeof(Tile), hipMemcpyHostToDevice));
     float tot = 0.f;
     for (int rep = 0; rep < 4; ++rep) {
       BHIP(hipEventRecord(e0, b->st));
-      rhs_tiles(b, d, list.size(), c[0], c[1], b->V, b->P, RHS_SCALE_OM, b->tV, none, nullptr);
+      rhs_lists_launch(b, d1, b->closed_ok ? closed_count(rs) : 0, rs.size(), d2, rg.size(), c[0], c[1], b->V, b->P,
+                       RHS_SCALE_OM, b->tV, none, nullptr);
       BHIP(hipEventRecord(e1, b->st));
       BHIP(hipEventSynchronize(e1));
       float ms = 0.f;
       BHIP(hipEventElapsedTime(&ms, e0, e1));
       if (rep > 0) tot += ms;   // the first launch warms caches
     }
-    BHIP(hipFree(d));
+    BHIP(hipFree(d1));
+    BHIP(hipFree(d2));
     if (tot < best) {
       best = tot;
       bx = c[0];
       by = c[1];
-      best_list.swap(list);
     }
   }
   BHIP(hipEventDestroy(e0));
   BHIP(hipEventDestroy(e1));
-  if (best_list.size() > b->rg.size()) b->drg = balloc<Tile>(b, best_list.size());
-  if (!best_list.empty())
-    BHIP(hipMemcpy(b->drg, best_list.data(), best_list.size() * sizeof(Tile), hipMemcpyHostToDevice));
-  b->rg.swap(best_list);
+  std::vector<RTile> rs;
+  std::vector<Tile> rg;
+  build_rhs_lists(b, bx, by, rs, rg);
+  upload_rhs_lists(b, rs, rg);
   b->tx = bx;
   b->ty = by;
   if (tx) *tx = bx;
@@ -1430,12 +1483,11 @@ int ins2d_batch_autotune(ins2d_batch* b, int* tx, int* ty) {
 int ins2d_batch_set_tile(ins2d_batch* b, int tx, int ty) {
   if (!b || tx * ty != 256 || !(tx == 16 || tx == 32 || tx == 64 || tx == 128 || tx == 256)) return INS2D_E_ARG;
   B_TRY
-  std::vector<Tile> list;
-  build_generic_rhs_tiles(b, tx, ty, list);
   BHIP(hipStreamSynchronize(b->st));
-  if (list.size() > b->rg.size()) b->drg = balloc<Tile>(b, list.size());
-  if (!list.empty()) BHIP(hipMemcpy(b->drg, list.data(), list.size() * sizeof(Tile), hipMemcpyHostToDevice));
-  b->rg.swap(list);
+  std::vector<RTile> rs;
+  std::vector<Tile> rg;
+  build_rhs_lists(b, tx, ty, rs, rg);
+  upload_rhs_lists(b, rs, rg);
   b->tx = tx;
   b->ty = ty;
   return INS2D_OK;

@@ -63,6 +63,16 @@ struct UniForm {
   const unsigned char* fast;
   double hx, hy, rhx, rhy, invRe;
   int ou, ov, nux, nuy, nvx, nvy;
+  // stretched tensor grids: cell widths hx (Nx), hy (Ny), line spacings gx
+  // (Nx + 1), gy (Ny + 1) and their reciprocals; null on uniform grids
+  const double *ahx, *ahy, *agx, *agy, *arhx, *arhy, *argx, *argy;
+};
+// a w x h block of points (w h <= 256) of one momentum component: comp bit
+// 0 the component, bit 1 set for points that take the closed form (the
+// interior rectangle), clear for the generic operators (the boundary ring);
+// inst: the ensemble member (0 for a single case)
+struct RTile {
+  int inst, comp, i0, j0, w, h;
 };
 struct DComp {
   int nx, ny, nconv, ndiff;

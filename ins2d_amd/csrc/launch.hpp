@@ -48,6 +48,10 @@ struct CGStateH {
 void launch_rhs_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,
                         const double* u, const double* v, const double* p, int mode, double* out_u, double* out_v,
                         const StageCombine& scu, const StageCombine& scv, double* vt_u, double* vt_v);
+// the same over split tiles (interior closed form, boundary ring generic)
+void launch_rhs_split(hipStream_t st, const DComp* dCu, const DComp* dCv, const RTile* tiles, int ntiles,
+                      const double* u, const double* v, const double* p, int mode, double* out_u, double* out_v,
+                      const StageCombine& scu, const StageCombine& scv, double* vt_u, double* vt_v);
 void launch_div_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int npx, int npy, const double* A,
                         const double* B, double ca, double cb, double ycoef, double oscale, int nu, double* f);
 void launch_proj_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,

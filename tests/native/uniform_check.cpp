@@ -12,13 +12,21 @@
 
 using namespace ins2d;
 
-static void run(const char* name, int Nx, int Ny, double Lx, const int bu[4], const int bv[4], double stretch) {
+// stretch: geometric growth factor of the cells in y; stretch_x the same in x
+static void run(const char* name, int Nx, int Ny, double Lx, const int bu[4], const int bv[4], double stretch,
+                double stretch_x = 1.0) {
   CaseDesc cd;
   cd.Nx = Nx;
   cd.Ny = Ny;
   cd.x.resize(Nx + 1);
   cd.y.resize(Ny + 1);
-  for (int i = 0; i <= Nx; ++i) cd.x[i] = i * (Lx / Nx);
+  {
+    std::vector<double> dx(Nx);
+    double h = 1.0, tot = 0.0;
+    for (int i = 0; i < Nx; ++i) { dx[i] = h; tot += h; h *= stretch_x; }
+    cd.x[0] = 0.0;
+    for (int i = 0; i < Nx; ++i) cd.x[i + 1] = stretch_x == 1.0 ? (i + 1) * (Lx / Nx) : cd.x[i] + Lx * dx[i] / tot;
+  }
   std::vector<double> dy(Ny);
   double h = 1.0, tot = 0.0;
   for (int j = 0; j < Ny; ++j) { dy[j] = h; tot += h; h *= stretch; }
@@ -31,8 +39,9 @@ static void run(const char* name, int Nx, int Ny, double Lx, const int bu[4], co
   Plan P;
   build_plan(cd, P);
   UniForm g{};
+  UniArrays A;
   std::vector<unsigned char> fu, fv;
-  uniform_maps(P, g, fu, fv);
+  uniform_maps(P, g, A, fu, fv);
   size_t su = 0, sv = 0;
   for (auto m : fu) su += m;
   for (auto m : fv) sv += m;
@@ -46,5 +55,6 @@ int main() {
   run("channel", 96, 24, 4.0, chu, chv, 1.0);
   run("periodic_x", 64, 32, 2.0, per, per, 1.0);
   run("stretched", 48, 40, 1.0, dir, dir, 1.05);
+  run("stretched_xy", 40, 36, 2.0, chu, chv, 1.04, 0.97);
   return 0;
 }

@@ -102,7 +102,7 @@ def test_fdm_ensemble_converges_in_two_iterations():
 
 
 def _stretched(m, s):
-    """The member on a geometrically stretched grid (no closed form there)."""
+    """The member on a geometrically stretched grid."""
     base = m.parameters
 
     def parameters():
@@ -115,11 +115,10 @@ def _stretched(m, s):
 
 @pytest.mark.parametrize('tile', [(64, 4), (32, 8), (16, 16), (128, 2), (256, 1)])
 def test_ensemble_stage_tile_shapes_match_oracle(tile):
-    """Uniform members run the closed form over their interior and the generic
-    operators over the boundary ring (split tiles); a stretched member runs the
-    generic path over tiles of every stage-tile shape (its face sharing in 64-,
-    32- and 16-lane row segments, and tiles wider than the member leave lanes
-    past the row's end).  All give the oracle's result."""
+    """Members run the closed form over their interior rectangle, in tiles of
+    every stage-tile shape, and the generic operators over the boundary ring
+    (split tiles); the stretched member's closed form reads its per-index
+    spacings.  All give the oracle's result."""
     _say('ensemble tile', tile)
     kw = dict(cg_tol=1e-13, precond='fdm')
     members = [ensemble.member('cavity', 24, 16, 1.5, 400.0, 1.0, **kw),

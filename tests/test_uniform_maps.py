@@ -7,8 +7,8 @@ On a uniform all-wall cavity exactly the interior points (one point away
 from every wall in the direction of each component's boundary closure) are
 marked, and the grid-line offsets are those of the reference layout
 (operator_mesh.m:185-320: u on x(2:end-1), v on y(2:end-1)); a channel with
-a 'pres' outflow and a periodic direction keep their interiors; a stretched
-grid marks nothing.  The device kernels are checked against the oracle by the
+a 'pres' outflow and a periodic direction keep their interiors; stretched
+grids (the closed form with per-index spacings) mark their interiors too.  The device kernels are checked against the oracle by the
 GPU parity tests; this pins the classification itself."""
 import os
 import subprocess
@@ -47,5 +47,13 @@ def test_channel_and_periodic_keep_their_interiors(rows):
         assert 0.85 * nu < su < nu and 0.85 * nv < sv < nv
 
 
-def test_stretched_grid_marks_nothing(rows):
-    assert rows['stretched'][4:6] == [0, 0]
+def test_stretched_grids_mark_their_interior(rows):
+    """The closed form with per-index spacings (stretched tensor grids, as the
+    backward-facing step's tanh mesh) reproduces the generic operators at the
+    interior points: a cavity stretched in y marks exactly its interior, a
+    channel stretched in x and y its interior minus the outflow column."""
+    ou, ov, nu, nv, su, sv, nux, nvy = rows['stretched']
+    assert (nux, nvy) == (47, 39)
+    assert su == 45 * 38 and sv == 46 * 37
+    ou, ov, nu, nv, su, sv, nux, nvy = rows['stretched_xy']
+    assert 0.8 * nu < su < nu and 0.8 * nv < sv < nv
