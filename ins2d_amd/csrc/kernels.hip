@@ -221,6 +221,7 @@ __global__ void __launch_bounds__(256) k_rhs_per(PerGrid G, const double* __rest
 // diffusive faces j+3/2) and takes the older ones from registers, where the
 // rows above left them; the x-direction terms are computed per row.
 constexpr int kStripW = 64;
+template <bool BAND>
 __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __restrict__ u,
                                                  const double* __restrict__ v, const double* __restrict__ p, int mode,
                                                  double* __restrict__ out, StageCombine sc,
@@ -229,7 +230,8 @@ __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __res
   __shared__ double sv[7][kStripW + 6];
   const int tx = threadIdx.x;
   const int Nx = G.Nx, Ny = G.Ny;
-  const int x0 = blockIdx.x * kStripW, j0 = blockIdx.y * H;
+  const int2 bxy = block_xy<BAND>();   // strips of one XCD side by side
+  const int x0 = bxy.x * kStripW, j0 = bxy.y * H;
   const int i = x0 + tx;
   // LDS column c holds x = x0 - 3 + c (periodic)
   const int xa = ((x0 - 3 + tx) % Nx + Nx) % Nx;
@@ -400,6 +402,7 @@ __global__ void __launch_bounds__(256) k_div_per(PerGrid G, const double* __rest
 // buffers, so one barrier per row); the v column values of rows j-1 .. j+2 of
 // this thread's column carried in registers from row to row.  The raw A and B
 // values of the next row are loaded one row ahead and combined when stored.
+template <bool BAND>
 __global__ void __launch_bounds__(64) k_div_per4s(PerGrid G, const double* __restrict__ A,
                                                  const double* __restrict__ B, double ca, double cb, double oscale,
                                                  int H, double* __restrict__ f) {
@@ -407,7 +410,8 @@ __global__ void __launch_bounds__(64) k_div_per4s(PerGrid G, const double* __res
   const int tx = threadIdx.x;
   const int Nx = G.Nx, Ny = G.Ny;
   const size_t N = (size_t)Nx * Ny;
-  const int x0 = blockIdx.x * kStripW, j0 = blockIdx.y * H;
+  const int2 bxy = block_xy<BAND>();   // strips of one XCD side by side
+  const int x0 = bxy.x * kStripW, j0 = bxy.y * H;
   const int i = x0 + tx;
   // LDS column c holds x = x0 - 1 + c (periodic); c = tx and, for tx < 3, 64 + tx
   const int xa = ((x0 - 1 + tx) % Nx + Nx) % Nx;
@@ -1296,6 +1300,7 @@ static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / 
 thread_local KernelTimer* g_ktimer = nullptr;
 
 
+
 // rows per strip of the row-streaming kernels: 64, halved (down to 8) while
 // the grid would give fewer than 4096 strips (one wave each, 16 per CU)
 static int strip_height(int Nx, int Ny) {
@@ -1348,7 +1353,7 @@ void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double
   if (order == 2)
     launch_timed(k_rhs_per<2, true>, grid2(Nx, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
   else
-    launch_timed(k_rhs_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + H - 1) / H), dim3(kStripW), 0, st, G, u, v, p,
+    launch_timed(k_rhs_per4s<true>, dim3((Nx + kStripW - 1) / kStripW, (Ny + H - 1) / H), dim3(kStripW), 0, st, G, u, v, p,
                  mode, out, sc, vt, H);
 }
 
@@ -1360,7 +1365,7 @@ void launch_div_per(hipStream_t st, int order, int Nx, int Ny, double hx, double
   if (order == 2)
     launch_timed(k_div_per<2, true>, grid2(Nx, Ny, b), b, 0, st, G, A, B, ca, cb, oscale, f);
   else
-    launch_timed(k_div_per4s, dim3((Nx + kStripW - 1) / kStripW, (Ny + H - 1) / H), dim3(kStripW), 0, st, G, A, B,
+    launch_timed(k_div_per4s<true>, dim3((Nx + kStripW - 1) / kStripW, (Ny + H - 1) / H), dim3(kStripW), 0, st, G, A, B,
                  ca, cb, oscale, H, f);
 }
 
