@@ -1049,13 +1049,34 @@ void rhs_tiles(ins2d_batch* b, const Tile* tiles, size_t ntiles, int tx, int ty,
     hipLaunchKernelGGL(k_b_rhs<0>, g, t, 0, b->st, b->dinst, tiles, V, P, mode, out, sc, vt);
 }
 
+// Launch order of a tile list: blocks reach the 8 XCDs round-robin (block i
+// on XCD i mod 8), so XCD x is given the contiguous chunk of n / 8 tiles
+// (one more for x < n mod 8) starting at x (n / 8) + min(x, n mod 8):
+// neighbouring tiles of a member, whose halos share cache lines, then run
+// on one XCD.  A permutation of [begin, end).
+template <class T>
+void xcd_order(std::vector<T>& v, size_t begin, size_t end) {
+  const size_t n = end - begin;
+  if (n < 16) return;
+  const size_t q = n / 8, r = n % 8;
+  std::vector<T> out(n);
+  for (size_t i = 0; i < n; ++i) {
+    const size_t x = i & 7;
+    out[i] = v[begin + x * q + std::min(x, r) + (i >> 3)];
+  }
+  std::copy(out.begin(), out.end(), v.begin() + begin);
+}
+
 void build_rhs_lists(ins2d_batch* b, int tx, int ty, std::vector<RTile>& rs, std::vector<Tile>& rg) {
   rs.clear();
   rg.clear();
   for (int k = 0; k < b->n; ++k)
     if (!append_split_tiles(b->hinst[k], k, b->ufu[k], b->ufv[k], tx, ty, rs))
       append_velocity_tiles(b->hinst[k], k, tx, ty, rg);
-  order_split_tiles(rs);
+  const size_t nc = (size_t)order_split_tiles(rs);
+  xcd_order(rs, 0, nc);
+  xcd_order(rs, nc, rs.size());
+  xcd_order(rg, 0, rg.size());
 }
 
 size_t closed_count(const std::vector<RTile>& rs) {
@@ -1340,6 +1361,7 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
     b_upload_bc(b.get());
     b->dinst = balloc<BInst>(b.get(), n);
     BHIP(hipMemcpy(b->dinst, b->hinst.data(), n * sizeof(BInst), hipMemcpyHostToDevice));
+    xcd_order(b->uvt, 0, b->uvt.size());
     b->duv = balloc<Tile>(b.get(), b->uvt.size());
     BHIP(hipMemcpy(b->duv, b->uvt.data(), b->uvt.size() * sizeof(Tile), hipMemcpyHostToDevice));
     {
