@@ -633,6 +633,29 @@ void poisson(ins2d_handle* h, const double* f, double* dp) {
   if (flag[2]) throw NoConv("CG did not converge in maxit iterations");
 }
 
+// A dp = f with f = oscale (M (ca A + cb B) + ycoef yM); acc: out += dp,
+// else out = dp.  With the register-transform FFT solve the update is done
+// by the last transform, and on the second-order periodic kernels f is
+// computed inside the first transform (never stored); h->f and h->dp then
+// keep their earlier contents.
+void div_poisson(ins2d_handle* h, const double* A, const double* B, double ca, double cb, double ycoef, double oscale,
+                 double* out, bool acc) {
+  const bool reg = h->poisson == INS2D_POISSON_FFT && fft_register_lengths(h->Nx, h->Ny);
+  if (reg && h->path == 2) {
+    ProfScope ps(h, 2);
+    launch_fft_poisson_fused(h->st, h->fft, nullptr, h->hx, h->hy, A, B, ca, cb, oscale, out, acc);
+    return;
+  }
+  div(h, A, B, ca, cb, ycoef, oscale, h->f);
+  if (reg && acc) {
+    ProfScope ps(h, 2);
+    launch_fft_poisson_fused(h->st, h->fft, h->f, h->hx, h->hy, nullptr, nullptr, 0.0, 0.0, 0.0, out, true);
+    return;
+  }
+  poisson(h, h->f, acc ? h->dp : out);
+  if (acc) launch_axpy(h->st, (size_t)h->Np, 1.0, h->dp, out);
+}
+
 // p <- p + A^-1 (M Om^-1 F(V, p) + ydM)   (pressure_additional_solve.m:19-25);
 // rate_cu/rate_cv: operator records whose yM is ydM (null: ydM = 0)
 void additional_solve(ins2d_handle* h, const double* V, DComp* rate_cu = nullptr, DComp* rate_cv = nullptr) {
@@ -644,11 +667,11 @@ void additional_solve(ins2d_handle* h, const double* V, DComp* rate_cu = nullptr
     div(h, h->tV, nullptr, 1.0, 0.0, 1.0, 1.0, h->f);
     h->dCu = cu;
     h->dCv = cv;
+    poisson(h, h->f, h->dp);
+    launch_axpy(h->st, (size_t)h->Np, 1.0, h->dp, h->p);
   } else {
-    div(h, h->tV, nullptr, 1.0, 0.0, 0.0, 1.0, h->f);
+    div_poisson(h, h->tV, nullptr, 1.0, 0.0, 0.0, 1.0, h->p, true);
   }
-  poisson(h, h->f, h->dp);
-  launch_axpy(h->st, (size_t)h->Np, 1.0, h->dp, h->p);
 }
 
 // one step of time_ERK.m.  ver: for time-dependent boundary values, the
@@ -688,8 +711,7 @@ void erk_step(ins2d_handle* h, double tn, double dt, DComp* ver = nullptr) {
     if (ver) use(i + 1);
     if (ver || i > 0) {
       // f = (M (Vn/dt + Vtemp) + yM/dt) / c_i                 (time_ERK.m:117)
-      div(h, Vn, h->Vt, 1.0 / dt, 1.0, 1.0 / dt, 1.0 / c, h->f);
-      poisson(h, h->f, h->dp);
+      div_poisson(h, Vn, h->Vt, 1.0 / dt, 1.0, 1.0 / dt, 1.0 / c, h->dp, false);
       dpp = h->dp;
     } else {
       dpp = h->p;   // steady BC, first stage: dp = pn (time_ERK.m:125-126)

@@ -16,6 +16,8 @@
 #include "launch.hpp"
 #include "stencil.hpp"
 
+#include <stdexcept>
+
 namespace ins2d {
 
 // ===========================================================================
@@ -784,21 +786,57 @@ __device__ __forceinline__ int xcd_row_pair(int b, int nb) {
   return x * q + min(x, r) + (b >> 3);
 }
 
+// Right-hand side of a forward row transform taken straight from the
+// velocity: the second-order periodic divergence of k_div_per,
+// f = oscale (hy (W_u(i+1, j) - W_u(i, j)) + hx (W_v(i, j+1) - W_v(i, j))),
+// W = ca A + cb B (B null: ca A), u at A[i + Nx j], v at A[Nx Ny + i + Nx j]
+struct DivSrc {
+  const double* A;
+  const double* B;
+  double ca, cb, oscale, hx, hy;
+};
+
 // rows y0 = 2b and y1 = 2b+1 packed as one complex row, half spectra out
 // (power-of-two Nx in [E, 4096], max(64, Nx / E) threads).  TR: the spectra
 // are stored transposed, H[k Ny + y] (Nh x Ny), ready for the column solve.
-template <int E, bool TR>
+// DIV: the rows are the divergence of D (computed in the first pass, f is
+// not read), else the rows of f.
+template <int E, bool TR, bool DIV>
 __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_r2c_rows(int Nx, int Ny, int Nh, int log2N,
                                                        const double2* __restrict__ tw, const double* __restrict__ f,
-                                                       double2* __restrict__ H) {
+                                                       DivSrc D, double2* __restrict__ H) {
   extern __shared__ double2 z[];
   const int y0 = 2 * (TR ? xcd_row_pair(blockIdx.x, gridDim.x) : (int)blockIdx.x), y1 = y0 + 1;
   const bool has1 = y1 < Ny;
-  const double* f0 = f + (size_t)y0 * Nx;
-  const double* f1 = f + (size_t)(has1 ? y1 : y0) * Nx;
-  auto ld = [&](int, int n) { return make_double2(f0[n], has1 ? f1[n] : 0.0); };
   auto st = [&](int, int n, double2 x) { z[swz(n)] = x; };
-  fft_reg<E, false, true>(z, Nx, log2N, tw, ld, st);
+  if constexpr (DIV) {
+    // rows ya = y0 + 1 and yb = y0 + 2 (periodic): ya = y1 when has1; the
+    // v values of row ya enter both divergences.  The two rows are staged
+    // in LDS first (computed inside the first pass, the 14 loads per
+    // element of all its slots at once exceed the register budget), and
+    // transformed in place
+    const int ya = y0 + 1 == Ny ? 0 : y0 + 1, yb = ya + 1 == Ny ? 0 : ya + 1;
+    const double* __restrict__ A = D.A;
+    const double* __restrict__ B = D.B;
+    const size_t N = (size_t)Nx * Ny;
+    const size_t r0 = (size_t)y0 * Nx, ra = (size_t)ya * Nx, rb = (size_t)yb * Nx;
+    auto W = [&](size_t k) { return B ? D.ca * A[k] + D.cb * B[k] : D.ca * A[k]; };
+    for (int n = threadIdx.x; n < Nx; n += blockDim.x) {
+      const int np = (n + 1) & (Nx - 1);
+      const double va = W(N + n + ra);
+      const double s0 = D.hy * (W(np + r0) - W(n + r0)) + D.hx * (va - W(N + n + r0));
+      const double s1 = has1 ? D.hy * (W(np + ra) - W(n + ra)) + D.hx * (W(N + n + rb) - va) : 0.0;
+      z[swz(n)] = make_double2(D.oscale * s0, D.oscale * s1);
+    }
+    __syncthreads();
+    auto ld = [&](int, int n) { return z[swz(n)]; };
+    fft_reg<E, false, true>(z, Nx, log2N, tw, ld, st);
+  } else {
+    const double* f0 = f + (size_t)y0 * Nx;
+    const double* f1 = f + (size_t)(has1 ? y1 : y0) * Nx;
+    auto ld = [&](int, int n) { return make_double2(f0[n], has1 ? f1[n] : 0.0); };
+    fft_reg<E, false, true>(z, Nx, log2N, tw, ld, st);
+  }
   for (int k = threadIdx.x; k < Nh; k += blockDim.x) {
     const double2 a = z[swz(k)], b = conjd(z[swz((Nx - k) & (Nx - 1))]);
     const double2 h0 = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
@@ -817,8 +855,9 @@ __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_r2c_rows(int Nx, int N
 
 // inverse of k_fftr_r2c_rows: the two half spectra as one complex row
 // a + i b, transformed and split into the real rows y0 and y1
-// (TR: the spectra are read transposed, H[k Ny + y])
-template <int E, bool TR>
+// (TR: the spectra are read transposed, H[k Ny + y]; ACC: the rows are
+// added into out instead of stored)
+template <int E, bool TR, bool ACC>
 __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_c2r_rows(int Nx, int Ny, int Nh, int log2N,
                                                        const double2* __restrict__ tw, const double2* __restrict__ H,
                                                        double* __restrict__ out) {
@@ -830,6 +869,24 @@ __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_c2r_rows(int Nx, int N
   const double2* H1 = H + (TR ? (size_t)(has1 ? y1 : y0) : (size_t)(has1 ? y1 : y0) * Nh);
   double* o0 = out + (size_t)y0 * Nx;
   double* o1 = out + (size_t)(has1 ? y1 : y0) * Nx;
+  // ACC: the values the last pass adds to, loaded up front into the
+  // register slots of its stores (slot s = c R + m takes element
+  // tid + c T + m Q, R the last pass's radix, Q = Nx / R: the remainder
+  // radix when log2 Nx is not a multiple of log2 E, else E), so that the
+  // loads overlap the transform instead of waiting behind the stores
+  double pre0[E], pre1[E];
+  if constexpr (ACC) {
+    constexpr int lgE = E == 16 ? 4 : (E == 8 ? 3 : 2);
+    const int rem = log2N % lgE;
+    const int R = rem ? (1 << rem) : E, Q = Nx / R, T = blockDim.x;
+#pragma unroll
+    for (int sl = 0; sl < E; ++sl) {
+      const int c = sl / R, m = sl % R, jq = threadIdx.x + c * T;
+      const bool ok = c < E / R && jq < Q;
+      pre0[sl] = ok ? o0[jq + m * Q] : 0.0;
+      pre1[sl] = ok ? o1[jq + m * Q] : 0.0;
+    }
+  }
   // the full row a + i b, staged in LDS (the half spectra read once)
   for (int k = threadIdx.x; k < Nh; k += blockDim.x) {
     double2 a = H0[k * ks], b = has1 ? H1[k * ks] : make_double2(0.0, 0.0);
@@ -839,9 +896,9 @@ __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_c2r_rows(int Nx, int N
   }
   __syncthreads();
   auto ld = [&](int, int n) { return z[swz(n)]; };
-  auto st = [&](int, int n, double2 x) {
-    o0[n] = x.x;
-    if (has1) o1[n] = x.y;
+  auto st = [&](int sl, int n, double2 x) {
+    o0[n] = ACC ? pre0[sl] + x.x : x.x;
+    if (has1) o1[n] = ACC ? pre1[sl] + x.y : x.y;
   };
   fft_reg<E, true, false>(z, Nx, log2N, tw, ld, st);
 }
@@ -1403,34 +1460,59 @@ static int fft_elems(int N, bool cols) {
   return 8;
 }
 
-// TR: the row transforms store (load) the spectra transposed
+// TR: the row transforms store (load) the spectra transposed; D: the first
+// transform computes its rows as the divergence of D->A, D->B (f unused);
+// acc: the last transform adds dp into its output
 template <int E, bool TR>
-static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, double* dp, int which) {
+static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, double* dp, int which, const DivSrc* D,
+                        bool acc) {
   const int Nx = P.Nx, Ny = P.Ny, Nh = P.Nh;
   double2* H = (double2*)(TR ? P.HT : P.H);
   double2* HT = (double2*)P.HT;
-  if (which == 0)
-    hipLaunchKernelGGL((k_fftr_r2c_rows<E, TR>), dim3((Ny + 1) / 2), dim3(std::max(64, Nx / E)),
-                       (size_t)Nx * sizeof(double2), st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx, f, H);
-  else if (which == 1)
+  const dim3 gr((Ny + 1) / 2), bl(std::max(64, Nx / E));
+  const size_t sh = (size_t)Nx * sizeof(double2);
+  if (which == 0) {
+    if (D)
+      hipLaunchKernelGGL((k_fftr_r2c_rows<E, TR, true>), gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx),
+                         (const double2*)P.twx, nullptr, *D, H);
+    else
+      hipLaunchKernelGGL((k_fftr_r2c_rows<E, TR, false>), gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx),
+                         (const double2*)P.twx, f, DivSrc{}, H);
+  } else if (which == 1) {
     hipLaunchKernelGGL(k_fftr_cols_solve<E>, dim3(Nh), dim3(std::max(64, Ny / E)), (size_t)Ny * sizeof(double2), st,
                        Nx, Ny, ilog2(Ny), (const double2*)P.twy, P.ax, P.ay, P.c, HT);
-  else
-    hipLaunchKernelGGL((k_fftr_c2r_rows<E, TR>), dim3((Ny + 1) / 2), dim3(std::max(64, Nx / E)),
-                       (size_t)Nx * sizeof(double2), st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx, H, dp);
+  } else if (acc) {
+    hipLaunchKernelGGL((k_fftr_c2r_rows<E, TR, true>), gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx,
+                       H, dp);
+  } else {
+    hipLaunchKernelGGL((k_fftr_c2r_rows<E, TR, false>), gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx,
+                       H, dp);
+  }
 }
 
 template <bool TR>
-static void fftr(hipStream_t st, const FFTPlan& P, const double* f, double* dp, int which) {
+static void fftr(hipStream_t st, const FFTPlan& P, const double* f, double* dp, int which,
+                 const DivSrc* D = nullptr, bool acc = false) {
   const int E = which == 1 ? fft_elems(P.Ny, true) : fft_elems(P.Nx, false);
-  if (E == 4) fftr_launch<4, TR>(st, P, f, dp, which);
-  else if (E == 8) fftr_launch<8, TR>(st, P, f, dp, which);
-  else fftr_launch<16, TR>(st, P, f, dp, which);
+  if (E == 4) fftr_launch<4, TR>(st, P, f, dp, which, D, acc);
+  else if (E == 8) fftr_launch<8, TR>(st, P, f, dp, which, D, acc);
+  else fftr_launch<16, TR>(st, P, f, dp, which, D, acc);
 }
 
 bool fft_register_lengths(int Nx, int Ny) {
   auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
   return fftr_ok(Nx, pow2(Nx)) && fftr_ok(Ny, pow2(Ny));
+}
+
+void launch_fft_poisson_fused(hipStream_t st, const FFTPlan& P, const double* f, double hx, double hy,
+                              const double* A, const double* B, double ca, double cb, double oscale, double* out,
+                              bool acc) {
+  if (!fft_register_lengths(P.Nx, P.Ny))
+    throw std::runtime_error("fused FFT solve: needs power-of-two lengths 16 .. 4096");
+  const DivSrc D{A, B, ca, cb, oscale, hx, hy};
+  fftr<true>(st, P, f, out, 0, A ? &D : nullptr);
+  fftr<true>(st, P, f, out, 1);
+  fftr<true>(st, P, f, out, 2, nullptr, acc);
 }
 
 void launch_fft_poisson(hipStream_t st, const FFTPlan& P, const double* f, double* dp) {

@@ -63,6 +63,14 @@ void launch_div_per(hipStream_t st, int order, int Nx, int Ny, double hx, double
 void launch_proj_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* Vn,
                      const double* Vt, const double* dp, double dt, double c, double* V);
 void launch_fft_poisson(hipStream_t st, const FFTPlan& P, const double* f, double* dp);
+// the register-transform solve (fft_register_lengths only) with its ends
+// fused: A non-null: the right-hand side is the second-order periodic
+// divergence oscale (hy dx + hx dy)(ca A + cb B) of the velocity A, B
+// (B may be null), computed in the first transform (f unused), else f;
+// acc: out += dp, else out = dp
+void launch_fft_poisson_fused(hipStream_t st, const FFTPlan& P, const double* f, double hx, double hy,
+                              const double* A, const double* B, double ca, double cb, double oscale, double* out,
+                              bool acc);
 // both lengths take the register transforms (no row-major spectra buffer)
 bool fft_register_lengths(int Nx, int Ny);
 void launch_lap(hipStream_t st, const DLap& L, const double* p, double* q);

@@ -239,6 +239,17 @@ def test_periodic_run_matches_oracle(order4, rk):
     _compare_runs(case, 10, False)
 
 
+@pytest.mark.parametrize('Nx,Ny,kernels', [(128, 32, 0), (32, 128, 0), (64, 32, 1)])
+def test_periodic_rectangular_run_matches_oracle(Nx, Ny, kernels):
+    """Periodic runs with hx != hy through the fused ends of the FFT solve:
+    on the second-order periodic kernels (kernels=0) the first row transform
+    computes the divergence of the stage velocity itself, and on every path
+    the additional solve's last transform adds dp into p."""
+    _say('periodic rectangular run', Nx, Ny, kernels)
+    case = load('taylor_green', Nx=Nx, Ny=Ny, Re=500.0, noise=1e-2, seed=3, dt=0.01, t_end=0.05, kernels=kernels)
+    _compare_runs(case, 10, False)
+
+
 def test_large_periodic_invariants():
     """1024^2 Taylor-Green with noise, inviscid: after steps the field stays
     discretely divergence-free, total momentum is conserved to round-off and
