@@ -797,10 +797,11 @@ struct DivSrc {
 };
 
 // rows y0 = 2b and y1 = 2b+1 packed as one complex row, half spectra out
-// (power-of-two Nx in [E, 4096], max(64, Nx / E) threads).  TR: the spectra
+// (power-of-two Nx in [E, 4096], at least max(64, Nx / E) threads: the
+// passes leave any beyond Nx / E idle).  TR: the spectra
 // are stored transposed, H[k Ny + y] (Nh x Ny), ready for the column solve.
-// DIV: the rows are the divergence of D (computed in the first pass, f is
-// not read), else the rows of f.
+// DIV: the rows are the divergence of D (computed into LDS before the
+// transform, f is not read), else the rows of f.
 template <int E, bool TR, bool DIV>
 __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_r2c_rows(int Nx, int Ny, int Nh, int log2N,
                                                        const double2* __restrict__ tw, const double* __restrict__ f,
@@ -1472,8 +1473,11 @@ static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, doubl
   const dim3 gr((Ny + 1) / 2), bl(std::max(64, Nx / E));
   const size_t sh = (size_t)Nx * sizeof(double2);
   if (which == 0) {
+    // the divergence rows are staged by a plain loop before the transform:
+    // as many threads as the launch bound allows keep more of its loads in
+    // flight (the transform's passes leave the extra threads idle)
     if (D)
-      hipLaunchKernelGGL((k_fftr_r2c_rows<E, TR, true>), gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx),
+      hipLaunchKernelGGL((k_fftr_r2c_rows<E, TR, true>), gr, dim3(fft_block<E>(0)), sh, st, Nx, Ny, Nh, ilog2(Nx),
                          (const double2*)P.twx, nullptr, *D, H);
     else
       hipLaunchKernelGGL((k_fftr_r2c_rows<E, TR, false>), gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx),
@@ -1481,12 +1485,16 @@ static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, doubl
   } else if (which == 1) {
     hipLaunchKernelGGL(k_fftr_cols_solve<E>, dim3(Nh), dim3(std::max(64, Ny / E)), (size_t)Ny * sizeof(double2), st,
                        Nx, Ny, ilog2(Ny), (const double2*)P.twy, P.ax, P.ay, P.c, HT);
-  } else if (acc) {
-    hipLaunchKernelGGL((k_fftr_c2r_rows<E, TR, true>), gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx,
-                       H, dp);
   } else {
-    hipLaunchKernelGGL((k_fftr_c2r_rows<E, TR, false>), gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx,
-                       H, dp);
+    // the half spectra are staged by a plain loop before the transform (as
+    // the divergence rows above): threads up to the launch bound
+    const dim3 blc(fft_block<E>(0));
+    if (acc)
+      hipLaunchKernelGGL((k_fftr_c2r_rows<E, TR, true>), gr, blc, sh, st, Nx, Ny, Nh, ilog2(Nx),
+                         (const double2*)P.twx, H, dp);
+    else
+      hipLaunchKernelGGL((k_fftr_c2r_rows<E, TR, false>), gr, blc, sh, st, Nx, Ny, Nh, ilog2(Nx),
+                         (const double2*)P.twx, H, dp);
   }
 }
 

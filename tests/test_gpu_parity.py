@@ -67,12 +67,15 @@ def test_rhs_matches_oracle(name, kernels):
     solver.close()
 
 
-@pytest.mark.parametrize('Nx,Ny', [(1024, 1024), (4096, 128), (128, 4096), (2048, 32), (16, 16), (512, 256)])
+@pytest.mark.parametrize('Nx,Ny', [(1024, 1024), (4096, 128), (128, 4096), (2048, 32), (16, 16), (512, 256),
+                                   (64, 48), (48, 64)])
 def test_fft_solve_register_transform_lengths(Nx, Ny):
     """The power-of-two FFT kernels pick 4, 8 or 16 elements per thread by
     length and kernel (row transforms: 16 at 4096, 4 at 1024, else 8; column
     solve: 4 at 1024 and 4096, else 8); each choice and each remainder pass
-    (log2 N mod 2, 3, 4) against the oracle's numpy FFT solve.  The solve's
+    (log2 N mod 2, 3, 4) against the oracle's numpy FFT solve; (64, 48) and
+    (48, 64) mix them with the LDS transforms and explicit transposes of the
+    other lengths (row-major spectra).  The solve's
     condition number grows like N^2, so the large grids are compared at
     1e-11 (max norm, relative) and the residual at 1e-10 max|f|."""
     _say('fft lengths', Nx, Ny)
