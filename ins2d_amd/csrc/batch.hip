@@ -768,6 +768,9 @@ struct ins2d_batch {
   std::vector<bool> keep;
   hipGraphExec_t cg_chunk = nullptr;
   hipGraph_t cg_graph = nullptr;
+  // fast-diagonalization PCG: the tail of an iteration (cg_chunk holds its head)
+  hipGraphExec_t cg_tail = nullptr;
+  hipGraph_t cg_tail_graph = nullptr;
   // stage-kernel tile shape (velocity tiles), chosen by ins2d_batch_autotune
   int tx = TX, ty = TY;
   // two-level PCG (cg_precond = 2)
@@ -1130,22 +1133,28 @@ void launch_tiles_proj(ins2d_batch* b, const double* Vn, const double* Vt, const
                      dt, c, V);
 }
 
+// fast-diagonalization PCG iteration, head: q = A p, alpha, x and r, the
+// convergence test (converged instances stop here)
+void fdm_iter_head(ins2d_batch* b) {
+  const unsigned np = (unsigned)b->pt.size();
+  hipLaunchKernelGGL(k_b_cg_apply, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q);
+  hipLaunchKernelGGL(k_b_cg_alpha, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
+  hipLaunchKernelGGL(k_bf_update, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q, b->x,
+                     b->r);
+  hipLaunchKernelGGL(k_bf_test, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
+}
+
+// tail: z = A^-1 r for the instances still iterating, r.z, beta, direction
+void fdm_iter_tail(ins2d_batch* b) {
+  const unsigned np = (unsigned)b->pt.size();
+  fdm_apply(b, b->r, b->z);
+  hipLaunchKernelGGL(k_bf_rz, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->r, b->z);
+  hipLaunchKernelGGL(k_bf_beta, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
+  hipLaunchKernelGGL(k_b_cg_dir, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->z, b->pp);
+}
+
+// one iteration of the Jacobi or two-level PCG (replayed in chunks)
 void cg_iter_launches(ins2d_batch* b) {
-  if (b->fdm) {
-    const unsigned np = (unsigned)b->pt.size();
-    hipLaunchKernelGGL(k_b_cg_apply, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q);
-    hipLaunchKernelGGL(k_b_cg_alpha, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
-    hipLaunchKernelGGL(k_bf_update, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q, b->x,
-                       b->r);
-    // converged instances stop here: the preconditioner is applied only to
-    // those still iterating
-    hipLaunchKernelGGL(k_bf_test, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
-    fdm_apply(b, b->r, b->z);
-    hipLaunchKernelGGL(k_bf_rz, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->r, b->z);
-    hipLaunchKernelGGL(k_bf_beta, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
-    hipLaunchKernelGGL(k_b_cg_dir, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->z, b->pp);
-    return;
-  }
   if (b->two_level) {
     const unsigned na = (unsigned)b->at.size();
     const dim3 tb(kAgg, kAgg);
@@ -1190,18 +1199,42 @@ void b_poisson(ins2d_batch* b, const double* f) {
                        b->pp);
     hipLaunchKernelGGL(k_b_cg_init_fin, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tol * b->tol);
   }
-  if (!b->cg_chunk) {
-    BHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-    for (int c = 0; c < b->chunk; ++c) cg_iter_launches(b);
-    BHIP(hipStreamEndCapture(b->st, &b->cg_graph));
-    BHIP(hipGraphInstantiate(&b->cg_chunk, b->cg_graph, nullptr, nullptr, 0));
-  }
   int active = b->n;
-  while (true) {
-    BHIP(hipGraphLaunch(b->cg_chunk, b->st));
-    BHIP(hipMemcpyAsync(&active, b->cg.nactive, sizeof(int), hipMemcpyDeviceToHost, b->st));
-    BHIP(hipStreamSynchronize(b->st));
-    if (active <= 0) break;
+  if (b->fdm) {
+    // with the exact inverse as preconditioner one iteration is the rule:
+    // the host checks after the head of each iteration, so a converged
+    // solve ends before the tail (whose launches would find every instance
+    // done); the tail runs only for another iteration
+    if (!b->cg_chunk) {
+      BHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+      fdm_iter_head(b);
+      BHIP(hipStreamEndCapture(b->st, &b->cg_graph));
+      BHIP(hipGraphInstantiate(&b->cg_chunk, b->cg_graph, nullptr, nullptr, 0));
+      BHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+      fdm_iter_tail(b);
+      BHIP(hipStreamEndCapture(b->st, &b->cg_tail_graph));
+      BHIP(hipGraphInstantiate(&b->cg_tail, b->cg_tail_graph, nullptr, nullptr, 0));
+    }
+    while (true) {
+      BHIP(hipGraphLaunch(b->cg_chunk, b->st));
+      BHIP(hipMemcpyAsync(&active, b->cg.nactive, sizeof(int), hipMemcpyDeviceToHost, b->st));
+      BHIP(hipStreamSynchronize(b->st));
+      if (active <= 0) break;
+      BHIP(hipGraphLaunch(b->cg_tail, b->st));
+    }
+  } else {
+    if (!b->cg_chunk) {
+      BHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+      for (int c = 0; c < b->chunk; ++c) cg_iter_launches(b);
+      BHIP(hipStreamEndCapture(b->st, &b->cg_graph));
+      BHIP(hipGraphInstantiate(&b->cg_chunk, b->cg_graph, nullptr, nullptr, 0));
+    }
+    while (true) {
+      BHIP(hipGraphLaunch(b->cg_chunk, b->st));
+      BHIP(hipMemcpyAsync(&active, b->cg.nactive, sizeof(int), hipMemcpyDeviceToHost, b->st));
+      BHIP(hipStreamSynchronize(b->st));
+      if (active <= 0) break;
+    }
   }
   std::vector<int> fl(4 * (size_t)b->n);
   BHIP(hipMemcpy(fl.data(), b->cg.f, fl.size() * sizeof(int), hipMemcpyDeviceToHost));
@@ -1298,12 +1331,7 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
     else if (d0.cg_precond == 3) b->fdm = true;
     else if (d0.cg_precond != 0)
       throw BErr(INS2D_E_UNSUPPORTED, "the ensemble runs Jacobi, two-level or fast-diagonalization PCG");
-    // with the exact inverse as preconditioner a solve takes one or two
-    // iterations: replay short chunks
-    if (b->fdm) {
-      b->chunk = 2;
-      build_fdm_factors(b.get());
-    }
+    if (b->fdm) build_fdm_factors(b.get());
     build_uniform_maps(b.get());
     BHIP(hipStreamCreate(&b->st));
     b->hcomp.assign(2 * (size_t)n, DComp{});
@@ -1521,6 +1549,8 @@ void ins2d_batch_destroy(ins2d_batch* b) {
   if (b->st) (void)hipStreamSynchronize(b->st);
   if (b->cg_chunk) (void)hipGraphExecDestroy(b->cg_chunk);
   if (b->cg_graph) (void)hipGraphDestroy(b->cg_graph);
+  if (b->cg_tail) (void)hipGraphExecDestroy(b->cg_tail);
+  if (b->cg_tail_graph) (void)hipGraphDestroy(b->cg_tail_graph);
   for (void* p : b->allocs) (void)hipFree(p);
   for (void* p : b->yv_allocs) (void)hipFree(p);
   if (b->st) (void)hipStreamDestroy(b->st);
