@@ -576,18 +576,19 @@ void poisson(ins2d_handle* h, const double* f, double* dp) {
     launch_cgx(h->st, h->lap, h->cg, 0, f, h->cr, nullptr, nullptr, tol2);
     launch_fdm_poisson(h->st, h->fdm, h->cr, h->cz, h->fw1, h->fw2);
     launch_cgx(h->st, h->lap, h->cg, 1, h->cr, h->cz, h->cp, nullptr, tol2);
-    // one iteration between host checks; its kernels after the convergence
-    // test (the GEMMs included) do nothing once the done flag is set
+    // the host checks after each convergence test (one iteration is the
+    // rule with the exact inverse as preconditioner): the preconditioner
+    // apply and the direction update run only when another iteration follows
     int flag[3] = {0, 0, 0};
     double rel = 0;
     while (true) {
       launch_cgx(h->st, h->lap, h->cg, 2, h->cp, h->cq, dp, h->cr, tol2);
-      launch_fdm_poisson(h->st, h->fdm, h->cr, h->cz, h->fw1, h->fw2, h->cg.flag);
-      launch_cgx(h->st, h->lap, h->cg, 3, h->cr, h->cz, h->cp, nullptr, tol2);
       HIPCHK(hipMemcpyAsync(flag, h->cg.flag, 3 * sizeof(int), hipMemcpyDeviceToHost, h->st));
       HIPCHK(hipMemcpyAsync(&rel, h->cg.s + 4, sizeof(double), hipMemcpyDeviceToHost, h->st));
       HIPCHK(hipStreamSynchronize(h->st));
       if (flag[0]) break;
+      launch_fdm_poisson(h->st, h->fdm, h->cr, h->cz, h->fw1, h->fw2, h->cg.flag);
+      launch_cgx(h->st, h->lap, h->cg, 3, h->cr, h->cz, h->cp, nullptr, tol2);
     }
     h->last_it = flag[1];
     h->last_rel = std::sqrt(rel);

@@ -355,6 +355,27 @@ def test_fdm_pcg_meets_the_plain_stopping_rule(name):
     solver.close()
 
 
+def test_fdm_pcg_second_iteration():
+    """At 1e-14 the fast-diagonalization PCG on the stretched 40x30 cavity
+    takes a second iteration (the preconditioner apply and direction update
+    after the first convergence test run) and reaches the tolerance, with
+    the solution of the oracle's sparse LU."""
+    _say('fdm pcg second iteration')
+    case = _wall_case('ldc', 40, 30, sx=1.03, poisson=3, CG_acc=1e-14, CG_maxit=30, CG_precond='fdm')
+    opts = O.build(case)
+    solver, _, _, _ = host.setup(case)
+    ref = O.build(_wall_case('ldc', 40, 30, sx=1.03, poisson=1))
+    for seed in range(3):
+        f = np.random.default_rng(seed).standard_normal(opts.grid.Np)
+        f -= f.mean()
+        dp, it, rel = solver.poisson(f)
+        dpo = O.pressure_poisson(f, 0.0, ref)
+        _say('seed', seed, 'iterations', it, 'relres', rel, 'vs LU', _rel(dp - dp.mean(), dpo - dpo.mean()))
+        assert it == 2 and rel <= 1e-14
+        assert _rel(dp - dp.mean(), dpo - dpo.mean()) < 1e-9
+    solver.close()
+
+
 def test_ldc_run_fdm_pcg_matches_oracle():
     """LDC 20x20 fixture configuration, PCG with the fast-diagonalization
     preconditioner on the GPU against the oracle's direct solve."""
