@@ -861,6 +861,7 @@ struct DivSrc {
   const double* A;
   const double* B;
   double ca, cb, oscale, hx, hy;
+  int pair;   // A, B 16-byte aligned: two elements per thread, 16-byte loads
 };
 
 // rows y0 = 2b and y1 = 2b+1 packed as one complex row, half spectra out
@@ -889,7 +890,29 @@ __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_r2c_rows(int Nx, int N
     const size_t N = (size_t)Nx * Ny;
     const size_t r0 = (size_t)y0 * Nx, ra = (size_t)ya * Nx, rb = (size_t)yb * Nx;
     auto W = [&](size_t k) { return B ? D.ca * A[k] + D.cb * B[k] : D.ca * A[k]; };
-    for (int n = threadIdx.x; n < Nx; n += blockDim.x) {
+    auto W2 = [&](size_t k) {   // elements k, k + 1 (k even)
+      const double2 a = *(const double2*)(A + k);
+      if (!B) return make_double2(D.ca * a.x, D.ca * a.y);
+      const double2 b = *(const double2*)(B + k);
+      return make_double2(D.ca * a.x + D.cb * b.x, D.ca * a.y + D.cb * b.y);
+    };
+    for (int n = 2 * threadIdx.x; D.pair && n < Nx; n += 2 * blockDim.x) {
+      const int n2 = (n + 2) & (Nx - 1);
+      const double2 u0 = W2(n + r0), ua = W2(n + ra), v0 = W2(N + n + r0), va = W2(N + n + ra);
+      const double u0e = W(n2 + r0);
+      const double s00 = D.hy * (u0.y - u0.x) + D.hx * (va.x - v0.x);
+      const double s01 = D.hy * (u0e - u0.y) + D.hx * (va.y - v0.y);
+      double s10 = 0.0, s11 = 0.0;
+      if (has1) {
+        const double2 vb = W2(N + n + rb);
+        const double uae = W(n2 + ra);
+        s10 = D.hy * (ua.y - ua.x) + D.hx * (vb.x - va.x);
+        s11 = D.hy * (uae - ua.y) + D.hx * (vb.y - va.y);
+      }
+      z[swz(n)] = make_double2(D.oscale * s00, D.oscale * s10);
+      z[swz(n + 1)] = make_double2(D.oscale * s01, D.oscale * s11);
+    }
+    for (int n = threadIdx.x; !D.pair && n < Nx; n += blockDim.x) {
       const int np = (n + 1) & (Nx - 1);
       const double va = W(N + n + ra);
       const double s0 = D.hy * (W(np + r0) - W(n + r0)) + D.hx * (va - W(N + n + r0));
@@ -1591,7 +1614,7 @@ void launch_fft_poisson_fused(hipStream_t st, const FFTPlan& P, const double* f,
                               bool acc) {
   if (!fft_register_lengths(P.Nx, P.Ny))
     throw std::runtime_error("fused FFT solve: needs power-of-two lengths 16 .. 4096");
-  const DivSrc D{A, B, ca, cb, oscale, hx, hy};
+  const DivSrc D{A, B, ca, cb, oscale, hx, hy, aligned16(A) && aligned16(B) ? 1 : 0};
   fftr<true>(st, P, f, out, 0, A ? &D : nullptr);
   fftr<true>(st, P, f, out, 1);
   fftr<true>(st, P, f, out, 2, nullptr, acc);
