@@ -36,7 +36,9 @@ extern "C" {
 #define INS2D_BC_SYM 3
 
 /* pressure solvers (pressure_poisson.m:42-104 numbering) */
-#define INS2D_POISSON_DIRECT 1 /* emulated: PCG to the requested tolerance */
+#define INS2D_POISSON_DIRECT 1 /* direct solve by fast diagonalization of the
+                                  tensor-product A (singular all-Neumann A:
+                                  the solution of zero area-weighted mean)   */
 #define INS2D_POISSON_CG 3     /* preconditioned CG (cg.F slot)             */
 #define INS2D_POISSON_CG_IC 4  /* preconditioned CG (cg_matlab.m slot)      */
 #define INS2D_POISSON_FFT 6    /* periodic uniform grids                    */
@@ -169,8 +171,10 @@ int ins2d_pressure_rhs(ins2d_handle* h, double t, double* f);
 
 /* kernel timing with HIP events on the handle's stream:
  * enable != 0 starts accumulating; read returns the accumulated milliseconds
- * and launch count of one kernel class (0 momentum RHS, 1 divergence,
- * 2 Poisson solve, 3 projection) */
+ * and launch count of one kernel class: 0 momentum RHS, 1 divergence,
+ * 2 Poisson solve, 3 projection of the RK stages (time_ERK.m:78-132);
+ * 4 momentum RHS, 5 divergence, 6 Poisson solve of the additional solve
+ * (pressure_additional_solve.m:19-25) */
 int ins2d_profile(ins2d_handle* h, int enable);
 int ins2d_profile_read(const ins2d_handle* h, int kind, double* ms, long long* count);
 

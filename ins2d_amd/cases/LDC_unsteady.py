@@ -1,7 +1,10 @@
 """Lid-driven cavity (case_files/LDC_unsteady/LDC_unsteady_*.m).
 
-Defaults reproduce the reference run logged in
-results/LDC_unsteady_1.000e+02_20x20_0 (Re=100, 20x20, dt=0.01, t=1, RK44).
+Defaults are those of case_files/LDC_unsteady/LDC_unsteady_parameters.m
+(RK44, direct pressure solve) at 20x20, Re=100, dt=0.01, t=1.  The reference
+run logged in results/LDC_unsteady_1.000e+02_20x20_0 is reproduced with
+RK='RK33P2' and the direct solve: its log matches that tableau row by row, not
+RK44.
 """
 import numpy as np
 

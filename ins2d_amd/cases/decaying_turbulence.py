@@ -7,6 +7,12 @@ case_files/decaying_turbulence/decaying_turbulence_IC.m:33).  psi has the
 energy spectrum E(k) ~ k^4 exp(-2 (k/k0)^2), k0 = 10, with seeded uniform
 phases; V is scaled so the domain-averaged kinetic energy 1/2 <|u|^2> is
 ``energy`` (default 1).
+
+The pressure solve uses the exact Fourier symbol of the assembled
+fourth-order A (``fft_symbol = 1``): the reference's FFT branch always takes
+the second-order symbol (spatial/operators/operator_divergence.m:283), which
+leaves the fourth-order projection with a divergence that grows until the run
+diverges; ``fft_symbol = 0`` reproduces the reference's choice.
 """
 import math
 
@@ -18,7 +24,7 @@ from ._common import periodic, uniform_mesh, zeros_bc
 def parameters():
     return dict(Re=1e4, Nx=4096, Ny=4096, x1=0.0, x2=2 * math.pi, y1=0.0, y2=2 * math.pi, sx=1.0, sy=1.0,
                 order4=1, dt=None, CFL=0.4, t_start=0.0, nsteps=500, RK='RK44',
-                poisson=6, p_initial=1, p_add_solve=1, seed=2024, k0=10.0, energy=1.0)
+                poisson=6, fft_symbol=1, p_initial=1, p_add_solve=1, seed=2024, k0=10.0, energy=1.0)
 
 
 def BCtype():

@@ -132,8 +132,12 @@ struct ins2d_handle {
   long long total_it = 0;
   // profiling
   bool prof = false;
-  double prof_ms[4] = {0, 0, 0, 0};
-  long long prof_n[4] = {0, 0, 0, 0};
+  // kinds 0-3: momentum RHS, divergence, Poisson solve, projection of the RK
+  // stages; 4-6: RHS, divergence and Poisson solve of the additional solve
+  static constexpr int kProfKinds = 8;
+  double prof_ms[kProfKinds] = {};
+  long long prof_n[kProfKinds] = {};
+  int prof_off = 0;   // 4 inside the additional solve
   std::vector<hipEvent_t> ev_free;
   struct Pending {
     int kind;
@@ -254,7 +258,7 @@ struct ProfScope {
   bool kernel;
   hipEvent_t a{}, b{};
   KernelTimer kt;
-  ProfScope(ins2d_handle* h_, int k, bool kernel_ts = false) : h(h_), kind(k), kernel(kernel_ts) {
+  ProfScope(ins2d_handle* h_, int k, bool kernel_ts = false) : h(h_), kind(k + h_->prof_off), kernel(kernel_ts) {
     if (!h->prof) return;
     a = get_ev();
     b = get_ev();
@@ -660,6 +664,11 @@ void div_poisson(ins2d_handle* h, const double* A, const double* B, double ca, d
 // p <- p + A^-1 (M Om^-1 F(V, p) + ydM)   (pressure_additional_solve.m:19-25);
 // rate_cu/rate_cv: operator records whose yM is ydM (null: ydM = 0)
 void additional_solve(ins2d_handle* h, const double* V, DComp* rate_cu = nullptr, DComp* rate_cv = nullptr) {
+  struct Off {
+    ins2d_handle* h;
+    explicit Off(ins2d_handle* h_) : h(h_) { h->prof_off = 4; }
+    ~Off() { h->prof_off = 0; }
+  } off(h);
   rhs(h, V, h->p, RHS_WITH_P | RHS_SCALE_OM, h->tV, kNoCombine, nullptr);
   if (rate_cu && !h->path) {
     DComp *cu = h->dCu, *cv = h->dCv;
@@ -1448,13 +1457,13 @@ int ins2d_profile(ins2d_handle* h, int enable) {
   API_TRY
   prof_collect(h);
   h->prof = enable != 0;
-  for (int k = 0; k < 4; ++k) { h->prof_ms[k] = 0.0; h->prof_n[k] = 0; }
+  for (int k = 0; k < ins2d_handle::kProfKinds; ++k) { h->prof_ms[k] = 0.0; h->prof_n[k] = 0; }
   return INS2D_OK;
   API_CATCH(h)
 }
 
 int ins2d_profile_read(const ins2d_handle* h, int kind, double* ms, long long* count) {
-  if (!h || kind < 0 || kind > 3) return INS2D_E_ARG;
+  if (!h || kind < 0 || kind >= ins2d_handle::kProfKinds) return INS2D_E_ARG;
   if (ms) *ms = h->prof_ms[kind];
   if (count) *count = h->prof_n[kind];
   return INS2D_OK;

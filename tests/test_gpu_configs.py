@@ -1,0 +1,219 @@
+"""GPU runs of each benchmark workload at its stated parameters: parity with
+the CPU oracle where the oracle finishes in seconds, size-independent
+properties at full size (discrete divergence, momentum, kinetic energy,
+pressure residual), and the exactness of the momentum right-hand side on
+stretched grids with a 'pres' outlet.
+
+Tolerances (max norm, relative): velocities 1e-12; pressure 1e-12 modulo its
+constant where the pressure matrix is singular."""
+import math
+
+import numpy as np
+import pytest
+
+from ins2d_amd import host
+from ins2d_amd.cases import ensemble, load
+from oracle import ins2d_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def _say(*a):
+    print('[gpu-test]', *a, flush=True)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# ---------------------------------------------------------------- config 1
+@pytest.mark.parametrize('precond', ['jacobi', 'fdm'])
+def test_ldc_64_re1000_rk44_pcg_matches_oracle(precond):
+    """Lid-driven cavity 64 x 64, Re = 1000, zero initial velocity, RK44,
+    dt = 1e-3, PCG to a relative residual of 1e-10, 50 steps, against the
+    oracle with the reference's direct solve (poisson = 1).  Each solve stops
+    at ||r|| <= 1e-10 ||f||; the pressure increments of the additional solve
+    are small against p and the stage solves enter V only through dt G dp,
+    so the difference a tolerance-limited solve leaves is far below the
+    tolerance (the oracle's own IC(0)-PCG at 1e-10 ends 2.8e-13 (V) and
+    1.3e-13 (p) from its direct solve after these 50 steps)."""
+    _say('ldc 64 Re1000 RK44 pcg', precond)
+    case = load('LDC_unsteady', Nx=64, Ny=64, Re=1000.0, dt=1e-3, RK='RK44', poisson=3, CG_acc=1e-10,
+                CG_maxit=20000, CG_precond=precond)
+    Vo, po, rows_o, _ = O.run(case, nsteps=50, overrides={'poisson': 1})
+    V, p, rows = host.main(case, nsteps=50)
+    ev = _rel(V, Vo)
+    ep = _rel(p - p.mean(), po - po.mean())
+    _say('ldc 64', precond, f'V {ev:.2e} p {ep:.2e}')
+    assert ev < RTOL and ep < RTOL
+    ro, rg = np.asarray(rows_o), np.asarray(rows)
+    assert np.all(np.abs(rg[:, 7] - ro[:, 7]) <= RTOL * np.abs(ro[:, 7]).max())
+
+
+# ---------------------------------------------------------------- config 3
+@pytest.mark.parametrize('N', [64, 128])
+def test_decaying_turbulence_defaults_match_oracle(N):
+    """The decaying-turbulence case with its own defaults (fourth order,
+    exact FFT symbol, streamfunction initial field, Re = 1e4, CFL 0.4) at
+    N x N, 10 steps."""
+    _say('decaying turbulence defaults', N)
+    case = load('decaying_turbulence', Nx=N, Ny=N)
+    assert case.parameters()['fft_symbol'] == 1 and case.parameters()['order4'] == 1
+    Vo, po, rows_o, _ = O.run(case, nsteps=10)
+    V, p, rows = host.main(case, nsteps=10)
+    ev, ep = _rel(V, Vo), _rel(p, po)
+    _say('decaying turbulence', N, f'V {ev:.2e} p {ep:.2e}')
+    assert ev < RTOL and ep < RTOL
+    rg = np.asarray(rows)
+    assert np.all(rg[:, 4] < 1e-12)
+    assert np.all(np.diff(rg[:, 7]) < 0)
+
+
+def test_decaying_turbulence_4096_properties():
+    """Full size, 4096 x 4096, fourth order: after the initial projection and
+    3 steps the field is discretely divergence-free, the total momentum
+    (periodic: no boundary terms) stays at round-off, and the kinetic energy
+    decreases (Re = 1e4)."""
+    _say('decaying turbulence 4096 properties')
+    solver, opts, V0, p0 = host.setup('decaying_turbulence')
+    assert solver.Np == 4096 * 4096 and solver.path == 4
+    solver.set_state(V0, p0)
+    solver.input_check(0.0, 1)
+    d0 = solver.diagnostics(0.0)
+    _say('4096 initial', d0)
+    ks = [d0[4]]
+    t = 0.0
+    for n in range(3):
+        solver.step(t, opts.dt, 1)
+        t += opts.dt
+        d = solver.diagnostics(t)
+        _say('4096 step', n + 1, d)
+        assert d[1] < 1e-12
+        assert abs(d[2]) < 1e-9 and abs(d[3]) < 1e-9
+        ks.append(d[4])
+    assert all(np.diff(ks) < 0)
+    assert ks[-1] > 0.99 * ks[0]
+    solver.close()
+
+
+# ---------------------------------------------------------------- config 4
+def test_ensemble_generator_members_match_oracle():
+    """Members drawn by the benchmark's own generator (seeded; long side
+    48..96 cells so that the oracle finishes in seconds), advanced together
+    with the fast-diagonalization PCG at 1e-13, against the oracle with the
+    direct solve."""
+    ms = ensemble.members(8, seed=11, L_min=48, L_max=96, precond='fdm', cg_tol=1e-13)
+    _say('ensemble generator members', [(m.parameters()['kind'], m.parameters()['Nx'], m.parameters()['Ny'])
+                                         for m in ms])
+    nsteps = 4
+    B = host.Batch(ms)
+    B.input_check(1)
+    B.step(nsteps)
+    for k, m in enumerate(ms):
+        Vb, pb = B.get_state(k)
+        Vo, po, _, _ = O.run(m, nsteps=nsteps, overrides={'poisson': 1})
+        if m.parameters()['kind'] == 'cavity':
+            pb = pb - pb.mean(); po = po - po.mean()
+        ev, ep = _rel(Vb, Vo), _rel(pb, po)
+        _say(k, m.parameters()['kind'], f'V {ev:.2e} p {ep:.2e}')
+        assert ev < RTOL and ep < RTOL
+    B.close()
+
+
+def _divergence(m, V, sizes):
+    """Discrete divergence M V + yM of a cavity or channel member on its
+    uniform grid (operator_divergence.m:84-89): face fluxes h (u_right -
+    u_left) + h (v_up - v_low), boundary values from the member's BC."""
+    prm = m.parameters()
+    nux, nuy, nvx, nvy, Nx, Ny = sizes
+    hx = prm['x2'] / Nx
+    hy = 1.0 / Ny
+    u = V[:nux * nuy].reshape(nux, nuy, order='F')
+    v = V[nux * nuy:].reshape(nvx, nvy, order='F')
+    zc = np.zeros((1, Ny))
+    if prm['kind'] == 'cavity':
+        uf = np.vstack([zc, u, zc])
+    else:
+        yp = (np.arange(Ny) + 0.5) * hy
+        uin = 4.0 * prm['speed'] * yp * (1.0 - yp)
+        uf = np.vstack([uin[None, :], u])
+    zr = np.zeros((Nx, 1))
+    vf = np.hstack([zr, v, zr])
+    return hy * (uf[1:, :] - uf[:-1, :]) + hx * (vf[:, 1:] - vf[:, :-1])
+
+
+def test_ensemble_512_members_properties():
+    """The full benchmark ensemble (512 members, 48..512 cells a side,
+    fast-diagonalization PCG at 1e-10): after 2 steps every member is finite
+    and discretely divergence-free to the solver tolerance, and every solve
+    met the stopping rule within two iterations."""
+    ms = ensemble.members(512, precond='fdm')
+    B = host.Batch(ms)
+    B.input_check(1)
+    B.step(2)
+    tot, mx, cells = B.stats()
+    _say('ensemble 512: PCG iterations total', tot, 'max per solve', mx, 'cells', cells)
+    assert mx <= 2 and cells > 10_000_000
+    worst = 0.0
+    for k, m in enumerate(ms):
+        V, p = B.get_state(k)
+        assert np.all(np.isfinite(V)) and np.all(np.isfinite(p))
+        div = _divergence(m, V, B.sizes[k])
+        prm = m.parameters()
+        scale = prm['speed'] * min(prm['x2'] / prm['Nx'], 1.0 / prm['Ny'])
+        worst = max(worst, np.abs(div).max() / scale)
+    _say('ensemble 512: max |M V + yM| / (U h)', worst)
+    assert worst < 1e-9
+    B.close()
+
+
+# ---------------------------------------------------------------- config 5
+def test_bfs_1024x256_properties():
+    """Backward-facing step 1024 x 256 (tanh stretching 1.5), the direct
+    (fast-diagonalization) solve: after 3 steps the field is discretely
+    divergence-free and a pressure solve of the current right-hand side meets
+    a true relative residual of 1e-10."""
+    _say('bfs 1024x256 properties')
+    solver, opts, V0, p0 = host.setup('BFS_unsteady')
+    solver.set_state(V0, p0)
+    solver.input_check(0.0, 1)
+    t = 0.0
+    for n in range(3):
+        solver.step(t, opts.dt, 1)
+        t += opts.dt
+        d = solver.diagnostics(t)
+        _say('bfs step', n + 1, d)
+        assert np.all(np.isfinite(d)) and d[1] < 1e-12
+    f = solver.pressure_rhs(t)
+    dp, _, _ = solver.poisson(f)
+    rel = solver.residual(f, dp)
+    _say('bfs pressure true relative residual', rel)
+    assert rel <= 1e-10
+    solver.close()
+
+
+# ------------------------------------------- exact RHS on stretched grids
+@pytest.mark.parametrize('name', ['x_stretched_uy_right_outlet', 'y_stretched_vx_top_outlet',
+                                  'y_stretched_ux_closed', 'x_stretched_vy_closed'])
+def test_rhs_exact_on_fixed_ratio_stretched_grids(name):
+    """The momentum right-hand side of libins2d_hip on geometric grids whose
+    cells grow by a fixed factor (25-30 % per cell) with a 'pres' outlet, for
+    fields the operators differentiate exactly: Om^-1 F equals the continuous
+    -div(u u) + lap(u)/Re at every point, the outlet included (the flux
+    weights of the stretched interpolation and the outlet's half cell)."""
+    from test_oracle_outflow_stretched import _exact_checks
+    case, eu, ev = _exact_checks()[name]
+    opts = O.build(case)
+    g = opts.grid
+    solver, _, V0, _ = host.setup(case)
+    _, Fr = solver.rhs(opts.V_start, np.zeros(g.Np))
+    Fu, Fv = Fr[:g.Nu] / g.Omu, Fr[g.Nu:] / g.Omv
+    ex_u = (eu(g.xu, g.yu) + 0.0 * g.xu).ravel(order='F')
+    ex_v = (ev(g.xv, g.yv) + 0.0 * g.xv).ravel(order='F')
+    sc = max(np.abs(ex_u).max(), np.abs(ex_v).max(), 1.0)
+    err = max(np.abs(Fu - ex_u).max(), np.abs(Fv - ex_v).max()) / sc
+    _say('exact stretched rhs', name, err)
+    assert err < RTOL
+    solver.close()

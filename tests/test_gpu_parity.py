@@ -182,10 +182,11 @@ def _compare_runs(case, nsteps, const_mode_p):
     if const_mode_p:
         p = p - p.mean()
         po = po - po.mean()
-    assert _rel(p, po) < 1e-11
+    _say('run parity', f'V {_rel(V, Vo):.2e} p {_rel(p, po):.2e}')
+    assert _rel(p, po) < RTOL
     ro, rg = np.asarray(rows_o), np.asarray(rows)
     for c in (3, 7):   # maxres, kinetic energy
-        assert np.all(np.abs(rg[:, c] - ro[:, c]) <= 1e-11 * np.abs(ro[:, c]).max())
+        assert np.all(np.abs(rg[:, c] - ro[:, c]) <= RTOL * np.abs(ro[:, c]).max())
     return rows
 
 
