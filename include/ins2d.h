@@ -174,7 +174,9 @@ int ins2d_pressure_rhs(ins2d_handle* h, double t, double* f);
  * and launch count of one kernel class: 0 momentum RHS, 1 divergence,
  * 2 Poisson solve, 3 projection of the RK stages (time_ERK.m:78-132);
  * 4 momentum RHS, 5 divergence, 6 Poisson solve of the additional solve
- * (pressure_additional_solve.m:19-25) */
+ * (pressure_additional_solve.m:19-25); the kernels of the register-transform
+ * FFT solve over all solves: 8 forward row transform, 9 column solve,
+ * 10 inverse row transform, 11 inverse row transform adding dp into p */
 int ins2d_profile(ins2d_handle* h, int enable);
 int ins2d_profile_read(const ins2d_handle* h, int kind, double* ms, long long* count);
 

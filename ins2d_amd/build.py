@@ -25,16 +25,24 @@ FLAGS = [
 def build(verbose=False):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for s in SOURCES:
         src = os.path.join(CSRC, s)
         obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
-        lang = ["-x", "hip"] if s.endswith(".hip") else ["-x", "hip"]
-        cmd = [HIPCC] + FLAGS + lang + ["-c", src, "-o", obj]
+        cmds.append([HIPCC] + FLAGS + ["-x", "hip", "-c", src, "-o", obj])
+        objs.append(obj)
+    # the translation units compile independently: in parallel, at most
+    # MAX_JOBS (or 8) at a time
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "8"))))
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+    with ThreadPoolExecutor(jobs) as ex:
+        for fut in [ex.submit(run, c) for c in cmds]:
+            fut.result()
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
     if verbose:

@@ -134,7 +134,10 @@ struct ins2d_handle {
   bool prof = false;
   // kinds 0-3: momentum RHS, divergence, Poisson solve, projection of the RK
   // stages; 4-6: RHS, divergence and Poisson solve of the additional solve
-  static constexpr int kProfKinds = 8;
+  // 8-11: the forward row transform, column solve, inverse row transform and
+  // inverse row transform with p += dp of the register-transform FFT solve
+  // (every solve, kernel timestamps)
+  static constexpr int kProfKinds = 12;
   double prof_ms[kProfKinds] = {};
   long long prof_n[kProfKinds] = {};
   int prof_off = 0;   // 4 inside the additional solve
@@ -293,6 +296,42 @@ struct ProfScope {
       return;
     }
     if (hipEventRecord(b, h->st) == hipSuccess) h->pend.push_back({kind, a, b});
+  }
+};
+
+// kernel timestamps of the FFT solve's kernels (profile kinds 8-11)
+struct FftProf {
+  ins2d_handle* h;
+  KernelTimer kt[4];
+  explicit FftProf(ins2d_handle* h_) : h(h_) {
+    if (!h->prof) return;
+    for (int k = 0; k < 4; ++k) {
+      kt[k].start = get_ev();
+      kt[k].stop = get_ev();
+      g_fft_timer[k] = &kt[k];
+    }
+  }
+  hipEvent_t get_ev() {
+    if (!h->ev_free.empty()) {
+      hipEvent_t e = h->ev_free.back();
+      h->ev_free.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+  ~FftProf() {
+    if (!h->prof) return;
+    for (int k = 0; k < 4; ++k) {
+      g_fft_timer[k] = nullptr;
+      if (kt[k].launches > 0) {
+        h->pend.push_back({8 + k, kt[k].start, kt[k].stop});
+      } else {
+        h->ev_free.push_back(kt[k].start);
+        h->ev_free.push_back(kt[k].stop);
+      }
+    }
   }
 };
 
@@ -544,6 +583,7 @@ void accel_poisson(ins2d_handle* h, const double* f, double* dp) {
 void poisson(ins2d_handle* h, const double* f, double* dp) {
   ProfScope ps(h, 2);
   if (h->poisson == INS2D_POISSON_FFT) {
+    FftProf fp(h);
     launch_fft_poisson(h->st, h->fft, f, dp);
     return;
   }
@@ -648,12 +688,14 @@ void div_poisson(ins2d_handle* h, const double* A, const double* B, double ca, d
   const bool reg = h->poisson == INS2D_POISSON_FFT && fft_register_lengths(h->Nx, h->Ny);
   if (reg && h->path == 2) {
     ProfScope ps(h, 2);
+    FftProf fp(h);
     launch_fft_poisson_fused(h->st, h->fft, nullptr, h->hx, h->hy, A, B, ca, cb, oscale, out, acc);
     return;
   }
   div(h, A, B, ca, cb, ycoef, oscale, h->f);
   if (reg && acc) {
     ProfScope ps(h, 2);
+    FftProf fp(h);
     launch_fft_poisson_fused(h->st, h->fft, h->f, h->hx, h->hy, nullptr, nullptr, 0.0, 0.0, 0.0, out, true);
     return;
   }

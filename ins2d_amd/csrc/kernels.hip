@@ -1449,6 +1449,19 @@ static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / 
 static inline bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 thread_local KernelTimer* g_ktimer = nullptr;
+thread_local KernelTimer* g_fft_timer[4] = {nullptr, nullptr, nullptr, nullptr};
+
+// launch one kernel of the FFT solve, with kernel timestamps into timer k
+// while profiling
+template <typename F, typename... Args>
+static inline void fft_launch(int k, F kernel, dim3 grid, dim3 block, unsigned shmem, hipStream_t st, Args... args) {
+  if (KernelTimer* t = g_fft_timer[k]) {
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, st, t->launches == 0 ? t->start : nullptr, t->stop, 0u, args...);
+    ++t->launches;
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, shmem, st, args...);
+  }
+}
 
 
 
@@ -1574,24 +1587,24 @@ static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, doubl
     // as many threads as the launch bound allows keep more of its loads in
     // flight (the transform's passes leave the extra threads idle)
     if (D)
-      hipLaunchKernelGGL((k_fftr_r2c_rows<E, TR, true>), gr, dim3(fft_block<E>(0)), sh, st, Nx, Ny, Nh, ilog2(Nx),
-                         (const double2*)P.twx, nullptr, *D, H);
+      fft_launch(0, k_fftr_r2c_rows<E, TR, true>, gr, dim3(fft_block<E>(0)), sh, st, Nx, Ny, Nh, ilog2(Nx),
+                 (const double2*)P.twx, (const double*)nullptr, *D, H);
     else
-      hipLaunchKernelGGL((k_fftr_r2c_rows<E, TR, false>), gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx),
-                         (const double2*)P.twx, f, DivSrc{}, H);
+      fft_launch(0, k_fftr_r2c_rows<E, TR, false>, gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx, f,
+                 DivSrc{}, H);
   } else if (which == 1) {
-    hipLaunchKernelGGL(k_fftr_cols_solve<E>, dim3(Nh), dim3(std::max(64, Ny / E)), (size_t)Ny * sizeof(double2), st,
-                       Nx, Ny, ilog2(Ny), (const double2*)P.twy, P.ax, P.ay, P.c, HT);
+    fft_launch(1, k_fftr_cols_solve<E>, dim3(Nh), dim3(std::max(64, Ny / E)), (unsigned)(Ny * sizeof(double2)), st,
+               Nx, Ny, ilog2(Ny), (const double2*)P.twy, (const double*)P.ax, (const double*)P.ay, P.c, HT);
   } else {
     // the half spectra are staged by a plain loop before the transform (as
     // the divergence rows above): threads up to the launch bound
     const dim3 blc(fft_block<E>(0));
     if (acc)
-      hipLaunchKernelGGL((k_fftr_c2r_rows<E, TR, true>), gr, blc, sh, st, Nx, Ny, Nh, ilog2(Nx),
-                         (const double2*)P.twx, H, dp);
+      fft_launch(3, k_fftr_c2r_rows<E, TR, true>, gr, blc, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx,
+                 (const double2*)H, dp);
     else
-      hipLaunchKernelGGL((k_fftr_c2r_rows<E, TR, false>), gr, blc, sh, st, Nx, Ny, Nh, ilog2(Nx),
-                         (const double2*)P.twx, H, dp);
+      fft_launch(2, k_fftr_c2r_rows<E, TR, false>, gr, blc, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx,
+                 (const double2*)H, dp);
   }
 }
 

@@ -26,6 +26,10 @@ struct KernelTimer {
   int launches = 0;
 };
 extern thread_local KernelTimer* g_ktimer;
+// the same for the three kernels of the register-transform FFT solve, one
+// timer each: 0 forward row transform, 1 column solve, 2 inverse row
+// transform, 3 inverse row transform that adds dp into its output
+extern thread_local KernelTimer* g_fft_timer[4];
 
 template <typename F, typename... Args>
 inline void launch_timed(F kernel, dim3 grid, dim3 block, unsigned shmem, hipStream_t st, Args... args) {
