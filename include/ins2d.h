@@ -90,6 +90,7 @@ typedef struct {
  * the solver preparation of operator_divergence.m:219-291 */
 int ins2d_create(const ins2d_case_desc* desc, ins2d_handle** out);
 void ins2d_destroy(ins2d_handle* h);
+/* h NULL: the error of the last failed ins2d_create or handle-free call */
 const char* ins2d_last_error(const ins2d_handle* h);
 
 /* sizes: out[0..6] = Nux_in, Nuy_in, Nvx_in, Nvy_in, Npx, Npy, kernel path
@@ -270,6 +271,23 @@ int ins2d_nn_shape(const ins2d_nn* n, int* H, int* W);
 int ins2d_set_poisson_accel(ins2d_handle* h, ins2d_nn* nn, int flags, int agg);
 /* number of warm-start fallbacks so far, aggregate size in use */
 int ins2d_poisson_accel_stats(const ins2d_handle* h, long long* fallbacks, int* agg);
+
+/* ---- the periodic FFT pressure solve without a handle and without
+ * allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (work captured into HIP graphs, tight
+ * serving loops, harnesses that forbid hipMalloc).  Solves A dp = f on the
+ * periodic uniform Nx x Ny grid with spacings hx, hy as ins2d_poisson does
+ * for a handle created with poisson = INS2D_POISSON_FFT and the same order4
+ * and fft_symbol, with f and dp device arrays (x fastest) and the work queued
+ * on the given HIP stream (NULL: the null stream); it returns once the work
+ * is queued.  Nx and Ny must be powers of two from 16 to 4096 with the half
+ * spectrum (Nx/2 + 1) * Ny at most 525312 complex values (e.g. 1024 x 1024 or
+ * 4096 x 256): larger grids return INS2D_E_ARG.  The scratch memory (about
+ * 8.7 MB per GPU, held until the process ends) is one fixed device array
+ * that every call on a GPU shares: calls that overlap on one GPU race.  The
+ * error text of a failed call is ins2d_last_error(NULL). */
+int ins2d_poisson_fft_noalloc(int Nx, int Ny, double hx, double hy, int order4, int fft_symbol, const double* f,
+                              double* dp, void* stream);
 
 #ifdef __cplusplus
 }

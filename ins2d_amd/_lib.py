@@ -79,6 +79,8 @@ SIGNATURES = {
     "ins2d_pressure_rhs": (c_int, [c_void_p, c_double, P_double]),
     "ins2d_residual": (c_int, [c_void_p, P_double, P_double, P_double]),
     "ins2d_fdm_mfma_probe": (c_int, [P_double, P_double, P_double]),
+    "ins2d_poisson_fft_noalloc": (c_int, [c_int, c_int, c_double, c_double, c_int, c_int, c_void_p, c_void_p,
+                                          c_void_p]),
 }
 
 _LIB = None

@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libins2d_hip.so")
-SOURCES = ["ops1d.cpp", "kernels.hip", "api.hip", "batch.hip", "unet.hip", "fdm.hip", "fdm_host.cpp", "pcg2.hip"]
+SOURCES = ["ops1d.cpp", "kernels.hip", "api.hip", "batch.hip", "unet.hip", "fdm.hip", "fdm_host.cpp", "pcg2.hip",
+           "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

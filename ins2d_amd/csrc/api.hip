@@ -892,9 +892,14 @@ int fail(ins2d_handle* h, const std::exception& e, int code) {
 
 }  // namespace
 
-extern "C" {
-
 static std::string g_create_err;
+
+namespace ins2d {
+// the error text of the handle-free entry points (ins2d_last_error(NULL))
+void set_handle_free_error(const std::string& msg) { g_create_err = msg; }
+}  // namespace ins2d
+
+extern "C" {
 
 const char* ins2d_last_error(const ins2d_handle* h) { return h ? h->err.c_str() : g_create_err.c_str(); }
 

@@ -40,3 +40,17 @@ def test_binding_loads_and_reports_create_errors(lib):
     h = ctypes.c_void_p()
     assert L.ins2d_create(ctypes.byref(d), ctypes.byref(h)) == 1
     assert b'bad grid' in L.ins2d_last_error(None)
+
+
+def test_noalloc_fft_solve_rejects_oversized_grids(lib):
+    """The allocation-free FFT solve checks its sizes before any device call:
+    a grid whose half spectrum exceeds the fixed scratch array is refused
+    with a clear error (no GPU needed for the refusal)."""
+    L = _lib.load()
+    dummy = ctypes.c_void_p(8)   # never dereferenced on the refusal path
+    rc = L.ins2d_poisson_fft_noalloc(2048, 2048, 1.0, 1.0, 0, 0, dummy, dummy, None)
+    assert rc == 1
+    assert b'exceeds the scratch capacity' in L.ins2d_last_error(None)
+    rc = L.ins2d_poisson_fft_noalloc(48, 64, 1.0, 1.0, 0, 0, dummy, dummy, None)
+    assert rc == 4
+    assert b'powers of two' in L.ins2d_last_error(None)
