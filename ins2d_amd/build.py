@@ -50,7 +50,24 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_mex_harness(verbose)
     return LIB
+
+
+def build_mex_harness(verbose=False):
+    """The MATLAB gateway (matlab/ins2d_mex.c) linked with the test harness
+    of tests/native/mex_mock.c, which stands in for MATLAB's MEX API: the
+    tests drive the gateway through it (MATLAB itself is not installed)."""
+    out = os.path.join(ROOT, "tests", "native", "libmex_mock.so")
+    cmd = ["gcc", "-O1", "-std=c11", "-shared", "-fPIC", "-Wall",
+           "-I" + os.path.join(ROOT, "tests", "native", "mex_mock"), "-I" + os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "matlab", "ins2d_mex.c"), os.path.join(ROOT, "tests", "native", "mex_mock.c"),
+           "-L" + HERE, "-lins2d_hip", "-Wl,-rpath,$ORIGIN/../../ins2d_amd", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":
