@@ -39,6 +39,10 @@ extern "C" {
 #define INS2D_POISSON_DIRECT 1 /* direct solve by fast diagonalization of the
                                   tensor-product A (singular all-Neumann A:
                                   the solution of zero area-weighted mean)   */
+#define INS2D_POISSON_PCG_ILU 2 /* CG preconditioned by the unit lower ILU(0)
+                                  factor L as M = L L^T (pcg + ilu 'nofill',
+                                  pressure_poisson.m:52-64); at most 2^20
+                                  cells, one workgroup                        */
 #define INS2D_POISSON_CG 3     /* preconditioned CG (cg.F slot)             */
 #define INS2D_POISSON_CG_IC 4  /* preconditioned CG (cg_matlab.m slot)      */
 #define INS2D_POISSON_FFT 6    /* periodic uniform grids                    */

@@ -91,6 +91,7 @@ struct ins2d_handle {
   double* d_rel = nullptr;
   double* part = nullptr;
   int nred = 0;
+  double* ee_part = nullptr;   // partial maxima of |f| (early exit of the matrix solvers)
   double* ox = nullptr;   // volume factors (all paths)
   double* oy = nullptr;
   double* ovx = nullptr;
@@ -579,7 +580,11 @@ void accel_poisson(ins2d_handle* h, const double* f, double* dp) {
   if (flag[2]) throw NoConv("CG did not converge in maxit iterations");
 }
 
-// A dp = f, x0 = 0 (pressure_poisson.m:12)
+void poisson_matrix(ins2d_handle* h, const double* f, double* dp);
+
+// A dp = f, x0 = 0 (pressure_poisson.m:12); dp = 0 when the right-hand side
+// is negligible (pressure_poisson.m:21-39: the FFT kernels test max |fft2(f)|
+// themselves, the matrix solvers max |f|, both against 1e-14)
 void poisson(ins2d_handle* h, const double* f, double* dp) {
   ProfScope ps(h, 2);
   if (h->poisson == INS2D_POISSON_FFT) {
@@ -587,6 +592,13 @@ void poisson(ins2d_handle* h, const double* f, double* dp) {
     launch_fft_poisson(h->st, h->fft, f, dp);
     return;
   }
+  launch_absmax_part(h->st, f, (size_t)h->Np, h->ee_part);
+  poisson_matrix(h, f, dp);
+  launch_zero_if_small(h->st, h->ee_part, 1e-14, dp, (size_t)h->Np);
+}
+
+// the direct and CG solvers of pressure_poisson.m:42-90
+void poisson_matrix(ins2d_handle* h, const double* f, double* dp) {
   if (h->poisson == INS2D_POISSON_DIRECT) {
     launch_fdm_poisson(h->st, h->fdm, f, dp, h->cr, h->cz);
     h->last_it = 0;
@@ -860,6 +872,7 @@ void build_fft(ins2d_handle* h, int order) {
   // power of two in [16, 4096]) sits on one side of the transposes
   F.H = fft_register_lengths(F.Nx, F.Ny) ? nullptr : dalloc<double>(h, 2 * (size_t)F.Nh * h->Ny);
   F.HT = dalloc<double>(h, 2 * (size_t)F.Nh * h->Ny);
+  F.fmax2 = dalloc<unsigned long long>(h, 1);
   h->has_fft = true;
 }
 
@@ -943,7 +956,7 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
     if (h->poisson != INS2D_POISSON_FFT && d->order4)
       throw Unsupported("fourth order pressure is solved by FFT");
     if (h->poisson != INS2D_POISSON_FFT && h->poisson != INS2D_POISSON_CG && h->poisson != INS2D_POISSON_CG_IC &&
-        h->poisson != INS2D_POISSON_DIRECT)
+        h->poisson != INS2D_POISSON_DIRECT && h->poisson != INS2D_POISSON_PCG_ILU)
       throw ArgError("unknown poisson mode");
     if (!d->order4) {
       CaseDesc cd;
@@ -1065,8 +1078,9 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
       h->cg.flag = dalloc<int>(h.get(), 4);
       h->cg.part = dalloc<double>(h.get(), 2 * (size_t)h->cg.nblk);
       h->cg.maxit = h->maxit;
-      if (d->cg_precond == 1 && (h->poisson == INS2D_POISSON_CG || h->poisson == INS2D_POISSON_CG_IC)) {
-        if (h->Np > (1 << 20)) throw Unsupported("IC(0) preconditioner: at most 2^20 pressure cells");
+      if ((d->cg_precond == 1 && (h->poisson == INS2D_POISSON_CG || h->poisson == INS2D_POISSON_CG_IC)) ||
+          h->poisson == INS2D_POISSON_PCG_ILU) {
+        if (h->Np > (1 << 20)) throw Unsupported("IC(0) / ILU(0) preconditioner: at most 2^20 pressure cells");
         IC0Host I;
         build_ic0_host(*h->plan, I);
         h->dic.nx = h->Nx;
@@ -1076,6 +1090,9 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
         h->dic.cs = dupload(h.get(), I.cs);
         h->dic.cwx = I.cwx.empty() ? nullptr : dupload(h.get(), I.cwx);
         h->dic.cwy = I.cwy.empty() ? nullptr : dupload(h.get(), I.cwy);
+        // the ILU(0) pivots of a symmetric 5-point matrix are those of IC(0)
+        // (the same recurrence); mode 2 uses M = L1 L1^T of the unit factor
+        h->dic.lu = h->poisson == INS2D_POISSON_PCG_ILU ? 1 : 0;
         h->ic0 = true;
       } else if (d->cg_precond == 3 && (h->poisson == INS2D_POISSON_CG || h->poisson == INS2D_POISSON_CG_IC)) {
         h->fdm_pcg = true;
@@ -1101,6 +1118,7 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
     }
     h->nred = (int)std::min<long>(1024, std::max<long>(1, ((long)std::max(h->NV, h->Np) + 255) / 256));
     h->part = dalloc<double>(h.get(), 4 * (size_t)h->nred);
+    h->ee_part = dalloc<double>(h.get(), 64);
     HIPCHK(hipDeviceSynchronize());
   } catch (const std::exception& e) {
     g_create_err = e.what();

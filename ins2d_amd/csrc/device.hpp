@@ -114,6 +114,11 @@ struct FFTPlan {
   double c;
   double* H;                 // Ny x Nh complex (row transforms)
   double* HT;                // Nh x Ny complex (transposed)
+  // max |fhat|^2 of the solve in progress (as the bits of a non-negative
+  // double): zeroed by the forward row transform, raised by the column
+  // solve, read by the inverse row transform, which returns dp = 0 below
+  // 1e-28 (the early exit of pressure_poisson.m:21-31)
+  unsigned long long* fmax2;
 };
 
 }  // namespace ins2d

@@ -677,6 +677,30 @@ __device__ __forceinline__ void lds_transform(double2* z, double2* scratch, int 
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int swz(int n) { return n ^ ((n >> 4) & 15); }
 
+// the early exit of pressure_poisson.m:21-31 (FFT branch): dp = 0 when
+// max |fhat| < 1e-14, fhat = fft2(f) (unnormalized)
+constexpr double kFftExit2 = 1e-28;   // (1e-14)^2, compared with |fhat|^2
+__device__ __forceinline__ void fmax2_reset(unsigned long long* m) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *m = 0ull;
+}
+// block maximum of m2 (>= 0) raised into *m (one atomic per block); every
+// thread of the block must call it
+__device__ __forceinline__ void fmax2_raise(unsigned long long* m, double m2) {
+  __shared__ double wmax[16];
+  for (int o = 32; o > 0; o >>= 1) m2 = fmax(m2, __shfl_xor(m2, o));
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) wmax[w] = m2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+    for (int k = 0; k < nw; ++k) b = fmax(b, wmax[k]);
+    atomicMax(m, (unsigned long long)__double_as_longlong(b));
+  }
+}
+__device__ __forceinline__ bool fmax2_tiny(const unsigned long long* m) {
+  return __longlong_as_double((long long)*m) < kFftExit2;
+}
+
 // x * exp(-+2 pi i m / 16) for a compile-time m in 0 .. 9 (sign + for INV)
 template <bool INV>
 __device__ __forceinline__ double2 mul_w16(double2 x, int m) {
@@ -873,8 +897,10 @@ struct DivSrc {
 template <int E, bool TR, bool DIV>
 __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_r2c_rows(int Nx, int Ny, int Nh, int log2N,
                                                        const double2* __restrict__ tw, const double* __restrict__ f,
-                                                       DivSrc D, double2* __restrict__ H) {
+                                                       DivSrc D, double2* __restrict__ H,
+                                                       unsigned long long* __restrict__ fmax2) {
   extern __shared__ double2 z[];
+  fmax2_reset(fmax2);
   const int y0 = 2 * (TR ? xcd_row_pair(blockIdx.x, gridDim.x) : (int)blockIdx.x), y1 = y0 + 1;
   const bool has1 = y1 < Ny;
   auto st = [&](int, int n, double2 x) { z[swz(n)] = x; };
@@ -951,10 +977,20 @@ __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_r2c_rows(int Nx, int N
 template <int E, bool TR, bool ACC>
 __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_c2r_rows(int Nx, int Ny, int Nh, int log2N,
                                                        const double2* __restrict__ tw, const double2* __restrict__ H,
-                                                       double* __restrict__ out) {
+                                                       double* __restrict__ out,
+                                                       const unsigned long long* __restrict__ fmax2) {
   extern __shared__ double2 z[];
   const int y0 = 2 * (TR ? xcd_row_pair(blockIdx.x, gridDim.x) : (int)blockIdx.x), y1 = y0 + 1;
   const bool has1 = y1 < Ny;
+  if (fmax2_tiny(fmax2)) {
+    // dp = 0: nothing to add, or zero rows
+    if (!ACC)
+      for (int n = threadIdx.x; n < Nx; n += blockDim.x) {
+        out[(size_t)y0 * Nx + n] = 0.0;
+        if (has1) out[(size_t)y1 * Nx + n] = 0.0;
+      }
+    return;
+  }
   const size_t ks = TR ? (size_t)Ny : 1;   // stride of k
   const double2* H0 = H + (TR ? (size_t)y0 : (size_t)y0 * Nh);
   const double2* H1 = H + (TR ? (size_t)(has1 ? y1 : y0) : (size_t)(has1 ? y1 : y0) * Nh);
@@ -1000,21 +1036,25 @@ __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_c2r_rows(int Nx, int N
 template <int E>
 __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_cols_solve(int Nx, int Ny, int log2N, const double2* __restrict__ tw,
                                                          const double* __restrict__ ax, const double* __restrict__ ay,
-                                                         double c, double2* __restrict__ HT) {
+                                                         double c, double2* __restrict__ HT,
+                                                         unsigned long long* __restrict__ fmax2) {
   extern __shared__ double2 z[];
   const int kx = blockIdx.x;
   double2* row = HT + (size_t)kx * Ny;
   const double scale = 1.0 / ((double)Nx * (double)Ny);
   const double axk = ax[kx];
   double2 v[E];
+  double m2 = 0.0;
   auto ldg = [&](int, int n) { return row[n]; };
   auto stv = [&](int s, int ky, double2 x) {
+    m2 = fmax(m2, x.x * x.x + x.y * x.y);
     double Ah = c * (axk + ay[ky]);
     if (kx == 0 && ky == 0) Ah = 1.0;
     const double sc = -scale / Ah;
     v[s] = make_double2(x.x * sc, x.y * sc);
   };
   fft_reg<E, false, false>(z, Ny, log2N, tw, ldg, stv);
+  fmax2_raise(fmax2, m2);
   __syncthreads();
   auto ldv = [&](int s, int) { return v[s]; };
   auto stg = [&](int, int n, double2 x) { row[n] = x; };
@@ -1025,10 +1065,11 @@ __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_cols_solve(int Nx, int
 template <int C>
 __global__ void __launch_bounds__(C == 1 ? 1024 : 256) k_fft_r2c_rows(int Nx, int Ny, int Nh, bool pow2, int log2N,
                                                      const double2* __restrict__ tw, const double* __restrict__ f,
-                                                     double2* __restrict__ H) {
+                                                     double2* __restrict__ H, unsigned long long* __restrict__ fmax2) {
   extern __shared__ double2 sm[];
   double2* z = sm;
   double2* scratch = sm + Nx;
+  fmax2_reset(fmax2);
   const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
   const bool has1 = y1 < Ny;
   for (int n = threadIdx.x; n < Nx; n += blockDim.x)
@@ -1046,12 +1087,20 @@ __global__ void __launch_bounds__(C == 1 ? 1024 : 256) k_fft_r2c_rows(int Nx, in
 template <int C>
 __global__ void __launch_bounds__(C == 1 ? 1024 : 256) k_fft_c2r_rows(int Nx, int Ny, int Nh, bool pow2, int log2N,
                                                      const double2* __restrict__ tw, const double2* __restrict__ H,
-                                                     double* __restrict__ out) {
+                                                     double* __restrict__ out,
+                                                     const unsigned long long* __restrict__ fmax2) {
   extern __shared__ double2 sm[];
   double2* z = sm;
   double2* scratch = sm + Nx;
   const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
   const bool has1 = y1 < Ny;
+  if (fmax2_tiny(fmax2)) {
+    for (int n = threadIdx.x; n < Nx; n += blockDim.x) {
+      out[(size_t)y0 * Nx + n] = 0.0;
+      if (has1) out[(size_t)y1 * Nx + n] = 0.0;
+    }
+    return;
+  }
   for (int k = threadIdx.x; k < Nx; k += blockDim.x) {
     double2 a, b;
     if (k < Nh) {
@@ -1093,7 +1142,8 @@ template <int C>
 __global__ void __launch_bounds__(C == 1 ? 1024 : 256) k_fft_cols_solve(int Nx, int Ny, bool pow2, int log2N,
                                                        const double2* __restrict__ tw, const double* __restrict__ ax,
                                                        const double* __restrict__ ay, double c,
-                                                       double2* __restrict__ HT) {
+                                                       double2* __restrict__ HT,
+                                                       unsigned long long* __restrict__ fmax2) {
   extern __shared__ double2 sm[];
   double2* z = sm;
   double2* scratch = sm + Ny;
@@ -1102,6 +1152,9 @@ __global__ void __launch_bounds__(C == 1 ? 1024 : 256) k_fft_cols_solve(int Nx, 
   for (int n = threadIdx.x; n < Ny; n += blockDim.x) z[n] = row[n];
   __syncthreads();
   lds_transform<false, C>(z, scratch, Ny, pow2, log2N, tw);
+  double m2 = 0.0;
+  for (int ky = threadIdx.x; ky < Ny; ky += blockDim.x) m2 = fmax(m2, z[ky].x * z[ky].x + z[ky].y * z[ky].y);
+  fmax2_raise(fmax2, m2);
   const double scale = 1.0 / ((double)Nx * (double)Ny);
   for (int ky = threadIdx.x; ky < Ny; ky += blockDim.x) {
     double Ah = c * (ax[kx] + ay[ky]);
@@ -1588,23 +1641,24 @@ static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, doubl
     // flight (the transform's passes leave the extra threads idle)
     if (D)
       fft_launch(0, k_fftr_r2c_rows<E, TR, true>, gr, dim3(fft_block<E>(0)), sh, st, Nx, Ny, Nh, ilog2(Nx),
-                 (const double2*)P.twx, (const double*)nullptr, *D, H);
+                 (const double2*)P.twx, (const double*)nullptr, *D, H, P.fmax2);
     else
       fft_launch(0, k_fftr_r2c_rows<E, TR, false>, gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx, f,
-                 DivSrc{}, H);
+                 DivSrc{}, H, P.fmax2);
   } else if (which == 1) {
     fft_launch(1, k_fftr_cols_solve<E>, dim3(Nh), dim3(std::max(64, Ny / E)), (unsigned)(Ny * sizeof(double2)), st,
-               Nx, Ny, ilog2(Ny), (const double2*)P.twy, (const double*)P.ax, (const double*)P.ay, P.c, HT);
+               Nx, Ny, ilog2(Ny), (const double2*)P.twy, (const double*)P.ax, (const double*)P.ay, P.c, HT, P.fmax2);
   } else {
     // the half spectra are staged by a plain loop before the transform (as
     // the divergence rows above): threads up to the launch bound
     const dim3 blc(fft_block<E>(0));
+    const unsigned long long* fm = P.fmax2;
     if (acc)
       fft_launch(3, k_fftr_c2r_rows<E, TR, true>, gr, blc, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx,
-                 (const double2*)H, dp);
+                 (const double2*)H, dp, fm);
     else
       fft_launch(2, k_fftr_c2r_rows<E, TR, false>, gr, blc, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx,
-                 (const double2*)H, dp);
+                 (const double2*)H, dp, fm);
   }
 }
 
@@ -1653,20 +1707,58 @@ void launch_fft_poisson(hipStream_t st, const FFTPlan& P, const double* f, doubl
     fftr<false>(st, P, f, dp, 0);
   else
     hipLaunchKernelGGL(k_fft_r2c_rows<1>, dim3((Ny + 1) / 2), dim3(Tx), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
-                       (const double2*)P.twx, f, H);
+                       (const double2*)P.twx, f, H, P.fmax2);
   dim3 tb(32, 8);
   hipLaunchKernelGGL(k_transpose_c, dim3((Nh + 31) / 32, (Ny + 31) / 32), tb, 0, st, H, HT, Ny, Nh);
   if (fy)
     fftr<false>(st, P, f, dp, 1);   // in place on HT
   else
     hipLaunchKernelGGL(k_fft_cols_solve<1>, dim3(Nh), dim3(Ty), shy, st, Nx, Ny, P.pow2y, ilog2(Ny),
-                       (const double2*)P.twy, P.ax, P.ay, P.c, HT);
+                       (const double2*)P.twy, P.ax, P.ay, P.c, HT, P.fmax2);
   hipLaunchKernelGGL(k_transpose_c, dim3((Ny + 31) / 32, (Nh + 31) / 32), tb, 0, st, HT, H, Nh, Ny);
   if (fx)
     fftr<false>(st, P, f, dp, 2);
   else
     hipLaunchKernelGGL(k_fft_c2r_rows<1>, dim3((Ny + 1) / 2), dim3(Tx), shx, st, Nx, Ny, Nh, P.pow2x, ilog2(Nx),
-                       (const double2*)P.twx, H, dp);
+                       (const double2*)P.twx, H, dp, (const unsigned long long*)P.fmax2);
+}
+
+// the early exit of pressure_poisson.m:32-38 for the matrix solvers: max |f|
+// as kExitBlocks partial maxima, then dp = 0 when their maximum is below tol
+constexpr int kExitBlocks = 64;
+__global__ void __launch_bounds__(256) k_absmax_part(const double* __restrict__ f, size_t n,
+                                                     double* __restrict__ part) {
+  double m = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    m = fmax(m, fabs(f[i]));
+  __shared__ double w[4];
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(w[0], w[1]), fmax(w[2], w[3]));
+}
+
+__global__ void __launch_bounds__(256) k_zero_if_small(const double* __restrict__ part, double tol,
+                                                      double* __restrict__ x, size_t n) {
+  __shared__ int small;
+  if (threadIdx.x < 64) {
+    double m = part[threadIdx.x];
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if (threadIdx.x == 0) small = m < tol;
+  }
+  __syncthreads();
+  if (!small) return;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = 0.0;
+}
+
+void launch_absmax_part(hipStream_t st, const double* f, size_t n, double* part) {
+  hipLaunchKernelGGL(k_absmax_part, dim3(kExitBlocks), dim3(256), 0, st, f, n, part);
+}
+
+void launch_zero_if_small(hipStream_t st, const double* part, double tol, double* x, size_t n) {
+  const unsigned nb = (unsigned)std::min<size_t>(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(k_zero_if_small, dim3(std::max(1u, nb)), dim3(256), 0, st, part, tol, x, n);
 }
 
 void launch_lap(hipStream_t st, const DLap& L, const double* p, double* q) {

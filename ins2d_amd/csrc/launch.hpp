@@ -78,6 +78,10 @@ void launch_fft_poisson_fused(hipStream_t st, const FFTPlan& P, const double* f,
 // both lengths take the register transforms (no row-major spectra buffer)
 bool fft_register_lengths(int Nx, int Ny);
 void launch_lap(hipStream_t st, const DLap& L, const double* p, double* q);
+// the early exit of pressure_poisson.m:32-38: max |f| as 64 partial maxima
+// (part: 64 doubles), then x = 0 when their maximum is below tol
+void launch_absmax_part(hipStream_t st, const double* f, size_t n, double* part);
+void launch_zero_if_small(hipStream_t st, const double* part, double tol, double* x, size_t n);
 void launch_cg_single(hipStream_t st, const DLap& L, const double* f, double* x, double* r, double* z, double* p,
                       double* q, double tol2, int maxit, int x0_zero, int* out_it, double* out_rel);
 void launch_cg_init(hipStream_t st, const DLap& L, const CGStateH& S, const double* f, const double* x, double* r,
@@ -125,6 +129,10 @@ struct TLPlan {
 struct DIC {
   const double *D, *cw, *cs, *cwx, *cwy;   // cwx / cwy null when not periodic
   int nx, ny;
+  // 0: M = (D + L) D^-1 (D + L^T), the IC(0) factor of make_cholinc.m;
+  // 1: M = L1 L1^T with L1 = (D + L) D^-1 the unit lower ILU(0) factor
+  // (pressure_poisson.m:52-64: pcg(-A, -f, tol, maxit, L, L'))
+  int lu;
 };
 void launch_cg_ic0(hipStream_t st, const DLap& L, const DIC& P, const double* f, double* x, double* r, double* z,
                    double* p, double* q, double tol2, int maxit, int x0_zero, int* out_it, double* out_rel);

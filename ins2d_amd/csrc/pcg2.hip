@@ -265,7 +265,9 @@ __global__ void k_tl_true(TL T, int nblk) {
 // z = M^-1 r for M = (D + L) D^-1 (D + L^T) (cg_matlab.m:135-198): forward
 // (D + L) q = r, then (I + D^-1 L^T) z = q in place; cell (i, j) depends on
 // its west/south (forward) or east/north (backward) neighbours and on the
-// periodic wrap cells, all on earlier anti-diagonals i + j in sweep order
+// periodic wrap cells, all on earlier anti-diagonals i + j in sweep order.
+// P.lu: M = (D + L) D^-2 (D + L^T), the backward sweep solves
+// (I + D^-1 L^T) z = D q
 __device__ void ic0_apply(const DIC& P, const double* __restrict__ r, double* __restrict__ z) {
   const int nx = P.nx, ny = P.ny, T = blockDim.x;
   const int ns = nx + ny - 1;
@@ -293,7 +295,7 @@ __device__ void ic0_apply(const DIC& P, const double* __restrict__ r, double* __
       if (j < ny - 1) acc += P.cs[c + nx] * z[c + nx];
       if (P.cwx && i == 0) acc += P.cwx[j] * z[c + nx - 1];
       if (P.cwy && j == 0) acc += P.cwy[i] * z[(size_t)i + (size_t)nx * (ny - 1)];
-      z[c] -= acc / P.D[c];
+      z[c] = (P.lu ? P.D[c] * z[c] : z[c]) - acc / P.D[c];
     }
     __syncthreads();
   }

@@ -777,6 +777,9 @@ def operator_divergence(opts):
         s.decomp = _singular_direct(d.A)
     if s.poisson in (3, 4):
         s.A_pc = make_cholinc_diag(d.A)
+    if s.poisson == 2:
+        # operator_divergence.m:243-247
+        s.L, s.U = ilu0(d.A)
 
 
 def _singular_direct(A):
@@ -1180,6 +1183,65 @@ def cg_matlab(A, y, tol, x_in, maxiter, D=None):
         p = z + (rz / rz_old) * p
 
 
+def ilu0(A):
+    """[L, U] = ilu(A, struct('type', 'nofill')) as operator_divergence.m:
+    243-247 calls it: incomplete LU without fill and without pivoting, row by
+    row (the IKJ form: for every row i, eliminate with the earlier rows k of
+    its own pattern, updating only entries in the pattern).  L is unit lower
+    triangular, U upper triangular, (L U)_ij = A_ij on the pattern of A."""
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    N = A.shape[0]
+    rows = [dict(zip(A.indices[A.indptr[i]:A.indptr[i + 1]], A.data[A.indptr[i]:A.indptr[i + 1]]))
+            for i in range(N)]
+    for i in range(N):
+        ri = rows[i]
+        for k in sorted(c for c in ri if c < i):
+            ri[k] = ri[k] / rows[k][k]
+            lik = ri[k]
+            for j, ukj in rows[k].items():
+                if j > k and j in ri:
+                    ri[j] -= lik * ukj
+    Li, Lj, Lv, Ui, Uj, Uv = [], [], [], [], [], []
+    for i, ri in enumerate(rows):
+        for j, v in ri.items():
+            if j < i:
+                Li.append(i); Lj.append(j); Lv.append(v)
+            else:
+                Ui.append(i); Uj.append(j); Uv.append(v)
+    L = sp.csr_matrix((Lv, (Li, Lj)), shape=(N, N)) + sp.identity(N, format='csr')
+    U = sp.csr_matrix((Uv, (Ui, Uj)), shape=(N, N))
+    return L.tocsr(), U.tocsr()
+
+
+def pcg_ilu(A, f, tol, maxit, L):
+    """pressure_poisson.m:52-64: [dp, flag, norm1, iter] = pcg(-A, -f, CG_acc,
+    CG_maxit, L, L', dp) with dp = 0: CG on -A with the preconditioner
+    M = L L^T (L from ilu0(A)), stopping when the residual norm falls to
+    tol ||f||.  Returns x, iterations."""
+    K = (-sp.csr_matrix(A)).tocsr()
+    b = -f
+    Lt = L.T.tocsr()
+    n2b = float(np.linalg.norm(b))
+    x = np.zeros_like(b)
+    if n2b == 0.0:
+        return x, 0
+    r = b.copy()
+    rho_old, p = None, None
+    for it in range(1, maxit + 1):
+        z = spla.spsolve_triangular(Lt, spla.spsolve_triangular(L, r, lower=True), lower=False)
+        rho = float(r @ z)
+        p = z.copy() if p is None else z + (rho / rho_old) * p
+        q = K @ p
+        alpha = rho / float(p @ q)
+        x = x + alpha * p
+        r = r - alpha * q
+        rho_old = rho
+        if np.linalg.norm(r) <= tol * n2b:
+            return x, it
+    raise RuntimeError(f'pcg did not converge in {maxit} iterations')
+
+
 def pressure_poisson(f, t, opts):
     """solvers/pressure/pressure_poisson.m:1-106."""
     g = opts.grid; s = opts.solversettings
@@ -1196,6 +1258,10 @@ def pressure_poisson(f, t, opts):
         return s.decomp(f)
     if s.poisson in (3, 4):
         x, it, nrm = cg_matlab(opts.discretization.A, f, s.CG_acc, dp, s.CG_maxit, s.A_pc)
+        s.last_iters = it
+        return x
+    if s.poisson == 2:
+        x, it = pcg_ilu(opts.discretization.A, f, s.CG_acc, s.CG_maxit, s.L)
         s.last_iters = it
         return x
     raise ValueError('wrong poisson method')

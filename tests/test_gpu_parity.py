@@ -491,3 +491,76 @@ def test_unsteady_step_rejects_periodic_kernels():
         solver.step_unsteady(0.0, 0.01, 1, np.zeros(1000), np.zeros(100))
     assert e.value.code == 4
     solver.close()
+
+
+@pytest.mark.parametrize('name', ['fft_reg', 'fft_lds', 'direct', 'cg_jacobi', 'cg_ic0', 'cg_fdm'])
+def test_poisson_early_exit(name):
+    """pressure_poisson.m:21-39: a negligible right-hand side returns dp = 0
+    exactly (FFT branch: max |fft2(f)| < 1e-14; matrix solvers: max |f| <
+    1e-14), as the oracle does; just above the threshold the solve runs."""
+    _say('poisson early exit', name)
+    if name == 'fft_reg':
+        case = _case_periodic_noise(32)
+    elif name == 'fft_lds':
+        case = load('shear_layer', Nx=40, Ny=24)
+    elif name == 'direct':
+        case = _wall_case('ldc', 24, 20, poisson=1)
+    else:
+        case = _wall_case('outflow', 24, 20, poisson=3, CG_acc=1e-10, CG_maxit=5000,
+                          CG_precond={'cg_jacobi': 'jacobi', 'cg_ic0': 'ic0', 'cg_fdm': 'fdm'}[name])
+    opts = O.build(case)
+    solver, _, _, _ = host.setup(case)
+    rng = np.random.default_rng(9)
+    f = rng.standard_normal(opts.grid.Np)
+    f -= f.mean()
+    # below the threshold: |fft2(f)| <= sum |f|, so max |f| * Np < 1e-14
+    tiny = f * (0.5e-14 / (np.abs(f).max() * opts.grid.Np))
+    dp, _, _ = solver.poisson(tiny)
+    dpo = O.pressure_poisson(tiny, 0.0, opts)
+    assert np.all(dpo == 0.0) and np.all(dp == 0.0)
+    # above it the solve runs (and matches the oracle); the CG modes also
+    # return x0 = 0 while ||f||^2 <= tol^2 (cg.F:225-231, cg_matlab.m:61-66),
+    # so the right-hand side is taken well above that too
+    big = f * (1e-6 / np.abs(f).max())
+    dp, _, _ = solver.poisson(big)
+    dpo = O.pressure_poisson(big, 0.0, opts)
+    assert np.abs(dp).max() > 0.0
+    if name != 'direct' and name.startswith('fft'):
+        assert _rel(dp, dpo) < 1e-11
+    solver.close()
+
+
+@pytest.mark.parametrize('name', ['ldc', 'outflow', 'channel'])
+def test_ilu_pcg_matches_oracle(name):
+    """poisson = 2 (pressure_poisson.m:52-64, pcg with the ILU(0) factor L as
+    M = L L^T): the GPU's wavefront triangular solves with the IC(0) pivots
+    (equal to ILU(0)'s for these symmetric 5-point matrices) reach the
+    oracle's iteration count within one and its solution."""
+    _say('ilu pcg', name)
+    kw = dict(poisson=2, CG_acc=1e-12, CG_maxit=5000)
+    if name == 'ldc':
+        case = _wall_case('ldc', 40, 30, sx=1.03, **kw)
+    elif name == 'outflow':
+        case = _wall_case('outflow', 48, 20, sx=1.03, **kw)
+    else:
+        case = _wall_case('channel', 36, 20, sy=1.04, **kw)
+    opts = O.build(case)
+    solver, _, _, _ = host.setup(case)
+    rng = np.random.default_rng(8)
+    f = rng.standard_normal(opts.grid.Np)
+    if name in ('ldc', 'channel'):
+        f -= f.mean()
+    dp, it, rel = solver.poisson(f)
+    xo, ito = O.pcg_ilu(opts.discretization.A, f, 1e-12, 5000, opts.solversettings.L)
+    _say(name, 'iterations gpu', it, 'oracle', ito, 'rel diff', _rel(dp, xo))
+    assert abs(it - ito) <= 1
+    assert _rel(dp, xo) < 1e-9
+    solver.close()
+
+
+def test_ldc_run_ilu_pcg_matches_oracle():
+    """results/LDC_unsteady_1.000e+02_20x20_0 configuration with poisson = 2
+    at a tight tolerance: the whole run against the oracle's."""
+    _say('ldc 20x20 run ilu pcg')
+    case = load('LDC_unsteady', RK='RK33P2', poisson=2, CG_acc=1e-13, CG_maxit=2000)
+    _compare_runs(case, 20, True)

@@ -201,3 +201,42 @@ def test_fourth_order_projection_gives_divergence_free_field():
     V, p, row = O.input_check(opts)
     d = opts.discretization
     assert np.abs(d.M @ V + d.yM).max() < 1e-12
+
+
+@pytest.mark.parametrize('maker', ['ldc', 'outflow', 'channel'])
+def test_ilu0_factor_and_pcg(maker):
+    """poisson = 2 (pressure_poisson.m:52-64 with operator_divergence.m:243-247):
+    the oracle's ILU(0) reproduces A on its pattern (the defining property
+    of a no-fill incomplete LU), is the transpose-symmetric factor of a
+    symmetric matrix, and its pivots are make_cholinc.m's IC(0) diagonal (two
+    restatements: row-by-row elimination against the diagonal-storage
+    recurrence); its PCG agrees with SciPy's CG given the same
+    preconditioner M = L L^T."""
+    import scipy.sparse.linalg as spla
+    if maker == 'ldc':
+        opts = O.build(load('LDC_unsteady', Nx=14, Ny=10, poisson=2, CG_acc=1e-10, CG_maxit=2000))
+    elif maker == 'outflow':
+        opts = O.build(_outflow(poisson=2, CG_acc=1e-10, CG_maxit=2000))
+    else:
+        opts = O.build(_periodic_channel(poisson=2, CG_acc=1e-10, CG_maxit=2000))
+    A = opts.discretization.A.tocsr()
+    L, U = opts.solversettings.L, opts.solversettings.U
+    LU = (L @ U).tocsr()
+    coo = A.tocoo()
+    assert np.abs(np.asarray(LU[coo.row, coo.col]).ravel() - coo.data).max() < 1e-12 * np.abs(coo.data).max()
+    assert abs(U - sp.diags(U.diagonal()) @ L.T).max() < 1e-12 * np.abs(coo.data).max()
+    np.testing.assert_allclose(U.diagonal(), O.make_cholinc_diag(A), rtol=1e-13)
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal(opts.grid.Np)
+    if maker != 'outflow':
+        f -= f.mean()
+    x, it = O.pcg_ilu(A, f, 1e-10, 2000, L)
+    K = (-A).tocsr()
+    Lt = L.T.tocsr()
+    Minv = spla.LinearOperator(K.shape, matvec=lambda r: spla.spsolve_triangular(
+        Lt, spla.spsolve_triangular(L, r, lower=True), lower=False))
+    count = []
+    xs, info = spla.cg(K, -f, rtol=1e-10, atol=0.0, maxiter=2000, M=Minv, callback=lambda xk: count.append(1))
+    assert info == 0 and abs(len(count) - it) <= 1
+    np.testing.assert_allclose(x, xs, atol=1e-7 * np.abs(xs).max())
+    assert np.linalg.norm(A @ x - f) <= 1.01e-10 * np.linalg.norm(f)
