@@ -681,6 +681,19 @@ __global__ void k_b_clear_flags(int* f, int n) {
   if (k < n) f[4 * k] = 0;
 }
 
+// end of a device-iterated solve: per instance the iteration count into the
+// running totals, and a failure when the stopping rule was not met (or the
+// iteration limit reached) within the solve's iterations
+__global__ void k_bf_stats(BCG S, int n, unsigned long long* __restrict__ it_sum, int* __restrict__ it_max,
+                           int* __restrict__ nfail) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int it = S.f[4 * k + 1];
+    atomicAdd(it_sum, (unsigned long long)it);
+    atomicMax(it_max, it);
+    if (!S.f[4 * k] || S.f[4 * k + 2]) atomicAdd(nfail, 1);
+  }
+}
+
 }  // namespace ins2d
 
 using namespace ins2d;
@@ -771,6 +784,19 @@ struct ins2d_batch {
   // fast-diagonalization PCG: the tail of an iteration (cg_chunk holds its head)
   hipGraphExec_t cg_tail = nullptr;
   hipGraph_t cg_tail_graph = nullptr;
+  // fast-diagonalization PCG with device-side iterations: a whole RK step
+  // (every stage and solve) captured as one graph per parity of the state
+  // buffers (the step swaps V and Vb), replayed without host checks;
+  // iteration totals and failures accumulate on the device
+  bool device_iter = false;
+  int fixed_iters = 1;
+  double *Vsave = nullptr, *Psave = nullptr;   // state at the start of a call (for the fallback)
+  hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+  hipGraph_t step_graph[2] = {nullptr, nullptr};
+  const double* step_V[2] = {nullptr, nullptr};
+  unsigned long long* d_it_sum = nullptr;
+  int* d_it_max = nullptr;   // d_it_max[0]: largest count, d_it_max[1]: failures
+  unsigned long long h_it_sum = 0;
   // stage-kernel tile shape (velocity tiles), chosen by ins2d_batch_autotune
   int tx = TX, ty = TY;
   // two-level PCG (cg_precond = 2)
@@ -806,7 +832,20 @@ T* balloc(ins2d_batch* b, size_t n) {
   return (T*)p;
 }
 
+// the captured steps hold the addresses of the tile lists and boundary
+// vectors: dropped whenever those are rebuilt
+void drop_step_graphs(ins2d_batch* b) {
+  for (int k = 0; k < 2; ++k) {
+    if (b->step_exec[k]) (void)hipGraphExecDestroy(b->step_exec[k]);
+    if (b->step_graph[k]) (void)hipGraphDestroy(b->step_graph[k]);
+    b->step_exec[k] = nullptr;
+    b->step_graph[k] = nullptr;
+    b->step_V[k] = nullptr;
+  }
+}
+
 void b_upload_bc(ins2d_batch* b) {
+  drop_step_graphs(b);
   for (void* p : b->yv_allocs) (void)hipFree(p);
   b->yv_allocs.clear();
   // size, then fill one buffer with every boundary term of every instance
@@ -1102,6 +1141,7 @@ void rhs_lists_launch(ins2d_batch* b, const RTile* drs, size_t nclosed, size_t n
 
 // upload rs and rg into the device lists (grown as needed)
 void upload_rhs_lists(ins2d_batch* b, std::vector<RTile>& rs, std::vector<Tile>& rg) {
+  drop_step_graphs(b);
   if (!b->drs || rs.size() > b->rs_cap) {
     b->drs = balloc<RTile>(b, rs.size());
     b->rs_cap = rs.size();
@@ -1198,6 +1238,18 @@ void b_poisson(ins2d_batch* b, const double* f) {
     hipLaunchKernelGGL(k_b_cg_init, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, f, b->x, b->r, b->z,
                        b->pp);
     hipLaunchKernelGGL(k_b_cg_init_fin, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tol * b->tol);
+  }
+  if (b->fdm && b->device_iter) {
+    // a fixed number of iterations on the device (the exact inverse as
+    // preconditioner meets the stopping rule in one): instances that have
+    // converged skip the rest; no host check, so the solve can be captured
+    for (int it = 0; it < b->fixed_iters; ++it) {
+      fdm_iter_head(b);
+      if (it + 1 < b->fixed_iters) fdm_iter_tail(b);
+    }
+    hipLaunchKernelGGL(k_bf_stats, dim3((b->n + 255) / 256), dim3(256), 0, b->st, b->cg, b->n, b->d_it_sum,
+                       b->d_it_max, b->d_it_max + 1);
+    return;
   }
   int active = b->n;
   if (b->fdm) {
@@ -1446,6 +1498,15 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
     b->cg.part = balloc<double>(b.get(), 2 * b->pt.size());
     b->cg.nactive = balloc<int>(b.get(), 1);
     b->cg.maxit = b->maxit;
+    b->d_it_sum = balloc<unsigned long long>(b.get(), 1);
+    b->d_it_max = balloc<int>(b.get(), 2);
+    BHIP(hipMemset(b->d_it_sum, 0, sizeof(unsigned long long)));
+    BHIP(hipMemset(b->d_it_max, 0, 2 * sizeof(int)));
+    b->fixed_iters = 1;
+    if (b->fdm) {
+      b->Vsave = balloc<double>(b.get(), b->NVt);
+      b->Psave = balloc<double>(b.get(), b->Npt);
+    }
     BHIP(hipMemset(b->V, 0, sizeof(double) * b->NVt));
     BHIP(hipMemset(b->P, 0, sizeof(double) * b->Npt));
     BHIP(hipDeviceSynchronize());
@@ -1551,6 +1612,7 @@ void ins2d_batch_destroy(ins2d_batch* b) {
   if (b->cg_graph) (void)hipGraphDestroy(b->cg_graph);
   if (b->cg_tail) (void)hipGraphExecDestroy(b->cg_tail);
   if (b->cg_tail_graph) (void)hipGraphDestroy(b->cg_tail_graph);
+  drop_step_graphs(b);
   for (void* p : b->allocs) (void)hipFree(p);
   for (void* p : b->yv_allocs) (void)hipFree(p);
   if (b->st) (void)hipStreamDestroy(b->st);
@@ -1633,6 +1695,55 @@ int ins2d_batch_step(ins2d_batch* b, const double* dt, int nsteps) {
     b->bc_dirty = false;
   }
   BHIP(hipMemcpyAsync(b->dt, dt, sizeof(double) * b->n, hipMemcpyHostToDevice, b->st));
+  if (!b->fdm) {
+    for (int n = 0; n < nsteps; ++n) b_step(b);
+    BHIP(hipStreamSynchronize(b->st));
+    return INS2D_OK;
+  }
+  // fast-diagonalization PCG: each step is one graph replay (captured on
+  // first use for each parity of the state buffers) with one PCG iteration
+  // per solve on the device, and one host wait at the end.  Should any solve
+  // miss the stopping rule in that iteration, the call starts over from the
+  // saved state on the host-checked path (iterations up to cg_maxit).
+  double* V_start = b->V;
+  BHIP(hipMemcpyAsync(b->Vsave, b->V, sizeof(double) * b->NVt, hipMemcpyDeviceToDevice, b->st));
+  BHIP(hipMemcpyAsync(b->Psave, b->P, sizeof(double) * b->Npt, hipMemcpyDeviceToDevice, b->st));
+  for (int n = 0; n < nsteps; ++n) {
+    int par = -1;
+    for (int k = 0; k < 2; ++k)
+      if (b->step_exec[k] && b->step_V[k] == b->V) par = k;
+    if (par < 0) {
+      par = b->step_exec[0] ? 1 : 0;
+      b->device_iter = true;
+      BHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+      double* V0 = b->V;
+      b_step(b);   // swaps V and Vb
+      std::swap(b->V, b->Vb);
+      BHIP(hipStreamEndCapture(b->st, &b->step_graph[par]));
+      b->device_iter = false;
+      BHIP(hipGraphInstantiate(&b->step_exec[par], b->step_graph[par], nullptr, nullptr, 0));
+      b->step_V[par] = V0;
+    }
+    BHIP(hipGraphLaunch(b->step_exec[par], b->st));
+    std::swap(b->V, b->Vb);
+  }
+  unsigned long long sum = 0;
+  int mx[2] = {0, 0};
+  BHIP(hipMemcpyAsync(&sum, b->d_it_sum, sizeof(sum), hipMemcpyDeviceToHost, b->st));
+  BHIP(hipMemcpyAsync(mx, b->d_it_max, sizeof(mx), hipMemcpyDeviceToHost, b->st));
+  BHIP(hipStreamSynchronize(b->st));
+  if (!mx[1]) {
+    b->total_it += (long long)(sum - b->h_it_sum);
+    b->h_it_sum = sum;
+    b->max_it = std::max(b->max_it, mx[0]);
+    return INS2D_OK;
+  }
+  // fallback: the counts of the discarded attempt are dropped
+  b->h_it_sum = sum;
+  BHIP(hipMemsetAsync(b->d_it_max + 1, 0, sizeof(int), b->st));
+  if (b->V != V_start) std::swap(b->V, b->Vb);
+  BHIP(hipMemcpyAsync(b->V, b->Vsave, sizeof(double) * b->NVt, hipMemcpyDeviceToDevice, b->st));
+  BHIP(hipMemcpyAsync(b->P, b->Psave, sizeof(double) * b->Npt, hipMemcpyDeviceToDevice, b->st));
   for (int n = 0; n < nsteps; ++n) b_step(b);
   BHIP(hipStreamSynchronize(b->st));
   return INS2D_OK;

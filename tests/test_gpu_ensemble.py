@@ -142,3 +142,49 @@ def test_ensemble_stage_tile_shapes_match_oracle(tile):
         _say(k, f'V {err:.2e}')
         assert err < 1e-12
     B.close()
+
+
+def test_fdm_ensemble_graph_steps_across_calls():
+    """With the fast-diagonalization preconditioner each RK step is one
+    replay of a captured graph (one per parity of the state buffers), with
+    the PCG iterations on the device: steps split over calls (2 + 3, so both
+    parities are captured and each is replayed from either buffer) give the
+    oracle's 5-step result, and the iteration statistics read back at the
+    end of each call."""
+    _say('ensemble graph steps across calls')
+    kw = dict(cg_tol=1e-12, precond='fdm')
+    members = [ensemble.member('cavity', 24, 16, 1.5, 400.0, 1.0, **kw),
+               ensemble.member('channel', 40, 20, 2.0, 3000.0, 1.1, **kw),
+               ensemble.member('cavity', 33, 27, 33 / 27, 900.0, 0.8, **kw)]
+    B = host.Batch(members)
+    B.input_check(1)
+    B.step(2)
+    t1, m1, _ = B.stats()
+    B.step(3)
+    t2, m2, _ = B.stats()
+    _say('iterations after 2 steps', t1, 'after 5', t2, 'max per solve', m2)
+    # every member solves 4 times per step (3 stages + the additional solve)
+    assert t2 > t1 >= 2 * 4 * len(members) and m2 >= 1
+    for k, m in enumerate(members):
+        Vb, pb = B.get_state(k)
+        Vo, po, _, _ = O.run(m, nsteps=5, overrides={'poisson': 1})
+        if m.parameters()['kind'] == 'cavity':
+            pb = pb - pb.mean(); po = po - po.mean()
+        ev = np.abs(Vb - Vo).max() / np.abs(Vo).max()
+        ep = np.abs(pb - po).max() / np.abs(po).max()
+        _say(k, f'V {ev:.2e} p {ep:.2e}')
+        assert ev < 1e-12 and ep < 1e-12
+    B.close()
+
+
+def test_fdm_ensemble_reports_nonconvergence():
+    """An unreachable tolerance: the device-side iteration misses it, the
+    call starts over on the host-checked path, which reports
+    INS2D_E_NOCONV after cg_maxit iterations."""
+    _say('ensemble nonconvergence')
+    members = [ensemble.member('cavity', 24, 16, 1.5, 400.0, 1.0, cg_tol=1e-30, cg_maxit=2, precond='fdm')]
+    B = host.Batch(members)
+    with pytest.raises(host.Ins2dError) as e:
+        B.step(1)
+    assert e.value.code == 3
+    B.close()
