@@ -38,7 +38,10 @@ extern "C" {
 /* pressure solvers (pressure_poisson.m:42-104 numbering) */
 #define INS2D_POISSON_DIRECT 1 /* direct solve by fast diagonalization of the
                                   tensor-product A (singular all-Neumann A:
-                                  the solution of zero area-weighted mean)   */
+                                  the solution of zero area-weighted mean);
+                                  fourth order with walls: the dense inverse
+                                  of A (at most 8192 cells; singular A: the
+                                  zero-mean solution)                        */
 #define INS2D_POISSON_PCG_ILU 2 /* CG preconditioned by the unit lower ILU(0)
                                   factor L as M = L L^T (pcg + ilu 'nofill',
                                   pressure_poisson.m:52-64); at most 2^20
@@ -60,7 +63,12 @@ typedef struct {
   int bc_u[4];               /* u: left, right, low, up                         */
   int bc_v[4];               /* v: left, right, low, up                         */
   double Re;                 /* Reynolds number (1e100 = inviscid)              */
-  int order4;                /* 0: second order, 1: fourth order (periodic)     */
+  int order4;                /* 0: second order, 1: fourth order: periodic
+                                uniform grids with the FFT solve take the
+                                closed-form kernels, any other grid whose
+                                ends are Dirichlet or periodic the assembled
+                                operators (CSR) with poisson = DIRECT and
+                                steady boundary values                          */
   double alfa, beta;         /* fourth-order parameters (81, 9/8)               */
   int rk_stages;             /* s                                               */
   const double* rk_A;        /* s*s, row-major                                  */
@@ -98,7 +106,8 @@ void ins2d_destroy(ins2d_handle* h);
 const char* ins2d_last_error(const ins2d_handle* h);
 
 /* sizes: out[0..6] = Nux_in, Nuy_in, Nvx_in, Nvy_in, Npx, Npy, kernel path
- * (0 generic, 2 periodic second order, 4 periodic fourth order); xin (Nux_in),
+ * (0 generic or assembled fourth order, 2 periodic second order, 4 periodic
+ * fourth order); xin (Nux_in),
  * yin (Nvy_in) are the interior velocity grid lines (operator_mesh.m:185-320) */
 int ins2d_grid_info(const ins2d_handle* h, int* out, double* xin, double* yin);
 

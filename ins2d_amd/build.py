@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libins2d_hip.so")
 SOURCES = ["ops1d.cpp", "kernels.hip", "api.hip", "batch.hip", "unet.hip", "fdm.hip", "fdm_host.cpp", "pcg2.hip",
-           "noalloc.hip"]
+           "noalloc.hip", "ops4.cpp", "w4.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -51,6 +51,7 @@ def build(verbose=False):
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     build_mex_harness(verbose)
+    build_w4_harness(verbose)
     return LIB
 
 
@@ -63,6 +64,22 @@ def build_mex_harness(verbose=False):
            "-I" + os.path.join(ROOT, "tests", "native", "mex_mock"), "-I" + os.path.join(ROOT, "include"),
            os.path.join(ROOT, "matlab", "ins2d_mex.c"), os.path.join(ROOT, "tests", "native", "mex_mock.c"),
            "-L" + HERE, "-lins2d_hip", "-Wl,-rpath,$ORIGIN/../../ins2d_amd", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_w4_harness(verbose=False):
+    """Host-only build of the fourth-order wall operator assembly
+    (csrc/ops4.cpp, csrc/ops1d.cpp) with the test harness of
+    tests/native/w4_dump.cpp, which hands the assembled matrices to the CPU
+    tests."""
+    out = os.path.join(ROOT, "tests", "native", "libw4_dump.so")
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall",
+           os.path.join(CSRC, "ops4.cpp"), os.path.join(CSRC, "ops1d.cpp"),
+           os.path.join(ROOT, "tests", "native", "w4_dump.cpp"), "-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

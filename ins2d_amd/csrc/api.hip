@@ -16,6 +16,7 @@
 #include "device.hpp"
 #include "launch.hpp"
 #include "ops1d.hpp"
+#include "ops4.hpp"
 
 using namespace ins2d;
 
@@ -50,6 +51,12 @@ struct ins2d_handle {
   int nux = 0, nuy = 0, nvx = 0, nvy = 0;
   double hx = 0, hy = 0, Re = 1, invRe = 0;
   std::unique_ptr<Plan> plan;
+  // fourth-order operators on grids with walls (CSR on the device) and the
+  // dense inverse of their pressure matrix; null on the other paths
+  std::unique_ptr<W4Ops> w4;
+  W4Dev w4d{};
+  double* one = nullptr;   // a device 1.0 (unit weights of the reductions)
+  int bc_u[4] = {0, 0, 0, 0}, bc_v[4] = {0, 0, 0, 0};
   std::vector<void*> allocs;
   // operators (generic)
   DComp hCu{}, hCv{};
@@ -251,6 +258,40 @@ void push_comps(ins2d_handle* h) {
 }
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// the dense inverse of the fourth-order pressure matrix with walls holds
+// Np^2 doubles (8192 cells: 512 MiB)
+constexpr long kW4MaxNp = 8192;
+
+DCsr up_csr(ins2d_handle* h, const Sp& A) {
+  std::vector<int> rp(A.m + 1, 0), ci;
+  std::vector<double> v;
+  for (int i = 0; i < A.m; ++i) {
+    for (auto& kv : A.r[i]) {
+      ci.push_back(kv.first);
+      v.push_back(kv.second);
+    }
+    rp[i + 1] = (int)ci.size();
+  }
+  DCsr d{};
+  d.m = A.m;
+  d.n = A.n;
+  d.rp = dupload(h, rp);
+  d.ci = dupload(h, ci);
+  d.v = dupload(h, v);
+  return d;
+}
+
+// boundary vectors of the fourth-order wall operators at the current samples
+void w4_upload_bc(ins2d_handle* h) {
+  std::vector<double> yA, yI, yD, yM;
+  w4_bc_vectors(*h->w4, h->samp, yA, yI, yD, yM);
+  const W4Dev& D = h->w4d;
+  HIPCHK(hipMemcpy(const_cast<double*>(D.yA), yA.data(), yA.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(const_cast<double*>(D.yI), yI.data(), yI.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(const_cast<double*>(D.yD), yD.data(), yD.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(const_cast<double*>(D.yM), yM.data(), yM.size() * sizeof(double), hipMemcpyHostToDevice));
+}
 
 // timing with HIP events on the handle's stream.  KERNEL scopes wrap one
 // launcher of the stencil kernels: its kernels record their own start and end
@@ -488,7 +529,9 @@ const StageCombine kNoCombine{0, 0.0, {0}, {nullptr}};
 
 void rhs(ins2d_handle* h, const double* V, const double* p, int mode, double* out, const StageCombine& sc, double* vt) {
   ProfScope ps(h, 0, true);
-  if (h->path) {
+  if (h->w4) {
+    launch_w4_rhs(h->st, h->w4d, V, p, mode, out, sc, vt);
+  } else if (h->path) {
     launch_rhs_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, h->invRe, V, V + h->Nu, p, mode, out, sc, vt);
   } else {
     StageCombine scv = sc;
@@ -506,7 +549,9 @@ void rhs(ins2d_handle* h, const double* V, const double* p, int mode, double* ou
 void div(ins2d_handle* h, const double* A, const double* B, double ca, double cb, double ycoef, double oscale,
          double* f) {
   ProfScope ps(h, 1, true);
-  if (h->path)
+  if (h->w4)
+    launch_w4_div(h->st, h->w4d, A, B, ca, cb, ycoef, oscale, f);
+  else if (h->path)
     launch_div_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, A, B, ca, cb, oscale, f);
   else
     launch_div_generic(h->st, h->dCu, h->dCv, h->Nx, h->Ny, A, B, ca, cb, ycoef, oscale, h->Nu, f);
@@ -514,7 +559,9 @@ void div(ins2d_handle* h, const double* A, const double* B, double ca, double cb
 
 void proj(ins2d_handle* h, const double* Vn, const double* Vt, const double* dp, double dt, double c, double* V) {
   ProfScope ps(h, 3, true);
-  if (h->path)
+  if (h->w4)
+    launch_w4_proj(h->st, h->w4d, Vn, Vt, dp, dt, c, V);
+  else if (h->path)
     launch_proj_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, Vn, Vt, dp, dt, c, V);
   else
     launch_proj_generic(h->st, h->dCu, h->dCv, h->nux, h->nuy, h->nvx, h->nvy, h->Nu, Vn, Vt, dp, dt, c, V);
@@ -600,7 +647,10 @@ void poisson(ins2d_handle* h, const double* f, double* dp) {
 // the direct and CG solvers of pressure_poisson.m:42-90
 void poisson_matrix(ins2d_handle* h, const double* f, double* dp) {
   if (h->poisson == INS2D_POISSON_DIRECT) {
-    launch_fdm_poisson(h->st, h->fdm, f, dp, h->cr, h->cz);
+    if (h->w4)
+      launch_w4_solve(h->st, h->w4d, f, dp);
+    else
+      launch_fdm_poisson(h->st, h->fdm, f, dp, h->cr, h->cz);
     h->last_it = 0;
     h->last_rel = 0.0;
     return;
@@ -943,18 +993,25 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
     for (int j = 0; j < d->Ny; ++j) { double t = d->y[j + 1] - d->y[j]; hymin = std::min(hymin, t); hymax = std::max(hymax, t); }
     const bool uniform = (hxmax - hxmin) <= 1e-14 && (hymax - hymin) <= 1e-14;
     if (allper && (d->Nx < 8 || d->Ny < 8)) throw Unsupported("periodic grids need at least 8 cells per direction");
+    for (int k = 0; k < 4; ++k) { h->bc_u[k] = d->bc_u[k]; h->bc_v[k] = d->bc_v[k]; }
+    // fourth order: the periodic uniform kernels with the FFT solve, else the
+    // assembled operators with the direct solve
+    bool w4 = false;
     if (d->order4) {
-      if (!allper || !uniform) throw Unsupported("fourth order is implemented for periodic uniform grids");
-      if (d->alfa != 81.0 || d->beta != 9.0 / 8.0) throw Unsupported("fourth order needs alfa = 81, beta = 9/8");
-      if (d->kernels == 1) throw Unsupported("fourth order has no generic kernel path");
-      h->path = 4;
+      if (allper && uniform && d->kernels != 1 && h->poisson == INS2D_POISSON_FFT) {
+        if (d->alfa != 81.0 || d->beta != 9.0 / 8.0)
+          throw Unsupported("the periodic fourth-order kernels need alfa = 81, beta = 9/8");
+        h->path = 4;
+      } else {
+        if (h->poisson != INS2D_POISSON_DIRECT)
+          throw Unsupported("fourth order solves the pressure by FFT (periodic uniform grids) or directly (poisson = 1)");
+        w4 = true;
+      }
     } else if (allper && uniform && d->kernels != 1) {
       h->path = 2;
     }
     if (h->poisson == INS2D_POISSON_FFT && !(allper && uniform))
       throw Unsupported("FFT Poisson needs periodic boundaries and a uniform grid (operator_divergence.m:268-290)");
-    if (h->poisson != INS2D_POISSON_FFT && d->order4)
-      throw Unsupported("fourth order pressure is solved by FFT");
     if (h->poisson != INS2D_POISSON_FFT && h->poisson != INS2D_POISSON_CG && h->poisson != INS2D_POISSON_CG_IC &&
         h->poisson != INS2D_POISSON_DIRECT && h->poisson != INS2D_POISSON_PCG_ILU)
       throw ArgError("unknown poisson mode");
@@ -1013,9 +1070,51 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
       h->ovy = dupload(h.get(), P.cv.oy);
       h->hx = P.hx[0];
       h->hy = P.hy[0];
+    } else if (w4) {
+      if ((long)d->Nx * d->Ny > kW4MaxNp)
+        throw Unsupported("fourth order with the direct solve: at most " + std::to_string(kW4MaxNp) +
+                          " pressure cells (dense inverse)");
+      CaseDesc cd;
+      cd.Nx = d->Nx;
+      cd.Ny = d->Ny;
+      cd.x.assign(d->x, d->x + d->Nx + 1);
+      cd.y.assign(d->y, d->y + d->Ny + 1);
+      cd.u_left = d->bc_u[0]; cd.u_right = d->bc_u[1]; cd.u_low = d->bc_u[2]; cd.u_up = d->bc_u[3];
+      cd.v_left = d->bc_v[0]; cd.v_right = d->bc_v[1]; cd.v_low = d->bc_v[2]; cd.v_up = d->bc_v[3];
+      cd.Re = d->Re;
+      cd.order4 = 1;
+      cd.alfa = d->alfa;
+      cd.beta = d->beta;
+      h->w4.reset(new W4Ops());
+      try {
+        build_w4(cd, *h->w4);
+      } catch (const std::runtime_error& e) {
+        throw Unsupported(e.what());
+      }
+      const W4Ops& W = *h->w4;
+      h->nux = W.nux; h->nuy = W.nuy; h->nvx = W.nvx; h->nvy = W.nvy;
+      W4Dev& D = h->w4d;
+      D.NV = W.NV;
+      D.Np = W.Np;
+      D.Nf = W.Nf;
+      D.Af = up_csr(h.get(), W.Af);
+      D.If = up_csr(h.get(), W.If);
+      D.Cw = up_csr(h.get(), W.Cw);
+      D.Diff = up_csr(h.get(), W.Diff);
+      D.G = up_csr(h.get(), W.G);
+      D.M = up_csr(h.get(), W.M);
+      D.Om = dupload(h.get(), W.Om);
+      D.yA = dalloc<double>(h.get(), W.Nf);
+      D.yI = dalloc<double>(h.get(), W.Nf);
+      D.yD = dalloc<double>(h.get(), W.NV);
+      D.yM = dalloc<double>(h.get(), W.Np);
+      D.w = dalloc<double>(h.get(), W.Nf);
+      fill_samples_default(h.get());
+      w4_upload_bc(h.get());
     } else {
       h->nux = h->Nx; h->nuy = h->Ny; h->nvx = h->Nx; h->nvy = h->Ny;
     }
+    h->one = dupload(h.get(), std::vector<double>{1.0});
     if (h->path) {
       h->nux = h->Nx; h->nuy = h->Ny; h->nvx = h->Nx; h->nvy = h->Ny;
       h->hx = (d->x[d->Nx] - d->x[0]) / d->Nx;
@@ -1101,7 +1200,21 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
       } else if (d->cg_precond < 0 || d->cg_precond > 3) {
         throw ArgError("cg_precond must be 0 (Jacobi), 1 (IC(0)), 2 (two-level) or 3 (fast diagonalization)");
       }
-      if (h->poisson == INS2D_POISSON_DIRECT || h->fdm_pcg) {
+      if (h->w4) {
+        // dense inverse of the pressure matrix: Gauss-Jordan on C = -A, plus
+        // e e^T / Np when A has the constant null vector; the solve then
+        // returns the zero-mean solution (that of the bordered system
+        // [A e; e^T 0])
+        const int n = h->Np;
+        const double shift = h->w4->singular ? 1.0 / n : 0.0;
+        std::vector<double> C((size_t)n * n, shift);
+        for (int i = 0; i < n; ++i)
+          for (auto& kv : h->w4->A.r[i]) C[(size_t)i * n + kv.first] -= kv.second;
+        h->w4d.K = dupload(h.get(), C);
+        w4_invert(h->st, h->w4d.K, n, h->cr, h->cz, shift);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->st));
+      } else if (h->poisson == INS2D_POISSON_DIRECT || h->fdm_pcg) {
         FDMHost F;
         build_fdm_host(*h->plan, F);
         h->fdm.nx = F.nx;
@@ -1171,6 +1284,9 @@ int ins2d_grid_info(const ins2d_handle* h, int* out, double* xin, double* yin) {
   if (h->plan) {
     if (xin) std::copy(h->plan->xin.begin(), h->plan->xin.end(), xin);
     if (yin) std::copy(h->plan->yin.begin(), h->plan->yin.end(), yin);
+  } else if (h->w4) {
+    if (xin) std::copy(h->w4->xin.begin(), h->w4->xin.end(), xin);
+    if (yin) std::copy(h->w4->yin.begin(), h->w4->yin.end(), yin);
   }
   return INS2D_OK;
 }
@@ -1185,6 +1301,7 @@ int ins2d_set_bc(ins2d_handle* h, double t, const double* const* samples) {
     std::copy(samples[s], samples[s] + h->samp.s[s].size(), h->samp.s[s].begin());
   }
   h->have_bc = true;
+  if (h->w4) w4_upload_bc(h);
   if (h->plan) {
     for (void* p : h->yv_allocs) (void)hipFree(p);
     h->yv_allocs.clear();
@@ -1200,6 +1317,7 @@ int ins2d_set_bc(ins2d_handle* h, double t, const double* const* samples) {
 int ins2d_set_bc_rate(ins2d_handle* h, const double* const* rates) {
   if (!h) return INS2D_E_ARG;
   API_TRY
+  if (h->w4 && rates) throw Unsupported("time-dependent boundary values are not provided with fourth-order walls");
   HIPCHK(hipStreamSynchronize(h->st));
   if (!rates) {
     h->have_rate = false;
@@ -1307,7 +1425,7 @@ int ins2d_step_unsteady(ins2d_handle* h, double tn, double dt, int nsteps, const
   if (!h || nsteps < 0 || !(dt > 0) || (nsteps > 0 && (!samples || !rates))) return INS2D_E_ARG;
   API_TRY
   if (h->path || !h->plan)
-    throw Unsupported("time-dependent boundary values need the kernels with boundary terms (this handle runs the periodic ones)");
+    throw Unsupported("time-dependent boundary values need the second-order kernels with boundary terms");
   if (nsteps == 0) return INS2D_OK;
   const int s = h->s, nv = s + 2;
   size_t set_len = 0, rate_len = 0;
@@ -1385,21 +1503,31 @@ int ins2d_diagnostics(ins2d_handle* h, double t, double* out) {
   div(h, h->V, nullptr, 1.0, 0.0, 1.0, 1.0, h->f);
   stats_reduce(h, h->f, h->Nx, h->Ny, nullptr, nullptr, a);
   out[1] = a[0];
-  stats_reduce(h, h->V, h->nux, h->nuy, h->ox, h->oy, a);
-  stats_reduce(h, h->V + h->Nu, h->nvx, h->nvy, h->ovx, h->ovy, b);
+  if (h->w4) {
+    // volumes Om = alfa Om1 - Om3 do not factor: full weight arrays as one row
+    stats_reduce(h, h->V, h->Nu, 1, h->w4d.Om, h->one, a);
+    stats_reduce(h, h->V + h->Nu, h->Nv, 1, h->w4d.Om + h->Nu, h->one, b);
+  } else {
+    stats_reduce(h, h->V, h->nux, h->nuy, h->ox, h->oy, a);
+    stats_reduce(h, h->V + h->Nu, h->nvx, h->nvy, h->ovx, h->ovy, b);
+  }
   double umom = a[1], vmom = b[1], k = 0.5 * a[2] + 0.5 * b[2];
-  // Dirichlet boundary contributions (check_conservation.m:45-76)
-  if (h->plan) {
-    const Plan& P = *h->plan;
+  // Dirichlet boundary contributions with the second-order widths
+  // (check_conservation.m:45-76)
+  if (h->plan || h->w4) {
+    const std::vector<double>& hx = h->plan ? h->plan->hx : h->w4->hx;
+    const std::vector<double>& hy = h->plan ? h->plan->hy : h->w4->hy;
+    const std::vector<double>& gx = h->plan ? h->plan->gx : h->w4->gx;
+    const std::vector<double>& gy = h->plan ? h->plan->gy : h->w4->gy;
     const auto& S = h->samp.s;
-    if (P.cd.u_left == BC_DIR)
-      for (int j = 0; j < h->Ny; ++j) { umom += S[S_uLe_i][j] * P.hy[j] * P.gx[0]; k += 0.5 * S[S_uLe_i][j] * S[S_uLe_i][j] * P.hy[j] * P.gx[0]; }
-    if (P.cd.u_right == BC_DIR)
-      for (int j = 0; j < h->Ny; ++j) { umom += S[S_uRi_i][j] * P.hy[j] * P.gx[h->Nx]; k += 0.5 * S[S_uRi_i][j] * S[S_uRi_i][j] * P.hy[j] * P.gx[h->Nx]; }
-    if (P.cd.v_low == BC_DIR)
-      for (int i = 0; i < h->Nx; ++i) { vmom += S[S_vLo_i][i] * P.hx[i] * P.gy[0]; k += 0.5 * S[S_vLo_i][i] * S[S_vLo_i][i] * P.hx[i] * P.gy[0]; }
-    if (P.cd.v_up == BC_DIR)
-      for (int i = 0; i < h->Nx; ++i) { vmom += S[S_vUp_i][i] * P.hx[i] * P.gy[h->Ny]; k += 0.5 * S[S_vUp_i][i] * S[S_vUp_i][i] * P.hx[i] * P.gy[h->Ny]; }
+    if (h->bc_u[0] == BC_DIR)
+      for (int j = 0; j < h->Ny; ++j) { umom += S[S_uLe_i][j] * hy[j] * gx[0]; k += 0.5 * S[S_uLe_i][j] * S[S_uLe_i][j] * hy[j] * gx[0]; }
+    if (h->bc_u[1] == BC_DIR)
+      for (int j = 0; j < h->Ny; ++j) { umom += S[S_uRi_i][j] * hy[j] * gx[h->Nx]; k += 0.5 * S[S_uRi_i][j] * S[S_uRi_i][j] * hy[j] * gx[h->Nx]; }
+    if (h->bc_v[2] == BC_DIR)
+      for (int i = 0; i < h->Nx; ++i) { vmom += S[S_vLo_i][i] * hx[i] * gy[0]; k += 0.5 * S[S_vLo_i][i] * S[S_vLo_i][i] * hx[i] * gy[0]; }
+    if (h->bc_v[3] == BC_DIR)
+      for (int i = 0; i < h->Nx; ++i) { vmom += S[S_vUp_i][i] * hx[i] * gy[h->Ny]; k += 0.5 * S[S_vUp_i][i] * S[S_vUp_i][i] * hx[i] * gy[h->Ny]; }
   }
   out[2] = umom;
   out[3] = vmom;

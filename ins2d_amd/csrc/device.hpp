@@ -95,6 +95,22 @@ struct DLap {        // A = kron(L1.Y, L1.X) + kron(L2.Y, L2.X)
   int nx, ny;
 };
 
+// ---- fourth-order operators on grids with walls, CSR (row pointers rp,
+// column indices ci, values v)
+struct DCsr {
+  int m, n;
+  const int* rp;
+  const int* ci;
+  const double* v;
+};
+struct W4Dev {
+  int NV, Np, Nf;
+  DCsr Af, If, Cw, Diff, G, M;
+  const double *yA, *yI, *yD, *yM, *Om;
+  double* w;   // the Nf face products
+  double* K;   // Np x Np inverse of the pressure matrix (row-major)
+};
+
 // ---- stage combination: Vtemp = sum_j a[j] k[j] (+ aself * k_i)
 struct StageCombine {
   int n;

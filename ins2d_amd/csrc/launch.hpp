@@ -145,6 +145,18 @@ void launch_tl_coarse_correct(hipStream_t st, const TLPlan& P, const double* f, 
 // 2 * sweeps damped Jacobi sweeps on x (tmp: work array)
 void launch_tl_smooth(hipStream_t st, const TLPlan& P, const double* f, double* x, double* tmp, int sweeps,
                       double omega);
+// fourth-order operators on grids with walls (w4.hip)
+void launch_w4_rhs(hipStream_t st, const W4Dev& D, const double* V, const double* p, int mode, double* out,
+                   const StageCombine& sc, double* vt);
+void launch_w4_div(hipStream_t st, const W4Dev& D, const double* A, const double* B, double ca, double cb,
+                   double ycoef, double oscale, double* f);
+void launch_w4_proj(hipStream_t st, const W4Dev& D, const double* Vn, const double* Vt, const double* dp, double dt,
+                    double c, double* V);
+// dp = K f
+void launch_w4_solve(hipStream_t st, const W4Dev& D, const double* f, double* dp);
+// a (n x n, symmetric positive definite) <- -a^-1 + shift, by n Gauss-Jordan
+// sweeps (rowk, colk: n doubles of work space each)
+void w4_invert(hipStream_t st, double* a, int n, double* rowk, double* colk, double shift);
 void launch_reduce_stats(hipStream_t st, const double* x, int nx, int ny, const double* wx, const double* wy,
                          double* part, int nblk);
 

@@ -105,8 +105,8 @@ Sp spdiags_const(const std::vector<double>& vals, const std::vector<int>& offs, 
 
 // u_ext = (Bin - Btemp Bbc Bin) u_in + Btemp ybc, Btemp = Bb (Bbc Bb)^-1
 // (spatial/boundaryconditions/proposed/BC_general.m:1-7,124-125)
-static Closure make_closure(int Nt, int Nin, int Nb, int bin_off, const Sp& Bb, const Sp& Bbc,
-                            const std::vector<double>& y1, const std::vector<double>& y2) {
+Closure make_closure(int Nt, int Nin, int Nb, int bin_off, const Sp& Bb, const Sp& Bbc,
+                     const std::vector<double>& y1, const std::vector<double>& y2) {
   Sp Bin = spdiags_const({1.0}, {bin_off}, Nt, Nin);
   // S = Bbc Bb (Nb x Nb), dense inverse by Gauss-Jordan
   Sp S = mul(Bbc, Bb);

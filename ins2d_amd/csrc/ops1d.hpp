@@ -61,6 +61,10 @@ struct Closure {
   Sp B1D, Btemp;
   std::vector<double> ybc1, ybc2;
 };
+// the closure of the boundary rows Bbc u_ext = ybc (Bb: Nt x Nb places the Nb
+// boundary unknowns in u_ext, interior unknown r sits at row r - bin_off)
+Closure make_closure(int Nt, int Nin, int Nb, int bin_off, const Sp& Bb, const Sp& Bbc,
+                     const std::vector<double>& y1, const std::vector<double>& y2);
 Closure bc_general(int Nt, int Nin, int Nb, int bc1, int bc2, double h1, double h2);
 Closure bc_general_stag(int Nt, int Nin, int Nb, int bc1, int bc2, double h1, double h2, bool diff_variant);
 
