@@ -459,7 +459,18 @@ def main():
     }
     s = stages
     if it1 > it0:
-        out['config']['pcg_iterations_per_solve'] = (it1 - it0) / (a.steps * (s - 1 + 1))
+        its = (it1 - it0) / (a.steps * (s - 1 + 1))
+        out['config']['pcg_iterations_per_solve'] = its
+        us_solve = kernels['poisson']['us_per_launch']
+        if us_solve and opts.prm.get('CG_precond', 'jacobi') == 'jacobi' and cells > 32768:
+            # the multi-block Jacobi-PCG iteration (two launches): read z and p,
+            # write p and q; read p, q, x, r and 1/diag(A), write x, r, z
+            us_it = us_solve / its
+            gbs = 96.0 * cells / (us_it * 1e-6) / 1e9
+            out['pcg_iteration'] = {'us_per_iteration': us_it, 'bytes_per_cell': 96, 'achieved_GBs': gbs,
+                                    'frac': gbs / HBM_PEAK_GBS,
+                                    'byte_model': 'k_cg2_apply: read z, p, write p, q; k_cg2_update: read p, q, '
+                                                  'x, r, 1/diag(A), write x, r, z'}
     solver.close()
     progress('profile pass done')
     if not a.no_cpu_baseline:

@@ -93,6 +93,7 @@ struct ins2d_handle {
   double *V = nullptr, *Vb = nullptr, *p = nullptr, *Vt = nullptr, *f = nullptr, *dp = nullptr, *tV = nullptr;
   std::vector<double*> kv;
   double *cx = nullptr, *cr = nullptr, *cz = nullptr, *cp = nullptr, *cq = nullptr;
+  double* cp2 = nullptr;   // second search-direction buffer of the two-launch PCG iteration
   CGStateH cg{};
   int* d_it = nullptr;
   double* d_rel = nullptr;
@@ -721,7 +722,12 @@ void poisson_matrix(ins2d_handle* h, const double* f, double* dp) {
   if (!h->cg_exec || h->cg_x != dp) {
     if (h->cg_exec) { HIPCHK(hipGraphExecDestroy(h->cg_exec)); HIPCHK(hipGraphDestroy(h->cg_graph)); }
     HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < 16; ++k) launch_cg_iter(h->st, h->lap, h->cg, dp, h->cr, h->cz, h->cp, h->cq);
+    // 16 iterations of two launches each; the direction alternates between
+    // cp and cp2 and is back in cp after the chunk
+    for (int k = 0; k < 16; ++k)
+      launch_cg_iter2(h->st, h->lap, h->cg, dp, h->cr, h->cz, (k & 1) ? h->cp2 : h->cp, (k & 1) ? h->cp : h->cp2,
+                      h->cq);
+    launch_cg2_check(h->st, h->cg);
     HIPCHK(hipStreamEndCapture(h->st, &h->cg_graph));
     HIPCHK(hipGraphInstantiate(&h->cg_exec, h->cg_graph, nullptr, nullptr, 0));
     h->cg_x = dp;
@@ -1170,12 +1176,14 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
       h->cz = dalloc<double>(h.get(), h->Np);
       h->cp = dalloc<double>(h.get(), h->Np);
       h->cq = dalloc<double>(h.get(), h->Np);
+      h->cp2 = dalloc<double>(h.get(), h->Np);
       h->d_it = dalloc<int>(h.get(), 1);
       h->d_rel = dalloc<double>(h.get(), 1);
       h->cg.nblk = (int)std::min<long>(2048, std::max<long>(1, (h->Np + 255) / 256));
       h->cg.s = dalloc<double>(h.get(), 8);
       h->cg.flag = dalloc<int>(h.get(), 4);
       h->cg.part = dalloc<double>(h.get(), 2 * (size_t)h->cg.nblk);
+      h->cg.e = dalloc<double>(h.get(), 4 + (size_t)h->cg.nblk);
       h->cg.maxit = h->maxit;
       if ((d->cg_precond == 1 && (h->poisson == INS2D_POISSON_CG || h->poisson == INS2D_POISSON_CG_IC)) ||
           h->poisson == INS2D_POISSON_PCG_ILU) {

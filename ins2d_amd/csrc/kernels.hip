@@ -1185,6 +1185,7 @@ struct CGState {
   double* part;   // per-block partial sums (2 per block)
   int nblk;
   int maxit;
+  double* e;      // two-launch iteration: scalar slots and p.q partials (4 + nblk)
 };
 
 // 1-D grid over n points, 256 threads: q = A p on (i, j) = (idx % nx, idx / nx)
@@ -1266,6 +1267,204 @@ __global__ void __launch_bounds__(256) k_cg_dir(CGState S, size_t n, const doubl
     p[idx] = z[idx] + beta * p[idx];
 }
 
+// ---- the same iteration in two launches --------------------------------
+// Each launch first reduces the partial sums the previous launch left, in
+// every block (the same order in all of them, so every block forms the same
+// scalars; no inter-block synchronisation inside a launch):
+// (1) k_cg2_apply: from the partials of r.r and r.z, the stopping rule of the
+// previous iteration and beta = rz / rz_prev (0 before the first iteration:
+// p = z); then p = z + beta p, q = A p = A z + beta A p_old and the partials
+// of p.q.  p is double-buffered (p_in -> p_out): A p_old reads p at the
+// stencil neighbours, which other threads of the launch rewrite.
+// (2) k_cg2_update: alpha = rz / p.q from those partials, then x += alpha p,
+// r -= alpha q, z = dinv r and the partials of r.r and r.z.
+// Scalars pass between the launches through two slots written by block 0
+// of one launch and read by the next: e[0..1] (rz_prev, iterations done) for
+// k_cg2_apply, e[2..3] (rz, iterations done) for k_cg2_update; e + 4 holds
+// the p.q partials.  k_cg2_check applies the stopping rule at the end of a
+// replayed chunk.
+// the block sum handed to every thread of the block
+__device__ __forceinline__ double block_sum_all(double v, double* red, double* out) {
+  const double s = block_sum(v, red);
+  if (threadIdx.x == 0) *out = s;
+  __syncthreads();
+  return *out;
+}
+
+// the stopping rule after an update (cg.F:247): true when the solve ends;
+// block 0 records the outcome
+__device__ __forceinline__ bool cg2_stop(const CGState& S, double rr, int it) {
+  const double rel = rr / S.s[1];
+  const bool conv = rel <= S.s[5], fail = !conv && it >= S.maxit;
+  if ((conv || fail) && blockIdx.x == 0 && threadIdx.x == 0) {
+    S.s[4] = rel;
+    S.flag[1] = it;
+    S.flag[2] = fail ? 1 : 0;
+    S.flag[0] = 1;
+  }
+  return conv || fail;
+}
+
+// kron(Y, X) at (f, g) with KA entries per row of Y and KB of X known at
+// compile time: the index and weight loads of all entries issue together
+template <int KA, int KB>
+__device__ __forceinline__ double kron_fixed_at(const DKron& K, const double* __restrict__ in, int f, int g) {
+  const int ld = K.X.ncols;
+  int cy[KA], cx[KB];
+  double wy[KA], wx[KB];
+#pragma unroll
+  for (int a = 0; a < KA; ++a) { cy[a] = K.Y.col[(size_t)g * KA + a]; wy[a] = K.Y.val[(size_t)g * KA + a]; }
+#pragma unroll
+  for (int b = 0; b < KB; ++b) { cx[b] = K.X.col[(size_t)f * KB + b]; wx[b] = K.X.val[(size_t)f * KB + b]; }
+  double acc = 0.0;
+#pragma unroll
+  for (int a = 0; a < KA; ++a) {
+    const double* row = in + (size_t)cy[a] * ld;
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < KB; ++b) s += wx[b] * row[cx[b]];
+    acc += wy[a] * s;
+  }
+  return acc;
+}
+
+// A x at (i, j); TRI: the pressure Laplacian's factor widths
+// (kron(diag, tridiagonal) + kron(tridiagonal, diag)) fixed at compile time
+template <bool TRI>
+__device__ __forceinline__ double lap_cg2(const DLap& L, const double* __restrict__ x, int i, int j) {
+  if (TRI) return kron_fixed_at<1, 3>(L.L1, x, i, j) + kron_fixed_at<3, 1>(L.L2, x, i, j);
+  return lap_at(L, x, i, j);
+}
+
+// ONE: the grid covers every cell once (one cell per thread), so the cell's
+// loads and stencils are issued before the reduction of the partials and
+// overlap its latency; otherwise a grid-stride loop after the reduction.
+// A p = A z + beta A p_old (linearity: A z and A p_old need no beta)
+template <bool ONE, bool TRI>
+__global__ void __launch_bounds__(256) k_cg2_apply(DLap L, CGState S, const double* __restrict__ z,
+                                                  const double* __restrict__ p_in, double* __restrict__ p_out,
+                                                  double* __restrict__ q) {
+  __shared__ double red[4];
+  __shared__ double bc;
+  if (S.flag[0]) return;
+  const size_t n = (size_t)L.nx * L.ny;
+  const size_t idx0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  double zc = 0.0, pc = 0.0, az = 0.0, ap = 0.0;
+  if (ONE && idx0 < n) {
+    const int i = (int)(idx0 % L.nx), j = (int)(idx0 / L.nx);
+    zc = z[idx0];
+    pc = p_in[idx0];
+    az = lap_cg2<TRI>(L, z, i, j);
+    ap = lap_cg2<TRI>(L, p_in, i, j);
+  }
+  const int it = (int)S.e[1];
+  double ta = 0.0, tb = 0.0;
+  for (int k = threadIdx.x; k < S.nblk; k += blockDim.x) { ta += S.part[2 * k]; tb += S.part[2 * k + 1]; }
+  const double rr = block_sum_all(ta, red, &bc);
+  const double rz = block_sum_all(tb, red, &bc);
+  double beta = 0.0;
+  if (it > 0) {
+    if (cg2_stop(S, rr, it)) return;
+    beta = rz / S.e[0];
+  }
+  double acc = 0.0;
+  if (ONE) {
+    if (idx0 < n) {
+      const double pn = zc + beta * pc;
+      p_out[idx0] = pn;
+      const double a = az + beta * ap;
+      q[idx0] = a;
+      acc = pn * a;
+    }
+  } else {
+    for (size_t idx = idx0; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+      const int i = (int)(idx % L.nx), j = (int)(idx / L.nx);
+      const double pn = z[idx] + beta * p_in[idx];
+      p_out[idx] = pn;
+      const double a = lap_cg2<TRI>(L, z, i, j) + beta * lap_cg2<TRI>(L, p_in, i, j);
+      q[idx] = a;
+      acc += pn * a;
+    }
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    S.e[4 + blockIdx.x] = s;
+    if (blockIdx.x == 0) {
+      S.e[2] = rz;
+      S.e[3] = (double)it;
+    }
+  }
+}
+
+template <bool ONE>
+__global__ void __launch_bounds__(256) k_cg2_update(DLap L, CGState S, const double* __restrict__ p,
+                                                   const double* __restrict__ q, double* __restrict__ x,
+                                                   double* __restrict__ r, double* __restrict__ z) {
+  __shared__ double red[4];
+  __shared__ double bc;
+  if (S.flag[0]) return;
+  const size_t n = (size_t)L.nx * L.ny;
+  const size_t idx0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  double pv = 0.0, qv = 0.0, xv = 0.0, rv = 0.0, dv = 0.0;
+  if (ONE && idx0 < n) {
+    pv = p[idx0];
+    qv = q[idx0];
+    xv = x[idx0];
+    rv = r[idx0];
+    dv = L.dinv[idx0];
+  }
+  double t = 0.0;
+  for (int k = threadIdx.x; k < S.nblk; k += blockDim.x) t += S.e[4 + k];
+  const double pq = block_sum_all(t, red, &bc);
+  const double rz_cur = S.e[2];
+  const double alpha = rz_cur / pq;
+  double rr = 0.0, rz = 0.0;
+  if (ONE) {
+    if (idx0 < n) {
+      x[idx0] = xv + alpha * pv;
+      const double rn = rv - alpha * qv;
+      r[idx0] = rn;
+      const double zn = dv * rn;
+      z[idx0] = zn;
+      rr = rn * rn;
+      rz = rn * zn;
+    }
+  } else {
+    for (size_t idx = idx0; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+      x[idx] += alpha * p[idx];
+      const double rn = r[idx] - alpha * q[idx];
+      r[idx] = rn;
+      const double zn = L.dinv[idx] * rn;
+      z[idx] = zn;
+      rr += rn * rn;
+      rz += rn * zn;
+    }
+  }
+  const double a = block_sum(rr, red);
+  const double b = block_sum(rz, red);
+  if (threadIdx.x == 0) {
+    S.part[2 * blockIdx.x] = a;
+    S.part[2 * blockIdx.x + 1] = b;
+    if (blockIdx.x == 0) {
+      S.e[0] = rz_cur;
+      S.e[1] = S.e[3] + 1.0;
+    }
+  }
+}
+
+// the stopping rule after the last update of a replayed chunk (one block)
+__global__ void __launch_bounds__(256) k_cg2_check(CGState S) {
+  __shared__ double red[4];
+  __shared__ double bc;
+  if (S.flag[0]) return;
+  const int it = (int)S.e[1];
+  if (it == 0) return;
+  double ta = 0.0;
+  for (int k = threadIdx.x; k < S.nblk; k += blockDim.x) ta += S.part[2 * k];
+  const double rr = block_sum_all(ta, red, &bc);
+  cg2_stop(S, rr, it);
+}
+
 // start from x0: r = f - A x0, z = dinv r, p = z; partials of f.f and r.z
 __global__ void __launch_bounds__(256) k_cg_init(DLap L, CGState S, const double* __restrict__ f,
                                                 const double* __restrict__ x, double* __restrict__ r,
@@ -1301,6 +1500,10 @@ __global__ void k_cg_init_fin(CGState S, double tol2) {
     S.s[5] = tol2;
     S.flag[1] = 0;
     S.flag[2] = 0;
+    if (S.e) {       // the two-launch iteration starts at iteration 0
+      S.e[0] = rz;
+      S.e[1] = 0.0;
+    }
     // cg.F:222: an already tiny right-hand side returns x0 (absolute test)
     S.flag[0] = (ff <= tol2) ? 1 : 0;
   }
@@ -1774,14 +1977,14 @@ void launch_cg_single(hipStream_t st, const DLap& L, const double* f, double* x,
 
 void launch_cg_init(hipStream_t st, const DLap& L, const CGStateH& S, const double* f, const double* x, double* r,
                     double* z, double* p, int x0_zero, double tol2) {
-  CGState D{S.s, S.flag, S.part, S.nblk, S.maxit};
+  CGState D{S.s, S.flag, S.part, S.nblk, S.maxit, S.e};
   hipLaunchKernelGGL(k_cg_init, dim3(S.nblk), dim3(256), 0, st, L, D, f, x, r, z, p, x0_zero);
   hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(256), 0, st, D, tol2);
 }
 
 void launch_cg_iter(hipStream_t st, const DLap& L, const CGStateH& S, double* x, double* r, double* z, double* p,
                     double* q) {
-  CGState D{S.s, S.flag, S.part, S.nblk, S.maxit};
+  CGState D{S.s, S.flag, S.part, S.nblk, S.maxit, S.e};
   const size_t n = (size_t)L.nx * L.ny;
   hipLaunchKernelGGL(k_cg_apply, dim3(S.nblk), dim3(256), 0, st, L, D, p, q);
   hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, D);
@@ -1790,9 +1993,32 @@ void launch_cg_iter(hipStream_t st, const DLap& L, const CGStateH& S, double* x,
   hipLaunchKernelGGL(k_cg_dir, dim3(S.nblk), dim3(256), 0, st, D, n, z, p);
 }
 
+void launch_cg_iter2(hipStream_t st, const DLap& L, const CGStateH& S, double* x, double* r, double* z,
+                     const double* p_in, double* p_out, double* q) {
+  CGState D{S.s, S.flag, S.part, S.nblk, S.maxit, S.e};
+  const size_t n = (size_t)L.nx * L.ny;
+  const bool one = (size_t)S.nblk * 256 >= n;
+  // the pressure Laplacian: kron(diag, tridiagonal) + kron(tridiagonal, diag)
+  const bool tri = L.L1.Y.K == 1 && L.L1.X.K == 3 && L.L2.Y.K == 3 && L.L2.X.K == 1;
+  const dim3 g(S.nblk), t(256);
+  const double* zc = z;
+  if (one && tri) hipLaunchKernelGGL((k_cg2_apply<true, true>), g, t, 0, st, L, D, zc, p_in, p_out, q);
+  else if (one) hipLaunchKernelGGL((k_cg2_apply<true, false>), g, t, 0, st, L, D, zc, p_in, p_out, q);
+  else if (tri) hipLaunchKernelGGL((k_cg2_apply<false, true>), g, t, 0, st, L, D, zc, p_in, p_out, q);
+  else hipLaunchKernelGGL((k_cg2_apply<false, false>), g, t, 0, st, L, D, zc, p_in, p_out, q);
+  const double *pc = p_out, *qc = q;
+  if (one) hipLaunchKernelGGL(k_cg2_update<true>, g, t, 0, st, L, D, pc, qc, x, r, z);
+  else hipLaunchKernelGGL(k_cg2_update<false>, g, t, 0, st, L, D, pc, qc, x, r, z);
+}
+
+void launch_cg2_check(hipStream_t st, const CGStateH& S) {
+  CGState D{S.s, S.flag, S.part, S.nblk, S.maxit, S.e};
+  hipLaunchKernelGGL(k_cg2_check, dim3(1), dim3(256), 0, st, D);
+}
+
 void launch_cgx(hipStream_t st, const DLap& L, const CGStateH& S, int phase, const double* a, double* b, double* c,
                 double* d, double tol2) {
-  CGState D{S.s, S.flag, S.part, S.nblk, S.maxit};
+  CGState D{S.s, S.flag, S.part, S.nblk, S.maxit, S.e};
   const size_t n = (size_t)L.nx * L.ny;
   const dim3 g(S.nblk), t(256);
   switch (phase) {

@@ -47,6 +47,7 @@ struct CGStateH {
   double* part;
   int nblk;
   int maxit;
+  double* e = nullptr;   // 4 + nblk doubles for launch_cg_iter2 (null: not used)
 };
 
 void launch_rhs_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,
@@ -94,6 +95,15 @@ void launch_cgx(hipStream_t st, const DLap& L, const CGStateH& S, int phase, con
                 double* d, double tol2);
 void launch_cg_iter(hipStream_t st, const DLap& L, const CGStateH& S, double* x, double* r, double* z, double* p,
                     double* q);
+// the same iteration in two launches (k_cg2_apply, k_cg2_update; S.e set
+// before launch_cg_init); p is double-buffered: the iteration reads p_in and
+// leaves the new direction in p_out.  The stopping rule of an iteration is
+// applied by the next iteration's first launch, or by launch_cg2_check after
+// the last iteration of a chunk.  The scalars are launch_cg_iter's, with
+// A p formed as A z + beta A p_old (equal up to rounding).
+void launch_cg_iter2(hipStream_t st, const DLap& L, const CGStateH& S, double* x, double* r, double* z,
+                     const double* p_in, double* p_out, double* q);
+void launch_cg2_check(hipStream_t st, const CGStateH& S);
 void launch_axpy(hipStream_t st, size_t n, double a, const double* x, double* y);
 
 // fast-diagonalization direct Poisson solve (fdm.hip; factors from

@@ -564,3 +564,53 @@ def test_ldc_run_ilu_pcg_matches_oracle():
     _say('ldc 20x20 run ilu pcg')
     case = load('LDC_unsteady', RK='RK33P2', poisson=2, CG_acc=1e-13, CG_maxit=2000)
     _compare_runs(case, 20, True)
+
+
+def _jacobi_pcg(A, f, tol, maxit):
+    """Jacobi-preconditioned CG with the stopping rule of cg.F:247
+    (||r||^2 / ||f||^2 <= tol^2, x0 = 0), the iteration the multi-block GPU
+    solver runs: returns x and the iteration count."""
+    A = A.tocsr()
+    dinv = 1.0 / A.diagonal()
+    x = np.zeros_like(f)
+    r = f.copy()
+    z = dinv * r
+    p = z.copy()
+    nf = f @ f
+    rz = r @ z
+    for it in range(1, maxit + 1):
+        q = A @ p
+        alpha = rz / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        if (r @ r) / nf <= tol * tol:
+            return x, it
+        z = dinv * r
+        rz, rz_old = r @ z, rz
+        p = z + (rz / rz_old) * p
+    raise RuntimeError('no convergence')
+
+
+@pytest.mark.parametrize('shape', [(256, 160), (512, 256)])
+def test_two_launch_pcg_matches_jacobi_pcg(shape):
+    """The multi-block Jacobi-PCG (more than 32768 cells: two launches per
+    iteration, alpha and beta formed by the last block of each launch)
+    against the same iteration in numpy: the iteration count within one, the
+    solutions within the tolerance."""
+    _say('two-launch pcg', shape)
+    tol = 1e-10
+    case = _wall_case('ldc', shape[0], shape[1], poisson=3, CG_acc=tol, CG_maxit=20000)
+    opts = O.build(case)
+    solver, _, _, _ = host.setup(case)
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal(opts.grid.Np)
+    f -= f.mean()
+    xr, itr = _jacobi_pcg(opts.discretization.A, f, tol, 20000)
+    for rep in range(2):   # the second solve replays the captured chunks
+        dp, it, rel = solver.poisson(f)
+        _say('two-launch pcg', shape, 'gpu iterations', it, 'numpy', itr, 'relres', rel)
+        assert abs(it - itr) <= 1
+        assert rel <= tol
+        d1, d2 = dp - dp.mean(), xr - xr.mean()
+        assert np.abs(d1 - d2).max() <= 1e-8 * np.abs(d2).max()
+    solver.close()
