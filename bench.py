@@ -44,7 +44,8 @@ def parse():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-steps', type=int, default=10)
     ap.add_argument('--members', type=int, default=512)
-    ap.add_argument('--train-steps', type=int, default=1500)
+    ap.add_argument('--train-steps', type=int, default=4000)
+    ap.add_argument('--pairs', type=int, default=20000)
     return ap.parse_args()
 
 
@@ -174,26 +175,45 @@ def bench_ensemble(a):
 def bench_poisson_learned(a):
     """Per-solve pressure time, learned vs plain, on the backward-facing step
     1024x256 (tanh-stretched, Re 800 grid), PCG to a true relative residual
-    of 1e-10.  Training pairs come from plain-solver runs at Re 200..1200,
-    the held-out right-hand sides from runs at Re 300, 700, 1100 at later
-    times.  Plain = Jacobi-PCG from x0 = 0; learned = network start vector
-    (smoothed) + PCG with the two-level coarse correction; the two parts are
-    also timed alone."""
+    of 1e-10.  Training pairs (a.pairs, default 20,000) come from plain-solver
+    runs at Re 200..1200, generated into device memory with the coarse-grid
+    solution of each right-hand side; the held-out right-hand sides from runs
+    at Re 300, 700, 1100 at later times.  Variants: plain Jacobi-PCG from
+    x0 = 0; the two-level PCG from x0 = 0; the two-level PCG started from the
+    coarse-grid solution (the same method without the network); learned =
+    the network's correction of the coarse-grid solution as the start of the
+    two-level PCG."""
     from ins2d_amd import host, learn
     Nx, Ny = (a.N or 1024), (a.N // 4 if a.N else 256)
+    Re_train = [200.0, 400.0, 600.0, 800.0, 1000.0, 1200.0]
+    per_run = max(1, a.pairs // len(Re_train))
+    n = per_run * len(Re_train)
+    batch = 4
     t0 = time.perf_counter()
-    pairs = learn.generate_pairs(Nx, Ny, [200.0, 400.0, 600.0, 800.0, 1000.0, 1200.0], steps=600, every=6, skip=100,
-                                 progress=progress)
+    net = learn.make_net(Nx, Ny, out_scale=1.0, seed=0)
+    net.data_reserve(n + batch // 2)
+    learn.generate_pairs_dev(net, Nx, Ny, Re_train, per_run, every=3, skip=100, progress=progress)
+    out_scale = learn.out_scale_from_net(net, n)
+    net.init(0, out_scale)
     held = learn.generate_pairs(Nx, Ny, [300.0, 700.0, 1100.0], steps=340, every=10, skip=400, progress=progress)
     t_data = time.perf_counter() - t0
-    net = learn.make_net(Nx, Ny, out_scale=learn.out_scale_of(pairs), seed=0)
     t0 = time.perf_counter()
-    losses = learn.train(net, pairs, steps=a.train_steps, batch=4, lr=2e-3, progress=progress, every=100)
+    losses = learn.train_stored(net, n, steps=a.train_steps, batch=batch, lr=2e-3, progress=progress, every=250)
     t_train = time.perf_counter() - t0
     solver, _, _, _ = host.setup('BFS_unsteady', Nx=Nx, Ny=Ny, poisson=3, CG_acc=1e-10, CG_maxit=100000)
     rhs = [h[0] for h in held]
-    variants = [('plain', False, False), ('two_level', False, True), ('warm_start', True, False),
-                ('learned', True, True)]
+    # accuracy of the start vectors on the held-out set (relative L2 error)
+    solver.set_poisson_accel(None, two_level=True)
+    err_c, err_n = [], []
+    for f, p, *_ in held[::10]:
+        xc = solver.poisson_coarse(f)
+        pn = net.forward(f.reshape(Ny, Nx), xc.reshape(Ny, Nx)).ravel()
+        err_c.append(np.linalg.norm(xc - p) / np.linalg.norm(p))
+        err_n.append(np.linalg.norm(pn - p) / np.linalg.norm(p))
+    progress(f'held-out start error: coarse {np.median(err_c):.3e}, learned {np.median(err_n):.3e}')
+    variants = [('plain', {}), ('two_level', dict(two_level=True)),
+                ('coarse_start', dict(two_level=True, coarse_start=True)),
+                ('learned', dict(warm_start=True, two_level=True, coarse_input=True))]
     res = learn.evaluate(solver, net, rhs, variants, progress=progress)
     med = {k: float(np.median(v['ms'])) for k, v in res.items()}
     out = {'metric': 'Poisson time learned vs plain PCG @1e-10 residual', 'value': med['plain'] / med['learned'],
@@ -201,13 +221,16 @@ def bench_poisson_learned(a):
            'warmup': 1, 'ms_per_step': med['learned'], 'higher_is_better': True, 'scaling': 'weak',
            'vs_baseline': None, 'dtype': 'f64 (solver), bf16/fp32 (network)', 'data': 'synthetic',
            'config': {'workload': f'backward-facing step {Nx}x{Ny} tanh-stretched, pressure RHS from plain runs',
-                      'training_pairs': len(pairs), 'held_out_rhs': len(rhs), 'train_steps': a.train_steps,
-                      'train_batch': 4, 'train_s': t_train, 'data_s': t_data,
-                      'loss_first': losses[0], 'loss_last': float(np.mean(losses[-20:]))},
+                      'training_pairs': n, 'held_out_rhs': len(rhs), 'train_steps': a.train_steps,
+                      'train_batch': batch, 'train_s': t_train, 'data_s': t_data, 'out_scale': out_scale,
+                      'loss_first': float(np.mean(losses[:10])), 'loss_last': float(np.mean(losses[-50:])),
+                      'held_out_rel_error_coarse': float(np.median(err_c)),
+                      'held_out_rel_error_learned': float(np.median(err_n))},
            'variants': {k: {'median_ms': med[k], 'median_iterations': float(np.median(v['it'])),
                             'max_true_relres': float(np.max(v['rel'])), 'fallbacks': v['fallbacks']}
                         for k, v in res.items()},
-           'network_gain_over_two_level': med['two_level'] / med['learned']}
+           'network_gain_over_two_level': med['two_level'] / med['learned'],
+           'network_gain_over_coarse_start': med['coarse_start'] / med['learned']}
     agg = solver.accel_stats()[1]
     out['config']['aggregate'] = agg
     solver.close()

@@ -232,8 +232,10 @@ int ins2d_batch_stats(const ins2d_batch* b, long long* total_iters, int* max_ite
  * CNN (4 levels, 16/32/64/64 channels, 3x3 convolutions, BF16 weights and
  * activations, FP32 accumulation on the matrix cores) maps a Poisson
  * right-hand side f on an H x W pressure grid (x fastest, H, W multiples of 8)
- * to p0 = out_scale * rms(f) * net(f / rms(f)), the start vector of PCG.
- * Parameters are one flat FP32 vector (ins2d_nn_num_params entries). */
+ * and optionally a coarse-grid solution xc (ins2d_poisson_coarse) to
+ * p0 = xc + out_scale * rms(f) * net(f / rms(f), xc / (out_scale rms(f))),
+ * the start vector of PCG (xc NULL: zero).  Parameters are one flat FP32
+ * vector (ins2d_nn_num_params entries). */
 typedef struct ins2d_nn ins2d_nn;
 int ins2d_nn_create(int H, int W, ins2d_nn** out);
 void ins2d_nn_destroy(ins2d_nn* n);
@@ -243,22 +245,33 @@ int ins2d_nn_set_params(ins2d_nn* n, const float* params, double out_scale);
 int ins2d_nn_get_params(ins2d_nn* n, float* params);
 /* He-normal weights from a seed, zero biases, fresh optimizer state */
 int ins2d_nn_init(ins2d_nn* n, unsigned long long seed, double out_scale);
-/* host arrays of H*W doubles */
-int ins2d_nn_forward(ins2d_nn* n, const double* f, double* p);
-/* device arrays, queued on the given HIP stream (NULL: the null stream) */
-int ins2d_nn_forward_dev(ins2d_nn* n, const double* f, double* p, void* stream);
-/* mean-squared-error loss of net(f_b / rms(f_b)) against p_b / (out_scale
- * rms(f_b)) over a batch of device arrays; gradients into the gradient
- * vector; mode 1 also takes one Adam step (lr, beta 0.9 / 0.999) */
-int ins2d_nn_train_step(ins2d_nn* n, const double* const* f, const double* const* p, int batch, float lr, int mode,
-                        double* loss_out, void* stream);
+/* host arrays of H*W doubles (xc optional) */
+int ins2d_nn_forward(ins2d_nn* n, const double* f, const double* xc, double* p);
+/* device arrays, queued on the given HIP stream (NULL: the null stream); xc
+ * optional and may be p itself */
+int ins2d_nn_forward_dev(ins2d_nn* n, const double* f, const double* xc, double* p, void* stream);
+/* mean-squared-error loss of net(f_b / rms(f_b), xc_b / (out_scale rms(f_b)))
+ * against (p_b - xc_b) / (out_scale rms(f_b)) over a batch of device arrays
+ * (xc NULL, or an entry NULL: zero); gradients into the gradient vector;
+ * mode 1 also takes one Adam step (lr, beta 0.9 / 0.999) */
+int ins2d_nn_train_step(ins2d_nn* n, const double* const* f, const double* const* p, const double* const* xc,
+                        int batch, float lr, int mode, double* loss_out, void* stream);
 int ins2d_nn_get_grad(ins2d_nn* n, float* grad);
 /* training pairs held by the network object in device memory: reserve
- * room for count pairs, upload pair k from host arrays, and train on the
- * pairs idx[0..batch) (as ins2d_nn_train_step) */
+ * room for count pairs, upload pair k from host arrays (xc NULL: zero), and
+ * train on the pairs idx[0..batch) (as ins2d_nn_train_step) */
 int ins2d_nn_data_reserve(ins2d_nn* n, int count);
-int ins2d_nn_data_set(ins2d_nn* n, int k, const double* f, const double* p);
+int ins2d_nn_data_set(ins2d_nn* n, int k, const double* f, const double* p, const double* xc);
 int ins2d_nn_train(ins2d_nn* n, const int* idx, int batch, float lr, int mode, double* loss_out);
+/* the device array of pair k: which 0 f, 1 p, 2 xc (NULL when out of
+ * range); pairs can be written there directly (ins2d_pressure_rhs_dev,
+ * ins2d_poisson_dev, ins2d_poisson_coarse_dev) */
+double* ins2d_nn_data_ptr(ins2d_nn* n, int k, int which);
+/* copy pair k to host arrays (any of them NULL: skipped) */
+int ins2d_nn_data_get(ins2d_nn* n, int k, double* f, double* p, double* xc);
+/* pair dst = a pair_i / rms(f_i) + b pair_j / rms(f_j) on all three arrays
+ * (A is linear, so this is again an exact pair) */
+int ins2d_nn_data_combine(ins2d_nn* n, int dst, int i, int j, double a, double b);
 /* one convolution layer's kernels on host FP32 arrays (inputs, weights and
  * gradients rounded to BF16 where the kernels read BF16): out = act(conv(x)
  * + bias); din, dw, db = the input, weight and bias gradients for the output
@@ -275,13 +288,31 @@ int ins2d_nn_shape(const ins2d_nn* n, int* H, int* W);
 
 /* ---- accelerated PCG for the pressure modes 3/4 (no reference
  * counterpart).  flags bit 0: start from the network's p0 (nn on the handle's
- * pressure grid; when ||f - A p0|| > ||f|| the solve falls back to x0 = 0);
- * bit 1: two-level preconditioner, Jacobi plus an aggregation coarse-grid
- * correction over agg x agg cells solved exactly (agg 0: automatic).  The
- * stopping rule stays ||r|| <= cg_tol ||f|| (cg.F:247); ins2d_solver_stats then
- * reports the true relative residual ||f - A x|| / ||f||.  flags 0 restores
- * the plain PCG.  The network is borrowed and must outlive its use. */
+ * pressure grid); bit 1: two-level preconditioner, Jacobi plus an aggregation
+ * coarse-grid correction over agg x agg cells solved exactly (agg 0:
+ * automatic); bit 2 (with bits 0 and 1): the network takes the coarse-grid
+ * solution P Ac^-1 P^T f as its second input and predicts its correction;
+ * bit 3 (with bit 1, without bit 0): start from the coarse-grid solution
+ * alone; bits 8-15: damped Jacobi sweep pairs after a bit-2 start (0: 4).  A
+ * start x0 is kept only when it is closer to the solution than x0 = 0 in the
+ * (-A)-norm that CG minimises (x0 (f + f - A x0) < 0), else the solve falls
+ * back to x0 = 0.  The stopping rule stays ||r|| <= cg_tol ||f|| (cg.F:247);
+ * ins2d_solver_stats then reports the true relative residual
+ * ||f - A x|| / ||f||.  flags 0 restores the plain PCG.  The network is
+ * borrowed and must outlive its use. */
 int ins2d_set_poisson_accel(ins2d_handle* h, ins2d_nn* nn, int flags, int agg);
+/* device-array forms (HIP device pointers; each call returns after its
+ * work): the pressure right-hand side of the resident velocity (as
+ * ins2d_pressure_rhs), a pressure solve (as ins2d_poisson) and the coarse-grid
+ * solution (as ins2d_poisson_coarse).  For generating training pairs in
+ * device memory. */
+int ins2d_pressure_rhs_dev(ins2d_handle* h, double t, double* f);
+int ins2d_poisson_dev(ins2d_handle* h, const double* f, double* dp);
+int ins2d_poisson_coarse_dev(ins2d_handle* h, const double* f, double* xc);
+/* xc = P Ac^-1 P^T f, the coarse-grid solution of the two-level
+ * preconditioner (needs bit 1 of the ins2d_set_poisson_accel flags); host
+ * arrays */
+int ins2d_poisson_coarse(ins2d_handle* h, const double* f, double* xc);
 /* number of warm-start fallbacks so far, aggregate size in use */
 int ins2d_poisson_accel_stats(const ins2d_handle* h, long long* fallbacks, int* agg);
 

@@ -62,13 +62,13 @@ SIGNATURES = {
     "ins2d_nn_set_params": (c_int, [c_void_p, P_float, c_double]),
     "ins2d_nn_get_params": (c_int, [c_void_p, P_float]),
     "ins2d_nn_init": (c_int, [c_void_p, ctypes.c_ulonglong, c_double]),
-    "ins2d_nn_forward": (c_int, [c_void_p, P_double, P_double]),
-    "ins2d_nn_forward_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
-    "ins2d_nn_train_step": (c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int,
-                                    ctypes.c_float, c_int, P_double, c_void_p]),
+    "ins2d_nn_forward": (c_int, [c_void_p, P_double, P_double, P_double]),
+    "ins2d_nn_forward_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ins2d_nn_train_step": (c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                    ctypes.POINTER(c_void_p), c_int, ctypes.c_float, c_int, P_double, c_void_p]),
     "ins2d_nn_get_grad": (c_int, [c_void_p, P_float]),
     "ins2d_nn_data_reserve": (c_int, [c_void_p, c_int]),
-    "ins2d_nn_data_set": (c_int, [c_void_p, c_int, P_double, P_double]),
+    "ins2d_nn_data_set": (c_int, [c_void_p, c_int, P_double, P_double, P_double]),
     "ins2d_nn_train": (c_int, [c_void_p, P_int, c_int, ctypes.c_float, c_int, P_double]),
     "ins2d_nn_layer_check": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, P_float, P_float, P_float,
                                      P_float, P_float, P_float, P_float, P_float, P_float]),
@@ -76,6 +76,13 @@ SIGNATURES = {
     "ins2d_nn_shape": (c_int, [c_void_p, P_int, P_int]),
     "ins2d_set_poisson_accel": (c_int, [c_void_p, c_void_p, c_int, c_int]),
     "ins2d_poisson_accel_stats": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong), P_int]),
+    "ins2d_poisson_coarse": (c_int, [c_void_p, P_double, P_double]),
+    "ins2d_pressure_rhs_dev": (c_int, [c_void_p, c_double, c_void_p]),
+    "ins2d_poisson_dev": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ins2d_poisson_coarse_dev": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ins2d_nn_data_ptr": (c_void_p, [c_void_p, c_int, c_int]),
+    "ins2d_nn_data_get": (c_int, [c_void_p, c_int, P_double, P_double, P_double]),
+    "ins2d_nn_data_combine": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_double]),
     "ins2d_pressure_rhs": (c_int, [c_void_p, c_double, P_double]),
     "ins2d_residual": (c_int, [c_void_p, P_double, P_double, P_double]),
     "ins2d_fdm_mfma_probe": (c_int, [P_double, P_double, P_double]),

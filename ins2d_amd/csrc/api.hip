@@ -80,6 +80,7 @@ struct ins2d_handle {
   DIC dic{};
   // accelerated PCG: 1 learned warm start, 2 two-level preconditioner
   int accel = 0;
+  int ws_sweeps = 4;   // damped Jacobi sweep pairs after a start with the coarse-solution input
   ins2d_nn* nn = nullptr;
   TLPlan tl{};
   long long fallbacks = 0;
@@ -572,18 +573,28 @@ void proj(ins2d_handle* h, const double* Vn, const double* Vt, const double* dp,
 // the stopping rule and tolerance are the plain solver's (cg.F:247)
 void accel_poisson(ins2d_handle* h, const double* f, double* dp) {
   const double tol2 = h->tol * h->tol;
+  const bool net = (h->accel & 1) != 0, xc_in = (h->accel & 4) != 0, coarse_start = (h->accel & 8) != 0;
   int x0_zero = 1;
-  if (h->accel & 1) {
-    if (ins2d_nn_forward_dev(h->nn, f, dp, h->st) != INS2D_OK)
-      throw HipError(std::string("warm start: ") + ins2d_nn_last_error(h->nn));
-    // the network's BF16 rounding noise is rough and A amplifies rough
-    // error most: damped Jacobi sweeps remove it before CG starts; with the
-    // two-level preconditioner the smooth (global) error of the start vector
-    // is also corrected on the coarse grid, between the sweeps
-    launch_tl_smooth(h->st, h->tl, f, dp, h->cz, 12, 0.6);
-    if (h->tl.coarse) {
-      launch_tl_coarse_correct(h->st, h->tl, f, dp, h->cr);
-      launch_tl_smooth(h->st, h->tl, f, dp, h->cz, 12, 0.6);
+  if (net || coarse_start) {
+    // the coarse-grid solution P Ac^-1 P^T f: the network's second input
+    // (the network then predicts its correction), or the start itself
+    if (xc_in || coarse_start) launch_tl_coarse_solution(h->st, h->tl, f, dp, h->cr);
+    if (net) {
+      if (ins2d_nn_forward_dev(h->nn, f, xc_in ? dp : nullptr, dp, h->st) != INS2D_OK)
+        throw HipError(std::string("warm start: ") + ins2d_nn_last_error(h->nn));
+      // the network's BF16 rounding noise is rough and A amplifies rough
+      // error most: damped Jacobi sweeps remove it before CG starts; without
+      // the coarse input the start's smooth (global) error is also corrected
+      // on the coarse grid, between the sweeps
+      if (xc_in) {
+        if (h->ws_sweeps > 0) launch_tl_smooth(h->st, h->tl, f, dp, h->cz, h->ws_sweeps, 0.6);
+      } else {
+        launch_tl_smooth(h->st, h->tl, f, dp, h->cz, 12, 0.6);
+        if (h->tl.coarse) {
+          launch_tl_coarse_correct(h->st, h->tl, f, dp, h->cr);
+          launch_tl_smooth(h->st, h->tl, f, dp, h->cz, 12, 0.6);
+        }
+      }
     }
     x0_zero = 0;
   } else {
@@ -591,11 +602,17 @@ void accel_poisson(ins2d_handle* h, const double* f, double* dp) {
   }
   launch_tl_init(h->st, h->tl, f, dp, h->cr, h->cp, x0_zero, tol2);
   if (!x0_zero) {
-    double sc[8];
-    HIPCHK(hipMemcpyAsync(sc, h->tl.s, sizeof(sc), hipMemcpyDeviceToHost, h->st));
+    // keep the start only when it is closer to the solution than x0 = 0 in
+    // the norm CG minimises: phi(x0) = x0 (f + r0) / 2 < phi(0) = 0, with
+    // phi(x) - phi(A^-1 f) = |x - A^-1 f|^2_(-A) / 2
+    launch_tl_energy(h->st, h->tl, f, dp, h->cr, h->tl.en);
+    const int nblk = h->tl.C.ncx * h->tl.C.ncy;
+    std::vector<double> en(nblk);
+    HIPCHK(hipMemcpyAsync(en.data(), h->tl.en, nblk * sizeof(double), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
-    if (sc[6] > sc[1]) {
-      // the warm start raised the residual above ||f||: start from x0 = 0
+    double e = 0.0;
+    for (double v : en) e += v;
+    if (!(e < 0.0)) {
       ++h->fallbacks;
       HIPCHK(hipMemsetAsync(dp, 0, sizeof(double) * h->Np, h->st));
       launch_tl_init(h->st, h->tl, f, dp, h->cr, h->cp, 1, tol2);
@@ -1545,8 +1562,12 @@ int ins2d_diagnostics(ins2d_handle* h, double t, double* out) {
 }
 
 int ins2d_set_poisson_accel(ins2d_handle* h, ins2d_nn* nn, int flags, int agg) {
-  if (!h || flags < 0 || flags > 3) return INS2D_E_ARG;
+  if (!h || flags < 0 || (flags & ~0xff0f)) return INS2D_E_ARG;
   API_TRY
+  const int sweeps = (flags >> 8) & 0xff;
+  flags &= 0xf;
+  if (((flags & 4) && (flags & 3) != 3) || ((flags & 8) && ((flags & 1) || !(flags & 2))))
+    throw ArgError("flag 4 needs flags 1 and 2; flag 8 needs flag 2 and excludes flag 1");
   if (flags == 0) {
     h->accel = 0;
     h->nn = nullptr;
@@ -1594,6 +1615,7 @@ int ins2d_set_poisson_accel(ins2d_handle* h, ins2d_nn* nn, int flags, int agg) {
   T.s = dalloc<double>(h, 8);
   T.flag = dalloc<int>(h, 4);
   T.part = dalloc<double>(h, 3 * nc);
+  T.en = dalloc<double>(h, nc);
   HIPCHK(hipMemset(T.s, 0, 8 * sizeof(double)));
   HIPCHK(hipMemset(T.flag, 0, 4 * sizeof(int)));
   h->tl = T;
@@ -1605,6 +1627,50 @@ int ins2d_set_poisson_accel(ins2d_handle* h, ins2d_nn* nn, int flags, int agg) {
   }
   h->accel = flags;
   h->nn = (flags & 1) ? nn : nullptr;
+  h->ws_sweeps = sweeps ? sweeps : 4;
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_pressure_rhs_dev(ins2d_handle* h, double t, double* f) {
+  if (!h || !f) return INS2D_E_ARG;
+  (void)t;
+  API_TRY
+  if (h->order4) throw Unsupported("pressure right-hand side is built for second order");
+  rhs(h, h->V, h->p, RHS_SCALE_OM, h->tV, kNoCombine, nullptr);
+  div(h, h->tV, nullptr, 1.0, 0.0, 0.0, 1.0, f);
+  HIPCHK(hipStreamSynchronize(h->st));
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_poisson_dev(ins2d_handle* h, const double* f, double* dp) {
+  if (!h || !f || !dp) return INS2D_E_ARG;
+  API_TRY
+  poisson(h, f, dp);
+  HIPCHK(hipStreamSynchronize(h->st));
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_poisson_coarse_dev(ins2d_handle* h, const double* f, double* xc) {
+  if (!h || !f || !xc) return INS2D_E_ARG;
+  API_TRY
+  if (!(h->accel & 2)) throw Unsupported("the coarse-grid solution needs the two-level preconditioner (flag 2)");
+  launch_tl_coarse_solution(h->st, h->tl, f, xc, h->cr);
+  HIPCHK(hipStreamSynchronize(h->st));
+  return INS2D_OK;
+  API_CATCH(h)
+}
+
+int ins2d_poisson_coarse(ins2d_handle* h, const double* f, double* xc) {
+  if (!h || !f || !xc) return INS2D_E_ARG;
+  API_TRY
+  if (!(h->accel & 2)) throw Unsupported("the coarse-grid solution needs the two-level preconditioner (flag 2)");
+  HIPCHK(hipMemcpyAsync(h->f, f, sizeof(double) * h->Np, hipMemcpyHostToDevice, h->st));
+  launch_tl_coarse_solution(h->st, h->tl, h->f, h->dp, h->cr);
+  HIPCHK(hipMemcpyAsync(xc, h->dp, sizeof(double) * h->Np, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
   return INS2D_OK;
   API_CATCH(h)
 }

@@ -133,6 +133,7 @@ struct TLPlan {
   double* part;   // 3 per aggregate block
   int maxit;
   int coarse;
+  double* en;     // 1 per aggregate block (energy partials of a start vector)
 };
 // IC(0)-preconditioned CG in one workgroup, triangular solves by
 // anti-diagonal wavefronts (pcg2.hip); factor from build_ic0_host
@@ -152,6 +153,12 @@ void launch_tl_iter(hipStream_t st, const TLPlan& P, double* x, double* r, doubl
 void launch_tl_true_residual(hipStream_t st, const TLPlan& P, const double* f, const double* x, double* r);
 // x += P Ac^-1 P^T (f - A x) (needs the coarse factors; r: work array)
 void launch_tl_coarse_correct(hipStream_t st, const TLPlan& P, const double* f, double* x, double* r);
+// x = P Ac^-1 P^T f, the coarse-grid solution (r: work array)
+void launch_tl_coarse_solution(hipStream_t st, const TLPlan& P, const double* f, double* x, double* r);
+// part[b] = sum over aggregate b of x (f + r), r = f - A x (one value per
+// aggregate; their sum is 2 phi(x), phi(x) = -x^T A x / 2 + f^T x)
+void launch_tl_energy(hipStream_t st, const TLPlan& P, const double* f, const double* x, const double* r,
+                      double* part);
 // 2 * sweeps damped Jacobi sweeps on x (tmp: work array)
 void launch_tl_smooth(hipStream_t st, const TLPlan& P, const double* f, double* x, double* tmp, int sweeps,
                       double omega);

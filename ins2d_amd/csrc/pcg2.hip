@@ -4,7 +4,8 @@
 // (cg.F:247), so the answer is judged exactly as the plain solver's:
 //  * a start vector x0 from the learned network (the caller fills x before
 //    the solve), smoothed by damped Jacobi sweeps, with a fallback to x0 = 0
-//    when ||f - A x0|| > ||f||;
+//    unless x0 is closer to the solution than 0 in the (-A)-norm
+//    (x0 . (f + r0) < 0, k_tl_energy);
 //  * a two-level preconditioner z = D^-1 r + P Ac^-1 P^T r, with P the 0/1
 //    aggregation over agg x agg blocks of cells and Ac = P^T A P the Galerkin
 //    coarse operator, solved exactly by fast diagonalization inside one
@@ -249,6 +250,32 @@ __global__ void k_tl_addcoarse(TL T, double* __restrict__ x) {
   x[(size_t)i + (size_t)T.L.nx * j] += T.C.ec[blk_id()];
 }
 
+// x = ec[aggregate] (the coarse-grid solution P Ac^-1 P^T f after an init
+// with x0 = 0)
+__global__ void k_tl_setcoarse(TL T, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= T.L.nx || j >= T.L.ny) return;
+  x[(size_t)i + (size_t)T.L.nx * j] = T.C.ec[blk_id()];
+}
+
+// per-aggregate partials of x . (f + r) with r = f - A x: twice the energy
+// phi(x) = -x^T A x / 2 + f^T x, whose difference to phi(A^-1 f) is half the
+// squared (-A)-norm of the error; phi(0) = 0
+__global__ void k_tl_energy(TL T, const double* __restrict__ f, const double* __restrict__ x,
+                            const double* __restrict__ r, double* __restrict__ part) {
+  __shared__ double red[16];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  double e = 0.0;
+  if (i < T.L.nx && j < T.L.ny) {
+    const size_t idx = (size_t)i + (size_t)T.L.nx * j;
+    e = x[idx] * (f[idx] + r[idx]);
+  }
+  e = block_sum(e, red);
+  if (lin_tid() == 0) part[blk_id()] = e;
+}
+
 // true relative residual of the final x: s[7] = ||f - A x||^2 / ||f||^2
 __global__ void k_tl_true(TL T, int nblk) {
   __shared__ double red[16];
@@ -434,6 +461,20 @@ void launch_tl_coarse_correct(hipStream_t st, const TLPlan& P, const double* f, 
   hipLaunchKernelGGL(k_tl_init, grid_of(P), block_of(P), 0, st, T, f, x, r, 0);
   hipLaunchKernelGGL(k_tl_coarse, dim3(1), dim3(1024), 0, st, T, nblk, 1, -1.0);
   hipLaunchKernelGGL(k_tl_addcoarse, grid_of(P), block_of(P), 0, st, T, x);
+}
+
+void launch_tl_coarse_solution(hipStream_t st, const TLPlan& P, const double* f, double* x, double* r) {
+  TL T = make(P);
+  const int nblk = P.C.ncx * P.C.ncy;
+  hipLaunchKernelGGL(k_tl_init, grid_of(P), block_of(P), 0, st, T, f, (const double*)nullptr, r, 1);
+  hipLaunchKernelGGL(k_tl_coarse, dim3(1), dim3(1024), 0, st, T, nblk, 1, -1.0);
+  hipLaunchKernelGGL(k_tl_setcoarse, grid_of(P), block_of(P), 0, st, T, x);
+}
+
+void launch_tl_energy(hipStream_t st, const TLPlan& P, const double* f, const double* x, const double* r,
+                      double* part) {
+  TL T = make(P);
+  hipLaunchKernelGGL(k_tl_energy, grid_of(P), block_of(P), 0, st, T, f, x, r, part);
 }
 
 void launch_tl_true_residual(hipStream_t st, const TLPlan& P, const double* f, const double* x, double* r) {

@@ -258,21 +258,24 @@ __global__ void k_wflip(const float* __restrict__ w, int taps, int cout, int cin
     wt[((long)(taps - 1 - t) * cin + ci) * cout + co] = f2bf(w[i]);
   }
 }
-// network input: x = f / s (fp64 -> 4-channel bf16 [x, 0, 0, 0]); s = rms(f)
-__global__ void k_input(const double* __restrict__ f, long n, const double* __restrict__ s, uint16_t* __restrict__ x) {
+// network input (fp64 -> 4-channel bf16): [f / s, xc / (scale s), 0, 0] with
+// s = rms(f); xc (optional) a coarse-grid solution the output corrects
+__global__ void k_input(const double* __restrict__ f, const double* __restrict__ xc, long n,
+                        const double* __restrict__ s, double scale, uint16_t* __restrict__ x) {
   const double inv = 1.0 / fmax(s[0], 1e-300);
+  const double invc = inv / scale;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     x[4 * i] = f2bf((float)(f[i] * inv));
-    x[4 * i + 1] = 0;
+    x[4 * i + 1] = xc ? f2bf((float)(xc[i] * invc)) : (uint16_t)0;
     x[4 * i + 2] = 0;
     x[4 * i + 3] = 0;
   }
 }
-// p = scale * s * y (network output fp32 -> fp64)
-__global__ void k_output(const float* __restrict__ y, long n, const double* __restrict__ s, double scale,
-                         double* __restrict__ p) {
+// p = xc + scale * s * y (network output fp32 -> fp64; xc optional, may alias p)
+__global__ void k_output(const float* __restrict__ y, const double* xc, long n, const double* __restrict__ s,
+                         double scale, double* p) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    p[i] = scale * s[0] * (double)y[i];
+    p[i] = (xc ? xc[i] : 0.0) + scale * s[0] * (double)y[i];
 }
 // s = sqrt(mean(f^2)) (single block)
 __global__ void k_rms(const double* __restrict__ f, long n, double* __restrict__ s) {
@@ -282,14 +285,15 @@ __global__ void k_rms(const double* __restrict__ f, long n, double* __restrict__
   a = block_sum(a, red);
   if (threadIdx.x == 0) s[0] = sqrt(a / (double)n);
 }
-// MSE loss of y against target t / (scale s): dy = 2 (y - tt) / n; loss += sum (y-tt)^2 / n
-__global__ void k_mse(const float* __restrict__ y, const double* __restrict__ t, long n, const double* __restrict__ s,
-                      double scale, float* __restrict__ dy, double* __restrict__ loss) {
+// MSE loss of y against the target (t - xc) / (scale s) (xc optional):
+// dy = 2 (y - tt) / n; loss += sum (y - tt)^2 / n
+__global__ void k_mse(const float* __restrict__ y, const double* __restrict__ t, const double* __restrict__ xc, long n,
+                      const double* __restrict__ s, double scale, float* __restrict__ dy, double* __restrict__ loss) {
   __shared__ double red[16];
   const double inv = 1.0 / (scale * fmax(s[0], 1e-300));
   double a = 0.0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const double d = (double)y[i] - t[i] * inv;
+    const double d = (double)y[i] - (t[i] - (xc ? xc[i] : 0.0)) * inv;
     dy[i] = (float)(2.0 * d / (double)n);
     a += d * d / (double)n;
   }
@@ -307,6 +311,14 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
     v[i] = vi;
     w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
   }
+}
+// pair dst = ca f_i + cb f_j with ca = a / rms(f_i), cb = b / rms(f_j)
+// (the same weights on p and xc: the three arrays of a pair are contiguous)
+__global__ void k_combine(const double* __restrict__ si, const double* __restrict__ sj, long n3,
+                          const double* __restrict__ rs, double a, double b, double* __restrict__ dst) {
+  const double ca = a / fmax(rs[0], 1e-300), cb = b / fmax(rs[1], 1e-300);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x)
+    dst[i] = ca * si[i] + cb * sj[i];
 }
 __global__ void k_zero_f(float* x, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = 0.f;
@@ -365,9 +377,10 @@ struct ins2d_nn {
   uint16_t *pool1 = nullptr, *pool2 = nullptr, *pool3 = nullptr, *up3 = nullptr, *up2 = nullptr, *up1 = nullptr;
   float *g0 = nullptr, *g1 = nullptr, *gs1 = nullptr, *gs2 = nullptr, *gs3 = nullptr, *gtmp = nullptr;
   double* scale = nullptr;   // rms of the current input
+  double* cscale = nullptr;  // rms of the two sources of a combined pair
   double* loss = nullptr;
   double out_scale = 1.0;    // p = out_scale * rms(f) * y
-  double* data = nullptr;    // training pairs: [k][0] = f, [k][1] = p (H*W each)
+  double* data = nullptr;    // training pairs: [k][0] = f, [k][1] = p, [k][2] = xc (H*W each)
   int data_count = 0;
   long step = 0;
   bool weights_ready = false;
@@ -554,10 +567,10 @@ void backward(hipStream_t st, ins2d_nn* N) {
   wgrad(0, bfin(N->x0, 4), none(), g0);
 }
 
-void prepare_input(hipStream_t st, ins2d_nn* N, const double* f) {
+void prepare_input(hipStream_t st, ins2d_nn* N, const double* f, const double* xc) {
   const long n = (long)N->H * N->W;
   hipLaunchKernelGGL(k_rms, dim3(1), dim3(1024), 0, st, f, n, N->scale);
-  hipLaunchKernelGGL(k_input, dim3(nb(n)), dim3(256), 0, st, f, n, N->scale, N->x0);
+  hipLaunchKernelGGL(k_input, dim3(nb(n)), dim3(256), 0, st, f, xc, n, N->scale, N->out_scale, N->x0);
 }
 
 int nn_fail(ins2d_nn* N, const std::exception& e) {
@@ -615,6 +628,7 @@ int ins2d_nn_create(int H, int W, ins2d_nn** out) {
     N->gs2 = nalloc<float>(N.get(), (p0 >> 2) * 32);
     N->gs3 = nalloc<float>(N.get(), (p0 >> 4) * 64);
     N->scale = nalloc<double>(N.get(), 1);
+    N->cscale = nalloc<double>(N.get(), 2);
     N->loss = nalloc<double>(N.get(), 1);
     NNCHK(hipMemset(N->param, 0, sizeof(float) * off));
     NNCHK(hipMemset(N->m, 0, sizeof(float) * off));
@@ -704,15 +718,16 @@ int ins2d_nn_get_params(ins2d_nn* N, float* params) {
   return INS2D_OK;
 }
 
-// p = out_scale * rms(f) * U(f / rms(f)); f, p device arrays (H*W doubles)
-int ins2d_nn_forward_dev(ins2d_nn* N, const double* f, double* p, void* stream) {
+// p = xc + out_scale * rms(f) * U(f / rms(f), xc / (out_scale rms(f))); f,
+// xc (optional, may be p itself), p device arrays (H*W doubles)
+int ins2d_nn_forward_dev(ins2d_nn* N, const double* f, const double* xc, double* p, void* stream) {
   if (!N || !f || !p) return INS2D_E_ARG;
   hipStream_t st = (hipStream_t)stream;
   try {
     const long n = (long)N->H * N->W;
-    prepare_input(st, N, f);
+    prepare_input(st, N, f, xc);
     forward(st, N);
-    hipLaunchKernelGGL(k_output, dim3(nb(n)), dim3(256), 0, st, N->out, n, N->scale, N->out_scale, p);
+    hipLaunchKernelGGL(k_output, dim3(nb(n)), dim3(256), 0, st, N->out, xc, n, N->scale, N->out_scale, p);
     NNCHK(hipGetLastError());
   } catch (const std::exception& e) {
     return nn_fail(N, e);
@@ -720,20 +735,25 @@ int ins2d_nn_forward_dev(ins2d_nn* N, const double* f, double* p, void* stream) 
   return INS2D_OK;
 }
 
-// host-array forward (tests)
-int ins2d_nn_forward(ins2d_nn* N, const double* f, double* p) {
+// host-array forward (tests); xc optional
+int ins2d_nn_forward(ins2d_nn* N, const double* f, const double* xc, double* p) {
   if (!N || !f || !p) return INS2D_E_ARG;
   try {
     const long n = (long)N->H * N->W;
-    double *df = nullptr, *dp = nullptr;
+    double *df = nullptr, *dp = nullptr, *dx = nullptr;
     NNCHK(hipMalloc(&df, n * sizeof(double)));
     NNCHK(hipMalloc(&dp, n * sizeof(double)));
     NNCHK(hipMemcpy(df, f, n * sizeof(double), hipMemcpyHostToDevice));
-    int rc = ins2d_nn_forward_dev(N, df, dp, nullptr);
+    if (xc) {
+      NNCHK(hipMalloc(&dx, n * sizeof(double)));
+      NNCHK(hipMemcpy(dx, xc, n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    int rc = ins2d_nn_forward_dev(N, df, dx, dp, nullptr);
     NNCHK(hipDeviceSynchronize());
     NNCHK(hipMemcpy(p, dp, n * sizeof(double), hipMemcpyDeviceToHost));
     (void)hipFree(df);
     (void)hipFree(dp);
+    if (dx) (void)hipFree(dx);
     return rc;
   } catch (const std::exception& e) {
     return nn_fail(N, e);
@@ -743,8 +763,8 @@ int ins2d_nn_forward(ins2d_nn* N, const double* f, double* p) {
 // loss and parameter gradients (accumulated over a batch of device samples
 // f[b], p[b]); mode 0: gradient only (into grad, zeroed first); mode 1: also
 // one Adam step with learning rate lr.  loss_out: mean loss over the batch.
-int ins2d_nn_train_step(ins2d_nn* N, const double* const* f, const double* const* p, int batch, float lr, int mode,
-                        double* loss_out, void* stream) {
+int ins2d_nn_train_step(ins2d_nn* N, const double* const* f, const double* const* p, const double* const* xc,
+                        int batch, float lr, int mode, double* loss_out, void* stream) {
   if (!N || !f || !p || batch < 1) return INS2D_E_ARG;
   hipStream_t st = (hipStream_t)stream;
   try {
@@ -752,9 +772,11 @@ int ins2d_nn_train_step(ins2d_nn* N, const double* const* f, const double* const
     hipLaunchKernelGGL(k_zero_f, dim3(nb(N->nparam)), dim3(256), 0, st, N->grad, N->nparam);
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, N->loss, 1L);
     for (int b = 0; b < batch; ++b) {
-      prepare_input(st, N, f[b]);
+      const double* x = xc ? xc[b] : nullptr;
+      prepare_input(st, N, f[b], x);
       forward(st, N);
-      hipLaunchKernelGGL(k_mse, dim3(nb(n)), dim3(256), 0, st, N->out, p[b], n, N->scale, N->out_scale, N->g0, N->loss);
+      hipLaunchKernelGGL(k_mse, dim3(nb(n)), dim3(256), 0, st, N->out, p[b], x, n, N->scale, N->out_scale, N->g0,
+                         N->loss);
       backward(st, N);
     }
     if (mode == 1) {
@@ -783,7 +805,7 @@ int ins2d_nn_data_reserve(ins2d_nn* N, int count) {
     N->data = nullptr;
     N->data_count = 0;
     if (count > 0) {
-      NNCHK(hipMalloc(&N->data, sizeof(double) * 2 * (size_t)count * N->H * N->W));
+      NNCHK(hipMalloc(&N->data, sizeof(double) * 3 * (size_t)count * N->H * N->W));
       N->data_count = count;
     }
   } catch (const std::exception& e) {
@@ -792,7 +814,7 @@ int ins2d_nn_data_reserve(ins2d_nn* N, int count) {
   return INS2D_OK;
 }
 
-int ins2d_nn_data_set(ins2d_nn* N, int k, const double* f, const double* p) {
+int ins2d_nn_data_set(ins2d_nn* N, int k, const double* f, const double* p, const double* xc) {
   if (!N || !f || !p) return INS2D_E_ARG;
   if (k < 0 || k >= N->data_count) {
     N->err = "training pair index out of range";
@@ -800,8 +822,58 @@ int ins2d_nn_data_set(ins2d_nn* N, int k, const double* f, const double* p) {
   }
   try {
     const size_t n = (size_t)N->H * N->W;
-    NNCHK(hipMemcpy(N->data + 2 * n * k, f, n * sizeof(double), hipMemcpyHostToDevice));
-    NNCHK(hipMemcpy(N->data + 2 * n * k + n, p, n * sizeof(double), hipMemcpyHostToDevice));
+    double* d = N->data + 3 * n * k;
+    NNCHK(hipMemcpy(d, f, n * sizeof(double), hipMemcpyHostToDevice));
+    NNCHK(hipMemcpy(d + n, p, n * sizeof(double), hipMemcpyHostToDevice));
+    if (xc) NNCHK(hipMemcpy(d + 2 * n, xc, n * sizeof(double), hipMemcpyHostToDevice));
+    else NNCHK(hipMemset(d + 2 * n, 0, n * sizeof(double)));
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+  return INS2D_OK;
+}
+
+double* ins2d_nn_data_ptr(ins2d_nn* N, int k, int which) {
+  if (!N || k < 0 || k >= N->data_count || which < 0 || which > 2) return nullptr;
+  const size_t n = (size_t)N->H * N->W;
+  return N->data + 3 * n * k + n * which;
+}
+
+int ins2d_nn_data_get(ins2d_nn* N, int k, double* f, double* p, double* xc) {
+  if (!N) return INS2D_E_ARG;
+  if (k < 0 || k >= N->data_count) {
+    N->err = "training pair index out of range";
+    return INS2D_E_ARG;
+  }
+  try {
+    const size_t n = (size_t)N->H * N->W;
+    const double* d = N->data + 3 * n * k;
+    NNCHK(hipDeviceSynchronize());
+    if (f) NNCHK(hipMemcpy(f, d, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (p) NNCHK(hipMemcpy(p, d + n, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (xc) NNCHK(hipMemcpy(xc, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost));
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+  return INS2D_OK;
+}
+
+int ins2d_nn_data_combine(ins2d_nn* N, int dst, int i, int j, double a, double b) {
+  if (!N) return INS2D_E_ARG;
+  if (dst < 0 || dst >= N->data_count || i < 0 || i >= N->data_count || j < 0 || j >= N->data_count || dst == i ||
+      dst == j) {
+    N->err = "training pair index out of range (or the destination is a source)";
+    return INS2D_E_ARG;
+  }
+  try {
+    const long n = (long)N->H * N->W;
+    const double* si = N->data + 3 * n * i;
+    const double* sj = N->data + 3 * n * j;
+    hipLaunchKernelGGL(k_rms, dim3(1), dim3(1024), 0, nullptr, si, n, N->cscale);
+    hipLaunchKernelGGL(k_rms, dim3(1), dim3(1024), 0, nullptr, sj, n, N->cscale + 1);
+    hipLaunchKernelGGL(k_combine, dim3(nb(3 * n)), dim3(256), 0, nullptr, si, sj, 3 * n, (const double*)N->cscale, a,
+                       b, N->data + 3 * n * dst);
+    NNCHK(hipGetLastError());
   } catch (const std::exception& e) {
     return nn_fail(N, e);
   }
@@ -811,16 +883,17 @@ int ins2d_nn_data_set(ins2d_nn* N, int k, const double* f, const double* p) {
 int ins2d_nn_train(ins2d_nn* N, const int* idx, int batch, float lr, int mode, double* loss_out) {
   if (!N || !idx || batch < 1) return INS2D_E_ARG;
   const size_t n = (size_t)N->H * N->W;
-  std::vector<const double*> f(batch), p(batch);
+  std::vector<const double*> f(batch), p(batch), xc(batch);
   for (int b = 0; b < batch; ++b) {
     if (idx[b] < 0 || idx[b] >= N->data_count) {
       N->err = "training pair index out of range";
       return INS2D_E_ARG;
     }
-    f[b] = N->data + 2 * n * idx[b];
+    f[b] = N->data + 3 * n * idx[b];
     p[b] = f[b] + n;
+    xc[b] = f[b] + 2 * n;
   }
-  return ins2d_nn_train_step(N, f.data(), p.data(), batch, lr, mode, loss_out, nullptr);
+  return ins2d_nn_train_step(N, f.data(), p.data(), xc.data(), batch, lr, mode, loss_out, nullptr);
 }
 
 int ins2d_nn_get_grad(ins2d_nn* N, float* grad) {

@@ -244,14 +244,40 @@ class Solver:
                    _dp(np.ascontiguousarray(x, dtype=np.float64)), ctypes.byref(rel))
         return rel.value
 
-    def set_poisson_accel(self, net=None, warm_start=False, two_level=False, agg=0):
+    def set_poisson_accel(self, net=None, warm_start=False, two_level=False, agg=0, coarse_input=False,
+                          coarse_start=False, sweeps=0):
         """Accelerated PCG (modes 3/4): learned start vector from ``net``
         (a warmstart.WarmStartNet on this pressure grid) and/or the two-level
-        preconditioner; both off restores the plain PCG."""
-        flags = (1 if warm_start else 0) | (2 if two_level else 0)
+        preconditioner; ``coarse_input``: the network also takes the coarse-grid
+        solution and predicts its correction (needs both); ``coarse_start``:
+        start from the coarse-grid solution alone (two-level, no network);
+        ``sweeps``: damped Jacobi sweep pairs after a coarse-input start (0:
+        the library's default).  All off restores the plain PCG."""
+        flags = ((1 if warm_start else 0) | (2 if two_level else 0) | (4 if coarse_input else 0) |
+                 (8 if coarse_start else 0) | (int(sweeps) << 8))
         self._accel_net = net if warm_start else None   # keep the network alive
         self._call(self.lib.ins2d_set_poisson_accel, net._h if warm_start else None, ctypes.c_int(flags),
                    ctypes.c_int(agg))
+
+    def pressure_rhs_dev(self, f_ptr, t=0.0):
+        """The pressure right-hand side of the resident velocity into the
+        device array at f_ptr."""
+        self._call(self.lib.ins2d_pressure_rhs_dev, ctypes.c_double(t), ctypes.c_void_p(f_ptr))
+
+    def poisson_dev(self, f_ptr, dp_ptr):
+        """A dp = f on device arrays."""
+        self._call(self.lib.ins2d_poisson_dev, ctypes.c_void_p(f_ptr), ctypes.c_void_p(dp_ptr))
+
+    def poisson_coarse_dev(self, f_ptr, xc_ptr):
+        """The coarse-grid solution of f into the device array at xc_ptr."""
+        self._call(self.lib.ins2d_poisson_coarse_dev, ctypes.c_void_p(f_ptr), ctypes.c_void_p(xc_ptr))
+
+    def poisson_coarse(self, f):
+        """The coarse-grid solution P Ac^-1 P^T f of the two-level
+        preconditioner (set_poisson_accel(two_level=True) first)."""
+        xc = np.zeros(self.Np)
+        self._call(self.lib.ins2d_poisson_coarse, _dp(np.ascontiguousarray(f, dtype=np.float64)), _dp(xc))
+        return xc
 
     def accel_stats(self):
         fb = ctypes.c_longlong()

@@ -72,27 +72,33 @@ class WarmStartNet:
         self._check(self._lib.ins2d_nn_get_grad(self._h, g.ctypes.data_as(_lib.P_float)))
         return g
 
-    def forward(self, f):
-        """p0 for a host right-hand side f of shape (H, W) (x fastest)."""
+    def forward(self, f, xc=None):
+        """p0 for a host right-hand side f of shape (H, W) (x fastest) and an
+        optional coarse-grid solution xc of the same shape."""
         f = np.ascontiguousarray(f, dtype=np.float64)
         if f.shape != (self.H, self.W):
             raise ValueError(f'expected shape {(self.H, self.W)}, got {f.shape}')
+        x = None if xc is None else np.ascontiguousarray(xc, dtype=np.float64).reshape(f.shape)
         p = np.empty_like(f)
         self._check(self._lib.ins2d_nn_forward(self._h, f.ctypes.data_as(_lib.P_double),
+                                               None if x is None else x.ctypes.data_as(_lib.P_double),
                                                p.ctypes.data_as(_lib.P_double)))
         return p
 
-    def forward_dev(self, f_ptr, p_ptr):
-        self._check(self._lib.ins2d_nn_forward_dev(self._h, ctypes.c_void_p(f_ptr), ctypes.c_void_p(p_ptr), None))
+    def forward_dev(self, f_ptr, p_ptr, xc_ptr=None):
+        self._check(self._lib.ins2d_nn_forward_dev(self._h, ctypes.c_void_p(f_ptr),
+                                                   None if xc_ptr is None else ctypes.c_void_p(xc_ptr),
+                                                   ctypes.c_void_p(p_ptr), None))
 
-    def train_step(self, f_ptrs, p_ptrs, lr=1e-3, update=True):
-        """Loss over a batch of device (f, p) pairs; with ``update`` one Adam
-        step.  Returns the mean loss."""
+    def train_step(self, f_ptrs, p_ptrs, lr=1e-3, update=True, xc_ptrs=None):
+        """Loss over a batch of device (f, p[, xc]) pairs; with ``update`` one
+        Adam step.  Returns the mean loss."""
         n = len(f_ptrs)
         fa = (ctypes.c_void_p * n)(*[ctypes.c_void_p(x) for x in f_ptrs])
         pa = (ctypes.c_void_p * n)(*[ctypes.c_void_p(x) for x in p_ptrs])
+        xa = None if xc_ptrs is None else (ctypes.c_void_p * n)(*[ctypes.c_void_p(x) for x in xc_ptrs])
         loss = ctypes.c_double()
-        self._check(self._lib.ins2d_nn_train_step(self._h, fa, pa, n, ctypes.c_float(lr), 1 if update else 0,
+        self._check(self._lib.ins2d_nn_train_step(self._h, fa, pa, xa, n, ctypes.c_float(lr), 1 if update else 0,
                                                   ctypes.byref(loss), None))
         return loss.value
 
@@ -100,13 +106,34 @@ class WarmStartNet:
     def data_reserve(self, count):
         self._check(self._lib.ins2d_nn_data_reserve(self._h, int(count)))
 
-    def data_set(self, k, f, p):
+    def data_set(self, k, f, p, xc=None):
         f = np.ascontiguousarray(f, dtype=np.float64)
         p = np.ascontiguousarray(p, dtype=np.float64)
-        if f.size != self.H * self.W or p.size != self.H * self.W:
+        x = None if xc is None else np.ascontiguousarray(xc, dtype=np.float64)
+        if f.size != self.H * self.W or p.size != self.H * self.W or (x is not None and x.size != f.size):
             raise ValueError('training pair has the wrong size')
         self._check(self._lib.ins2d_nn_data_set(self._h, int(k), f.ctypes.data_as(_lib.P_double),
-                                                p.ctypes.data_as(_lib.P_double)))
+                                                p.ctypes.data_as(_lib.P_double),
+                                                None if x is None else x.ctypes.data_as(_lib.P_double)))
+
+    def data_ptr(self, k, which):
+        """Device address of pair k's array (0 f, 1 p, 2 xc)."""
+        ptr = self._lib.ins2d_nn_data_ptr(self._h, int(k), int(which))
+        if not ptr:
+            raise IndexError(f'no training pair {k}')
+        return int(ptr)
+
+    def data_get(self, k):
+        """Host copies (f, p, xc) of pair k."""
+        f, p, xc = (np.empty(self.H * self.W) for _ in range(3))
+        P = lambda a: a.ctypes.data_as(_lib.P_double)
+        self._check(self._lib.ins2d_nn_data_get(self._h, int(k), P(f), P(p), P(xc)))
+        return f, p, xc
+
+    def data_combine(self, dst, i, j, a, b):
+        """pair dst = a pair_i / rms(f_i) + b pair_j / rms(f_j) (device)."""
+        self._check(self._lib.ins2d_nn_data_combine(self._h, int(dst), int(i), int(j), ctypes.c_double(a),
+                                                    ctypes.c_double(b)))
 
     def train(self, idx, lr=1e-3, update=True):
         """Loss over the stored pairs ``idx``; with ``update`` one Adam step."""

@@ -8,8 +8,10 @@ library evaluates:
   4-level U-shaped CNN, 3x3 convolutions with zero padding, ReLU, 2x2 average
   pooling, nearest x2 upsampling, skip connections by channel concatenation
   (upsampled decoder features first), a final 1x1 linear convolution;
-  channels 16/32/64/64; input [f / rms(f), 0, 0, 0]; output y with
-  p0 = out_scale * rms(f) * y; loss mean((y - p / (out_scale rms(f)))^2).
+  channels 16/32/64/64; input [f / rms(f), xc / (out_scale rms(f)), 0, 0]
+  with xc an optional coarse-grid solution (zero when absent); output y with
+  p0 = xc + out_scale * rms(f) * y; loss
+  mean((y - (p - xc) / (out_scale rms(f)))^2).
 
 Tensors are (H, W, C) arrays, W fastest over pixels as in the solver's
 pressure layout.  Parameters are one flat vector: for each layer the weights
@@ -150,14 +152,18 @@ def rms(f):
     return float(np.sqrt(np.mean(np.asarray(f, dtype=np.float64) ** 2)))
 
 
-def forward(theta, f, bf16_emulation=False, keep=False):
-    """Network output y (H, W) for a right-hand side f (H, W)."""
+def forward(theta, f, bf16_emulation=False, keep=False, xc=None, out_scale=1.0):
+    """Network output y (H, W) for a right-hand side f (H, W) and an optional
+    coarse-grid solution xc (H, W)."""
     R = bf16 if bf16_emulation else _ident
     P = _layer_params(theta, R)
     s = max(rms(f), 1e-300)
     H, W = f.shape
     x0 = np.zeros((H, W, 4))
     x0[..., 0] = R(np.float32(np.asarray(f, dtype=np.float64) / s)) if bf16_emulation else f / s
+    if xc is not None:
+        c = np.asarray(xc, dtype=np.float64).reshape(H, W) / (out_scale * s)
+        x0[..., 1] = R(np.float32(c)) if bf16_emulation else c
     T = {'x0': x0, 's': s}
 
     def layer(l, a, b=None):
@@ -193,23 +199,27 @@ def forward(theta, f, bf16_emulation=False, keep=False):
     return (y, T) if keep else y
 
 
-def predict(theta, f, out_scale, bf16_emulation=False):
-    """p0 = out_scale * rms(f) * y(f)."""
-    return out_scale * rms(f) * forward(theta, f, bf16_emulation)
+def predict(theta, f, out_scale, bf16_emulation=False, xc=None):
+    """p0 = xc + out_scale * rms(f) * y(f, xc) (xc = 0 when absent)."""
+    y = forward(theta, f, bf16_emulation, xc=xc, out_scale=out_scale)
+    base = 0.0 if xc is None else np.asarray(xc, dtype=np.float64).reshape(y.shape)
+    return base + out_scale * rms(f) * y
 
 
-def loss_and_grad(theta, fs, ps, out_scale, bf16_emulation=False):
-    """Mean over the batch of mean((y - p / (out_scale rms f))^2) and its
-    gradient with respect to the flat parameter vector."""
+def loss_and_grad(theta, fs, ps, out_scale, bf16_emulation=False, xcs=None):
+    """Mean over the batch of mean((y - (p - xc) / (out_scale rms f))^2) and
+    its gradient with respect to the flat parameter vector."""
     R = bf16 if bf16_emulation else _ident
     P = _layer_params(theta, R)
     sl, n = layer_slices()
     grad = np.zeros(n)
     total = 0.0
-    for f, p in zip(fs, ps):
-        y, T = forward(theta, f, bf16_emulation, keep=True)
+    for k, (f, p) in enumerate(zip(fs, ps)):
+        xc = None if xcs is None else xcs[k]
+        y, T = forward(theta, f, bf16_emulation, keep=True, xc=xc, out_scale=out_scale)
         npx = y.size
-        tt = np.asarray(p, dtype=np.float64) / (out_scale * T['s'])
+        base = 0.0 if xc is None else np.asarray(xc, dtype=np.float64).reshape(y.shape)
+        tt = (np.asarray(p, dtype=np.float64) - base) / (out_scale * T['s'])
         d = y - tt
         total += float(np.sum(d * d) / npx)
         g_out = (2.0 * d / npx)[..., None]

@@ -94,38 +94,52 @@ def test_warm_start_fallback_and_neutral_start():
 
 
 def test_trained_network_gain():
-    """Train on (f, p) pairs from BFS runs at three Reynolds numbers, then
+    """Train on pairs from BFS runs at three Reynolds numbers, generated in
+    device memory with the coarse-grid solution of each right-hand side, then
     solve held-out right-hand sides (another Re, later times) with plain
-    Jacobi-PCG, the two-level PCG, and both with the network's start vector.
-    Prints what the network gains over the same method without it.  On this
-    data the smoothed, coarse-corrected start still leaves ||f - A x0|| above
-    ||f|| (its BF16 rounding noise and ~30 % error), so the fallback to x0 = 0
-    triggers; the test pins that this guard holds and that every variant
-    reaches the tolerance."""
+    Jacobi-PCG, the two-level PCG from zero, the two-level PCG from the
+    coarse-grid solution (the method without the network) and the two-level
+    PCG from the network's correction of it.  Pins: every variant reaches the
+    tolerance; the network's start is far closer to the solution than the
+    coarse-grid solution it corrects; the energy test keeps a start only when
+    it is closer to the solution than x0 = 0, so the learned start never needs
+    more iterations than the start from zero; and prints the time gains."""
     from ins2d_amd import learn
     _say('trained network gain')
     Nx, Ny = 128, 64
-    pairs = learn.generate_pairs(Nx, Ny, [300.0, 600.0, 900.0], steps=240, every=4, skip=20, progress=_say)
+    per_run, Re_train = 80, [300.0, 600.0, 900.0]
+    n = per_run * len(Re_train)
+    net = learn.make_net(Nx, Ny, out_scale=1.0, seed=0)
+    net.data_reserve(n + 2)
+    assert learn.generate_pairs_dev(net, Nx, Ny, Re_train, per_run, every=3, skip=20, progress=_say) == n
+    out_scale = learn.out_scale_from_net(net, n)
+    net.init(0, out_scale)
     held = learn.generate_pairs(Nx, Ny, [750.0], steps=160, every=8, skip=100, progress=_say)
-    net = learn.make_net(Nx, Ny, out_scale=learn.out_scale_of(pairs), seed=0)
-    losses = learn.train(net, pairs, steps=300, batch=4, lr=2e-3, progress=_say, every=100)
-    assert np.mean(losses[-20:]) < 0.35 * np.mean(losses[:5])
+    losses = learn.train_stored(net, n, steps=400, batch=4, lr=2e-3, progress=_say, every=100)
+    assert np.mean(losses[-20:]) < 0.1 * np.mean(losses[:5])
     solver, _, _, _ = host.setup(_bfs(Nx, Ny))
     rhs = [h[0] for h in held]
-    res = learn.evaluate(solver, net, rhs, [('plain', False, False), ('two_level', False, True),
-                                            ('warm', True, False), ('learned', True, True)], progress=_say)
-    x0rel = [solver.residual(f, net.forward(f.reshape(Ny, Nx)).ravel()) for f in rhs]
-    _say('residual of the raw network output (before smoothing): median', np.median(x0rel))
+    solver.set_poisson_accel(None, two_level=True)
+    err_c, err_n = [], []
+    for f, p, *_ in held:
+        xc = solver.poisson_coarse(f)
+        pn = net.forward(f.reshape(Ny, Nx), xc.reshape(Ny, Nx)).ravel()
+        err_c.append(np.linalg.norm(xc - p) / np.linalg.norm(p))
+        err_n.append(np.linalg.norm(pn - p) / np.linalg.norm(p))
+    _say('held-out start error: coarse', np.median(err_c), 'learned', np.median(err_n))
+    assert np.median(err_n) < 0.2 * np.median(err_c)
+    res = learn.evaluate(solver, net, rhs, [('plain', {}), ('two_level', dict(two_level=True)),
+                                            ('coarse_start', dict(two_level=True, coarse_start=True)),
+                                            ('learned', dict(warm_start=True, two_level=True, coarse_input=True))],
+                         progress=_say)
     for k in res:
         assert max(res[k]['rel']) <= 1.5e-10, k
-    gain = np.median(res['two_level']['ms']) / np.median(res['learned']['ms'])
-    _say('network gain over the two-level PCG without it:', gain, 'fallbacks', res['learned']['fallbacks'],
-         'of', len(rhs) + 1)
-    # whatever the network's quality, the fallback keeps the accelerated
-    # solve from needing more iterations than the start from x0 = 0
+    med = {k: np.median(v['ms']) for k, v in res.items()}
+    _say('network gain over the two-level PCG from zero:', med['two_level'] / med['learned'],
+         'from the coarse-grid solution:', med['coarse_start'] / med['learned'],
+         'fallbacks', res['learned']['fallbacks'], 'of', len(rhs) + 1)
     assert np.median(res['learned']['it']) <= np.median(res['two_level']['it']) + 2
-    assert np.median(res['warm']['it']) <= np.median(res['plain']['it']) + 2
-    # the coarse correction alone must give the required gain
-    assert np.median(res['plain']['ms']) > 3 * np.median(res['two_level']['ms'])
+    # the coarse correction alone gives a large gain over plain PCG
+    assert med['plain'] > 3 * med['two_level']
     solver.close()
     net.close()

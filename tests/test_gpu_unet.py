@@ -130,3 +130,49 @@ def test_adam_step_and_loss_decrease():
     losses = [net.train([0], lr=lr, update=True) for _ in range(30)]
     assert losses[-1] < 0.5 * l0
     net.close()
+
+
+def test_coarse_input_forward_and_gradient_parity():
+    """The second input channel (a coarse-grid solution xc, scaled by
+    out_scale rms(f)) and the output p0 = xc + out_scale rms(f) y: forward
+    against the emulated oracle (the network part p0 - xc to 1e-2), and the
+    loss and per-layer gradients of pairs stored with xc to 3e-2 on the
+    16x32 grid."""
+    from ins2d_amd.warmstart import WarmStartNet
+    H, W = 16, 32
+    rng = np.random.default_rng(21)
+    th = _theta(4)
+    net = WarmStartNet(H, W)
+    net.set_params(th, 0.6)
+    f = rng.standard_normal((H, W))
+    xc = 0.8 * rng.standard_normal((H, W))
+    p_gpu = net.forward(f, xc)
+    p_ref = U.predict(th.astype(np.float64), f, 0.6, bf16_emulation=True, xc=xc)
+    print(f'forward with xc: rel {_rel(p_gpu - xc, p_ref - xc):.2e}', flush=True)
+    assert _rel(p_gpu - xc, p_ref - xc) < 1e-2
+    fs = [rng.standard_normal((H, W)) for _ in range(2)]
+    ps = [rng.standard_normal((H, W)) for _ in range(2)]
+    xcs = [0.5 * rng.standard_normal((H, W)) for _ in range(2)]
+    net.data_reserve(2)
+    for k in range(2):
+        net.data_set(k, fs[k], ps[k], xcs[k])
+    loss = net.train([0, 1], update=False)
+    g_gpu = net.get_grad().astype(np.float64) / 2.0
+    L_ref, g_ref = U.loss_and_grad(th.astype(np.float64), fs, ps, 0.6, bf16_emulation=True, xcs=xcs)
+    print(f'loss with xc: gpu {loss:.8e} oracle {L_ref:.8e}', flush=True)
+    assert abs(loss - L_ref) <= 1e-3 * L_ref
+    for l, (ws, _, bs) in enumerate(U.layer_slices()[0]):
+        ew, eb = _rel(g_gpu[ws], g_ref[ws]), _rel(g_gpu[bs], g_ref[bs])
+        assert ew < 3e-2 and eb < 3e-2, (l, ew, eb)
+    # stored pairs round-trip, and a device-side combination is exact
+    f0, p0, x0 = net.data_get(0)
+    assert np.array_equal(f0, fs[0].ravel()) and np.array_equal(x0, xcs[0].ravel())
+    net.data_reserve(3)
+    for k in range(2):
+        net.data_set(k, fs[k], ps[k], xcs[k])
+    net.data_combine(2, 0, 1, 0.5, -2.0)
+    fc, pc, xcc = net.data_get(2)
+    r0, r1 = np.sqrt(np.mean(fs[0] ** 2)), np.sqrt(np.mean(fs[1] ** 2))
+    for got, a, b in ((fc, fs[0], fs[1]), (pc, ps[0], ps[1]), (xcc, xcs[0], xcs[1])):
+        np.testing.assert_allclose(got, (0.5 / r0 * a - 2.0 / r1 * b).ravel(), rtol=1e-13, atol=1e-13)
+    net.close()

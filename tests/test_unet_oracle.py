@@ -81,3 +81,31 @@ def test_scale_equivariance():
     p1 = U.predict(th, f, 2.0)
     p2 = U.predict(th, 7.5 * f, 2.0)
     assert np.allclose(p2, 7.5 * p1, rtol=1e-12, atol=1e-14)
+
+
+def test_coarse_input_gradient_and_equivariance():
+    """With the coarse-grid solution as second input: the gradient against
+    central finite differences, p0 = xc + out_scale rms(f) y, and
+    p0(a f, a xc) = a p0(f, xc); the input channel matters (a planted zero
+    coarse input changes the output)."""
+    rng = np.random.default_rng(5)
+    th = U.init_params(9).astype(np.float64)
+    sl, n = U.layer_slices()
+    for (_, _, bs) in sl:
+        th[bs] = 0.05 * rng.standard_normal(bs.stop - bs.start)
+    fs = [rng.standard_normal((16, 24)) for _ in range(2)]
+    ps = [rng.standard_normal((16, 24)) for _ in range(2)]
+    xcs = [rng.standard_normal((16, 24)) for _ in range(2)]
+    L, g = U.loss_and_grad(th, fs, ps, 0.5, xcs=xcs)
+    for k in range(3):
+        d = rng.standard_normal(n)
+        lp, _ = U.loss_and_grad(th + 1e-7 * d, fs, ps, 0.5, xcs=xcs)
+        lm, _ = U.loss_and_grad(th - 1e-7 * d, fs, ps, 0.5, xcs=xcs)
+        fd = (lp - lm) / 2e-7
+        assert abs(fd - g @ d) <= 1e-6 * max(1.0, abs(fd)), (k, fd, g @ d)
+    f, xc = fs[0], xcs[0]
+    y = U.forward(th, f, xc=xc, out_scale=0.5)
+    assert np.allclose(U.predict(th, f, 0.5, xc=xc), xc + 0.5 * U.rms(f) * y, rtol=1e-14, atol=1e-14)
+    assert np.allclose(U.predict(th, 3.5 * f, 0.5, xc=3.5 * xc), 3.5 * U.predict(th, f, 0.5, xc=xc),
+                       rtol=1e-12, atol=1e-13)
+    assert np.abs(U.forward(th, f, xc=xc, out_scale=0.5) - U.forward(th, f)).max() > 1e-6
