@@ -140,39 +140,82 @@ __global__ void __launch_bounds__(256) k_b_rhs(const BInst* __restrict__ I, cons
   }
 }
 
+// The divergence and the projection run over tall tiles (TX x kTallRows; the
+// block is TX x TY, each thread takes the rows j, j + TY, ...): the chain of
+// dependent loads that starts every block (tile, instance, operator records)
+// is then paid once per kTallRows / TY rows.  The operator records arrive by
+// scalar loads (cst), the per-point arithmetic is branch-free (kron2_at_nb,
+// optional operands as template flags) and rows past the component's end are
+// clamped to its last row (valid addresses) and not stored, so the loads of
+// all rows of a thread issue together.
+constexpr int kTallRows = 16;
+constexpr int kTallR = kTallRows / TY;
+
 // f = oscale * (M (ca A + cb B) + ycoef yM); with dt != null: ca = ycoef = 1/dt_k
+template <bool HASB>
 __global__ void __launch_bounds__(256) k_b_div(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
                                               const double* __restrict__ A, const double* __restrict__ Bv,
                                               const double* __restrict__ dt, double ca, double cb, double ycoef,
                                               double oscale, double* __restrict__ f) {
   const Tile t = tiles[blockIdx.x];
   const BInst& B = I[t.inst];
-  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
-  if (i >= B.npx || j >= B.npy) return;
+  const int i = t.i0 + threadIdx.x;
+  if (i >= B.npx) return;
   double a = ca, y = ycoef;
   if (dt) { a = 1.0 / dt[t.inst]; y = a; }
-  const double s = div_point(*B.cu, *B.cv, A + B.voff, Bv ? Bv + B.voff : nullptr, a, cb, y, B.nu, i, j);
-  f[B.poff + (long)i + (long)B.npx * j] = oscale * s;
+  const DComp Cu = cst(B.cu), Cv = cst(B.cv);
+  const double* Au = A + B.voff;
+  const double* Bu = HASB ? Bv + B.voff : nullptr;
+  double s[kTallR];
+#pragma unroll
+  for (int r = 0; r < kTallR; ++r) {
+    const int j = min(t.j0 + (int)threadIdx.y + TY * r, B.npy - 1);
+    s[r] = kron2_at2_nb<HASB>(Cu.M, Au, a, Bu, cb, i, j);
+    s[r] += kron2_at2_nb<HASB>(Cv.M, Au + B.nu, a, HASB ? Bu + B.nu : nullptr, cb, i, j);
+  }
+  if (y != 0.0) {
+#pragma unroll
+    for (int r = 0; r < kTallR; ++r) {
+      const int j = min(t.j0 + (int)threadIdx.y + TY * r, B.npy - 1);
+      s[r] += y * (yv_at(Cu.yM, i, j) + yv_at(Cv.yM, i, j));
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kTallR; ++r) {
+    const int j = t.j0 + (int)threadIdx.y + TY * r;
+    if (j < B.npy) f[B.poff + (long)i + (long)B.npx * j] = oscale * s[r];
+  }
 }
 
-// V = Vn + dt_k (Vt - c Om^-1 G dp)  (dt null: dt_k = 1)
+// V = Vn + dt_k (Vt - c Om^-1 G dp)  (dt null: dt_k = 1; without Vt: Vt = 0)
+template <bool HASVT>
 __global__ void __launch_bounds__(256) k_b_proj(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
                                                const double* __restrict__ Vn, const double* __restrict__ Vt,
                                                const double* __restrict__ dp, const double* __restrict__ dt, double c,
                                                double* __restrict__ V) {
   const Tile t = tiles[blockIdx.x];
   const BInst& B = I[t.inst];
-  const int i = t.i0 + threadIdx.x, j = t.j0 + threadIdx.y;
+  const int i = t.i0 + threadIdx.x;
+  const int nx = t.comp == 0 ? B.nux : B.nvx, ny = t.comp == 0 ? B.nuy : B.nvy;
+  if (i >= nx) return;
   const double d = dt ? dt[t.inst] : 1.0;
   const double* pp = dp + B.poff;
-  if (t.comp == 0) {
-    if (i >= B.nux || j >= B.nuy) return;
-    const long o = B.voff;
-    V[o + (long)i + (long)B.nux * j] = proj_point(*B.cu, Vn + o, Vt ? Vt + o : nullptr, pp, d, c, i, j);
-  } else {
-    if (i >= B.nvx || j >= B.nvy) return;
-    const long o = B.voff + B.nu;
-    V[o + (long)i + (long)B.nvx * j] = proj_point(*B.cv, Vn + o, Vt ? Vt + o : nullptr, pp, d, c, i, j);
+  const DComp C = cst(t.comp == 0 ? B.cu : B.cv);
+  const long o = t.comp == 0 ? B.voff : B.voff + B.nu;
+  const auto vn = glb(Vn + o);
+  double w[kTallR];
+#pragma unroll
+  for (int r = 0; r < kTallR; ++r) {
+    const int j = min(t.j0 + (int)threadIdx.y + TY * r, ny - 1);
+    const unsigned idx = (unsigned)i + (unsigned)nx * (unsigned)j;
+    const double g = kron2_at_nb(C.G, pp, i, j) / (glb(C.ox)[i] * glb(C.oy)[j]);
+    const double vt = HASVT ? glb(Vt + o)[idx] : 0.0;
+    w[r] = vn[idx] + d * (vt - c * g);
+  }
+#pragma unroll
+  for (int r = 0; r < kTallR; ++r) {
+    const int j = t.j0 + (int)threadIdx.y + TY * r;
+    if (j < ny) V[o + (long)i + (long)nx * j] = w[r];
   }
 }
 
@@ -533,28 +576,26 @@ __global__ void __launch_bounds__(256) k_b2_dir(const BInst* __restrict__ I, con
 
 // ---- batched PCG with the fast-diagonalization preconditioner: z = A^-1 r
 // by the four GEMMs of the direct solve (fdm.hip), per instance, over a work
-// list of 32 x 32 tiles of the Npx x Npy pressure arrays; the per-instance
+// list of 64 x 64 tiles of the Npx x Npy pressure arrays; the per-instance
 // scalars and the convergence test (cg.F:247) are the Jacobi PCG's, so the
 // preconditioner (the exact inverse on these tensor-product grids, up to
 // rounding) changes only how many iterations reach the tolerance.
 // PH 0: dst = Vx^T src; 1: dst = -(src Vy) ./ (lx + ly); 2: dst = Vx src;
 // 3: dst = src Vy^T
 template <int PH>
-__global__ void __launch_bounds__(64 * kGemmWaves) k_bf_gemm(const BInst* __restrict__ I,
-                                                            const Tile* __restrict__ tiles, BCG S,
-                                                            const double* __restrict__ src,
-                                                            double* __restrict__ dst) {
-  __shared__ GemmPart part;
+__global__ void __launch_bounds__(256) k_bf_gemm(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
+                                                const double* __restrict__ src, double* __restrict__ dst) {
+  __shared__ Gemm64Smem sm;
   const Tile t = tiles[blockIdx.x];
   if (S.f[4 * t.inst]) return;
   const BInst& B = I[t.inst];
   const int m = B.npx, n = B.npy;
   const double* s = src + B.poff;
   double* d = dst + B.poff;
-  if (PH == 0) dgemm_tile<0>(m, n, m, B.fVxT, m, s, m, d, m, nullptr, nullptr, -1, -1, t.i0, t.j0, part);
-  if (PH == 1) dgemm_tile<1>(m, n, n, s, m, B.fVy, n, d, m, B.flx, B.fly, B.fnx0, B.fny0, t.i0, t.j0, part);
-  if (PH == 2) dgemm_tile<0>(m, n, m, B.fVx, m, s, m, d, m, nullptr, nullptr, -1, -1, t.i0, t.j0, part);
-  if (PH == 3) dgemm_tile<0>(m, n, n, s, m, B.fVyT, n, d, m, nullptr, nullptr, -1, -1, t.i0, t.j0, part);
+  if (PH == 0) dgemm_tile64<0>(m, n, m, B.fVxT, m, s, m, d, m, nullptr, nullptr, -1, -1, t.i0, t.j0, sm);
+  if (PH == 1) dgemm_tile64<1>(m, n, n, s, m, B.fVy, n, d, m, B.flx, B.fly, B.fnx0, B.fny0, t.i0, t.j0, sm);
+  if (PH == 2) dgemm_tile64<0>(m, n, m, B.fVx, m, s, m, d, m, nullptr, nullptr, -1, -1, t.i0, t.j0, sm);
+  if (PH == 3) dgemm_tile64<0>(m, n, n, s, m, B.fVyT, n, d, m, nullptr, nullptr, -1, -1, t.i0, t.j0, sm);
 }
 
 // x = 0, r = f; partial f.f per pressure tile
@@ -752,8 +793,12 @@ struct ins2d_batch {
   DComp* dcomp = nullptr;
   std::vector<BInst> hinst;
   BInst* dinst = nullptr;
-  std::vector<Tile> uvt, pt;
-  Tile *duv = nullptr, *dpt = nullptr;
+  std::vector<Tile> pt;   // pressure tiles, TX x TY
+  Tile* dpt = nullptr;
+  // velocity and pressure areas in tall tiles (TX x kTallRows) for the
+  // divergence and the projection
+  std::vector<Tile> uvt_tall, pt_tall;
+  Tile *duv_tall = nullptr, *dpt_tall = nullptr;
   // momentum right-hand side: members with a closed form over split tiles
   // (rs), the others over tiles of the tuned shape tx x ty (rg)
   std::vector<RTile> rs;
@@ -802,7 +847,7 @@ struct ins2d_batch {
   // two-level PCG (cg_precond = 2)
   bool two_level = false;
   // fast-diagonalization PCG (cg_precond = 3): per-instance factors, the
-  // 32 x 32 GEMM tile list, two work arrays
+  // 64 x 64 GEMM tile list, two work arrays
   bool fdm = false;
   std::vector<FDMHost> fh;
   // closed-form maps of the momentum kernels (uniform members), per member
@@ -810,7 +855,10 @@ struct ins2d_batch {
   std::vector<UniArrays> ua;
   std::vector<std::vector<unsigned char>> ufu, ufv;
   std::vector<Tile> gt;
-  Tile* dgt = nullptr;
+  // the GEMM tiles in two launch orders, longest inner dimension first (npx
+  // for the products with Vx, npy for those with Vy): the blocks of the
+  // largest members start first and the small ones fill in at the end
+  Tile *dgtx = nullptr, *dgty = nullptr;
   double *gw1 = nullptr, *gw2 = nullptr;
   std::vector<Tile> at;
   Tile* dat = nullptr;
@@ -1050,11 +1098,11 @@ void build_fdm_factors(ins2d_batch* b) {
 
 // z = A^-1 r for every active instance (four batched GEMM phases)
 void fdm_apply(ins2d_batch* b, const double* r, double* z) {
-  const dim3 g((unsigned)b->gt.size()), t(64 * kGemmWaves);
-  hipLaunchKernelGGL(k_bf_gemm<0>, g, t, 0, b->st, b->dinst, b->dgt, b->cg, r, b->gw1);
-  hipLaunchKernelGGL(k_bf_gemm<1>, g, t, 0, b->st, b->dinst, b->dgt, b->cg, b->gw1, b->gw2);
-  hipLaunchKernelGGL(k_bf_gemm<2>, g, t, 0, b->st, b->dinst, b->dgt, b->cg, b->gw2, b->gw1);
-  hipLaunchKernelGGL(k_bf_gemm<3>, g, t, 0, b->st, b->dinst, b->dgt, b->cg, b->gw1, z);
+  const dim3 g((unsigned)b->gt.size()), t(256);
+  hipLaunchKernelGGL(k_bf_gemm<0>, g, t, 0, b->st, b->dinst, b->dgtx, b->cg, r, b->gw1);
+  hipLaunchKernelGGL(k_bf_gemm<1>, g, t, 0, b->st, b->dinst, b->dgty, b->cg, b->gw1, b->gw2);
+  hipLaunchKernelGGL(k_bf_gemm<2>, g, t, 0, b->st, b->dinst, b->dgtx, b->cg, b->gw2, b->gw1);
+  hipLaunchKernelGGL(k_bf_gemm<3>, g, t, 0, b->st, b->dinst, b->dgty, b->cg, b->gw1, z);
 }
 
 // split tiles of member k (split_tiles for both components); false, with
@@ -1164,13 +1212,19 @@ void launch_tiles_rhs(ins2d_batch* b, const double* V, const double* P, int mode
 }
 void launch_tiles_div(ins2d_batch* b, const double* A, const double* B, const double* dt, double ca, double cb,
                       double yc, double os, double* f) {
-  hipLaunchKernelGGL(k_b_div, dim3((unsigned)b->pt.size()), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, A, B, dt, ca,
-                     cb, yc, os, f);
+  const dim3 g((unsigned)b->pt_tall.size()), t(TX, TY);
+  if (B)
+    hipLaunchKernelGGL(k_b_div<true>, g, t, 0, b->st, b->dinst, b->dpt_tall, A, B, dt, ca, cb, yc, os, f);
+  else
+    hipLaunchKernelGGL(k_b_div<false>, g, t, 0, b->st, b->dinst, b->dpt_tall, A, B, dt, ca, cb, yc, os, f);
 }
 void launch_tiles_proj(ins2d_batch* b, const double* Vn, const double* Vt, const double* dp, const double* dt,
                        double c, double* V) {
-  hipLaunchKernelGGL(k_b_proj, dim3((unsigned)b->uvt.size()), dim3(TX, TY), 0, b->st, b->dinst, b->duv, Vn, Vt, dp,
-                     dt, c, V);
+  const dim3 g((unsigned)b->uvt_tall.size()), t(TX, TY);
+  if (Vt)
+    hipLaunchKernelGGL(k_b_proj<true>, g, t, 0, b->st, b->dinst, b->duv_tall, Vn, Vt, dp, dt, c, V);
+  else
+    hipLaunchKernelGGL(k_b_proj<false>, g, t, 0, b->st, b->dinst, b->duv_tall, Vn, Vt, dp, dt, c, V);
 }
 
 // fast-diagonalization PCG iteration, head: q = A p, alpha, x and r, the
@@ -1411,7 +1465,9 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
       I.poff = poff;
       voff += P.NV;
       poff += P.Np;
-      append_velocity_tiles(I, k, TX, TY, b->uvt);
+      append_velocity_tiles(I, k, TX, kTallRows, b->uvt_tall);
+      for (int j = 0; j < I.npy; j += kTallRows)
+        for (int i = 0; i < I.npx; i += TX) b->pt_tall.push_back(Tile{k, 2, i, j});
       I.pt0 = (int)b->pt.size();
       for (int j = 0; j < I.npy; j += TY)
         for (int i = 0; i < I.npx; i += TX) b->pt.push_back(Tile{k, 2, i, j});
@@ -1425,8 +1481,8 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
       }
       I.at1 = (int)b->at.size();
       if (b->fdm)
-        for (int j = 0; j < I.npy; j += 32)
-          for (int i = 0; i < I.npx; i += 32) b->gt.push_back(Tile{k, 2, i, j});
+        for (int j = 0; j < I.npy; j += kG64)
+          for (int i = 0; i < I.npx; i += kG64) b->gt.push_back(Tile{k, 2, i, j});
       // samples default to zero
       BCSamples& S = b->samp[k];
       for (int s = 0; s < S_NSRC; ++s) S.s[s].assign(sample_len(P, s), 0.0);
@@ -1441,9 +1497,6 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
     b_upload_bc(b.get());
     b->dinst = balloc<BInst>(b.get(), n);
     BHIP(hipMemcpy(b->dinst, b->hinst.data(), n * sizeof(BInst), hipMemcpyHostToDevice));
-    xcd_order(b->uvt, 0, b->uvt.size());
-    b->duv = balloc<Tile>(b.get(), b->uvt.size());
-    BHIP(hipMemcpy(b->duv, b->uvt.data(), b->uvt.size() * sizeof(Tile), hipMemcpyHostToDevice));
     {
       std::vector<RTile> rs;
       std::vector<Tile> rg;
@@ -1452,6 +1505,12 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
     }
     b->dpt = balloc<Tile>(b.get(), b->pt.size());
     BHIP(hipMemcpy(b->dpt, b->pt.data(), b->pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    xcd_order(b->uvt_tall, 0, b->uvt_tall.size());
+    xcd_order(b->pt_tall, 0, b->pt_tall.size());
+    b->duv_tall = balloc<Tile>(b.get(), b->uvt_tall.size());
+    BHIP(hipMemcpy(b->duv_tall, b->uvt_tall.data(), b->uvt_tall.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    b->dpt_tall = balloc<Tile>(b.get(), b->pt_tall.size());
+    BHIP(hipMemcpy(b->dpt_tall, b->pt_tall.data(), b->pt_tall.size() * sizeof(Tile), hipMemcpyHostToDevice));
     if (b->two_level) {
       b->dat = balloc<Tile>(b.get(), b->at.size());
       BHIP(hipMemcpy(b->dat, b->at.data(), b->at.size() * sizeof(Tile), hipMemcpyHostToDevice));
@@ -1460,8 +1519,16 @@ int ins2d_batch_create(const ins2d_case_desc* descs, int n, ins2d_batch** out) {
       b->tl.ec = balloc<double>(b.get(), (size_t)coff);
     }
     if (b->fdm) {
-      b->dgt = balloc<Tile>(b.get(), b->gt.size());
-      BHIP(hipMemcpy(b->dgt, b->gt.data(), b->gt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+      for (int ph = 0; ph < 2; ++ph) {
+        std::vector<Tile> o = b->gt;
+        std::stable_sort(o.begin(), o.end(), [&](const Tile& x, const Tile& y) {
+          const BInst &X = b->hinst[x.inst], &Y = b->hinst[y.inst];
+          return ph == 0 ? X.npx > Y.npx : X.npy > Y.npy;
+        });
+        Tile*& d = ph == 0 ? b->dgtx : b->dgty;
+        d = balloc<Tile>(b.get(), o.size());
+        BHIP(hipMemcpy(d, o.data(), o.size() * sizeof(Tile), hipMemcpyHostToDevice));
+      }
       b->gw1 = balloc<double>(b.get(), b->Npt);
       b->gw2 = balloc<double>(b.get(), b->Npt);
     }

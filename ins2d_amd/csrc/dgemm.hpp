@@ -83,4 +83,105 @@ __device__ __forceinline__ void dgemm_tile(int M, int N, int K, const double* __
   }
 }
 
+// C = A B as above, LDS-tiled: a workgroup of 256 threads (4 waves) owns the
+// 64 x 64 tile (m0, n0) of C, each wave a 32 x 32 quarter (2 x 2 matrix-core
+// tiles).  K advances in stages of 16: every thread loads 4 values of the
+// 64 x 16 slice of A (a wave reads 64 consecutive rows of one column) and 4
+// of the 16 x 64 slice of B (a wave reads 16 consecutive k of 4 columns),
+// writes them to the LDS buffer the next stage reads, and the matrix-core
+// operands come from LDS (double-buffered: one barrier per stage).  Per
+// matrix instruction a wave issues half a memory load instead of one, and the
+// loads cover whole cache-line runs; the result tile is transposed through
+// LDS so that the stores of C run along its columns.  EPI as in dgemm_tile.
+constexpr int kG64 = 64, kG64K = 16;
+union Gemm64Smem {
+  struct {
+    double a[2][kG64K][kG64];       // a[buf][k][m]
+    double b[2][kG64][kG64K + 1];   // b[buf][n][k]
+  } s;
+  double c[kG64][kG64 + 1];         // c[n][m], the result tile
+};
+
+template <int EPI>
+__device__ __forceinline__ void dgemm_tile64(int M, int N, int K, const double* __restrict__ A, int lda,
+                                             const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc,
+                                             const double* __restrict__ lx, const double* __restrict__ ly, int nx0,
+                                             int ny0, int m0, int n0, Gemm64Smem& sm) {
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  // this thread's global load positions: A (row am, k rows ak + 4 r),
+  // B (k bk, columns bn + 16 r)
+  const int am = t & 63, ak = t >> 6;
+  const int bk = t & 15, bn = t >> 4;
+  const bool arow = m0 + am < M;
+  const double* Ap = A + (arow ? m0 + am : 0);
+  double ra[4], rb[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = k0 + ak + 4 * r;
+      ra[r] = (arow && k < K) ? Ap[(size_t)lda * k] : 0.0;
+      const int n = n0 + bn + 16 * r, kb = k0 + bk;
+      rb[r] = (n < N && kb < K) ? B[(size_t)ldb * n + kb] : 0.0;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      sm.s.a[buf][ak + 4 * r][am] = ra[r];
+      sm.s.b[buf][bn + 16 * r][bk] = rb[r];
+    }
+  };
+  double4_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+  const int nst = (K + kG64K - 1) / kG64K;
+  // a wave whose quarter lies wholly outside C (edge tiles of matrices whose
+  // sides are not multiples of 64) only loads and stores: its matrix
+  // instructions are skipped
+  const bool active = m0 + wm < M && n0 + wn < N;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int buf = st & 1;
+    if (st + 1 < nst) load((st + 1) * kG64K);
+    if (active) {
+#pragma unroll
+      for (int kk = 0; kk < kG64K; kk += 4) {
+        const double a0 = sm.s.a[buf][kk + lk][wm + lr], a1 = sm.s.a[buf][kk + lk][wm + 16 + lr];
+        const double b0 = sm.s.b[buf][wn + lr][kk + lk], b1 = sm.s.b[buf][wn + 16 + lr][kk + lk];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+    }
+    if (st + 1 < nst) store(buf ^ 1);
+    __syncthreads();
+  }
+  // result lane l, register r: row l / 16 + 4 r, column l % 16 of its tile
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sm.c[wn + 16 * b + lr][wm + 16 * a + lk + 4 * r] = acc[a][b][r];
+  __syncthreads();
+  const int gm = m0 + am;
+  if (gm >= M) return;
+#pragma unroll 4
+  for (int r = 0; r < kG64 / 4; ++r) {
+    const int nn = ak + 4 * r, gn = n0 + nn;
+    if (gn >= N) break;
+    double v = sm.c[nn][am];
+    if (EPI == 1) v = (gm == nx0 && gn == ny0) ? 0.0 : -v / (lx[gm] + ly[gn]);
+    C[(size_t)gm + (size_t)ldc * gn] = v;
+  }
+}
+
 }  // namespace ins2d

@@ -18,6 +18,19 @@ template <class T>
 __device__ __forceinline__ const __attribute__((address_space(1))) T* glb(const T* p) {
   return (const __attribute__((address_space(1))) T*)p;
 }
+// A record that no kernel writes, at a wave-uniform address (the operator
+// records of the batched kernels, reached through the instance table): read
+// through the constant address space, so its fields arrive by scalar loads
+// instead of one flat load per lane and field.
+// (The host pass of the compiler sees only the declaration's plain form.)
+template <class T>
+__device__ __forceinline__ T cst(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(4))) T*)p;
+#else
+  return *p;
+#endif
+}
 __device__ __forceinline__ double kron_at(const DKron& K, const double* __restrict__ in, int f, int g) {
   const int KY = K.Y.K, KX = K.X.K, ld = K.X.ncols;
   const int* cy = K.Y.col + (size_t)g * KY;
@@ -102,6 +115,40 @@ __device__ __forceinline__ double kron2_at2(const DKron& K, const double* __rest
     if (x2) s1 += x.w.y * w(r1 + x.c.y);
     acc += y.w.y * s1;
   }
+  return acc;
+}
+
+// Branch-free forms of kron2_at / kron2_at2: both stored entries of every row
+// are used (the host pads a row's second entry with the row's first column
+// and weight 0, so the padded term adds an exact zero), and with B as a
+// template flag there is no branch left: the loads of several points issue
+// together instead of waiting behind each point's branches.
+__device__ __forceinline__ double kron2_at_nb(const DKron& K, const double* __restrict__ in, int f, int g) {
+  const Row2 y = row2(K.Y, g), x = row2(K.X, f);
+  const unsigned ld = K.X.ncols;
+  const auto a = glb(in);
+  const unsigned r0 = (unsigned)y.c.x * ld, r1 = (unsigned)y.c.y * ld;
+  double s0 = x.w.x * a[r0 + (unsigned)x.c.x];
+  s0 += x.w.y * a[r0 + (unsigned)x.c.y];
+  double s1 = x.w.x * a[r1 + (unsigned)x.c.x];
+  s1 += x.w.y * a[r1 + (unsigned)x.c.y];
+  double acc = y.w.x * s0;
+  acc += y.w.y * s1;
+  return acc;
+}
+template <bool HASB>
+__device__ __forceinline__ double kron2_at2_nb(const DKron& K, const double* __restrict__ A, double ca,
+                                               const double* __restrict__ B, double cb, int f, int g) {
+  const Row2 y = row2(K.Y, g), x = row2(K.X, f);
+  const unsigned ld = K.X.ncols;
+  auto w = [&](unsigned o) { return HASB ? ca * glb(A)[o] + cb * glb(B)[o] : ca * glb(A)[o]; };
+  const unsigned r0 = (unsigned)y.c.x * ld, r1 = (unsigned)y.c.y * ld;
+  double s0 = x.w.x * w(r0 + (unsigned)x.c.x);
+  s0 += x.w.y * w(r0 + (unsigned)x.c.y);
+  double s1 = x.w.x * w(r1 + (unsigned)x.c.x);
+  s1 += x.w.y * w(r1 + (unsigned)x.c.y);
+  double acc = y.w.x * s0;
+  acc += y.w.y * s1;
   return acc;
 }
 
