@@ -73,10 +73,10 @@ __global__ void __launch_bounds__(256) k_b_rhs_closed(const BInst* __restrict__ 
   const double* p = P + B.poff;
   if ((t.comp & 1) == 0) {
     const long o = B.voff;
-    rhs_closed_point<0>(*B.cu, u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
+    rhs_closed_point<0>(cst(B.cu), u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
   } else {
     const long o = B.voff + B.nu;
-    rhs_closed_point<1>(*B.cv, u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
+    rhs_closed_point<1>(cst(B.cv), u, v, p, mode, out ? out + o : nullptr, sc, vtemp ? vtemp + o : nullptr, o, i, j);
   }
 }
 

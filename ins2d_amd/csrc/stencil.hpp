@@ -19,14 +19,21 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* glb(const 
   return (const __attribute__((address_space(1))) T*)p;
 }
 // A record that no kernel writes, at a wave-uniform address (the operator
-// records of the batched kernels, reached through the instance table): read
-// through the constant address space, so its fields arrive by scalar loads
-// instead of one flat load per lane and field.
-// (The host pass of the compiler sees only the declaration's plain form.)
+// records of the batched kernels, reached through the instance table): copied
+// word by word through the constant address space, so its fields arrive by
+// scalar loads instead of one flat load per lane and field (a plain copy
+// would bind the record to a generic reference and load it through flat
+// instructions).  The host pass of the compiler sees the plain copy.
 template <class T>
 __device__ __forceinline__ T cst(const T* p) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return *(const __attribute__((address_space(4))) T*)p;
+  static_assert(sizeof(T) % 8 == 0, "record size");
+  T v;
+  const auto* src = (const __attribute__((address_space(4))) unsigned long long*)p;
+  auto* dst = reinterpret_cast<unsigned long long*>(&v);
+#pragma unroll
+  for (unsigned k = 0; k < sizeof(T) / 8; ++k) dst[k] = src[k];
+  return v;
 #else
   return *p;
 #endif
