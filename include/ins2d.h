@@ -248,8 +248,15 @@ int ins2d_nn_init(ins2d_nn* n, unsigned long long seed, double out_scale);
 /* host arrays of H*W doubles (xc optional) */
 int ins2d_nn_forward(ins2d_nn* n, const double* f, const double* xc, double* p);
 /* device arrays, queued on the given HIP stream (NULL: the null stream); xc
- * optional and may be p itself */
+ * optional and may be p itself.  Inference form: 14 kernel launches, the input
+ * scaling, pooling, upsampling and the output layer fused into the
+ * convolutions */
 int ins2d_nn_forward_dev(ins2d_nn* n, const double* f, const double* xc, double* p, void* stream);
+/* the same map through the training forward (input tensor, pooled and
+ * upsampled tensors and the 1x1 output layer stored separately); equal to the
+ * fused form up to FP32 summation order */
+int ins2d_nn_forward_unfused(ins2d_nn* n, const double* f, const double* xc, double* p);
+int ins2d_nn_forward_unfused_dev(ins2d_nn* n, const double* f, const double* xc, double* p, void* stream);
 /* mean-squared-error loss of net(f_b / rms(f_b), xc_b / (out_scale rms(f_b)))
  * against (p_b - xc_b) / (out_scale rms(f_b)) over a batch of device arrays
  * (xc NULL, or an entry NULL: zero); gradients into the gradient vector;

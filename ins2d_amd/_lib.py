@@ -64,6 +64,8 @@ SIGNATURES = {
     "ins2d_nn_init": (c_int, [c_void_p, ctypes.c_ulonglong, c_double]),
     "ins2d_nn_forward": (c_int, [c_void_p, P_double, P_double, P_double]),
     "ins2d_nn_forward_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ins2d_nn_forward_unfused": (c_int, [c_void_p, P_double, P_double, P_double]),
+    "ins2d_nn_forward_unfused_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ins2d_nn_train_step": (c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
                                     ctypes.POINTER(c_void_p), c_int, ctypes.c_float, c_int, P_double, c_void_p]),
     "ins2d_nn_get_grad": (c_int, [c_void_p, P_float]),

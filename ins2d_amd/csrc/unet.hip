@@ -33,94 +33,285 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 
-// one input of a convolution (concatenation = two inputs)
-struct In {
+// Input of a convolution: channels [0, a.c) from a, [a.c, a.c + b.c) from b,
+// zeros up to the layer's cin.  Each source is read on the convolution's grid
+// (H x W) in one of four ways:
+enum : int {
+  SRC_DIRECT = 0,   // same grid
+  SRC_POOL = 1,     // a 2H x 2W tensor, 2x2 average rounded to bf16
+  SRC_UP = 2,       // an H/2 x W/2 tensor, nearest neighbour
+  SRC_F64 = 3       // one fp64 field, times 1 / (div * s), rounded to bf16
+};
+struct CvSrc {
   const void* p;
   int c;        // channels
-  int fp32;     // 1: float storage, 0: bf16
+  int fp32;     // storage of DIRECT/POOL/UP sources: 1 float, 0 bf16
+  int mode;     // SRC_*
+  double div;   // SRC_F64 divisor (besides s)
 };
 
 struct ConvArgs {
-  In a, b;              // channels [0, a.c) from a, [a.c, a.c + b.c) from b
+  CvSrc a, b;
+  int cin;              // input channels of the weights (>= a.c + b.c; the rest are zero)
   int H, W, cout, ks;   // ks 3 (padding 1) or 1
   const uint16_t* w;    // [taps][cout][cin] bf16
   const float* bias;    // cout (or null)
-  void* out;
+  void* out;            // [H][W][cout], bf16 or fp32 (null with the fused output layer)
   int out_fp32;
   int relu;
+  // s = sqrt(sum(sq[0..nsq)) / npix_s): the input scale of SRC_F64 sources and of the fused output
+  const double* sq;
+  int nsq;
+  long npix_s;
+  // fused 1x1 output layer (cout 16): y = b1 + sum_c bf16(act_c) w1[c], out64 = base + oscale * s * y
+  const uint16_t* w1;
+  const float* b1;
+  double* out64;
+  const double* base;
+  double oscale;
 };
 
-__device__ __forceinline__ float load_ch(const In& s, long pix, int c) {
+// Tiles: a block of 4 waves covers 4 rows x 16 MT pixels, wave w the row
+// y0 + w (MT MFMA row tiles of 16 pixels) times 16 NT output channels
+// (16 NT * blockIdx.z + [0, 16 NT)).  The input rows y0-1 .. y0+4 and columns
+// x0-1 .. x0+16 MT are staged in LDS as bf16, 32 channels at a time, with a
+// per-pixel stride of 36 channels (the 16 pixels a row tile reads land in
+// distinct banks); the chunk's weights are staged beside them (per output
+// channel the same stride).  Each thread issues all its staging loads before
+// its LDS stores, so a chunk's loads are in flight together.
+constexpr int CV_TY = 4, CV_KC = 32, CV_CP = 36;
+__device__ __forceinline__ float load_ch(const CvSrc& s, long pix, int c) {
   if (s.fp32) return ((const float*)s.p)[pix * s.c + c];
   return bf2f(((const uint16_t*)s.p)[pix * s.c + c]);
 }
 
+__device__ __forceinline__ float src_val(const CvSrc& S, int H, int W, int yy, int xx, int c) {
+  if (S.mode == SRC_POOL) {
+    const int W2 = 2 * W;
+    const long b = ((long)(2 * yy) * W2 + 2 * xx) * S.c + c;
+    const long C = S.c, R = (long)W2 * S.c;
+    float s;
+    if (S.fp32) {
+      const float* q = (const float*)S.p;
+      s = q[b] + q[b + C] + q[b + R] + q[b + R + C];
+    } else {
+      const uint16_t* q = (const uint16_t*)S.p;
+      s = bf2f(q[b]) + bf2f(q[b + C]) + bf2f(q[b + R]) + bf2f(q[b + R + C]);
+    }
+    return bf2f(f2bf(0.25f * s));
+  }
+  long pix;
+  if (S.mode == SRC_UP) pix = (long)(yy >> 1) * (W >> 1) + (xx >> 1);
+  else pix = (long)yy * W + xx;
+  const long o = pix * S.c + c;
+  return S.fp32 ? ((const float*)S.p)[o] : bf2f(((const uint16_t*)S.p)[o]);
+}
+
+// Staging kinds (a template parameter of the convolution, so that each kernel
+// instance holds only the loads it needs):
+enum : int {
+  SK_ANY = 0,    // any sources (fp32 and odd channel counts: the backward pass, layer checks)
+  SK_BF16 = 1,   // bf16 sources with 4k channels, DIRECT or UP
+  SK_POOL = 2,   // one bf16 POOL source with 4k channels
+  SK_F64 = 3     // SRC_F64 sources (the network input)
+};
+
+// the 4 channels c .. c+3 (c a multiple of 4) at (yy, xx), as bf16; zero outside
+// the grid and beyond the sources' channels
+template <int SK>
+__device__ __forceinline__ short4_t stage_quad(const ConvArgs& A, int yy, int xx, int c, double inv) {
+  short4_t v = short4_t{0, 0, 0, 0};
+  if (yy < 0 || yy >= A.H || xx < 0 || xx >= A.W) return v;
+  if (SK == SK_F64) {   // channel 0 from a, channel 1 from b (if any)
+    if (c == 0) {
+      const long pix = (long)yy * A.W + xx;
+      v[0] = (short)f2bf((float)(((const double*)A.a.p)[pix] * (inv / A.a.div)));
+      if (A.b.p) v[1] = (short)f2bf((float)(((const double*)A.b.p)[pix] * (inv / A.b.div)));
+    }
+    return v;
+  }
+  const int nc = A.a.c + A.b.c;
+  if (c >= nc) return v;
+  const bool ina = c < A.a.c;
+  const CvSrc& S = ina ? A.a : A.b;
+  const int cc = ina ? c : c - A.a.c;
+  if (SK == SK_BF16) {
+    const uint16_t* q = (const uint16_t*)S.p;
+    const long pix = S.mode == SRC_UP ? (long)(yy >> 1) * (A.W >> 1) + (xx >> 1) : (long)yy * A.W + xx;
+    return *(const short4_t*)(q + pix * S.c + cc);
+  }
+  if (SK == SK_POOL) {
+    const uint16_t* q = (const uint16_t*)S.p;
+    const long C = S.c, R = 2L * A.W * S.c;
+    const long b = ((long)(2 * yy) * (2 * A.W) + 2 * xx) * S.c + cc;
+    const short4_t v00 = *(const short4_t*)(q + b), v01 = *(const short4_t*)(q + b + C);
+    const short4_t v10 = *(const short4_t*)(q + b + R), v11 = *(const short4_t*)(q + b + R + C);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float s = bf2f((uint16_t)v00[i]) + bf2f((uint16_t)v01[i]) + bf2f((uint16_t)v10[i]) + bf2f((uint16_t)v11[i]);
+      v[i] = (short)f2bf(0.25f * s);
+    }
+    return v;
+  }
+  if (S.mode == SRC_DIRECT && S.fp32 && (S.c & 3) == 0) {
+    const float4_t f = *(const float4_t*)((const float*)S.p + ((long)yy * A.W + xx) * S.c + cc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = (short)f2bf(f[i]);
+    return v;
+  }
+  for (int i = 0; i < 4; ++i) {
+    const int ci = c + i;
+    if (ci >= nc) break;
+    const bool ia = ci < A.a.c;
+    v[i] = (short)f2bf(src_val(ia ? A.a : A.b, A.H, A.W, yy, xx, ia ? ci : ci - A.a.c));
+  }
+  return v;
+}
+
 // out[y][x][co] = act(bias[co] + sum_t sum_ci in[y+dy][x+dx][ci] w[t][co][ci])
-// grid (ceil(W/64), H, ceil(cout/64)); 256 threads = 4 waves; wave w owns
-// 16 pixels x 64 output channels (channels 64 * blockIdx.z + [0, 64))
+// grid (ceil(W / (16 MT)), ceil(H / 4), ceil(cout / (16 NT))), 256 threads
+template <int NT, int MT, int SK>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs A) {
+  constexpr int TW = 16 * MT;
+  constexpr int TP = (CV_TY + 2) * (TW + 2);   // staged pixels (ks 3)
+  constexpr int QA = (TP * 8 + 255) / 256;     // input quads per thread
+  constexpr int WR = 9 * 16 * NT;              // staged weight rows (tap, output channel)
+  constexpr int QW = (WR * 8 + 255) / 256;     // weight quads per thread
+  __shared__ __align__(16) uint16_t tile[TP * CV_CP];
+  __shared__ __align__(16) uint16_t wsh[WR * CV_CP];
+  __shared__ double red[16];
+  __shared__ double s_sh;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int y = blockIdx.y;
-  const int xw = blockIdx.x * 64 + wave * 16;
-  const int cin = A.a.c + A.b.c;
-  const int cob = 64 * blockIdx.z;
-  const int ntile = min(4, (A.cout - cob + 15) / 16);
-  float4_t acc[4];
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * CV_TY;
+  const int cob = 16 * NT * blockIdx.z;
+  const int r = A.ks / 2, taps = A.ks * A.ks;
+  const int tw = TW + 2 * r, th = CV_TY + 2 * r;
+  double s = 1.0;
+  if (A.sq) {
+    const double t = block_sum((int)threadIdx.x < A.nsq ? A.sq[threadIdx.x] : 0.0, red);
+    if (threadIdx.x == 0) s_sh = sqrt(t / (double)A.npix_s);
+    __syncthreads();
+    s = s_sh;
+  }
+  const double inv = 1.0 / fmax(s, 1e-300);
+  const int kq = 4 * (lane >> 4);
+  const int m = lane & 15;
+  float4_t acc[MT][NT];
 #pragma unroll
-  for (int n = 0; n < 4; ++n) acc[n] = float4_t{0.f, 0.f, 0.f, 0.f};
-  const int taps = A.ks * A.ks;
-  const int r = A.ks / 2;
-  const int xa = xw + (lane & 15);     // A-operand pixel of this lane
-  const int kq = 4 * (lane >> 4);      // K sub-index of this lane
-  for (int t = 0; t < taps; ++t) {
-    const int yy = y + t / A.ks - r, xx = xa + t % A.ks - r;
-    const bool inb = yy >= 0 && yy < A.H && xx >= 0 && xx < A.W;
-    const long pix = (long)yy * A.W + xx;
-    for (int c0 = 0; c0 < cin; c0 += 16) {
-      short4_t av;
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int ci = c0 + kq + i;
-        float v = 0.f;
-        if (inb && ci < cin) v = ci < A.a.c ? load_ch(A.a, pix, ci) : load_ch(A.b, pix, ci - A.a.c);
-        av[i] = (short)f2bf(v);
+    for (int n = 0; n < NT; ++n) acc[mt][n] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const bool wvec = (A.cin & 3) == 0;
+  for (int kc = 0; kc < A.cin; kc += CV_KC) {
+    const int nsub = min(2, (A.cin - kc + 15) / 16);
+    const int lq = nsub + 1;                    // log2 of the quads staged per pixel (4 or 8)
+    const int nq = (th * tw) << lq;
+    const int nw = (taps * 16 * NT) << lq;
+    short4_t ba[QA], bw[QW];
+#pragma unroll
+    for (int k = 0; k < QA; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      ba[k] = short4_t{0, 0, 0, 0};
+      if (q < nq) {
+        const int px = q >> lq, cq = q & ((1 << lq) - 1);
+        const int ty = px / tw, tx = px - ty * tw;
+        ba[k] = stage_quad<SK>(A, y0 - r + ty, x0 - r + tx, kc + 4 * cq, inv);
       }
+    }
 #pragma unroll
-      for (int n = 0; n < 4; ++n) {
-        if (n >= ntile) break;
-        const int co = cob + 16 * n + (lane & 15);
-        short4_t bv;
+    for (int k = 0; k < QW; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      bw[k] = short4_t{0, 0, 0, 0};
+      if (q < nw) {
+        const int row = q >> lq, cq = q & ((1 << lq) - 1);
+        const int t = row / (16 * NT), co = cob + row - t * (16 * NT);
+        const int ci = kc + 4 * cq;
+        if (co < A.cout && ci < A.cin) {
+          const uint16_t* wp = A.w + ((long)t * A.cout + co) * A.cin + ci;
+          if (wvec) bw[k] = *(const short4_t*)wp;
+          else
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int ci = c0 + kq + i;
-          bv[i] = (co < A.cout && ci < cin) ? (short)A.w[((long)t * A.cout + co) * cin + ci] : (short)0;
+            for (int i = 0; i < 4; ++i)
+              if (ci + i < A.cin) bw[k][i] = (short)wp[i];
         }
-        acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[n], 0, 0, 0);
+      }
+    }
+    if (kc) __syncthreads();   // the previous chunk's reads are done
+#pragma unroll
+    for (int k = 0; k < QA; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < nq) *(short4_t*)&tile[(q >> lq) * CV_CP + 4 * (q & ((1 << lq) - 1))] = ba[k];
+    }
+#pragma unroll
+    for (int k = 0; k < QW; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < nw) *(short4_t*)&wsh[(q >> lq) * CV_CP + 4 * (q & ((1 << lq) - 1))] = bw[k];
+    }
+    __syncthreads();
+    for (int t = 0; t < taps; ++t) {
+      const int dy = t / A.ks, dx = t - dy * A.ks;
+      for (int sub = 0; sub < nsub; ++sub) {
+        short4_t bv[NT];
+#pragma unroll
+        for (int n = 0; n < NT; ++n) bv[n] = *(const short4_t*)&wsh[(t * 16 * NT + 16 * n + m) * CV_CP + 16 * sub + kq];
+        const uint16_t* ap = &tile[((wave + dy) * tw + m + dx) * CV_CP + 16 * sub + kq];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          const short4_t av = *(const short4_t*)(ap + 16 * mt * CV_CP);
+#pragma unroll
+          for (int n = 0; n < NT; ++n) acc[mt][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[n], acc[mt][n], 0, 0, 0);
+        }
       }
     }
   }
+  const int y = y0 + wave;
+  if (A.w1) {   // fused output layer (NT 1, one z block): channel m of pixel x in lane (x, m)
+    const float bias = A.bias ? A.bias[m] : 0.f;
+    const float w1 = bf2f(A.w1[m]);
 #pragma unroll
-  for (int n = 0; n < 4; ++n) {
-    if (n >= ntile) break;
-    const int co = cob + 16 * n + (lane & 15);
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float v = acc[mt][0][i] + bias;
+        if (A.relu) v = fmaxf(v, 0.f);
+        float pr = bf2f(f2bf(v)) * w1;
+        pr += __shfl_xor(pr, 1, 64);
+        pr += __shfl_xor(pr, 2, 64);
+        pr += __shfl_xor(pr, 4, 64);
+        pr += __shfl_xor(pr, 8, 64);
+        const int x = x0 + 16 * mt + kq + i;
+        if (m == 0 && y < A.H && x < A.W) {
+          const long o = (long)y * A.W + x;
+          const float yv = pr + (A.b1 ? A.b1[0] : 0.f);
+          A.out64[o] = (A.base ? A.base[o] : 0.0) + A.oscale * s * (double)yv;
+        }
+      }
+    return;
+  }
+  if (y >= A.H) return;
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    const int co = cob + 16 * n + m;
     if (co >= A.cout) continue;
     const float bias = A.bias ? A.bias[co] : 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int x = xw + 4 * (lane >> 4) + i;
-      if (x >= A.W) continue;
-      float v = acc[n][i] + bias;
-      if (A.relu) v = fmaxf(v, 0.f);
-      const long o = ((long)y * A.W + x) * A.cout + co;
-      if (A.out_fp32) ((float*)A.out)[o] = v;
-      else ((uint16_t*)A.out)[o] = f2bf(v);
-    }
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = x0 + 16 * mt + kq + i;
+        if (x >= A.W) continue;
+        float v = acc[mt][n][i] + bias;
+        if (A.relu) v = fmaxf(v, 0.f);
+        const long o = ((long)y * A.W + x) * A.cout + co;
+        if (A.out_fp32) ((float*)A.out)[o] = v;
+        else ((uint16_t*)A.out)[o] = f2bf(v);
+      }
   }
 }
 
 // dw[t][co][ci] += sum_p g[p][co] in[p + off_t][ci]   (fp32 atomics)
 // grid (chunks, taps, tiles_co * tiles_ci); 64 threads
-__global__ void __launch_bounds__(64) k_conv_dw(In a, In b, const float* __restrict__ g, int H, int W, int cout,
+__global__ void __launch_bounds__(64) k_conv_dw(CvSrc a, CvSrc b, const float* __restrict__ g, int H, int W, int cout,
                                                int ks, int chunk, float* __restrict__ dw) {
   const int lane = threadIdx.x;
   const int cin = a.c + b.c;
@@ -277,13 +468,21 @@ __global__ void k_output(const float* __restrict__ y, const double* xc, long n, 
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     p[i] = (xc ? xc[i] : 0.0) + scale * s[0] * (double)y[i];
 }
-// s = sqrt(mean(f^2)) (single block)
-__global__ void k_rms(const double* __restrict__ f, long n, double* __restrict__ s) {
+// partial sums of f^2 over kSq blocks of 256 threads
+constexpr int kSq = 256;
+__global__ void k_sumsq(const double* __restrict__ f, long n, double* __restrict__ part) {
   __shared__ double red[16];
   double a = 0.0;
-  for (long i = threadIdx.x; i < n; i += blockDim.x) a += f[i] * f[i];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a += f[i] * f[i];
   a = block_sum(a, red);
-  if (threadIdx.x == 0) s[0] = sqrt(a / (double)n);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+// s = sqrt(sum(part) / n) (one block of 256 threads: the same sum the
+// convolution kernels form from the partials)
+__global__ void k_rms(const double* __restrict__ part, int np, long n, double* __restrict__ s) {
+  __shared__ double red[16];
+  const double t = block_sum((int)threadIdx.x < np ? part[threadIdx.x] : 0.0, red);
+  if (threadIdx.x == 0) s[0] = sqrt(t / (double)n);
 }
 // MSE loss of y against the target (t - xc) / (scale s) (xc optional):
 // dy = 2 (y - tt) / n; loss += sum (y - tt)^2 / n
@@ -377,6 +576,7 @@ struct ins2d_nn {
   uint16_t *pool1 = nullptr, *pool2 = nullptr, *pool3 = nullptr, *up3 = nullptr, *up2 = nullptr, *up1 = nullptr;
   float *g0 = nullptr, *g1 = nullptr, *gs1 = nullptr, *gs2 = nullptr, *gs3 = nullptr, *gtmp = nullptr;
   double* scale = nullptr;   // rms of the current input
+  double* sq = nullptr;      // partial sums of the input's squares
   double* cscale = nullptr;  // rms of the two sources of a combined pair
   double* loss = nullptr;
   double out_scale = 1.0;    // p = out_scale * rms(f) * y
@@ -404,27 +604,74 @@ T* nalloc(ins2d_nn* n, size_t count) {
 
 inline unsigned nb(long n) { return (unsigned)std::min<long>((n + 255) / 256, 8192); }
 
-void conv(hipStream_t st, ins2d_nn* N, int l, In a, In b, void* out, int out_fp32, const uint16_t* w,
-          const float* bias, int relu, int cout_override = -1) {
+// tile shape per launch: the largest pixel and channel tiles that still give
+// about 1024 blocks (the coarse levels of the network are small grids)
+template <int NT, int SK>
+void launch_conv_nt(hipStream_t st, const ConvArgs& A, int mt) {
+  const dim3 g((A.W + 16 * mt - 1) / (16 * mt), (A.H + CV_TY - 1) / CV_TY, (A.cout + 16 * NT - 1) / (16 * NT));
+  if (mt == 4) hipLaunchKernelGGL((k_conv<NT, 4, SK>), g, dim3(256), 0, st, A);
+  else if (mt == 2) hipLaunchKernelGGL((k_conv<NT, 2, SK>), g, dim3(256), 0, st, A);
+  else hipLaunchKernelGGL((k_conv<NT, 1, SK>), g, dim3(256), 0, st, A);
+}
+template <int SK>
+void launch_conv_sk(hipStream_t st, const ConvArgs& A, int nt, int mt) {
+  if (nt == 1) launch_conv_nt<1, SK>(st, A, mt);
+  else if (nt == 2) launch_conv_nt<2, SK>(st, A, mt);
+  else launch_conv_nt<4, SK>(st, A, mt);
+}
+int staging_kind(const ConvArgs& A) {
+  if (A.a.mode == SRC_F64) return SK_F64;
+  auto bf = [](const CvSrc& s) { return s.c == 0 || (!s.fp32 && (s.c & 3) == 0); };
+  if (!bf(A.a) || !bf(A.b)) return SK_ANY;
+  if (A.a.mode == SRC_POOL && A.b.c == 0) return SK_POOL;
+  if (A.a.mode != SRC_POOL && (A.b.c == 0 || A.b.mode != SRC_POOL)) return SK_BF16;
+  return SK_ANY;
+}
+void launch_conv(hipStream_t st, const ConvArgs& A) {
+  const long rows = (A.H + CV_TY - 1) / CV_TY;
+  int nt = std::min(4, (A.cout + 15) / 16);
+  if (nt == 3) nt = 4;
+  int mt = 4;
+  auto blocks = [&]() { return ((A.W + 16 * mt - 1) / (16 * mt)) * rows * ((A.cout + 16 * nt - 1) / (16 * nt)); };
+  if (A.w1) nt = 1;   // the fused output layer reads all 16 channels of a pixel in one block
+  while (blocks() < 1024 && nt > 1) nt /= 2;
+  while (blocks() < 1024 && mt > 1) mt /= 2;
+  switch (staging_kind(A)) {
+    case SK_F64: launch_conv_sk<SK_F64>(st, A, nt, mt); break;
+    case SK_BF16: launch_conv_sk<SK_BF16>(st, A, nt, mt); break;
+    case SK_POOL: launch_conv_sk<SK_POOL>(st, A, nt, mt); break;
+    default: launch_conv_sk<SK_ANY>(st, A, nt, mt); break;
+  }
+}
+
+ConvArgs layer_args(ins2d_nn* N, int l, CvSrc a, CvSrc b, void* out, int out_fp32, const uint16_t* w, const float* bias,
+                    int relu) {
   const LayerDef& L = kLayers[l];
-  ConvArgs A;
+  ConvArgs A{};
   A.a = a;
   A.b = b;
+  A.cin = L.cin_a + L.cin_b;
   A.H = N->Hl[L.level];
   A.W = N->Wl[L.level];
-  A.cout = cout_override > 0 ? cout_override : L.cout;
+  A.cout = L.cout;
   A.ks = L.ks;
   A.w = w;
   A.bias = bias;
   A.out = out;
   A.out_fp32 = out_fp32;
   A.relu = relu;
-  hipLaunchKernelGGL(k_conv, dim3((A.W + 63) / 64, A.H, (A.cout + 63) / 64), dim3(256), 0, st, A);
+  return A;
 }
 
-In bfin(const uint16_t* p, int c) { return In{p, c, 0}; }
-In f32in(const float* p, int c) { return In{p, c, 1}; }
-In none() { return In{nullptr, 0, 0}; }
+void conv(hipStream_t st, ins2d_nn* N, int l, CvSrc a, CvSrc b, void* out, int out_fp32, const uint16_t* w,
+          const float* bias, int relu) {
+  launch_conv(st, layer_args(N, l, a, b, out, out_fp32, w, bias, relu));
+}
+
+CvSrc bfin(const uint16_t* p, int c, int mode = SRC_DIRECT) { return CvSrc{p, c, 0, mode, 1.0}; }
+CvSrc f32in(const float* p, int c) { return CvSrc{p, c, 1, SRC_DIRECT, 1.0}; }
+CvSrc f64in(const double* p, double div) { return CvSrc{p, p ? 1 : 0, 0, SRC_F64, div}; }
+CvSrc none() { return CvSrc{nullptr, 0, 0, SRC_DIRECT, 1.0}; }
 
 void refresh_weights(hipStream_t st, ins2d_nn* N) {
   for (int l = 0; l < kNL; ++l) {
@@ -436,7 +683,8 @@ void refresh_weights(hipStream_t st, ins2d_nn* N) {
   N->weights_ready = true;
 }
 
-// forward pass on a prepared input tensor N->x0; output in N->out (fp32)
+// forward pass on a prepared input tensor N->x0; output in N->out (fp32); the
+// pooled and upsampled tensors are stored for the backward pass
 void forward(hipStream_t st, ins2d_nn* N) {
   uint16_t** A = N->act;
   auto W = [&](int l) { return N->wbf + N->woff[l]; };
@@ -464,6 +712,45 @@ void forward(hipStream_t st, ins2d_nn* N) {
   conv(st, N, 14, bfin(A[13], 16), none(), N->out, 1, W(14), B(14), 0);
 }
 
+// inference: p = xc + out_scale s U(f / s, xc / (out_scale s)) with s = rms(f),
+// 14 launches: the input scaling is done by the first convolution's staging,
+// pooling and upsampling by the staging of the convolutions that read them, and
+// the 1x1 output layer and the fp64 output by the epilogue of the last 3x3 one
+void forward_fused(hipStream_t st, ins2d_nn* N, const double* f, const double* xc, double* p) {
+  uint16_t** A = N->act;
+  auto W = [&](int l) { return N->wbf + N->woff[l]; };
+  auto B = [&](int l) { return N->param + N->boff[l]; };
+  const long n = (long)N->H * N->W;
+  hipLaunchKernelGGL(k_sumsq, dim3(kSq), dim3(256), 0, st, f, n, N->sq);
+  ConvArgs a0 = layer_args(N, 0, f64in(f, 1.0), f64in(xc, N->out_scale), A[0], 0, W(0), B(0), 1);
+  a0.sq = N->sq;
+  a0.nsq = kSq;
+  a0.npix_s = n;
+  launch_conv(st, a0);
+  conv(st, N, 1, bfin(A[0], 16), none(), A[1], 0, W(1), B(1), 1);
+  conv(st, N, 2, bfin(A[1], 16, SRC_POOL), none(), A[2], 0, W(2), B(2), 1);
+  conv(st, N, 3, bfin(A[2], 32), none(), A[3], 0, W(3), B(3), 1);
+  conv(st, N, 4, bfin(A[3], 32, SRC_POOL), none(), A[4], 0, W(4), B(4), 1);
+  conv(st, N, 5, bfin(A[4], 64), none(), A[5], 0, W(5), B(5), 1);
+  conv(st, N, 6, bfin(A[5], 64, SRC_POOL), none(), A[6], 0, W(6), B(6), 1);
+  conv(st, N, 7, bfin(A[6], 64), none(), A[7], 0, W(7), B(7), 1);
+  conv(st, N, 8, bfin(A[7], 64, SRC_UP), bfin(A[5], 64), A[8], 0, W(8), B(8), 1);
+  conv(st, N, 9, bfin(A[8], 64), none(), A[9], 0, W(9), B(9), 1);
+  conv(st, N, 10, bfin(A[9], 64, SRC_UP), bfin(A[3], 32), A[10], 0, W(10), B(10), 1);
+  conv(st, N, 11, bfin(A[10], 32), none(), A[11], 0, W(11), B(11), 1);
+  conv(st, N, 12, bfin(A[11], 32, SRC_UP), bfin(A[1], 16), A[12], 0, W(12), B(12), 1);
+  ConvArgs a13 = layer_args(N, 13, bfin(A[12], 16), none(), nullptr, 0, W(13), B(13), 1);
+  a13.sq = N->sq;
+  a13.nsq = kSq;
+  a13.npix_s = n;
+  a13.w1 = W(14);
+  a13.b1 = B(14);
+  a13.out64 = p;
+  a13.base = xc;
+  a13.oscale = N->out_scale;
+  launch_conv(st, a13);
+}
+
 // backward pass for the loss gradient dy (N->g0, fp32, 1 channel at level 0);
 // accumulates parameter gradients into N->grad
 void backward(hipStream_t st, ins2d_nn* N) {
@@ -474,7 +761,7 @@ void backward(hipStream_t st, ins2d_nn* N) {
   const long npx[4] = {(long)N->Hl[0] * N->Wl[0], (long)N->Hl[1] * N->Wl[1], (long)N->Hl[2] * N->Wl[2],
                        (long)N->Hl[3] * N->Wl[3]};
   // dW, db of layer l given output gradient g (already through the activation)
-  auto wgrad = [&](int l, In a, In b, const float* g) {
+  auto wgrad = [&](int l, CvSrc a, CvSrc b, const float* g) {
     const LayerDef& L = kLayers[l];
     const int lv = L.level;
     const int chunk = 4096;
@@ -487,19 +774,18 @@ void backward(hipStream_t st, ins2d_nn* N) {
   // input gradient of layer l: din = conv(g, flipped W), no bias, fp32
   auto dgrad = [&](int l, const float* g, float* din) {
     const LayerDef& L = kLayers[l];
-    ConvArgs C;
+    ConvArgs C{};
     C.a = f32in(g, L.cout);
     C.b = none();
+    C.cin = L.cout;
     C.H = N->Hl[L.level];
     C.W = N->Wl[L.level];
     C.cout = L.cin_a + L.cin_b;
     C.ks = L.ks;
     C.w = WT(l);
-    C.bias = nullptr;
     C.out = din;
     C.out_fp32 = 1;
-    C.relu = 0;
-    hipLaunchKernelGGL(k_conv, dim3((C.W + 63) / 64, C.H, (C.cout + 63) / 64), dim3(256), 0, st, C);
+    launch_conv(st, C);
   };
   auto relub = [&](float* g, const uint16_t* out, long n) {
     hipLaunchKernelGGL(k_relu_bwd, dim3(nb(n)), dim3(256), 0, st, g, out, n, g);
@@ -569,7 +855,8 @@ void backward(hipStream_t st, ins2d_nn* N) {
 
 void prepare_input(hipStream_t st, ins2d_nn* N, const double* f, const double* xc) {
   const long n = (long)N->H * N->W;
-  hipLaunchKernelGGL(k_rms, dim3(1), dim3(1024), 0, st, f, n, N->scale);
+  hipLaunchKernelGGL(k_sumsq, dim3(kSq), dim3(256), 0, st, f, n, N->sq);
+  hipLaunchKernelGGL(k_rms, dim3(1), dim3(256), 0, st, N->sq, kSq, n, N->scale);
   hipLaunchKernelGGL(k_input, dim3(nb(n)), dim3(256), 0, st, f, xc, n, N->scale, N->out_scale, N->x0);
 }
 
@@ -628,6 +915,7 @@ int ins2d_nn_create(int H, int W, ins2d_nn** out) {
     N->gs2 = nalloc<float>(N.get(), (p0 >> 2) * 32);
     N->gs3 = nalloc<float>(N.get(), (p0 >> 4) * 64);
     N->scale = nalloc<double>(N.get(), 1);
+    N->sq = nalloc<double>(N.get(), kSq);
     N->cscale = nalloc<double>(N.get(), 2);
     N->loss = nalloc<double>(N.get(), 1);
     NNCHK(hipMemset(N->param, 0, sizeof(float) * off));
@@ -724,6 +1012,20 @@ int ins2d_nn_forward_dev(ins2d_nn* N, const double* f, const double* xc, double*
   if (!N || !f || !p) return INS2D_E_ARG;
   hipStream_t st = (hipStream_t)stream;
   try {
+    forward_fused(st, N, f, xc, p);
+    NNCHK(hipGetLastError());
+  } catch (const std::exception& e) {
+    return nn_fail(N, e);
+  }
+  return INS2D_OK;
+}
+
+// the same output through the training forward (stored input tensor, pooled
+// and upsampled tensors, separate output layer)
+int ins2d_nn_forward_unfused_dev(ins2d_nn* N, const double* f, const double* xc, double* p, void* stream) {
+  if (!N || !f || !p) return INS2D_E_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  try {
     const long n = (long)N->H * N->W;
     prepare_input(st, N, f, xc);
     forward(st, N);
@@ -736,7 +1038,7 @@ int ins2d_nn_forward_dev(ins2d_nn* N, const double* f, const double* xc, double*
 }
 
 // host-array forward (tests); xc optional
-int ins2d_nn_forward(ins2d_nn* N, const double* f, const double* xc, double* p) {
+static int nn_forward_host(ins2d_nn* N, const double* f, const double* xc, double* p, int unfused) {
   if (!N || !f || !p) return INS2D_E_ARG;
   try {
     const long n = (long)N->H * N->W;
@@ -748,7 +1050,7 @@ int ins2d_nn_forward(ins2d_nn* N, const double* f, const double* xc, double* p) 
       NNCHK(hipMalloc(&dx, n * sizeof(double)));
       NNCHK(hipMemcpy(dx, xc, n * sizeof(double), hipMemcpyHostToDevice));
     }
-    int rc = ins2d_nn_forward_dev(N, df, dx, dp, nullptr);
+    int rc = unfused ? ins2d_nn_forward_unfused_dev(N, df, dx, dp, nullptr) : ins2d_nn_forward_dev(N, df, dx, dp, nullptr);
     NNCHK(hipDeviceSynchronize());
     NNCHK(hipMemcpy(p, dp, n * sizeof(double), hipMemcpyDeviceToHost));
     (void)hipFree(df);
@@ -758,6 +1060,13 @@ int ins2d_nn_forward(ins2d_nn* N, const double* f, const double* xc, double* p) 
   } catch (const std::exception& e) {
     return nn_fail(N, e);
   }
+}
+
+int ins2d_nn_forward(ins2d_nn* N, const double* f, const double* xc, double* p) {
+  return nn_forward_host(N, f, xc, p, 0);
+}
+int ins2d_nn_forward_unfused(ins2d_nn* N, const double* f, const double* xc, double* p) {
+  return nn_forward_host(N, f, xc, p, 1);
 }
 
 // loss and parameter gradients (accumulated over a batch of device samples
@@ -869,8 +1178,10 @@ int ins2d_nn_data_combine(ins2d_nn* N, int dst, int i, int j, double a, double b
     const long n = (long)N->H * N->W;
     const double* si = N->data + 3 * n * i;
     const double* sj = N->data + 3 * n * j;
-    hipLaunchKernelGGL(k_rms, dim3(1), dim3(1024), 0, nullptr, si, n, N->cscale);
-    hipLaunchKernelGGL(k_rms, dim3(1), dim3(1024), 0, nullptr, sj, n, N->cscale + 1);
+    hipLaunchKernelGGL(k_sumsq, dim3(kSq), dim3(256), 0, nullptr, si, n, N->sq);
+    hipLaunchKernelGGL(k_rms, dim3(1), dim3(256), 0, nullptr, N->sq, kSq, n, N->cscale);
+    hipLaunchKernelGGL(k_sumsq, dim3(kSq), dim3(256), 0, nullptr, sj, n, N->sq);
+    hipLaunchKernelGGL(k_rms, dim3(1), dim3(256), 0, nullptr, N->sq, kSq, n, N->cscale + 1);
     hipLaunchKernelGGL(k_combine, dim3(nb(3 * n)), dim3(256), 0, nullptr, si, sj, 3 * n, (const double*)N->cscale, a,
                        b, N->data + 3 * n * dst);
     NNCHK(hipGetLastError());
@@ -945,9 +1256,10 @@ int ins2d_nn_layer_check(int H, int W, int ca, int cb, int cout, int ks, int rel
     NNCHK(hipMemset(ddw, 0, nw * 4));
     NNCHK(hipMemset(ddb, 0, cout * 4));
     hipLaunchKernelGGL(k_wflip, dim3(nb(nw)), dim3(256), 0, nullptr, dwf, ks * ks, cout, (int)cin, dwt, dwb);
-    ConvArgs A;
+    ConvArgs A{};
     A.a = f32in(dxa, ca);
     A.b = cb ? f32in(dxb, cb) : none();
+    A.cin = (int)cin;
     A.H = H;
     A.W = W;
     A.cout = cout;
@@ -957,16 +1269,19 @@ int ins2d_nn_layer_check(int H, int W, int ca, int cb, int cout, int ks, int rel
     A.out = dout;
     A.out_fp32 = 1;
     A.relu = relu;
-    hipLaunchKernelGGL(k_conv, dim3((W + 63) / 64, H, (cout + 63) / 64), dim3(256), 0, nullptr, A);
-    ConvArgs C = A;
+    launch_conv(nullptr, A);
+    ConvArgs C{};
     C.a = f32in(dg, cout);
     C.b = none();
+    C.cin = cout;
+    C.H = H;
+    C.W = W;
     C.cout = (int)cin;
+    C.ks = ks;
     C.w = dwt;
-    C.bias = nullptr;
     C.out = ddin;
-    C.relu = 0;
-    hipLaunchKernelGGL(k_conv, dim3((W + 63) / 64, H, ((int)cin + 63) / 64), dim3(256), 0, nullptr, C);
+    C.out_fp32 = 1;
+    launch_conv(nullptr, C);
     const int chunk = 4096, tci = ((int)cin + 15) / 16, tco = (cout + 15) / 16;
     hipLaunchKernelGGL(k_conv_dw, dim3((unsigned)((np_ + chunk - 1) / chunk), ks * ks, tci * tco), dim3(64), 0, nullptr,
                        A.a, A.b, dg, H, W, cout, ks, chunk, ddw);

@@ -72,15 +72,17 @@ class WarmStartNet:
         self._check(self._lib.ins2d_nn_get_grad(self._h, g.ctypes.data_as(_lib.P_float)))
         return g
 
-    def forward(self, f, xc=None):
+    def forward(self, f, xc=None, unfused=False):
         """p0 for a host right-hand side f of shape (H, W) (x fastest) and an
-        optional coarse-grid solution xc of the same shape."""
+        optional coarse-grid solution xc of the same shape (``unfused``: through
+        the training forward)."""
         f = np.ascontiguousarray(f, dtype=np.float64)
         if f.shape != (self.H, self.W):
             raise ValueError(f'expected shape {(self.H, self.W)}, got {f.shape}')
         x = None if xc is None else np.ascontiguousarray(xc, dtype=np.float64).reshape(f.shape)
         p = np.empty_like(f)
-        self._check(self._lib.ins2d_nn_forward(self._h, f.ctypes.data_as(_lib.P_double),
+        fn = self._lib.ins2d_nn_forward_unfused if unfused else self._lib.ins2d_nn_forward
+        self._check(fn(self._h, f.ctypes.data_as(_lib.P_double),
                                                None if x is None else x.ctypes.data_as(_lib.P_double),
                                                p.ctypes.data_as(_lib.P_double)))
         return p

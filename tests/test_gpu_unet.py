@@ -85,6 +85,30 @@ def test_forward_parity(H, W):
     net.close()
 
 
+@pytest.mark.parametrize('H,W', [(16, 32), (24, 40), (64, 136), (256, 1024)])
+def test_fused_forward_equals_training_forward(H, W):
+    """The inference forward (input scaling, pooling, upsampling and the 1x1
+    output layer fused into the convolution kernels) reads the same bfloat16
+    values into every convolution as the training forward; only the output
+    layer's 16-term FP32 sum is taken in another order, so the two agree to
+    FP32 rounding (1e-5 here), with and without the coarse-grid input."""
+    from ins2d_amd.warmstart import WarmStartNet
+    rng = np.random.default_rng(5 * H + W)
+    th = _theta(6)
+    net = WarmStartNet(H, W)
+    net.set_params(th, 0.8)
+    f = rng.standard_normal((H, W)) * 2.0
+    xc = 0.3 * rng.standard_normal((H, W))
+    for x in (None, xc):
+        a = net.forward(f, x)
+        b = net.forward(f, x, unfused=True)
+        base = 0.0 if x is None else x
+        e = _rel(a - base, b - base)
+        print(f'fused vs unfused {H}x{W} xc={x is not None}: rel {e:.2e}', flush=True)
+        assert e < 1e-5
+    net.close()
+
+
 @pytest.mark.parametrize('H,W,tol', [(16, 32, 3e-2), (24, 40, 0.15)])
 def test_gradient_parity(H, W, tol):
     from ins2d_amd.warmstart import WarmStartNet
