@@ -260,7 +260,9 @@ int ins2d_nn_forward_unfused_dev(ins2d_nn* n, const double* f, const double* xc,
 /* mean-squared-error loss of net(f_b / rms(f_b), xc_b / (out_scale rms(f_b)))
  * against (p_b - xc_b) / (out_scale rms(f_b)) over a batch of device arrays
  * (xc NULL, or an entry NULL: zero); gradients into the gradient vector;
- * mode 1 also takes one Adam step (lr, beta 0.9 / 0.999) */
+ * mode bit 0: also one Adam step (lr, beta 0.9 / 0.999); bit 1: the relative
+ * loss sum (y - t)^2 / sum t^2 per pair instead of the mean square (pairs whose
+ * targets differ in scale weigh alike) */
 int ins2d_nn_train_step(ins2d_nn* n, const double* const* f, const double* const* p, const double* const* xc,
                         int batch, float lr, int mode, double* loss_out, void* stream);
 int ins2d_nn_get_grad(ins2d_nn* n, float* grad);

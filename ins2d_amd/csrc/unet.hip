@@ -484,20 +484,43 @@ __global__ void k_rms(const double* __restrict__ part, int np, long n, double* _
   const double t = block_sum((int)threadIdx.x < np ? part[threadIdx.x] : 0.0, red);
   if (threadIdx.x == 0) s[0] = sqrt(t / (double)n);
 }
-// MSE loss of y against the target (t - xc) / (scale s) (xc optional):
-// dy = 2 (y - tt) / n; loss += sum (y - tt)^2 / n
+// MSE loss of y against the target tt = (t - xc) / (scale s) (xc optional):
+// dy = 2 (y - tt) / n; loss += sum (y - tt)^2 / n.  With tpart (partial sums
+// of (t - xc)^2 over kSq blocks) the relative loss: dy = 2 (y - tt) / T,
+// loss += sum (y - tt)^2 / T with T = sum tt^2.
 __global__ void k_mse(const float* __restrict__ y, const double* __restrict__ t, const double* __restrict__ xc, long n,
-                      const double* __restrict__ s, double scale, float* __restrict__ dy, double* __restrict__ loss) {
+                      const double* __restrict__ s, double scale, const double* __restrict__ tpart,
+                      float* __restrict__ dy, double* __restrict__ loss) {
   __shared__ double red[16];
+  __shared__ double den_sh;
   const double inv = 1.0 / (scale * fmax(s[0], 1e-300));
+  double den = (double)n;
+  if (tpart) {
+    const double tt = block_sum((int)threadIdx.x < kSq ? tpart[threadIdx.x] : 0.0, red);
+    if (threadIdx.x == 0) den_sh = fmax(tt * inv * inv, 1e-300);
+    __syncthreads();
+    den = den_sh;
+  }
   double a = 0.0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const double d = (double)y[i] - (t[i] - (xc ? xc[i] : 0.0)) * inv;
-    dy[i] = (float)(2.0 * d / (double)n);
-    a += d * d / (double)n;
+    dy[i] = (float)(2.0 * d / den);
+    a += d * d / den;
   }
   a = block_sum(a, red);
   if (threadIdx.x == 0) atomicAdd(loss, a);
+}
+// partial sums of (t - xc)^2 over kSq blocks of 256 threads (xc optional)
+__global__ void k_sumsq_diff(const double* __restrict__ t, const double* __restrict__ xc, long n,
+                             double* __restrict__ part) {
+  __shared__ double red[16];
+  double a = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double d = t[i] - (xc ? xc[i] : 0.0);
+    a += d * d;
+  }
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 // Adam (fp32 master parameters)
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
@@ -577,6 +600,7 @@ struct ins2d_nn {
   float *g0 = nullptr, *g1 = nullptr, *gs1 = nullptr, *gs2 = nullptr, *gs3 = nullptr, *gtmp = nullptr;
   double* scale = nullptr;   // rms of the current input
   double* sq = nullptr;      // partial sums of the input's squares
+  double* tsq = nullptr;     // partial sums of the target's squares (relative loss)
   double* cscale = nullptr;  // rms of the two sources of a combined pair
   double* loss = nullptr;
   double out_scale = 1.0;    // p = out_scale * rms(f) * y
@@ -916,6 +940,7 @@ int ins2d_nn_create(int H, int W, ins2d_nn** out) {
     N->gs3 = nalloc<float>(N.get(), (p0 >> 4) * 64);
     N->scale = nalloc<double>(N.get(), 1);
     N->sq = nalloc<double>(N.get(), kSq);
+    N->tsq = nalloc<double>(N.get(), kSq);
     N->cscale = nalloc<double>(N.get(), 2);
     N->loss = nalloc<double>(N.get(), 1);
     NNCHK(hipMemset(N->param, 0, sizeof(float) * off));
@@ -1084,11 +1109,13 @@ int ins2d_nn_train_step(ins2d_nn* N, const double* const* f, const double* const
       const double* x = xc ? xc[b] : nullptr;
       prepare_input(st, N, f[b], x);
       forward(st, N);
-      hipLaunchKernelGGL(k_mse, dim3(nb(n)), dim3(256), 0, st, N->out, p[b], x, n, N->scale, N->out_scale, N->g0,
+      if (mode & 2) hipLaunchKernelGGL(k_sumsq_diff, dim3(kSq), dim3(256), 0, st, p[b], x, n, N->tsq);
+      hipLaunchKernelGGL(k_mse, dim3(nb(n)), dim3(256), 0, st, N->out, p[b], x, n, N->scale, N->out_scale,
+                         (mode & 2) ? (const double*)N->tsq : nullptr, N->g0,
                          N->loss);
       backward(st, N);
     }
-    if (mode == 1) {
+    if (mode & 1) {
       N->step += 1;
       const float b1 = 0.9f, b2 = 0.999f;
       const float c1 = 1.f - powf(b1, (float)N->step), c2 = 1.f - powf(b2, (float)N->step);

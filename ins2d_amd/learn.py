@@ -113,7 +113,8 @@ def train(net, pairs, steps, batch=4, lr=2e-3, lr_end=1e-4, seed=0, combos=True,
     return train_stored(net, n, steps, batch, lr, lr_end, seed, combos, progress, every)
 
 
-def train_stored(net, n, steps, batch=4, lr=2e-3, lr_end=1e-4, seed=0, combos=True, progress=None, every=50):
+def train_stored(net, n, steps, batch=4, lr=2e-3, lr_end=1e-4, seed=0, combos=True, progress=None, every=50,
+                 relative=False):
     """Adam steps on the first n stored pairs; combinations are formed on the
     device in slots n .. n + batch/2 (reserved by the caller).  Cosine
     learning-rate decay from lr to lr_end.  Returns the loss history."""
@@ -129,7 +130,7 @@ def train_stored(net, n, steps, batch=4, lr=2e-3, lr_end=1e-4, seed=0, combos=Tr
             a, b = rng.standard_normal(2)
             net.data_combine(n + c, int(i), int(j), float(a), float(b))
             idx.append(n + c)
-        losses.append(net.train(idx, lr=float(lr_s), update=True))
+        losses.append(net.train(idx, lr=float(lr_s), update=True, relative=relative))
         if progress and (s % every == 0 or s == steps - 1):
             progress(f'train step {s}/{steps}: loss {losses[-1]:.4e} ({time.perf_counter() - t0:.1f} s)')
     return losses

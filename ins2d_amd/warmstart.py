@@ -137,12 +137,14 @@ class WarmStartNet:
         self._check(self._lib.ins2d_nn_data_combine(self._h, int(dst), int(i), int(j), ctypes.c_double(a),
                                                     ctypes.c_double(b)))
 
-    def train(self, idx, lr=1e-3, update=True):
-        """Loss over the stored pairs ``idx``; with ``update`` one Adam step."""
+    def train(self, idx, lr=1e-3, update=True, relative=False):
+        """Loss over the stored pairs ``idx``; with ``update`` one Adam step;
+        ``relative``: per-pair loss sum (y - t)^2 / sum t^2."""
         ia = np.ascontiguousarray(idx, dtype=np.int32)
         loss = ctypes.c_double()
+        mode = (1 if update else 0) | (2 if relative else 0)
         self._check(self._lib.ins2d_nn_train(self._h, ia.ctypes.data_as(_lib.P_int), ia.size, ctypes.c_float(lr),
-                                             1 if update else 0, ctypes.byref(loss)))
+                                             mode, ctypes.byref(loss)))
         return loss.value
 
 
