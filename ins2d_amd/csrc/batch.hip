@@ -277,6 +277,9 @@ __global__ void k_b_cg_init_fin(const BInst* __restrict__ I, BCG S, double tol2)
   }
 }
 
+// TRI: every instance's pressure Laplacian has the factor widths of
+// lap_tri_at (the host checks), read by scalar loads from its record
+template <bool TRI>
 __global__ void __launch_bounds__(256) k_b_cg_apply(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
                                                    BCG S, const double* __restrict__ p, double* __restrict__ q) {
   __shared__ double red[4];
@@ -287,7 +290,7 @@ __global__ void __launch_bounds__(256) k_b_cg_apply(const BInst* __restrict__ I,
   double acc = 0.0;
   if (i < B.npx && j < B.npy) {
     const long k = B.poff + (long)i + (long)B.npx * j;
-    const double a = lap_at(B.lap, p + B.poff, i, j);
+    const double a = TRI ? lap_tri_at(cst(&B.lap), p + B.poff, i, j) : lap_at(B.lap, p + B.poff, i, j);
     q[k] = a;
     acc = p[k] * a;
   }
@@ -414,6 +417,7 @@ __global__ void __launch_bounds__(256) k_b2_init(const BInst* __restrict__ I, co
   (void)S;
 }
 
+template <bool TRI>
 __global__ void __launch_bounds__(256) k_b2_apply(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
                                                  BTL T, const double* __restrict__ p, double* __restrict__ q) {
   __shared__ double red[4];
@@ -424,7 +428,7 @@ __global__ void __launch_bounds__(256) k_b2_apply(const BInst* __restrict__ I, c
   double acc = 0.0;
   if (i < B.npx && j < B.npy) {
     const long k = B.poff + (long)i + (long)B.npx * j;
-    const double a = lap_at(B.lap, p + B.poff, i, j);
+    const double a = TRI ? lap_tri_at(cst(&B.lap), p + B.poff, i, j) : lap_at(B.lap, p + B.poff, i, j);
     q[k] = a;
     acc = p[k] * a;
   }
@@ -849,6 +853,7 @@ struct ins2d_batch {
   // fast-diagonalization PCG (cg_precond = 3): per-instance factors, the
   // 64 x 64 GEMM tile list, two work arrays
   bool fdm = false;
+  bool lap_tri = false;   // every instance's Laplacian in the widths of lap_tri_at
   std::vector<FDMHost> fh;
   // closed-form maps of the momentum kernels (uniform members), per member
   std::vector<UniForm> ug;
@@ -1028,6 +1033,9 @@ void build_ops(ins2d_batch* b, Arena& A) {
     I.lap.dinv = A.push(dinv);
     I.lap.nx = P.Npx;
     I.lap.ny = P.Npy;
+    const DLap& Lh = I.lap;
+    const bool tri = Lh.L1.Y.K == 1 && Lh.L1.X.K == 3 && Lh.L2.Y.K == 3 && Lh.L2.X.K == 1;
+    b->lap_tri = (k == 0 ? true : b->lap_tri) && tri;
     if (b->two_level) {
       FDMHost C;
       build_coarse_host(P, kAgg, C);
@@ -1229,9 +1237,15 @@ void launch_tiles_proj(ins2d_batch* b, const double* Vn, const double* Vt, const
 
 // fast-diagonalization PCG iteration, head: q = A p, alpha, x and r, the
 // convergence test (converged instances stop here)
+void launch_cg_apply(ins2d_batch* b, dim3 g, dim3 t, unsigned sh, hipStream_t st, const BInst* I, const Tile* tiles,
+                     BCG S, const double* p, double* q) {
+  if (b->lap_tri) hipLaunchKernelGGL(k_b_cg_apply<true>, g, t, sh, st, I, tiles, S, p, q);
+  else hipLaunchKernelGGL(k_b_cg_apply<false>, g, t, sh, st, I, tiles, S, p, q);
+}
+
 void fdm_iter_head(ins2d_batch* b) {
   const unsigned np = (unsigned)b->pt.size();
-  hipLaunchKernelGGL(k_b_cg_apply, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q);
+  launch_cg_apply(b, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q);
   hipLaunchKernelGGL(k_b_cg_alpha, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
   hipLaunchKernelGGL(k_bf_update, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q, b->x,
                      b->r);
@@ -1252,7 +1266,8 @@ void cg_iter_launches(ins2d_batch* b) {
   if (b->two_level) {
     const unsigned na = (unsigned)b->at.size();
     const dim3 tb(kAgg, kAgg);
-    hipLaunchKernelGGL(k_b2_apply, dim3(na), tb, 0, b->st, b->dinst, b->dat, b->cg, b->tl, b->pp, b->q);
+    if (b->lap_tri) hipLaunchKernelGGL(k_b2_apply<true>, dim3(na), tb, 0, b->st, b->dinst, b->dat, b->cg, b->tl, b->pp, b->q);
+    else hipLaunchKernelGGL(k_b2_apply<false>, dim3(na), tb, 0, b->st, b->dinst, b->dat, b->cg, b->tl, b->pp, b->q);
     hipLaunchKernelGGL(k_b2_alpha, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tl);
     hipLaunchKernelGGL(k_b2_update, dim3(na), tb, 0, b->st, b->dinst, b->dat, b->cg, b->tl, b->pp, b->q, b->x, b->r);
     hipLaunchKernelGGL(k_b2_coarse, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tl, 0, 0.0);
@@ -1260,7 +1275,7 @@ void cg_iter_launches(ins2d_batch* b) {
     return;
   }
   const unsigned np = (unsigned)b->pt.size();
-  hipLaunchKernelGGL(k_b_cg_apply, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q);
+  launch_cg_apply(b, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q);
   hipLaunchKernelGGL(k_b_cg_alpha, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
   hipLaunchKernelGGL(k_b_cg_update, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->pp, b->q, b->x,
                      b->r, b->z);

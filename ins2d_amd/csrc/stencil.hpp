@@ -362,6 +362,31 @@ __device__ __forceinline__ double proj_point(const DComp& C, const double* __res
 __device__ __forceinline__ double lap_at(const DLap& L, const double* __restrict__ p, int i, int j) {
   return kron_at(L.L1, p, i, j) + kron_at(L.L2, p, i, j);
 }
+// the same for the factor widths of the pressure Laplacian,
+// kron(diagonal, tridiagonal) + kron(tridiagonal, diagonal): every index and
+// weight load is independent of the others, so they issue together (the
+// generic form loads an entry's index, then its value, one entry at a time)
+__device__ __forceinline__ double lap_tri_at(const DLap& L, const double* __restrict__ p, int i, int j) {
+  const auto a = glb(p);
+  const unsigned ld = (unsigned)L.L1.X.ncols;
+  const unsigned i3 = 3u * (unsigned)i, j3 = 3u * (unsigned)j;
+  const auto c1x = glb(L.L1.X.col), c2y = glb(L.L2.Y.col);
+  const auto w1x = glb(L.L1.X.val), w2y = glb(L.L2.Y.val);
+  const unsigned r1 = (unsigned)glb(L.L1.Y.col)[j] * ld;
+  const double w1y = glb(L.L1.Y.val)[j];
+  const unsigned c2 = (unsigned)glb(L.L2.X.col)[i];
+  const double w2 = glb(L.L2.X.val)[i];
+  double s1 = w1x[i3] * a[r1 + (unsigned)c1x[i3]];
+  s1 += w1x[i3 + 1] * a[r1 + (unsigned)c1x[i3 + 1]];
+  s1 += w1x[i3 + 2] * a[r1 + (unsigned)c1x[i3 + 2]];
+  double s2 = w2y[j3] * a[(unsigned)c2y[j3] * ld + c2];
+  s2 += w2y[j3 + 1] * a[(unsigned)c2y[j3 + 1] * ld + c2];
+  s2 += w2y[j3 + 2] * a[(unsigned)c2y[j3 + 2] * ld + c2];
+  return w1y * s1 + w2 * s2;
+}
+__device__ __forceinline__ bool lap_is_tri(const DLap& L) {
+  return L.L1.Y.K == 1 && L.L1.X.K == 3 && L.L2.Y.K == 3 && L.L2.X.K == 1;
+}
 
 // block reductions over all threads of a (possibly 2-D) block; the result is
 // valid on the block's first thread
