@@ -1030,6 +1030,167 @@ __global__ void __launch_bounds__(fft_block<E>(0)) k_fftr_c2r_rows(int Nx, int N
   fft_reg<E, true, false>(z, Nx, log2N, tw, ld, st);
 }
 
+// P passes of radix E (N = E^P points) on data in registers: thread tid
+// (< N / E, the active threads) owns one butterfly per pass, slots m < E
+// holding elements tid + m N/E on input and on output (the last pass's
+// output index is again tid + m N/E).  Consecutive passes exchange through
+// an N-double LDS array, real parts first, then imaginary parts (half the LDS
+// of exchanging complex values, at two more barriers per exchange).  Every
+// thread of the block must call it.
+template <int E, bool INV>
+__device__ __forceinline__ void fft_split_passes(double2 (&a)[E], double* zr, int N, int P,
+                                                 const double2* __restrict__ tw) {
+  const int tid = threadIdx.x, Q = N / E;
+  const bool act = tid < Q;
+  int Ns = 1;
+  for (int p = 0; p < P; ++p) {
+    if (act) {
+      if (Ns > 1) {
+        const int ks = (tid & (Ns - 1)) * (N / (Ns * E));
+        double2 t[E];
+#pragma unroll
+        for (int b = 1; b < E; b *= 2) {
+          t[b] = tw[b * ks];
+          if (INV) t[b] = conjd(t[b]);
+        }
+#pragma unroll
+        for (int m = 3; m < E; ++m)
+          if (m & (m - 1)) t[m] = cmul(t[m & (m - 1)], t[m & -m]);
+#pragma unroll
+        for (int m = 1; m < E; ++m) a[m] = cmul(a[m], t[m]);
+      }
+      dft_reg<E, INV>(a);
+    }
+    if (p == P - 1) break;
+    const int k = tid & (Ns - 1);
+    const int o = (tid - k) * E + k;   // output element of slot m: o + m Ns
+    __syncthreads();   // the previous exchange's reads are done
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) zr[swz(o + m * Ns)] = a[m].x;
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) a[m].x = zr[swz(tid + m * Q)];
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) zr[swz(o + m * Ns)] = a[m].y;
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) a[m].y = zr[swz(tid + m * Q)];
+    }
+    Ns *= E;
+  }
+}
+
+// k_fftr_cols_solve for Ny = E^P: forward transform, division by the
+// symbol and inverse transform on one register array per thread, N / E
+// threads per column, N doubles of LDS (32 KB at 4096 instead of 64 KB)
+template <int E>
+__global__ void __launch_bounds__(4096 / E) k_fftr_cols_solve_s(int Nx, int Ny, int P, const double2* __restrict__ tw,
+                                                             const double* __restrict__ ax, const double* __restrict__ ay,
+                                                             double c, double2* __restrict__ HT,
+                                                             unsigned long long* __restrict__ fmax2) {
+  extern __shared__ double zr[];
+  const int kx = blockIdx.x, tid = threadIdx.x, Q = Ny / E;
+  const bool act = tid < Q;
+  double2* row = HT + (size_t)kx * Ny;
+  double2 a[E];
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) a[m] = row[tid + m * Q];
+  }
+  fft_split_passes<E, false>(a, zr, Ny, P, tw);
+  double m2 = 0.0;
+  if (act) {
+    const double scale = 1.0 / ((double)Nx * (double)Ny);
+    const double axk = ax[kx];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int ky = tid + m * Q;
+      m2 = fmax(m2, a[m].x * a[m].x + a[m].y * a[m].y);
+      double Ah = c * (axk + ay[ky]);
+      if (kx == 0 && ky == 0) Ah = 1.0;
+      const double sc = -scale / Ah;
+      a[m] = make_double2(a[m].x * sc, a[m].y * sc);
+    }
+  }
+  fmax2_raise(fmax2, m2);
+  fft_split_passes<E, true>(a, zr, Ny, P, tw);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) row[tid + m * Q] = a[m];
+  }
+}
+
+// k_fftr_r2c_rows (transposed spectra, rows of f) for Nx = E^P with the split
+// exchange: the rows y0 = 2b, y1 = 2b+1 packed as one complex row, loaded
+// straight into the registers of the first pass; after the transform the
+// partner element Nx - k of each k is fetched through LDS (real parts, then
+// imaginary parts) to split the two half spectra.  Nx / E threads, Nx doubles
+// of LDS.
+template <int E>
+__global__ void __launch_bounds__(4096 / E) k_fftr_r2c_rows_s(int Nx, int Ny, int Nh, int P,
+                                                           const double2* __restrict__ tw, const double* __restrict__ f,
+                                                           double2* __restrict__ H,
+                                                           unsigned long long* __restrict__ fmax2) {
+  extern __shared__ double zr[];
+  fmax2_reset(fmax2);
+  const int y0 = 2 * xcd_row_pair(blockIdx.x, gridDim.x), y1 = y0 + 1;
+  const bool has1 = y1 < Ny;
+  const int tid = threadIdx.x, Q = Nx / E;
+  const bool act = tid < Q;
+  const double* f0 = f + (size_t)y0 * Nx;
+  const double* f1 = f + (size_t)(has1 ? y1 : y0) * Nx;
+  double2 a[E];
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int n = tid + m * Q;
+      a[m] = make_double2(f0[n], has1 ? f1[n] : 0.0);
+    }
+  }
+  fft_split_passes<E, false>(a, zr, Nx, P, tw);
+  // element k = tid + m Q is in a[m]; its partner Nx - k (mod Nx) through LDS
+  double2 b[E];
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) zr[swz(tid + m * Q)] = a[m].x;
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) b[m].x = zr[swz((Nx - tid - m * Q) & (Nx - 1))];
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) zr[swz(tid + m * Q)] = a[m].y;
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int k = tid + m * Q;
+      b[m].y = -zr[swz((Nx - k) & (Nx - 1))];   // b = conj(z[Nx - k])
+      if (k < Nh) {
+        const double2 h0 = make_double2(0.5 * (a[m].x + b[m].x), 0.5 * (a[m].y + b[m].y));
+        // (a - b) / (2i) = -i/2 (a - b)
+        const double2 h1 = make_double2(0.5 * (a[m].y - b[m].y), -0.5 * (a[m].x - b[m].x));
+        double2* o = H + (size_t)k * Ny + y0;
+        o[0] = h0;
+        if (has1) o[1] = h1;
+      }
+    }
+  }
+}
+
 // one block per kx: forward FFT along y, division by -Ahat in registers,
 // inverse FFT along y, in place on HT (Nh x Ny) (power-of-two Ny in
 // [E, 4096], max(64, Ny / E) threads)
@@ -1827,6 +1988,15 @@ static int fft_elems(int N, bool cols) {
   return 8;
 }
 
+// Length 4096: the column solve and the forward row transform with the split
+// exchange (radix 16, 3 passes, 32 KB of LDS per transform instead of 64 KB:
+// three workgroups per CU instead of two; at 4096^2 column solve 110 -> 72 us,
+// forward rows 97 -> 92 us).  The inverse row transform keeps its staged form:
+// built in registers from the half spectra, every value of which two threads
+// need, it ran at 79 against 64 us (with p += dp: 158 against 97 us).
+constexpr int kSplitE = 16, kSplitP = 3;
+static bool split_fft(int N) { return N == 4096; }
+
 // TR: the row transforms store (load) the spectra transposed; D: the first
 // transform computes its rows as the divergence of D->A, D->B (f unused);
 // acc: the last transform adds dp into its output
@@ -1845,12 +2015,19 @@ static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, doubl
     if (D)
       fft_launch(0, k_fftr_r2c_rows<E, TR, true>, gr, dim3(fft_block<E>(0)), sh, st, Nx, Ny, Nh, ilog2(Nx),
                  (const double2*)P.twx, (const double*)nullptr, *D, H, P.fmax2);
+    else if (TR && split_fft(Nx))
+      fft_launch(0, k_fftr_r2c_rows_s<kSplitE>, gr, dim3(Nx / kSplitE), (unsigned)(Nx * sizeof(double)), st, Nx, Ny,
+                 Nh, kSplitP, (const double2*)P.twx, f, H, P.fmax2);
     else
       fft_launch(0, k_fftr_r2c_rows<E, TR, false>, gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx, f,
                  DivSrc{}, H, P.fmax2);
   } else if (which == 1) {
-    fft_launch(1, k_fftr_cols_solve<E>, dim3(Nh), dim3(std::max(64, Ny / E)), (unsigned)(Ny * sizeof(double2)), st,
-               Nx, Ny, ilog2(Ny), (const double2*)P.twy, (const double*)P.ax, (const double*)P.ay, P.c, HT, P.fmax2);
+    if (split_fft(Ny))
+      fft_launch(1, k_fftr_cols_solve_s<kSplitE>, dim3(Nh), dim3(Ny / kSplitE), (unsigned)(Ny * sizeof(double)), st,
+                 Nx, Ny, kSplitP, (const double2*)P.twy, (const double*)P.ax, (const double*)P.ay, P.c, HT, P.fmax2);
+    else
+      fft_launch(1, k_fftr_cols_solve<E>, dim3(Nh), dim3(std::max(64, Ny / E)), (unsigned)(Ny * sizeof(double2)), st,
+                 Nx, Ny, ilog2(Ny), (const double2*)P.twy, (const double*)P.ax, (const double*)P.ay, P.c, HT, P.fmax2);
   } else {
     // the half spectra are staged by a plain loop before the transform (as
     // the divergence rows above): threads up to the launch bound
