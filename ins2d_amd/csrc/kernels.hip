@@ -1191,6 +1191,96 @@ __global__ void __launch_bounds__(4096 / E) k_fftr_r2c_rows_s(int Nx, int Ny, in
   }
 }
 
+// k_fftr_c2r_rows (transposed spectra) for Nx = E^P with the split exchange:
+// the full row a + i b is staged through an Nx-double LDS array, real parts
+// first, then imaginary parts, each half-spectrum value read from memory once
+// (kept in registers between the two halves), then transformed in registers;
+// ACC: the rows are added into out (their old values loaded before the
+// transform).  Nx / E threads.
+template <int E, bool ACC>
+__global__ void __launch_bounds__(4096 / E) k_fftr_c2r_rows_s(int Nx, int Ny, int Nh, int P,
+                                                           const double2* __restrict__ tw, const double2* __restrict__ H,
+                                                           double* __restrict__ out,
+                                                           const unsigned long long* __restrict__ fmax2) {
+  extern __shared__ double zr[];
+  const int y0 = 2 * xcd_row_pair(blockIdx.x, gridDim.x), y1 = y0 + 1;
+  const bool has1 = y1 < Ny;
+  const int tid = threadIdx.x, T = blockDim.x, Q = Nx / E;
+  const bool act = tid < Q;
+  double* o0 = out + (size_t)y0 * Nx;
+  double* o1 = out + (size_t)(has1 ? y1 : y0) * Nx;
+  if (fmax2_tiny(fmax2)) {
+    // dp = 0: nothing to add, or zero rows
+    if (!ACC)
+      for (int n = tid; n < Nx; n += T) {
+        o0[n] = 0.0;
+        if (has1) o1[n] = 0.0;
+      }
+    return;
+  }
+  const double2* H0 = H + y0;
+  const double2* H1 = H + (has1 ? y1 : y0);
+  // half-spectrum values k = tid + s T, s < KS (k < Nh), loaded once
+  constexpr int KS = E / 2 + 1;
+  double2 ha[KS], hb[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const int k = tid + s * T;
+    ha[s] = hb[s] = make_double2(0.0, 0.0);
+    if (k < Nh) {
+      ha[s] = H0[(size_t)k * Ny];
+      hb[s] = has1 ? H1[(size_t)k * Ny] : make_double2(0.0, 0.0);
+      if (k == 0 || 2 * k == Nx) { ha[s].y = 0.0; hb[s].y = 0.0; }
+    }
+  }
+  double2 pre[E];
+  if (ACC && act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int n = tid + m * Q;
+      pre[m] = make_double2(o0[n], has1 ? o1[n] : 0.0);
+    }
+  }
+  double2 a[E];
+  // z[k] = a + i b, z[Nx - k] = conj(a) + i conj(b): real parts, then imaginary
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const int k = tid + s * T;
+    if (k < Nh) {
+      zr[swz(k)] = ha[s].x - hb[s].y;
+      if (k > 0 && 2 * k < Nx) zr[swz(Nx - k)] = ha[s].x + hb[s].y;
+    }
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) a[m].x = zr[swz(tid + m * Q)];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const int k = tid + s * T;
+    if (k < Nh) {
+      zr[swz(k)] = ha[s].y + hb[s].x;
+      if (k > 0 && 2 * k < Nx) zr[swz(Nx - k)] = hb[s].x - ha[s].y;
+    }
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) a[m].y = zr[swz(tid + m * Q)];
+  }
+  fft_split_passes<E, true>(a, zr, Nx, P, tw);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int n = tid + m * Q;
+      o0[n] = ACC ? pre[m].x + a[m].x : a[m].x;
+      if (has1) o1[n] = ACC ? pre[m].y + a[m].y : a[m].y;
+    }
+  }
+}
+
 // one block per kx: forward FFT along y, division by -Ahat in registers,
 // inverse FFT along y, in place on HT (Nh x Ny) (power-of-two Ny in
 // [E, 4096], max(64, Ny / E) threads)
@@ -1988,12 +2078,13 @@ static int fft_elems(int N, bool cols) {
   return 8;
 }
 
-// Length 4096: the column solve and the forward row transform with the split
-// exchange (radix 16, 3 passes, 32 KB of LDS per transform instead of 64 KB:
-// three workgroups per CU instead of two; at 4096^2 column solve 110 -> 72 us,
-// forward rows 97 -> 92 us).  The inverse row transform keeps its staged form:
-// built in registers from the half spectra, every value of which two threads
-// need, it ran at 79 against 64 us (with p += dp: 158 against 97 us).
+// Length 4096: the transforms with the split exchange (radix 16, 3 passes,
+// 32 KB of LDS per transform instead of 64 KB: three workgroups per CU instead
+// of two; at 4096^2 column solve 110 -> 72 us, forward rows 97 -> 92 us,
+// inverse rows 62 -> 62 us, with p += dp 96 -> 91 us).  The inverse rows stage
+// the full row through LDS as before, each half-spectrum value read once: built
+// in registers from the half spectra directly (every value needed by two
+// threads, read twice) they ran at 79 us (158 us with p += dp).
 constexpr int kSplitE = 16, kSplitP = 3;
 static bool split_fft(int N) { return N == 4096; }
 
@@ -2033,7 +2124,15 @@ static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, doubl
     // the divergence rows above): threads up to the launch bound
     const dim3 blc(fft_block<E>(0));
     const unsigned long long* fm = P.fmax2;
-    if (acc)
+    if (TR && split_fft(Nx)) {
+      const unsigned shs = (unsigned)(Nx * sizeof(double));
+      if (acc)
+        fft_launch(3, k_fftr_c2r_rows_s<kSplitE, true>, gr, dim3(Nx / kSplitE), shs, st, Nx, Ny, Nh, kSplitP,
+                   (const double2*)P.twx, (const double2*)H, dp, fm);
+      else
+        fft_launch(2, k_fftr_c2r_rows_s<kSplitE, false>, gr, dim3(Nx / kSplitE), shs, st, Nx, Ny, Nh, kSplitP,
+                   (const double2*)P.twx, (const double2*)H, dp, fm);
+    } else if (acc)
       fft_launch(3, k_fftr_c2r_rows<E, TR, true>, gr, blc, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx,
                  (const double2*)H, dp, fm);
     else
