@@ -949,17 +949,34 @@ void build_fft(ins2d_handle* h, int order) {
   h->has_fft = true;
 }
 
-void stats_reduce(ins2d_handle* h, const double* x, int nx, int ny, const double* wx, const double* wy, double out[3]) {
-  launch_reduce_stats(h->st, x, nx, ny, wx, wy, h->part, h->nred);
-  std::vector<double> part(4 * (size_t)h->nred);
+// max |x|, sum w x, sum w x^2 (w = wx[i] wy[j], or 1): partials of one
+// launch into slot `slot` of h->part (kStatSlots slots), read back together
+// by stats_collect (one host wait for several reductions)
+constexpr int kStatSlots = 5;
+void stats_launch(ins2d_handle* h, int slot, const double* x, int nx, int ny, const double* wx, const double* wy) {
+  launch_reduce_stats(h->st, x, nx, ny, wx, wy, h->part + 4 * (size_t)h->nred * slot, h->nred);
+}
+void stats_collect(ins2d_handle* h, int nslots, double (*out)[3]) {
+  std::vector<double> part(4 * (size_t)h->nred * nslots);
   HIPCHK(hipMemcpyAsync(part.data(), h->part, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
-  out[0] = out[1] = out[2] = 0.0;
-  for (int b = 0; b < h->nred; ++b) {
-    out[0] = std::max(out[0], part[4 * b]);
-    out[1] += part[4 * b + 1];
-    out[2] += part[4 * b + 2];
+  for (int s = 0; s < nslots; ++s) {
+    const double* q = part.data() + 4 * (size_t)h->nred * s;
+    out[s][0] = out[s][1] = out[s][2] = 0.0;
+    for (int b = 0; b < h->nred; ++b) {
+      out[s][0] = std::max(out[s][0], q[4 * b]);
+      out[s][1] += q[4 * b + 1];
+      out[s][2] += q[4 * b + 2];
+    }
   }
+}
+void stats_reduce(ins2d_handle* h, const double* x, int nx, int ny, const double* wx, const double* wy, double out[3]) {
+  stats_launch(h, 0, x, nx, ny, wx, wy);
+  double o[1][3];
+  stats_collect(h, 1, o);
+  out[0] = o[0][0];
+  out[1] = o[0][1];
+  out[2] = o[0][2];
 }
 
 int fail(ins2d_handle* h, const std::exception& e, int code) {
@@ -1255,7 +1272,7 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
       }
     }
     h->nred = (int)std::min<long>(1024, std::max<long>(1, ((long)std::max(h->NV, h->Np) + 255) / 256));
-    h->part = dalloc<double>(h.get(), 4 * (size_t)h->nred);
+    h->part = dalloc<double>(h.get(), 4 * (size_t)h->nred * kStatSlots);
     h->ee_part = dalloc<double>(h.get(), 64);
     HIPCHK(hipDeviceSynchronize());
   } catch (const std::exception& e) {
@@ -1520,23 +1537,26 @@ int ins2d_diagnostics(ins2d_handle* h, double t, double* out) {
   if (!h || !out) return INS2D_E_ARG;
   (void)t;
   API_TRY
-  double a[3], b[3];
+  // the five reductions queued behind the F and divergence launches, read
+  // back with one host wait
   rhs(h, h->V, h->p, RHS_WITH_P, h->tV, kNoCombine, nullptr);
-  stats_reduce(h, h->tV, h->nux, h->nuy, nullptr, nullptr, a);
-  stats_reduce(h, h->tV + h->Nu, h->nvx, h->nvy, nullptr, nullptr, b);
-  out[0] = std::max(a[0], b[0]);
+  stats_launch(h, 0, h->tV, h->nux, h->nuy, nullptr, nullptr);
+  stats_launch(h, 1, h->tV + h->Nu, h->nvx, h->nvy, nullptr, nullptr);
   div(h, h->V, nullptr, 1.0, 0.0, 1.0, 1.0, h->f);
-  stats_reduce(h, h->f, h->Nx, h->Ny, nullptr, nullptr, a);
-  out[1] = a[0];
+  stats_launch(h, 2, h->f, h->Nx, h->Ny, nullptr, nullptr);
   if (h->w4) {
     // volumes Om = alfa Om1 - Om3 do not factor: full weight arrays as one row
-    stats_reduce(h, h->V, h->Nu, 1, h->w4d.Om, h->one, a);
-    stats_reduce(h, h->V + h->Nu, h->Nv, 1, h->w4d.Om + h->Nu, h->one, b);
+    stats_launch(h, 3, h->V, h->Nu, 1, h->w4d.Om, h->one);
+    stats_launch(h, 4, h->V + h->Nu, h->Nv, 1, h->w4d.Om + h->Nu, h->one);
   } else {
-    stats_reduce(h, h->V, h->nux, h->nuy, h->ox, h->oy, a);
-    stats_reduce(h, h->V + h->Nu, h->nvx, h->nvy, h->ovx, h->ovy, b);
+    stats_launch(h, 3, h->V, h->nux, h->nuy, h->ox, h->oy);
+    stats_launch(h, 4, h->V + h->Nu, h->nvx, h->nvy, h->ovx, h->ovy);
   }
-  double umom = a[1], vmom = b[1], k = 0.5 * a[2] + 0.5 * b[2];
+  double S5[kStatSlots][3];
+  stats_collect(h, kStatSlots, S5);
+  out[0] = std::max(S5[0][0], S5[1][0]);
+  out[1] = S5[2][0];
+  double umom = S5[3][1], vmom = S5[4][1], k = 0.5 * S5[3][2] + 0.5 * S5[4][2];
   // Dirichlet boundary contributions with the second-order widths
   // (check_conservation.m:45-76)
   if (h->plan || h->w4) {
