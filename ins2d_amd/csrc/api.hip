@@ -657,6 +657,13 @@ void poisson(ins2d_handle* h, const double* f, double* dp) {
     launch_fft_poisson(h->st, h->fft, f, dp);
     return;
   }
+  if (h->poisson == INS2D_POISSON_DIRECT && !h->w4) {
+    // the early exit inside the first and last GEMM launches
+    launch_fdm_poisson(h->st, h->fdm, f, dp, h->cr, h->cz, nullptr, h->ee_part);
+    h->last_it = 0;
+    h->last_rel = 0.0;
+    return;
+  }
   launch_absmax_part(h->st, f, (size_t)h->Np, h->ee_part);
   poisson_matrix(h, f, dp);
   launch_zero_if_small(h->st, h->ee_part, 1e-14, dp, (size_t)h->Np);
@@ -1273,7 +1280,7 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
     }
     h->nred = (int)std::min<long>(1024, std::max<long>(1, ((long)std::max(h->NV, h->Np) + 255) / 256));
     h->part = dalloc<double>(h.get(), 4 * (size_t)h->nred * kStatSlots);
-    h->ee_part = dalloc<double>(h.get(), 64);
+    h->ee_part = dalloc<double>(h.get(), std::max(64, (h->Ny + 31) / 32));
     HIPCHK(hipDeviceSynchronize());
   } catch (const std::exception& e) {
     g_create_err = e.what();

@@ -22,11 +22,13 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 constexpr int kGemmWaves = 8, kGemmDepth = 4;
 typedef double GemmPart[kGemmWaves][32][33];
 
-template <int EPI>
+// BMAX: also the largest |B| among the operand values this thread loads
+// (all of the tile's 32 columns over the block's threads) into *bmax
+template <int EPI, bool BMAX = false>
 __device__ __forceinline__ void dgemm_tile(int M, int N, int K, const double* __restrict__ A, int lda,
                                            const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc,
                                            const double* __restrict__ lx, const double* __restrict__ ly, int nx0,
-                                           int ny0, int m0, int n0, GemmPart& part) {
+                                           int ny0, int m0, int n0, GemmPart& part, double* bmax = nullptr) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int ma = m0 + lr, mb = m0 + 16 + lr, na = n0 + lr, nb = n0 + 16 + lr;
@@ -42,12 +44,14 @@ __device__ __forceinline__ void dgemm_tile(int M, int N, int K, const double* __
     for (int b = 0; b < 2; ++b) acc[a][b] = double4_t{0.0, 0.0, 0.0, 0.0};
   constexpr int kStep = 4 * kGemmWaves;   // k advance per step of one wave
   double ra[kGemmDepth], rb[kGemmDepth], rc[kGemmDepth], rd[kGemmDepth];
+  double bm = 0.0;
   auto ld = [&](int k, int d) {
     const bool kin = k < K;
     ra[d] = (kin && ina) ? Aa[(size_t)lda * k] : 0.0;
     rb[d] = (kin && inb) ? Ab[(size_t)lda * k] : 0.0;
     rc[d] = (kin && jna) ? Ba[k] : 0.0;
     rd[d] = (kin && jnb) ? Bb[k] : 0.0;
+    if (BMAX) bm = fmax(bm, fmax(fabs(rc[d]), fabs(rd[d])));
   };
   const int kw = 4 * wave + lk;   // this lane's k in the wave's first step
 #pragma unroll
@@ -81,6 +85,7 @@ __device__ __forceinline__ void dgemm_tile(int M, int N, int K, const double* __
     if (EPI == 1) v = (gm == nx0 && gn == ny0) ? 0.0 : -v / (lx[gm] + ly[gn]);
     C[(size_t)gm + (size_t)ldc * gn] = v;
   }
+  if (BMAX) *bmax = bm;
 }
 
 // C = A B as above, LDS-tiled: a workgroup of 256 threads (4 waves) owns the

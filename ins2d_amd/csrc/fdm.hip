@@ -35,24 +35,62 @@ namespace ins2d {
 namespace {
 
 // skip: optional device flag (a converged PCG); the launch does nothing
-// while it is set
-template <int EPI>
+// while it is set.  The early exit of pressure_poisson.m:21-39 (dp = 0 when
+// max |f| < tol) rides on the first and last launches of a direct solve:
+// EXIT 1, the blocks of the first row tile also record the largest |B| of
+// their 32 columns (all K rows; B = f), taken from the operand values the
+// tile loads, in emax[blockIdx.y]; EXIT 2, every
+// block reads the ne partial maxima first and, when they are all below tol,
+// stores zeros instead of its product.
+template <int EPI, int EXIT>
 __global__ void __launch_bounds__(64 * kGemmWaves) k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
                                                           const double* __restrict__ B, int ldb,
                                                           double* __restrict__ C, int ldc,
                                                           const double* __restrict__ lx,
                                                           const double* __restrict__ ly, int nx0, int ny0,
-                                                          const int* __restrict__ skip) {
+                                                          const int* __restrict__ skip, double* __restrict__ emax,
+                                                          int ne, double tol) {
   __shared__ GemmPart part;
+  __shared__ double wmax[kGemmWaves];
+  __shared__ int tiny;
   if (skip && *skip) return;
-  dgemm_tile<EPI>(M, N, K, A, lda, B, ldb, C, ldc, lx, ly, nx0, ny0, blockIdx.x * 32, blockIdx.y * 32, part);
+  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+  if (EXIT == 2) {
+    if (threadIdx.x < 64) {
+      double m = 0.0;
+      for (int e = threadIdx.x; e < ne; e += 64) m = fmax(m, emax[e]);
+      for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+      if (threadIdx.x == 0) tiny = m < tol;
+    }
+    __syncthreads();
+    if (tiny) {
+      for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+        const int gm = m0 + (e & 31), gn = n0 + (e >> 5);
+        if (gm < M && gn < N) C[(size_t)gm + (size_t)ldc * gn] = 0.0;
+      }
+      return;
+    }
+  }
+  double m = 0.0;
+  dgemm_tile<EPI, EXIT == 1>(M, N, K, A, lda, B, ldb, C, ldc, lx, ly, nx0, ny0, m0, n0, part, &m);
+  if (EXIT == 1 && blockIdx.x == 0) {
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double b = 0.0;
+      for (int w = 0; w < kGemmWaves; ++w) b = fmax(b, wmax[w]);
+      emax[blockIdx.y] = b;
+    }
+  }
 }
 
-template <int EPI>
+template <int EPI, int EXIT = 0>
 void gemm(hipStream_t st, const int* skip, int M, int N, int K, const double* A, int lda, const double* B, int ldb,
-          double* C, int ldc, const double* lx = nullptr, const double* ly = nullptr, int nx0 = -1, int ny0 = -1) {
-  hipLaunchKernelGGL(k_dgemm<EPI>, dim3((M + 31) / 32, (N + 31) / 32), dim3(64 * kGemmWaves), 0, st, M, N, K, A, lda, B,
-                     ldb, C, ldc, lx, ly, nx0, ny0, skip);
+          double* C, int ldc, const double* lx = nullptr, const double* ly = nullptr, int nx0 = -1, int ny0 = -1,
+          double* emax = nullptr, double tol = 0.0) {
+  hipLaunchKernelGGL((k_dgemm<EPI, EXIT>), dim3((M + 31) / 32, (N + 31) / 32), dim3(64 * kGemmWaves), 0, st, M, N, K,
+                     A, lda, B, ldb, C, ldc, lx, ly, nx0, ny0, skip, emax, (N + 31) / 32, tol);
 }
 
 // one v_mfma_f64_16x16x4f64 with the operand layout the GEMM uses; the raw
@@ -81,14 +119,21 @@ int fdm_mfma_probe(const double* A, const double* B, double* Draw) {
   return e == hipSuccess ? 0 : 2;
 }
 
-// dp = A^-1 f; w1, w2: Npx*Npy work arrays; nothing while *skip is set
+// dp = A^-1 f; w1, w2: Npx*Npy work arrays; nothing while *skip is set;
+// emax (ceil(Npy / 32) doubles, optional): dp = 0 when max |f| < 1e-14
 void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2,
-                        const int* skip) {
+                        const int* skip, double* emax) {
   const int nx = F.nx, ny = F.ny;
-  gemm<0>(st, skip, nx, ny, nx, F.VxT, nx, f, nx, w1, nx);                           // Vx^T F
+  if (emax)
+    gemm<0, 1>(st, skip, nx, ny, nx, F.VxT, nx, f, nx, w1, nx, nullptr, nullptr, -1, -1, emax);   // Vx^T F, max |f|
+  else
+    gemm<0>(st, skip, nx, ny, nx, F.VxT, nx, f, nx, w1, nx);                                       // Vx^T F
   gemm<1>(st, skip, nx, ny, ny, w1, nx, F.Vy, ny, w2, nx, F.lx, F.ly, F.nx0, F.ny0);   // (.) Vy, scaled
   gemm<0>(st, skip, nx, ny, nx, F.Vx, nx, w2, nx, w1, nx);                            // Vx (.)
-  gemm<0>(st, skip, nx, ny, ny, w1, nx, F.VyT, ny, dp, nx);                           // (.) Vy^T
+  if (emax)
+    gemm<0, 2>(st, skip, nx, ny, ny, w1, nx, F.VyT, ny, dp, nx, nullptr, nullptr, -1, -1, emax, 1e-14);   // (.) Vy^T
+  else
+    gemm<0>(st, skip, nx, ny, ny, w1, nx, F.VyT, ny, dp, nx);                                             // (.) Vy^T
 }
 
 }  // namespace ins2d

@@ -113,9 +113,10 @@ struct FDMPlan {
   const double *Vx, *VxT, *Vy, *VyT, *lx, *ly;
 };
 // dp = A^-1 f by fast diagonalization; the launches do nothing while *skip
-// (optional device flag) is set
+// (optional device flag) is set; with emax (ceil(Npy / 32) doubles) dp = 0
+// when max |f| < 1e-14 (the early exit of pressure_poisson.m:21-39)
 void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2,
-                        const int* skip = nullptr);
+                        const int* skip = nullptr, double* emax = nullptr);
 
 // accelerated PCG (pcg2.hip): optional warm start and two-level preconditioner
 constexpr int kTLMaxCoarse = 2048;   // coarse unknowns solved in one workgroup

@@ -493,6 +493,30 @@ def test_unsteady_step_rejects_periodic_kernels():
     solver.close()
 
 
+def test_direct_early_exit_reads_every_column_tile():
+    """The direct solve's early exit rides on its GEMM launches: the first
+    records max |f| per 32-column tile, the last zeroes dp when all of them
+    are below 1e-14.  On a grid with three column tiles, f below the threshold
+    everywhere but one value just above it in the last tile must be solved
+    (and match the oracle); below it everywhere, dp = 0 exactly."""
+    _say('direct early exit, three column tiles')
+    case = _wall_case('outflow', 40, 70, poisson=1)
+    opts = O.build(case)
+    solver, _, _, _ = host.setup(case)
+    rng = np.random.default_rng(19)
+    f = rng.standard_normal(opts.grid.Np)
+    f *= 0.5e-14 / np.abs(f).max()
+    dp, _, _ = solver.poisson(f)
+    assert np.all(dp == 0.0) and np.all(O.pressure_poisson(f, 0.0, opts) == 0.0)
+    g = f.copy()
+    g[5 + 40 * 69] = 2e-14   # column 69, the third tile
+    dp, _, _ = solver.poisson(g)
+    dpo = O.pressure_poisson(g, 0.0, opts)
+    assert np.abs(dp).max() > 0.0
+    assert _rel(dp, dpo) < 1e-12
+    solver.close()
+
+
 @pytest.mark.parametrize('name', ['fft_reg', 'fft_lds', 'direct', 'cg_jacobi', 'cg_ic0', 'cg_fdm'])
 def test_poisson_early_exit(name):
     """pressure_poisson.m:21-39: a negligible right-hand side returns dp = 0
