@@ -277,9 +277,6 @@ __global__ void k_b_cg_init_fin(const BInst* __restrict__ I, BCG S, double tol2)
   }
 }
 
-// TRI: every instance's pressure Laplacian has the factor widths of
-// lap_tri_at (the host checks), read by scalar loads from its record
-template <bool TRI>
 __global__ void __launch_bounds__(256) k_b_cg_apply(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
                                                    BCG S, const double* __restrict__ p, double* __restrict__ q) {
   __shared__ double red[4];
@@ -290,12 +287,76 @@ __global__ void __launch_bounds__(256) k_b_cg_apply(const BInst* __restrict__ I,
   double acc = 0.0;
   if (i < B.npx && j < B.npy) {
     const long k = B.poff + (long)i + (long)B.npx * j;
-    const double a = TRI ? lap_tri_at(cst(&B.lap), p + B.poff, i, j) : lap_at(B.lap, p + B.poff, i, j);
+    const double a = lap_at(B.lap, p + B.poff, i, j);
     q[k] = a;
     acc = p[k] * a;
   }
   const double s = block_sum(acc, red);
   if (threadIdx.x == 0 && threadIdx.y == 0) S.part[2 * blockIdx.x] = s;
+}
+
+// Tall-tile forms of the per-tile reductions: a TX x kTallRows tile covers
+// kTallR of the TX x TY pressure tiles whose partial sums the per-instance
+// kernels add up over [pt0, pt1).  It writes its sum into the slot of the
+// first of them and zeros into the others, so those sums are unchanged.
+__device__ __forceinline__ void tall_partial(const BInst& B, const Tile& t, double v, double* __restrict__ part,
+                                             int comp) {
+  const int ntx = (B.npx + TX - 1) / TX;
+  const int s0 = B.pt0 + (t.j0 / TY) * ntx + t.i0 / TX;
+  part[2 * s0 + comp] = v;
+  for (int m = 1; m < kTallR; ++m)
+    if (t.j0 + m * TY < B.npy) part[2 * (s0 + m * ntx) + comp] = 0.0;
+}
+
+// k_b_cg_apply over tall tiles, every instance's Laplacian in the widths of
+// lap_tri_at: the x factors of the column loaded once for all its rows
+__global__ void __launch_bounds__(256) k_b_cg_apply_tall(const BInst* __restrict__ I, const Tile* __restrict__ tiles,
+                                                        BCG S, const double* __restrict__ p, double* __restrict__ q) {
+  __shared__ double red[4];
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x;
+  const int npx = B.npx, npy = B.npy;
+  double acc = 0.0;
+  if (i < npx) {
+    const DLap L = cst(&B.lap);
+    const auto a = glb(p + B.poff);
+    const unsigned ld = (unsigned)L.L1.X.ncols;
+    const unsigned i3 = 3u * (unsigned)i;
+    const auto c1x = glb(L.L1.X.col), c2y = glb(L.L2.Y.col);
+    const auto w1x = glb(L.L1.X.val), w2y = glb(L.L2.Y.val);
+    const unsigned x0 = (unsigned)c1x[i3], x1 = (unsigned)c1x[i3 + 1], x2 = (unsigned)c1x[i3 + 2];
+    const double u0 = w1x[i3], u1 = w1x[i3 + 1], u2 = w1x[i3 + 2];
+    const unsigned c2 = (unsigned)glb(L.L2.X.col)[i];
+    const double w2 = glb(L.L2.X.val)[i];
+    double qv[kTallR], pv[kTallR];
+#pragma unroll
+    for (int r = 0; r < kTallR; ++r) {
+      const int j = min(t.j0 + (int)threadIdx.y + TY * r, npy - 1);
+      const unsigned j3 = 3u * (unsigned)j;
+      const unsigned r1 = (unsigned)glb(L.L1.Y.col)[j] * ld;
+      const double w1y = glb(L.L1.Y.val)[j];
+      double s1 = u0 * a[r1 + x0];
+      s1 += u1 * a[r1 + x1];
+      s1 += u2 * a[r1 + x2];
+      double s2 = w2y[j3] * a[(unsigned)c2y[j3] * ld + c2];
+      s2 += w2y[j3 + 1] * a[(unsigned)c2y[j3 + 1] * ld + c2];
+      s2 += w2y[j3 + 2] * a[(unsigned)c2y[j3 + 2] * ld + c2];
+      qv[r] = w1y * s1 + w2 * s2;
+      pv[r] = a[(unsigned)j * ld + (unsigned)i];
+    }
+#pragma unroll
+    for (int r = 0; r < kTallR; ++r) {
+      const int j = t.j0 + (int)threadIdx.y + TY * r;
+      if (j < npy) {
+        q[B.poff + (long)i + (long)npx * j] = qv[r];
+        acc += pv[r] * qv[r];
+      }
+    }
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) tall_partial(B, t, s, S.part, 0);
 }
 
 __global__ void k_b_cg_alpha(const BInst* __restrict__ I, BCG S) {
@@ -637,6 +698,34 @@ __global__ void __launch_bounds__(256) k_bf_rz(const BInst* __restrict__ I, cons
   }
   const double a = block_sum(rz, red);
   if (threadIdx.x == 0 && threadIdx.y == 0) S.part[2 * blockIdx.x + 1] = a;
+}
+
+// k_bf_rz over tall tiles
+__global__ void __launch_bounds__(256) k_bf_rz_tall(const BInst* __restrict__ I, const Tile* __restrict__ tiles, BCG S,
+                                                   const double* __restrict__ r, const double* __restrict__ z) {
+  __shared__ double red[4];
+  const Tile t = tiles[blockIdx.x];
+  if (S.f[4 * t.inst]) return;
+  const BInst& B = I[t.inst];
+  const int i = t.i0 + threadIdx.x;
+  const int npx = B.npx, npy = B.npy;
+  double rz = 0.0;
+  if (i < npx) {
+    const auto rr = glb(r + B.poff), zz = glb(z + B.poff);
+    double a[kTallR], b[kTallR];
+#pragma unroll
+    for (int q = 0; q < kTallR; ++q) {
+      const int j = min(t.j0 + (int)threadIdx.y + TY * q, npy - 1);
+      const unsigned k = (unsigned)i + (unsigned)npx * (unsigned)j;
+      a[q] = rr[k];
+      b[q] = zz[k];
+    }
+#pragma unroll
+    for (int q = 0; q < kTallR; ++q)
+      if (t.j0 + (int)threadIdx.y + TY * q < npy) rz += a[q] * b[q];
+  }
+  const double s = block_sum(rz, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) tall_partial(B, t, s, S.part, 1);
 }
 
 // x += alpha p, r -= alpha q; partial r.r per pressure tile
@@ -1237,10 +1326,18 @@ void launch_tiles_proj(ins2d_batch* b, const double* Vn, const double* Vt, const
 
 // fast-diagonalization PCG iteration, head: q = A p, alpha, x and r, the
 // convergence test (converged instances stop here)
+// q = A p with partials p.q: over tall tiles when every instance's Laplacian
+// has the widths of lap_tri_at, else over the pressure tiles (g, tiles)
 void launch_cg_apply(ins2d_batch* b, dim3 g, dim3 t, unsigned sh, hipStream_t st, const BInst* I, const Tile* tiles,
                      BCG S, const double* p, double* q) {
-  if (b->lap_tri) hipLaunchKernelGGL(k_b_cg_apply<true>, g, t, sh, st, I, tiles, S, p, q);
-  else hipLaunchKernelGGL(k_b_cg_apply<false>, g, t, sh, st, I, tiles, S, p, q);
+  if (b->lap_tri)
+    hipLaunchKernelGGL(k_b_cg_apply_tall, dim3((unsigned)b->pt_tall.size()), t, sh, st, I, b->dpt_tall, S, p, q);
+  else hipLaunchKernelGGL(k_b_cg_apply, g, t, sh, st, I, tiles, S, p, q);
+}
+// partials r.z over tall tiles
+void launch_bf_rz(ins2d_batch* b, const double* r, const double* z) {
+  hipLaunchKernelGGL(k_bf_rz_tall, dim3((unsigned)b->pt_tall.size()), dim3(TX, TY), 0, b->st, b->dinst, b->dpt_tall,
+                     b->cg, r, z);
 }
 
 void fdm_iter_head(ins2d_batch* b) {
@@ -1256,7 +1353,7 @@ void fdm_iter_head(ins2d_batch* b) {
 void fdm_iter_tail(ins2d_batch* b) {
   const unsigned np = (unsigned)b->pt.size();
   fdm_apply(b, b->r, b->z);
-  hipLaunchKernelGGL(k_bf_rz, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->r, b->z);
+  launch_bf_rz(b, b->r, b->z);
   hipLaunchKernelGGL(k_bf_beta, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg);
   hipLaunchKernelGGL(k_b_cg_dir, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->z, b->pp);
 }
@@ -1292,7 +1389,7 @@ void b_poisson(ins2d_batch* b, const double* f) {
     hipLaunchKernelGGL(k_b_clear_flags, dim3((b->n + 255) / 256), dim3(256), 0, b->st, b->cg.f, b->n);
     hipLaunchKernelGGL(k_bf_init, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, f, b->x, b->r);
     fdm_apply(b, b->r, b->z);
-    hipLaunchKernelGGL(k_bf_rz, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->r, b->z);
+    launch_bf_rz(b, b->r, b->z);
     hipLaunchKernelGGL(k_b_cg_init_fin, dim3(b->n), dim3(256), 0, b->st, b->dinst, b->cg, b->tol * b->tol);
     hipLaunchKernelGGL(k_bf_dir0, dim3(np), dim3(TX, TY), 0, b->st, b->dinst, b->dpt, b->cg, b->z, b->pp);
   } else if (b->two_level) {
