@@ -952,7 +952,7 @@ void build_fft(ins2d_handle* h, int order) {
   // power of two in [16, 4096]) sits on one side of the transposes
   F.H = fft_register_lengths(F.Nx, F.Ny) ? nullptr : dalloc<double>(h, 2 * (size_t)F.Nh * h->Ny);
   F.HT = dalloc<double>(h, 2 * (size_t)F.Nh * h->Ny);
-  F.fmax2 = dalloc<unsigned long long>(h, 1);
+  F.fmax2 = dalloc<unsigned long long>(h, 64);   // the early-exit maxima (kFmaxSlots)
   h->has_fft = true;
 }
 

@@ -110,7 +110,7 @@ int ins2d_poisson_fft_noalloc(int Nx, int Ny, double hx, double hy, int order4, 
   const int Nh = Nx / 2 + 1;
   const size_t spec = 2 * (size_t)Nh * (size_t)Ny;
   const size_t tables = 2 * (size_t)Nx + 2 * (size_t)Ny + (size_t)Nh + (size_t)Ny;
-  if (spec > kSpecCap || tables > kTableCap - 1)
+  if (spec > kSpecCap || tables > kTableCap - 64)
     return fail(INS2D_E_ARG, "grid " + std::to_string(Nx) + "x" + std::to_string(Ny) +
                                  " exceeds the scratch capacity (half spectrum (Nx/2+1)*Ny at most 525312 complex "
                                  "values, e.g. 1024x1024)");
@@ -129,9 +129,9 @@ int ins2d_poisson_fft_noalloc(int Nx, int Ny, double hx, double hy, int order4, 
   P.ay = P.ax + Nh;
   P.HT = base + kTableCap;   // 256-byte aligned (kTableCap doubles)
   P.H = nullptr;             // register transforms store the spectra transposed
-  // the early-exit maximum: the last slot of the table region (the tables
-  // take at most 22529 of its 32768 doubles)
-  P.fmax2 = reinterpret_cast<unsigned long long*>(base + kTableCap - 1);
+  // the early-exit maxima: the last 64 slots of the table region (the
+  // tables take at most 22529 of its 32768 doubles)
+  P.fmax2 = reinterpret_cast<unsigned long long*>(base + kTableCap - 64);
   const bool exact4 = order4 && fft_symbol;
   P.c = exact4 ? 1.0 : 4.0 * hx * hy;
   const int n = Nx + Ny + Nh + Ny;
