@@ -74,6 +74,14 @@ struct ins2d_handle {
   // preconditioner of PCG (cg_precond = 3) with its two work arrays
   FDMPlan fdm{};
   bool fdm_pcg = false;
+  // device-iteration mode of the fast-diagonalization PCG inside ins2d_step:
+  // each solve runs one iteration (the rule with the exact inverse as
+  // preconditioner) and counts a missed stopping rule into d_missed; the
+  // call checks once at its end and redoes its steps from the saved state
+  // on the host-checked path when any solve missed
+  bool fdm_dev = false;
+  int* d_missed = nullptr;
+  double *Vsave = nullptr, *psave = nullptr;
   double *fw1 = nullptr, *fw2 = nullptr;
   // IC(0) preconditioner of the reference (cg_precond = 1)
   bool ic0 = false;
@@ -707,6 +715,12 @@ void poisson_matrix(ins2d_handle* h, const double* f, double* dp) {
     launch_cgx(h->st, h->lap, h->cg, 0, f, h->cr, nullptr, nullptr, tol2);
     launch_fdm_poisson(h->st, h->fdm, h->cr, h->cz, h->fw1, h->fw2);
     launch_cgx(h->st, h->lap, h->cg, 1, h->cr, h->cz, h->cp, nullptr, tol2);
+    if (h->fdm_dev) {
+      // one iteration, its outcome counted on the device (no host wait)
+      launch_cgx(h->st, h->lap, h->cg, 2, h->cp, h->cq, dp, h->cr, tol2);
+      launch_cgx_missed(h->st, h->cg, h->d_missed);
+      return;
+    }
     // the host checks after each convergence test (one iteration is the
     // rule with the exact inverse as preconditioner): the preconditioner
     // apply and the direction update run only when another iteration follows
@@ -1463,6 +1477,43 @@ int ins2d_input_check(ins2d_handle* h, double t, int p_initial) {
 int ins2d_step(ins2d_handle* h, double tn, double dt, int nsteps) {
   if (!h || nsteps < 0 || !(dt > 0)) return INS2D_E_ARG;
   API_TRY
+  if (h->fdm_pcg && !h->accel && !h->prof && nsteps > 0) {
+    // fast-diagonalization PCG: one device iteration per solve and one host
+    // wait per call (see fdm_dev); the saved state lets the call start over
+    // on the host-checked path should a solve need a second iteration
+    if (!h->d_missed) {
+      h->d_missed = dalloc<int>(h, 1);
+      h->Vsave = dalloc<double>(h, h->NV);
+      h->psave = dalloc<double>(h, h->Np);
+    }
+    double* V0 = h->V;
+    HIPCHK(hipMemcpyAsync(h->Vsave, h->V, sizeof(double) * h->NV, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->psave, h->p, sizeof(double) * h->Np, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemsetAsync(h->d_missed, 0, sizeof(int), h->st));
+    h->fdm_dev = true;
+    try {
+      for (int n = 0; n < nsteps; ++n) erk_step(h, tn + n * dt, dt);
+    } catch (...) {
+      h->fdm_dev = false;
+      throw;
+    }
+    h->fdm_dev = false;
+    int missed = 0;
+    double rel = 0.0;
+    HIPCHK(hipMemcpyAsync(&missed, h->d_missed, sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(&rel, h->cg.s + 4, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (!missed) {
+      const long long solves = (long long)nsteps * h->s;   // stages 2..s and the additional solve
+      h->last_it = 1;
+      h->last_rel = std::sqrt(rel);
+      h->total_it += solves;
+      return INS2D_OK;
+    }
+    if (h->V != V0) std::swap(h->V, h->Vb);
+    HIPCHK(hipMemcpyAsync(h->V, h->Vsave, sizeof(double) * h->NV, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->p, h->psave, sizeof(double) * h->Np, hipMemcpyDeviceToDevice, h->st));
+  }
   for (int n = 0; n < nsteps; ++n) erk_step(h, tn + n * dt, dt);
   if (h->prof) prof_collect(h);
   return INS2D_OK;

@@ -2328,6 +2328,16 @@ void launch_cgx(hipStream_t st, const DLap& L, const CGStateH& S, int phase, con
   }
 }
 
+// counts a solve whose convergence test has not passed (flag[0] = 0) or
+// that stopped at maxit (flag[2] = 1) into missed[0] (the device-iteration
+// mode of the fast-diagonalization PCG)
+__global__ void k_cgx_missed(const int* __restrict__ flag, int* __restrict__ missed) {
+  if (threadIdx.x == 0 && (!flag[0] || flag[2])) missed[0] += 1;
+}
+void launch_cgx_missed(hipStream_t st, const CGStateH& S, int* missed) {
+  hipLaunchKernelGGL(k_cgx_missed, dim3(1), dim3(64), 0, st, S.flag, missed);
+}
+
 void launch_axpy(hipStream_t st, size_t n, double a, const double* x, double* y) {
   int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(256), 0, st, n, a, x, y);

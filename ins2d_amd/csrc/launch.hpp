@@ -50,6 +50,10 @@ struct CGStateH {
   double* e = nullptr;   // 4 + nblk doubles for launch_cg_iter2 (null: not used)
 };
 
+// missed[0] += 1 when the PCG's convergence test has not passed or it
+// stopped at maxit (one thread)
+void launch_cgx_missed(hipStream_t st, const CGStateH& S, int* missed);
+
 void launch_rhs_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,
                         const double* u, const double* v, const double* p, int mode, double* out_u, double* out_v,
                         const StageCombine& scu, const StageCombine& scv, double* vt_u, double* vt_v);

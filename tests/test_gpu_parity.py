@@ -377,6 +377,39 @@ def test_fdm_pcg_second_iteration():
     solver.close()
 
 
+@pytest.mark.parametrize('acc,nsteps', [(1e-10, 7), (1e-14, 5)])
+def test_fdm_pcg_device_iteration_mode(acc, nsteps):
+    """ins2d_step with the fast-diagonalization PCG runs one iteration per
+    solve and checks the stopping rule once per call on the device; with
+    profiling on, every solve is checked on the host.  Both give the same V
+    and p bit for bit and the same iteration total: at 1e-10 every solve
+    meets the rule after one iteration (no restart), at 1e-14 the first
+    solve needs a second one and the call starts over from its saved state
+    on the host-checked path."""
+    _say('fdm pcg device iteration', acc, nsteps)
+    case = _wall_case('ldc', 40, 30, sx=1.03, poisson=3, CG_acc=acc, CG_maxit=30, CG_precond='fdm')
+    out = []
+    for prof in (True, False):
+        solver, opts, V0, p0 = host.setup(case)
+        solver.set_state(V0, p0)
+        solver.input_check(opts.t_start)
+        solver.profile(prof)
+        dt = opts.dt
+        total0 = solver.solver_stats()[2]   # the initial pressure solve
+        solver.step(opts.t_start, dt, nsteps)
+        solver.step(opts.t_start + nsteps * dt, dt, 2)
+        V, p = solver.get_state()
+        it, rel, total = solver.solver_stats()
+        _say('profile', prof, 'last it', it, 'relres', rel, 'total', total - total0)
+        out.append((V, p, it, total - total0))
+        solver.close()
+    (Va, pa, ita, ta), (Vb, pb, itb, tb) = out
+    assert np.array_equal(Va, Vb) and np.array_equal(pa, pb)
+    assert ta == tb and ita == itb
+    solves = (nsteps + 2) * 4   # RK44: three stage solves and the additional solve per step
+    assert ta == solves if acc == 1e-10 else ta > solves
+
+
 def test_ldc_run_fdm_pcg_matches_oracle():
     """LDC 20x20 fixture configuration, PCG with the fast-diagonalization
     preconditioner on the GPU against the oracle's direct solve."""
