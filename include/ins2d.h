@@ -90,11 +90,12 @@ typedef struct {
   int cg_precond;            /* PCG modes: 0 Jacobi; 1 IC(0) as the reference
                                 (make_cholinc.m, cg_matlab.m:135-198), one
                                 workgroup with wavefront triangular solves,
-                                up to 2^20 cells; 2 (ensemble only) Jacobi plus
-                                an aggregation coarse-grid correction, as
-                                ins2d_set_poisson_accel flag 2 (single case);
-                                3 (ensemble only) the fast-diagonalization
-                                inverse of each instance's pressure matrix */
+                                up to 2^20 cells; 2 Jacobi plus an
+                                aggregation coarse-grid correction (single
+                                case: ins2d_set_poisson_accel flag 2, set at
+                                create); 3 the fast-diagonalization inverse of
+                                the pressure matrix (of each instance in an
+                                ensemble) */
 } ins2d_case_desc;
 
 /* operator construction: replaces operators.m -> operator_mesh/averaging/
@@ -136,7 +137,12 @@ int ins2d_input_check(ins2d_handle* h, double t, int p_initial);
 
 /* nsteps of time_ERK.m:1-173 (with pressure_additional_solve.m at the end of
  * each step) on the resident state, t = tn, tn + dt, ... (steady boundary
- * values: those of the last ins2d_set_bc) */
+ * values: those of the last ins2d_set_bc).  With cg_precond 3 (and profiling
+ * off) each pressure solve runs one PCG iteration whose outcome the device
+ * records, and the call waits on the host once; when a solve missed the
+ * stopping rule, the call restores the state it started from and redoes its
+ * steps with the convergence test read after every iteration (the results
+ * are those of that path bit for bit). */
 int ins2d_step(ins2d_handle* h, double tn, double dt, int nsteps);
 
 /* time derivatives of the boundary values at the time of the last
