@@ -258,7 +258,7 @@ def stage_fields(RK):
     return [2 + 2 + (2 if keep[i] else 0) + 2 * sum(1 for j in range(i) if At[i, j] != 0) for i in range(s)]
 
 
-def kernel_table(raw, RK, path, poisson, cells, Nx, assembled=False):
+def kernel_table(raw, RK, poisson, cells, Nx, assembled=False):
     """Per kernel class: launches, time per launch and achieved bandwidth on
     algorithmic bytes (8 B per FP64 value a launch must read or write once
     per cell), weighted over the launches the profile pass made.  raw maps
@@ -278,12 +278,16 @@ def kernel_table(raw, RK, path, poisson, cells, Nx, assembled=False):
     # divergence: stage f = M (Vn/dt + Vtemp) (5 fields), additional f = M F (3)
     n1, n5 = raw[1][1], raw[5][1]
     div_fields = (n1 * 5 + n5 * 3) / max(n1 + n5, 1)
-    # FFT solve: lower bound read f, write dp (2 fields); on the second-order
-    # periodic kernels the stage solve reads Vn, Vtemp in place of f (5) and
-    # the additional solve reads F and p and writes p (4); at fourth order the
-    # additional solve reads f and p and writes p (3)
+    # FFT solve: lower bound read f, write dp (2 fields); with the divergence
+    # fused the stage solve reads Vn, Vtemp in place of f (5) and the
+    # additional solve reads F and p and writes p (4); else the additional
+    # solve reads f and p and writes p (3)
     n2, n6 = raw[2][1], raw[6][1]
-    if fft and path == 2:
+    # the divergence computed inside the forward row transform (second-order
+    # periodic kernels; fourth order with 4096-point rows): no divergence
+    # launches of their own
+    fused = fft and raw[8][1] > 0 and raw[1][1] + raw[5][1] == 0
+    if fused:
         sol_fields = (n2 * 5 + n6 * 4) / max(n2 + n6, 1)
         sol_model = 'stage: read Vn, Vtemp, write dp (divergence fused); additional: read F, p, write p'
     elif fft:
@@ -313,7 +317,7 @@ def kernel_table(raw, RK, path, poisson, cells, Nx, assembled=False):
         # the kernels of the register-transform FFT solve; the half spectrum
         # holds Nx/2 + 1 complex values per row of Nx cells
         spec = 2.0 * (Nx // 2 + 1) / Nx
-        if path == 2:
+        if fused:
             fwd = (n2 * (4 + spec) + n6 * (2 + spec)) / max(n2 + n6, 1)
             fwd_model = 'stage: read Vn, Vtemp (divergence fused), write the half spectrum; additional: read F'
         else:
@@ -453,7 +457,7 @@ def main():
     t += nprof * dt
     diag = solver.diagnostics(t)
     assembled = bool(opts.prm.get('order4', 0)) and solver.path == 0
-    kernels = kernel_table(raw, RK, solver.path, opts.prm.get('poisson', 6), cells, opts.grid.Nx, assembled)
+    kernels = kernel_table(raw, RK, opts.prm.get('poisson', 6), cells, opts.grid.Nx, assembled)
     # the dominant kernel: the most time per step among single kernels (the
     # pressure solve is a class of several kernels when it is an FFT)
     dom = max((v for v in kernels.values() if v['ms_total'] and v['single_kernel']), key=lambda v: v['ms_total'])

@@ -792,10 +792,10 @@ void poisson_matrix(ins2d_handle* h, const double* f, double* dp) {
 void div_poisson(ins2d_handle* h, const double* A, const double* B, double ca, double cb, double ycoef, double oscale,
                  double* out, bool acc) {
   const bool reg = h->poisson == INS2D_POISSON_FFT && fft_register_lengths(h->Nx, h->Ny);
-  if (reg && h->path == 2) {
+  if (reg && (h->path == 2 || (h->path == 4 && fft_fused_div4(h->Nx, h->Ny)))) {
     ProfScope ps(h, 2);
     FftProf fp(h);
-    launch_fft_poisson_fused(h->st, h->fft, nullptr, h->hx, h->hy, A, B, ca, cb, oscale, out, acc);
+    launch_fft_poisson_fused(h->st, h->fft, nullptr, h->hx, h->hy, A, B, ca, cb, oscale, out, acc, h->path);
     return;
   }
   div(h, A, B, ca, cb, ycoef, oscale, h->f);

@@ -894,7 +894,8 @@ struct DivSrc {
   const double* A;
   const double* B;
   double ca, cb, oscale, hx, hy;
-  int pair;   // A, B 16-byte aligned: two elements per thread, 16-byte loads
+  int pair;    // A, B 16-byte aligned: two elements per thread, 16-byte loads
+  int order;   // 2, or 4: the fourth-order divergence of k_div_per4s
 };
 
 // rows y0 = 2b and y1 = 2b+1 packed as one complex row, half spectra out
@@ -1143,29 +1144,15 @@ __global__ void __launch_bounds__(4096 / E) k_fftr_cols_solve_s(int Nx, int Ny, 
 // partner element Nx - k of each k is fetched through LDS (real parts, then
 // imaginary parts) to split the two half spectra.  Nx / E threads, Nx doubles
 // of LDS.
+// the two real rows y0, y1 transformed as one complex row (element
+// k = tid + m Q in a[m]) split into their half spectra, stored transposed
+// (H[k Ny + y]); zr: the Nx-double LDS array of the split exchange
 template <int E>
-__global__ void __launch_bounds__(4096 / E) k_fftr_r2c_rows_s(int Nx, int Ny, int Nh, int P,
-                                                           const double2* __restrict__ tw, const double* __restrict__ f,
-                                                           double2* __restrict__ H,
-                                                           unsigned long long* __restrict__ fmax2) {
-  extern __shared__ double zr[];
-  fmax2_reset(fmax2);
-  const int y0 = 2 * xcd_row_pair(blockIdx.x, gridDim.x), y1 = y0 + 1;
-  const bool has1 = y1 < Ny;
+__device__ __forceinline__ void r2c_split_store(double2 (&a)[E], double* zr, int Nx, int Ny, int Nh, int y0, bool has1,
+                                                double2* __restrict__ H) {
   const int tid = threadIdx.x, Q = Nx / E;
   const bool act = tid < Q;
-  const double* f0 = f + (size_t)y0 * Nx;
-  const double* f1 = f + (size_t)(has1 ? y1 : y0) * Nx;
-  double2 a[E];
-  if (act) {
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int n = tid + m * Q;
-      a[m] = make_double2(f0[n], has1 ? f1[n] : 0.0);
-    }
-  }
-  fft_split_passes<E, false>(a, zr, Nx, P, tw);
-  // element k = tid + m Q is in a[m]; its partner Nx - k (mod Nx) through LDS
+  // the partner Nx - k (mod Nx) of element k through LDS
   double2 b[E];
   __syncthreads();
   if (act) {
@@ -1198,6 +1185,101 @@ __global__ void __launch_bounds__(4096 / E) k_fftr_r2c_rows_s(int Nx, int Ny, in
       }
     }
   }
+}
+
+template <int E>
+__global__ void __launch_bounds__(4096 / E) k_fftr_r2c_rows_s(int Nx, int Ny, int Nh, int P,
+                                                           const double2* __restrict__ tw, const double* __restrict__ f,
+                                                           double2* __restrict__ H,
+                                                           unsigned long long* __restrict__ fmax2) {
+  extern __shared__ double zr[];
+  fmax2_reset(fmax2);
+  const int y0 = 2 * xcd_row_pair(blockIdx.x, gridDim.x), y1 = y0 + 1;
+  const bool has1 = y1 < Ny;
+  const int tid = threadIdx.x, Q = Nx / E;
+  const bool act = tid < Q;
+  const double* f0 = f + (size_t)y0 * Nx;
+  const double* f1 = f + (size_t)(has1 ? y1 : y0) * Nx;
+  double2 a[E];
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int n = tid + m * Q;
+      a[m] = make_double2(f0[n], has1 ? f1[n] : 0.0);
+    }
+  }
+  fft_split_passes<E, false>(a, zr, Nx, P, tw);
+  r2c_split_store<E>(a, zr, Nx, Ny, Nh, y0, has1, H);
+}
+
+// k_fftr_r2c_rows_s with its rows the fourth-order periodic divergence of
+// k_div_per4s, f = oscale (81 hy (W_u(x+1) - W_u(x)) - 3 hy (W_u(x+2) -
+// W_u(x-1)) + 81 hx (W_v(y+1) - W_v(y)) - 3 hx (W_v(y+2) - W_v(y-1))),
+// W = ca A + cb B, taken straight from the velocity (f is not stored), in
+// the same arithmetic order: u of each of the two rows staged through the
+// LDS array for its x halo, v of the rows y0-1 .. y0+3 loaded per element
+template <int E, bool HASB>
+__global__ void __launch_bounds__(4096 / E, 3) k_fftr_r2c_rows_sdiv4(int Nx, int Ny, int Nh, int P,
+                                                                  const double2* __restrict__ tw, DivSrc D,
+                                                                  double2* __restrict__ H,
+                                                                  unsigned long long* __restrict__ fmax2) {
+  extern __shared__ double zr[];
+  fmax2_reset(fmax2);
+  const int y0 = 2 * xcd_row_pair(blockIdx.x, gridDim.x), y1 = y0 + 1;
+  const bool has1 = y1 < Ny;
+  const int tid = threadIdx.x, Q = Nx / E;
+  const bool act = tid < Q;
+  const double* __restrict__ A = D.A;
+  const double* __restrict__ B = D.B;
+  const double ca = D.ca, cb = D.cb, hx = D.hx, hy = D.hy;
+  auto W = [&](size_t k) {
+    if constexpr (HASB) return ca * A[k] + cb * B[k];
+    else return ca * A[k];
+  };
+  double2 a[E];
+  // the u terms of rows y0 and y1
+  for (int r = 0; r < 2; ++r) {
+    const size_t row = (size_t)Nx * (r == 0 || !has1 ? y0 : y1);
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int n = tid + m * Q;
+        zr[n] = W(row + n);
+      }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int n = tid + m * Q;
+        const double um1 = zr[(n - 1) & (Nx - 1)], u0 = zr[n], up1 = zr[(n + 1) & (Nx - 1)],
+                     up2 = zr[(n + 2) & (Nx - 1)];
+        const double su = 81.0 * hy * (up1 - u0) - 3.0 * hy * (up2 - um1);
+        if (r == 0) a[m].x = su;
+        else a[m].y = has1 ? su : 0.0;
+      }
+    }
+    __syncthreads();
+  }
+  // the v terms: rows y0-1 .. y0+3 of this thread's columns
+  if (act) {
+    const size_t N = (size_t)Nx * Ny;
+    const size_t rm1 = N + (size_t)Nx * wrap(y0 - 1, Ny), r0 = N + (size_t)Nx * y0,
+                 r1 = N + (size_t)Nx * wrap(y0 + 1, Ny), r2 = N + (size_t)Nx * wrap(y0 + 2, Ny),
+                 r3 = N + (size_t)Nx * wrap(y0 + 3, Ny);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int n = tid + m * Q;
+      // branch-free (row y0 + 3 wraps; without a row y1 the value is
+      // dropped), so that both rows' terms are formed as the loads arrive
+      const double vm1 = W(rm1 + n), v0 = W(r0 + n), vp1 = W(r1 + n), vp2 = W(r2 + n), vp3 = W(r3 + n);
+      a[m].x = D.oscale * (a[m].x + 81.0 * hx * (vp1 - v0) - 3.0 * hx * (vp2 - vm1));
+      const double s1 = D.oscale * (a[m].y + 81.0 * hx * (vp2 - vp1) - 3.0 * hx * (vp3 - v0));
+      a[m].y = has1 ? s1 : 0.0;
+    }
+  }
+  fft_split_passes<E, false>(a, zr, Nx, P, tw);
+  r2c_split_store<E>(a, zr, Nx, Ny, Nh, y0, has1, H);
 }
 
 // k_fftr_c2r_rows (transposed spectra) for Nx = E^P with the split exchange:
@@ -2112,7 +2194,15 @@ static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, doubl
     // the divergence rows are staged by a plain loop before the transform:
     // as many threads as the launch bound allows keep more of its loads in
     // flight (the transform's passes leave the extra threads idle)
-    if (D)
+    if (D && D->order == 4) {
+      if (!TR || !split_fft(Nx)) throw std::runtime_error("fourth-order fused divergence: 4096-point rows only");
+      if (D->B)
+        fft_launch(0, k_fftr_r2c_rows_sdiv4<kSplitE, true>, gr, dim3(Nx / kSplitE), (unsigned)(Nx * sizeof(double)),
+                   st, Nx, Ny, Nh, kSplitP, (const double2*)P.twx, *D, H, P.fmax2);
+      else
+        fft_launch(0, k_fftr_r2c_rows_sdiv4<kSplitE, false>, gr, dim3(Nx / kSplitE), (unsigned)(Nx * sizeof(double)),
+                   st, Nx, Ny, Nh, kSplitP, (const double2*)P.twx, *D, H, P.fmax2);
+    } else if (D)
       fft_launch(0, k_fftr_r2c_rows<E, TR, true>, gr, dim3(fft_block<E>(0)), sh, st, Nx, Ny, Nh, ilog2(Nx),
                  (const double2*)P.twx, (const double*)nullptr, *D, H, P.fmax2);
     else if (TR && split_fft(Nx))
@@ -2120,7 +2210,7 @@ static void fftr_launch(hipStream_t st, const FFTPlan& P, const double* f, doubl
                  Nh, kSplitP, (const double2*)P.twx, f, H, P.fmax2);
     else
       fft_launch(0, k_fftr_r2c_rows<E, TR, false>, gr, bl, sh, st, Nx, Ny, Nh, ilog2(Nx), (const double2*)P.twx, f,
-                 DivSrc{}, H, P.fmax2);
+                 DivSrc{nullptr, nullptr, 0.0, 0.0, 0.0, 0.0, 0.0, 0, 2}, H, P.fmax2);
   } else if (which == 1) {
     if (split_fft(Ny))
       fft_launch(1, k_fftr_cols_solve_s<kSplitE>, dim3(Nh), dim3(Ny / kSplitE), (unsigned)(Ny * sizeof(double)), st,
@@ -2164,12 +2254,14 @@ bool fft_register_lengths(int Nx, int Ny) {
   return fftr_ok(Nx, pow2(Nx)) && fftr_ok(Ny, pow2(Ny));
 }
 
+bool fft_fused_div4(int Nx, int Ny) { return fft_register_lengths(Nx, Ny) && split_fft(Nx); }
+
 void launch_fft_poisson_fused(hipStream_t st, const FFTPlan& P, const double* f, double hx, double hy,
                               const double* A, const double* B, double ca, double cb, double oscale, double* out,
-                              bool acc) {
+                              bool acc, int order) {
   if (!fft_register_lengths(P.Nx, P.Ny))
     throw std::runtime_error("fused FFT solve: needs power-of-two lengths 16 .. 4096");
-  const DivSrc D{A, B, ca, cb, oscale, hx, hy, aligned16(A) && aligned16(B) ? 1 : 0};
+  const DivSrc D{A, B, ca, cb, oscale, hx, hy, aligned16(A) && aligned16(B) ? 1 : 0, order};
   fftr<true>(st, P, f, out, 0, A ? &D : nullptr);
   fftr<true>(st, P, f, out, 1);
   fftr<true>(st, P, f, out, 2, nullptr, acc);

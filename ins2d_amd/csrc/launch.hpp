@@ -73,15 +73,19 @@ void launch_proj_per(hipStream_t st, int order, int Nx, int Ny, double hx, doubl
                      const double* Vt, const double* dp, double dt, double c, double* V);
 void launch_fft_poisson(hipStream_t st, const FFTPlan& P, const double* f, double* dp);
 // the register-transform solve (fft_register_lengths only) with its ends
-// fused: A non-null: the right-hand side is the second-order periodic
-// divergence oscale (hy dx + hx dy)(ca A + cb B) of the velocity A, B
-// (B may be null), computed in the first transform (f unused), else f;
+// fused: A non-null: the right-hand side is the periodic divergence of
+// order 2 (oscale (hy dx + hx dy)(ca A + cb B)) or 4 (that of
+// launch_div_per, fft_fused_div4 lengths only) of the velocity A, B (B may
+// be null), computed in the first transform (f unused), else f;
 // acc: out += dp, else out = dp
 void launch_fft_poisson_fused(hipStream_t st, const FFTPlan& P, const double* f, double hx, double hy,
                               const double* A, const double* B, double ca, double cb, double oscale, double* out,
-                              bool acc);
+                              bool acc, int order = 2);
 // both lengths take the register transforms (no row-major spectra buffer)
 bool fft_register_lengths(int Nx, int Ny);
+// the lengths whose forward row transform computes the fourth-order
+// divergence itself (4096-point rows)
+bool fft_fused_div4(int Nx, int Ny);
 void launch_lap(hipStream_t st, const DLap& L, const double* p, double* q);
 // the early exit of pressure_poisson.m:32-38: max |f| as 64 partial maxima
 // (part: 64 doubles), then x = 0 when their maximum is below tol
