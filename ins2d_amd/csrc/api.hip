@@ -805,6 +805,14 @@ void div_poisson(ins2d_handle* h, const double* A, const double* B, double ca, d
     launch_fft_poisson_fused(h->st, h->fft, h->f, h->hx, h->hy, nullptr, nullptr, 0.0, 0.0, 0.0, out, true);
     return;
   }
+  if (acc && h->poisson == INS2D_POISSON_DIRECT && !h->w4) {
+    // out += dp in the last GEMM of the solve (left as it is on the early exit)
+    ProfScope ps(h, 2);
+    launch_fdm_poisson(h->st, h->fdm, h->f, out, h->cr, h->cz, nullptr, h->ee_part, true);
+    h->last_it = 0;
+    h->last_rel = 0.0;
+    return;
+  }
   poisson(h, h->f, acc ? h->dp : out);
   if (acc) launch_axpy(h->st, (size_t)h->Np, 1.0, h->dp, out);
 }

@@ -14,7 +14,8 @@ namespace ins2d {
 // other.  Operands come straight from memory (L2), kGemmDepth k-steps ahead of
 // their use (a register ring), which covers the L2 latency with one or two
 // waves per SIMD; the partial tiles are summed through LDS (part).  EPI 1
-// scales C_ij by -1 / (lx_i + ly_j) and zeroes the null mode (nx0, ny0).
+// scales C_ij by -1 / (lx_i + ly_j) and zeroes the null mode (nx0, ny0);
+// EPI 2 adds the product into C.
 // Operand layout of the 16x16x4 instruction: lane l holds A[l % 16][l / 16]
 // and B[l / 16][l % 16]; the result lane l, register r holds
 // D[l / 16 + 4 r][l % 16] (pinned by a GPU test through ins2d_fdm_mfma_probe).
@@ -83,7 +84,8 @@ __device__ __forceinline__ void dgemm_tile(int M, int N, int K, const double* __
 #pragma unroll
     for (int w = 0; w < kGemmWaves; ++w) v += part[w][mm][nn];
     if (EPI == 1) v = (gm == nx0 && gn == ny0) ? 0.0 : -v / (lx[gm] + ly[gn]);
-    C[(size_t)gm + (size_t)ldc * gn] = v;
+    if (EPI == 2) C[(size_t)gm + (size_t)ldc * gn] += v;
+    else C[(size_t)gm + (size_t)ldc * gn] = v;
   }
   if (BMAX) *bmax = bm;
 }

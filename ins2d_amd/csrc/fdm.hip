@@ -41,7 +41,8 @@ namespace {
 // their 32 columns (all K rows; B = f), taken from the operand values the
 // tile loads, in emax[blockIdx.y]; EXIT 2, every
 // block reads the ne partial maxima first and, when they are all below tol,
-// stores zeros instead of its product.
+// stores zeros instead of its product (EPI 2, which adds the product into
+// C: leaves C as it is).
 template <int EPI, int EXIT>
 __global__ void __launch_bounds__(64 * kGemmWaves) k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
                                                           const double* __restrict__ B, int ldb,
@@ -64,6 +65,7 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_dgemm(int M, int N, int K, 
     }
     __syncthreads();
     if (tiny) {
+      if (EPI == 2) return;   // adds zero
       for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
         const int gm = m0 + (e & 31), gn = n0 + (e >> 5);
         if (gm < M && gn < N) C[(size_t)gm + (size_t)ldc * gn] = 0.0;
@@ -122,7 +124,7 @@ int fdm_mfma_probe(const double* A, const double* B, double* Draw) {
 // dp = A^-1 f; w1, w2: Npx*Npy work arrays; nothing while *skip is set;
 // emax (ceil(Npy / 32) doubles, optional): dp = 0 when max |f| < 1e-14
 void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2,
-                        const int* skip, double* emax) {
+                        const int* skip, double* emax, bool acc) {
   const int nx = F.nx, ny = F.ny;
   if (emax)
     gemm<0, 1>(st, skip, nx, ny, nx, F.VxT, nx, f, nx, w1, nx, nullptr, nullptr, -1, -1, emax);   // Vx^T F, max |f|
@@ -130,8 +132,12 @@ void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, doubl
     gemm<0>(st, skip, nx, ny, nx, F.VxT, nx, f, nx, w1, nx);                                       // Vx^T F
   gemm<1>(st, skip, nx, ny, ny, w1, nx, F.Vy, ny, w2, nx, F.lx, F.ly, F.nx0, F.ny0);   // (.) Vy, scaled
   gemm<0>(st, skip, nx, ny, nx, F.Vx, nx, w2, nx, w1, nx);                            // Vx (.)
-  if (emax)
+  if (emax && acc)
+    gemm<2, 2>(st, skip, nx, ny, ny, w1, nx, F.VyT, ny, dp, nx, nullptr, nullptr, -1, -1, emax, 1e-14);   // += (.) Vy^T
+  else if (emax)
     gemm<0, 2>(st, skip, nx, ny, ny, w1, nx, F.VyT, ny, dp, nx, nullptr, nullptr, -1, -1, emax, 1e-14);   // (.) Vy^T
+  else if (acc)
+    throw std::runtime_error("fast-diagonalization solve: accumulation needs the early-exit partials");
   else
     gemm<0>(st, skip, nx, ny, ny, w1, nx, F.VyT, ny, dp, nx);                                             // (.) Vy^T
 }

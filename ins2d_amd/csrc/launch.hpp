@@ -122,9 +122,10 @@ struct FDMPlan {
 };
 // dp = A^-1 f by fast diagonalization; the launches do nothing while *skip
 // (optional device flag) is set; with emax (ceil(Npy / 32) doubles) dp = 0
-// when max |f| < 1e-14 (the early exit of pressure_poisson.m:21-39)
+// when max |f| < 1e-14 (the early exit of pressure_poisson.m:21-39); acc
+// (with emax only): dp += A^-1 f, dp unchanged on the early exit
 void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2,
-                        const int* skip = nullptr, double* emax = nullptr);
+                        const int* skip = nullptr, double* emax = nullptr, bool acc = false);
 
 // accelerated PCG (pcg2.hip): optional warm start and two-level preconditioner
 constexpr int kTLMaxCoarse = 2048;   // coarse unknowns solved in one workgroup
