@@ -1302,7 +1302,7 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
     }
     h->nred = (int)std::min<long>(1024, std::max<long>(1, ((long)std::max(h->NV, h->Np) + 255) / 256));
     h->part = dalloc<double>(h.get(), 4 * (size_t)h->nred * kStatSlots);
-    h->ee_part = dalloc<double>(h.get(), std::max(64, (h->Ny + 31) / 32));
+    h->ee_part = dalloc<double>(h.get(), std::max(64, (h->Ny + 15) / 16));
     HIPCHK(hipDeviceSynchronize());
   } catch (const std::exception& e) {
     g_create_err = e.what();

@@ -25,15 +25,19 @@ typedef double GemmPart[kGemmWaves][32][33];
 
 // BMAX: also the largest |B| among the operand values this thread loads
 // (all of the tile's 32 columns over the block's threads) into *bmax
-template <int EPI, bool BMAX = false>
+// TN: the tile's width, 32 or 16 columns (one 16-column block: half the
+// accumulators, twice the workgroups for a given C)
+template <int EPI, bool BMAX = false, int TN = 32>
 __device__ __forceinline__ void dgemm_tile(int M, int N, int K, const double* __restrict__ A, int lda,
                                            const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc,
                                            const double* __restrict__ lx, const double* __restrict__ ly, int nx0,
                                            int ny0, int m0, int n0, GemmPart& part, double* bmax = nullptr) {
+  static_assert(TN == 32 || TN == 16, "tile width 32 or 16");
+  constexpr int NB = TN / 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int ma = m0 + lr, mb = m0 + 16 + lr, na = n0 + lr, nb = n0 + 16 + lr;
-  const bool ina = ma < M, inb = mb < M, jna = na < N, jnb = nb < N;
+  const bool ina = ma < M, inb = mb < M, jna = na < N, jnb = NB == 2 && nb < N;
   const double* Aa = A + (ina ? ma : 0);
   const double* Ab = A + (inb ? mb : 0);
   const double* Ba = B + (size_t)ldb * (jna ? na : 0);
@@ -51,7 +55,7 @@ __device__ __forceinline__ void dgemm_tile(int M, int N, int K, const double* __
     ra[d] = (kin && ina) ? Aa[(size_t)lda * k] : 0.0;
     rb[d] = (kin && inb) ? Ab[(size_t)lda * k] : 0.0;
     rc[d] = (kin && jna) ? Ba[k] : 0.0;
-    rd[d] = (kin && jnb) ? Bb[k] : 0.0;
+    rd[d] = (NB == 2 && kin && jnb) ? Bb[k] : 0.0;
     if (BMAX) bm = fmax(bm, fmax(fabs(rc[d]), fabs(rd[d])));
   };
   const int kw = 4 * wave + lk;   // this lane's k in the wave's first step
@@ -62,9 +66,9 @@ __device__ __forceinline__ void dgemm_tile(int M, int N, int K, const double* __
     for (int d = 0; d < kGemmDepth; ++d) {
       if (k0 + d * kStep < K) {
         acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[d], rc[d], acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[d], rd[d], acc[0][1], 0, 0, 0);
+        if (NB == 2) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[d], rd[d], acc[0][1], 0, 0, 0);
         acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(rb[d], rc[d], acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(rb[d], rd[d], acc[1][1], 0, 0, 0);
+        if (NB == 2) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(rb[d], rd[d], acc[1][1], 0, 0, 0);
       }
       ld(k0 - 4 * wave + kw + (d + kGemmDepth) * kStep, d);
     }
@@ -72,11 +76,11 @@ __device__ __forceinline__ void dgemm_tile(int M, int N, int K, const double* __
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) part[wave][16 * a + lk + 4 * r][16 * b + lr] = acc[a][b][r];
   __syncthreads();
-  for (int e = threadIdx.x; e < 32 * 32; e += 64 * kGemmWaves) {
+  for (int e = threadIdx.x; e < 32 * TN; e += 64 * kGemmWaves) {
     const int mm = e & 31, nn = e >> 5;
     const int gm = m0 + mm, gn = n0 + nn;
     if (gm >= M || gn >= N) continue;

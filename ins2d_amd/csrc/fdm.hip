@@ -43,7 +43,7 @@ namespace {
 // block reads the ne partial maxima first and, when they are all below tol,
 // stores zeros instead of its product (EPI 2, which adds the product into
 // C: leaves C as it is).
-template <int EPI, int EXIT>
+template <int EPI, int EXIT, int TN>
 __global__ void __launch_bounds__(64 * kGemmWaves) k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
                                                           const double* __restrict__ B, int ldb,
                                                           double* __restrict__ C, int ldc,
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_dgemm(int M, int N, int K, 
   __shared__ double wmax[kGemmWaves];
   __shared__ int tiny;
   if (skip && *skip) return;
-  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * TN;
   if (EXIT == 2) {
     if (threadIdx.x < 64) {
       double m = 0.0;
@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_dgemm(int M, int N, int K, 
     __syncthreads();
     if (tiny) {
       if (EPI == 2) return;   // adds zero
-      for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+      for (int e = threadIdx.x; e < 32 * TN; e += blockDim.x) {
         const int gm = m0 + (e & 31), gn = n0 + (e >> 5);
         if (gm < M && gn < N) C[(size_t)gm + (size_t)ldc * gn] = 0.0;
       }
@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_dgemm(int M, int N, int K, 
     }
   }
   double m = 0.0;
-  dgemm_tile<EPI, EXIT == 1>(M, N, K, A, lda, B, ldb, C, ldc, lx, ly, nx0, ny0, m0, n0, part, &m);
+  dgemm_tile<EPI, EXIT == 1, TN>(M, N, K, A, lda, B, ldb, C, ldc, lx, ly, nx0, ny0, m0, n0, part, &m);
   if (EXIT == 1 && blockIdx.x == 0) {
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
@@ -87,12 +87,21 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_dgemm(int M, int N, int K, 
   }
 }
 
+// 32 x 16 tiles when 32 x 32 ones would give fewer than 128 workgroups (the
+// small grids: LDC 64^2 17.9 against 20.5 us per solve; at BFS 1024 x 256,
+// 256 workgroups of 32 x 32, the wider tiles: 62.9 against 69.9 us).  The
+// first and last launches of a solve share M and N, so they agree on the
+// number of early-exit partials.
 template <int EPI, int EXIT = 0>
 void gemm(hipStream_t st, const int* skip, int M, int N, int K, const double* A, int lda, const double* B, int ldb,
           double* C, int ldc, const double* lx = nullptr, const double* ly = nullptr, int nx0 = -1, int ny0 = -1,
           double* emax = nullptr, double tol = 0.0) {
-  hipLaunchKernelGGL((k_dgemm<EPI, EXIT>), dim3((M + 31) / 32, (N + 31) / 32), dim3(64 * kGemmWaves), 0, st, M, N, K,
-                     A, lda, B, ldb, C, ldc, lx, ly, nx0, ny0, skip, emax, (N + 31) / 32, tol);
+  if (((M + 31) / 32) * ((N + 31) / 32) < 128)
+    hipLaunchKernelGGL((k_dgemm<EPI, EXIT, 16>), dim3((M + 31) / 32, (N + 15) / 16), dim3(64 * kGemmWaves), 0, st, M,
+                       N, K, A, lda, B, ldb, C, ldc, lx, ly, nx0, ny0, skip, emax, (N + 15) / 16, tol);
+  else
+    hipLaunchKernelGGL((k_dgemm<EPI, EXIT, 32>), dim3((M + 31) / 32, (N + 31) / 32), dim3(64 * kGemmWaves), 0, st, M,
+                       N, K, A, lda, B, ldb, C, ldc, lx, ly, nx0, ny0, skip, emax, (N + 31) / 32, tol);
 }
 
 // one v_mfma_f64_16x16x4f64 with the operand layout the GEMM uses; the raw

@@ -121,7 +121,7 @@ struct FDMPlan {
   const double *Vx, *VxT, *Vy, *VyT, *lx, *ly;
 };
 // dp = A^-1 f by fast diagonalization; the launches do nothing while *skip
-// (optional device flag) is set; with emax (ceil(Npy / 32) doubles) dp = 0
+// (optional device flag) is set; with emax (ceil(Npy / 16) doubles) dp = 0
 // when max |f| < 1e-14 (the early exit of pressure_poisson.m:21-39); acc
 // (with emax only): dp += A^-1 f, dp unchanged on the early exit
 void launch_fdm_poisson(hipStream_t st, const FDMPlan& F, const double* f, double* dp, double* w1, double* w2,
