@@ -1116,7 +1116,8 @@ int ins2d_create(const ins2d_case_desc* d, ins2d_handle** out) {
         h->hCu.uni.fast = fu.empty() ? nullptr : dupload(h.get(), fu);
         h->hCv.uni.fast = fv.empty() ? nullptr : dupload(h.get(), fv);
         std::vector<RTile> tiles;
-        if (split_tiles(0, 0, P.Nux_in, P.Nuy_in, fu, 64, tiles) && split_tiles(0, 1, P.Nvx_in, P.Nvy_in, fv, 64, tiles)) {
+        if (split_tiles(0, 0, P.Nux_in, P.Nuy_in, fu, 64, tiles, rhs_split_rows()) &&
+            split_tiles(0, 1, P.Nvx_in, P.Nvy_in, fv, 64, tiles, rhs_split_rows())) {
           h->split = dupload(h.get(), tiles);
           h->nsplit = (int)tiles.size();
         }

@@ -40,9 +40,16 @@ __global__ void __launch_bounds__(256) k_rhs_generic(const DComp* __restrict__ C
 
 // the same over split tiles: the closed form over the interior tiles, the
 // generic operators over the ring (and at any point a boundary term reaches);
-// point q of a block is (i0 + q mod w, j0 + q div w).  One launch: on single
+// thread q of a block takes the points (i0 + q mod w, j0 + q div w + r P),
+// r < kRhsRows, P = 256 / w the rows of one pass (tiles of up to kRhsRows P
+// rows: the tile record and operator loads, and the launch's waves, shared
+// by kRhsRows points).  One launch: on single
 // grids a separate launch for the interior tiles measured slower at every
 // size up to 1024^2.
+#ifndef KRHS_ROWS
+#define KRHS_ROWS 4
+#endif
+constexpr int kRhsRows = KRHS_ROWS;
 __global__ void __launch_bounds__(256) k_rhs_split(const DComp* __restrict__ Cu, const DComp* __restrict__ Cv,
                                                   const RTile* __restrict__ tiles, const double* __restrict__ u,
                                                   const double* __restrict__ v, const double* __restrict__ p, int mode,
@@ -51,21 +58,28 @@ __global__ void __launch_bounds__(256) k_rhs_split(const DComp* __restrict__ Cu,
                                                   double* __restrict__ vt_v) {
   const RTile t = tiles[blockIdx.x];
   const int q = threadIdx.x;
-  if (q >= t.w * t.h) return;
-  const int i = t.i0 + q % t.w, j = t.j0 + q / t.w;
+  const int pass = 256 / t.w;   // rows per pass of the block
+  if (q >= t.w * pass) return;
+  const int i = t.i0 + q % t.w;
   const bool closed = (t.comp & 2) != 0;
-  if ((t.comp & 1) == 0) {
-    const DComp& C = *Cu;
-    if (closed && !(glb(C.bmx)[i] & glb(C.bmy)[j]))
-      rhs_closed_point<0>(C, u, v, p, mode, out_u, scu, vt_u, 0, i, j);
-    else
-      rhs_point<0, 0>(C, u, v, p, mode, out_u, scu, vt_u, 0, i, j);
-  } else {
-    const DComp& C = *Cv;
-    if (closed && !(glb(C.bmx)[i] & glb(C.bmy)[j]))
-      rhs_closed_point<1>(C, u, v, p, mode, out_v, scv, vt_v, 0, i, j);
-    else
-      rhs_point<1, 0>(C, u, v, p, mode, out_v, scv, vt_v, 0, i, j);
+#pragma unroll
+  for (int r = 0; r < kRhsRows; ++r) {
+    const int jj = q / t.w + r * pass;
+    if (jj >= t.h) break;
+    const int j = t.j0 + jj;
+    if ((t.comp & 1) == 0) {
+      const DComp& C = *Cu;
+      if (closed && !(glb(C.bmx)[i] & glb(C.bmy)[j]))
+        rhs_closed_point<0>(C, u, v, p, mode, out_u, scu, vt_u, 0, i, j);
+      else
+        rhs_point<0, 0>(C, u, v, p, mode, out_u, scu, vt_u, 0, i, j);
+    } else {
+      const DComp& C = *Cv;
+      if (closed && !(glb(C.bmx)[i] & glb(C.bmy)[j]))
+        rhs_closed_point<1>(C, u, v, p, mode, out_v, scv, vt_v, 0, i, j);
+      else
+        rhs_point<1, 0>(C, u, v, p, mode, out_v, scv, vt_v, 0, i, j);
+    }
   }
 }
 
@@ -2083,6 +2097,8 @@ void launch_rhs_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int 
   launch_timed(k_rhs_generic<0>, grid2(nux, nuy, b), b, 0, st, dCu, u, v, p, mode, out_u, scu, vt_u);
   launch_timed(k_rhs_generic<1>, grid2(nvx, nvy, b), b, 0, st, dCv, u, v, p, mode, out_v, scv, vt_v);
 }
+
+int rhs_split_rows() { return kRhsRows; }
 
 void launch_rhs_split(hipStream_t st, const DComp* dCu, const DComp* dCv, const RTile* tiles, int ntiles,
                       const double* u, const double* v, const double* p, int mode, double* out_u, double* out_v,

@@ -58,6 +58,9 @@ void launch_rhs_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int 
                         const double* u, const double* v, const double* p, int mode, double* out_u, double* out_v,
                         const StageCombine& scu, const StageCombine& scv, double* vt_u, double* vt_v);
 // the same over split tiles (interior closed form, boundary ring generic)
+// rows per thread of launch_rhs_split: its tiles are built with this
+// rows_per_thread (split_tiles)
+int rhs_split_rows();
 void launch_rhs_split(hipStream_t st, const DComp* dCu, const DComp* dCv, const RTile* tiles, int ntiles,
                       const double* u, const double* v, const double* p, int mode, double* out_u, double* out_v,
                       const StageCombine& scu, const StageCombine& scv, double* vt_u, double* vt_v);

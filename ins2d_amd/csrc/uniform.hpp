@@ -243,11 +243,12 @@ inline void uniform_maps(const Plan& P, UniForm& g, UniArrays& A, std::vector<un
 }
 
 // Split tiles of one component (nx x ny points, closed-form map f): tiles
-// over the rectangle where the closed form applies (tx wide, 256 / tx rows)
+// over the rectangle where the closed form applies (tx wide, 256 / tx rows
+// per pass of a 256-thread block)
 // and tiles over the ring around it (up to 64 wide); false (nothing
 // appended) when the marked points do not fill a rectangle.
 inline bool split_tiles(int inst, int comp, int nx, int ny, const std::vector<unsigned char>& f, int tx,
-                        std::vector<RTile>& out) {
+                        std::vector<RTile>& out, int rows_per_thread = 1) {
   if (f.size() != (size_t)nx * ny) return false;
   int i0 = nx, i1 = 0, j0 = ny, j1 = 0;
   for (int j = 0; j < ny; ++j)
@@ -263,9 +264,11 @@ inline bool split_tiles(int inst, int comp, int nx, int ny, const std::vector<un
   auto cover = [&](int a0, int a1, int b0, int b1, int kind) {
     if (a0 >= a1 || b0 >= b1) return;
     // interior tiles tx x (256 / tx) (more rows when the rectangle is
-    // narrower than tx), ring tiles up to 64 wide; w h <= 256 either way
+    // narrower than tx), ring tiles up to 64 wide; w h <= 256 either way,
+    // times rows_per_thread rows (each thread then takes rows q / w,
+    // q / w + 256 / w, ...)
     const int w = std::min(a1 - a0, kind ? tx : 64);
-    const int h = std::max(1, std::min(b1 - b0, 256 / w));
+    const int h = std::max(1, std::min(b1 - b0, rows_per_thread * (256 / w)));
     for (int b = b0; b < b1; b += h)
       for (int a = a0; a < a1; a += w)
         out.push_back(RTile{inst, comp | kind, a, b, std::min(w, a1 - a), std::min(h, b1 - b)});
