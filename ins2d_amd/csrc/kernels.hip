@@ -462,6 +462,197 @@ __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __res
 #undef V
 }
 
+// k_rhs_per4s with two cells per thread (columns x0 + 2 tx and x0 + 2 tx + 1
+// of a 128-wide strip; Nx a multiple of 128, fields 16-byte aligned): the
+// x-direction terms the two cells share are formed once (the same
+// expressions over the same ring values), the row fetch and the stores are
+// 16 bytes per field and lane
+constexpr int kStripW2 = 128;
+template <bool BAND>
+__global__ void __launch_bounds__(64) k_rhs_per4s2(PerGrid G, const double* __restrict__ u,
+                                                 const double* __restrict__ v, const double* __restrict__ p, int mode,
+                                                 double* __restrict__ out, StageCombine sc,
+                                                 double* __restrict__ vtemp, int H) {
+  __shared__ double su[7][kStripW2 + 6];
+  __shared__ double sv[7][kStripW2 + 6];
+  const int tx = threadIdx.x;
+  const int Nx = G.Nx, Ny = G.Ny;
+  const int2 bxy = block_xy<BAND>();   // strips of one XCD side by side
+  const int x0 = bxy.x * kStripW2, j0 = bxy.y * H;
+  const int i0 = x0 + 2 * tx;   // this lane's cells i0 and i0 + 1
+  // LDS column c holds x = x0 - 3 + c (periodic): the lane's pair at
+  // 3 + 2 tx, the halo columns 0..2 and 131..133 from lanes 0..5
+  const int xh = tx < 3 ? ((x0 - 3 + tx) % Nx + Nx) % Nx : (tx < 6 ? (x0 + kStripW2 + tx - 3) % Nx : 0);
+  const int ch = tx < 3 ? tx : kStripW2 + tx;
+  // a row is fetched into registers one iteration ahead (its memory latency
+  // overlaps the previous row's arithmetic) and stored to the ring after
+  double2 pu = make_double2(0.0, 0.0), pv = make_double2(0.0, 0.0);
+  double pu2 = 0.0, pv2 = 0.0;
+  auto fetch_row = [&](int y) {
+    const size_t r = (size_t)Nx * wrap(y, Ny);
+    pu = *(const double2*)(u + r + i0);
+    pv = *(const double2*)(v + r + i0);
+    if (tx < 6) {
+      pu2 = u[r + xh];
+      pv2 = v[r + xh];
+    }
+  };
+  auto store_row = [&](int slot) {
+    su[slot][3 + 2 * tx] = pu.x;
+    su[slot][4 + 2 * tx] = pu.y;
+    sv[slot][3 + 2 * tx] = pv.x;
+    sv[slot][4 + 2 * tx] = pv.y;
+    if (tx < 6) {
+      su[slot][ch] = pu2;
+      sv[slot][ch] = pv2;
+    }
+  };
+  const double hx = G.hx, hy = G.hy;
+  const double w1 = 9.0 / 16.0, w2 = -1.0 / 16.0;   // beta/2, (1-beta)/2
+  const double r72x = G.rhx * (1.0 / 72.0), r72y = G.rhy * (1.0 / 72.0);
+  RingAcc4 acc;
+  int cc = 0;   // the cell of the pair the stencil lambdas take (column offset)
+  // rows j-3 .. j+3 of the ring (s0: slot of row j - 3)
+  auto set_ring = [&](int s0) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int sl = s0 + k < 7 ? s0 + k : s0 + k - 7;
+      acc.ur[k] = &su[sl][2 * tx + 3];
+      acc.vr[k] = &sv[sl][2 * tx + 3];
+    }
+  };
+#define U(a, b) acc.U((a) + cc, b)
+#define V(a, b) acc.V((a) + cc, b)
+  // y-direction fluxes at the face j + d of the current row j
+  auto fy1u = [&](int d) {   // 1h u-flux v * u on grid line y_{j+d}
+    return (hx * (w2 * V(-2, d) + w1 * V(-1, d) + w1 * V(0, d) + w2 * V(1, d))) * (0.5 * (U(0, d - 1) + U(0, d)));
+  };
+  auto fy3u = [&](int d) {   // 3h u-flux on grid line y_{j+d-1}
+    return (3.0 * hx * (w2 * V(-2, d - 1) + w1 * V(-1, d - 1) + w1 * V(0, d - 1) + w2 * V(1, d - 1))) *
+           (0.5 * (U(0, d - 3) + U(0, d)));
+  };
+  auto pya = [&](int d) {    // du/dy between rows j+d and j+d+1
+    return (81.0 * (U(0, d + 1) - U(0, d)) - 3.0 * (U(0, d + 2) - U(0, d - 1))) * r72y;
+  };
+  auto gy1v = [&](int d) {   // 1h v-flux v * v
+    return (hx * (w2 * V(0, d - 2) + w1 * V(0, d - 1) + w1 * V(0, d) + w2 * V(0, d + 1))) *
+           (0.5 * (V(0, d - 1) + V(0, d)));
+  };
+  auto gy3v = [&](int d) {   // 3h v-flux
+    return (3.0 * hx * (w2 * V(0, d - 3) + w1 * V(0, d - 2) + w1 * V(0, d - 1) + w2 * V(0, d))) *
+           (0.5 * (V(0, d - 3) + V(0, d)));
+  };
+  auto qya = [&](int d) {    // dv/dy between rows j+d and j+d+1
+    return (81.0 * (V(0, d + 1) - V(0, d)) - 3.0 * (V(0, d + 2) - V(0, d - 1))) * r72y;
+  };
+  for (int k = 0; k < 6; ++k) {
+    fetch_row(j0 - 3 + k);
+    store_row(k);
+  }
+  fetch_row(j0 + 3);
+  __syncthreads();
+  // the fluxes of the faces at and above row j0 (rows j0-3 .. j0+2 of the ring)
+  set_ring(0);
+  double F1[2], F30[2], F31[2], F32[2], Pm2[2], Pm1[2], P0[2];
+  double G1[2], G30[2], G31[2], G32[2], Qm2[2], Qm1[2], Q0[2];
+#pragma unroll
+  for (cc = 0; cc < 2; ++cc) {
+    F1[cc] = fy1u(0);                                             // face j
+    F30[cc] = fy3u(0); F31[cc] = fy3u(1); F32[cc] = fy3u(2);      // faces j, j+1, j+2
+    Pm2[cc] = pya(-2); Pm1[cc] = pya(-1); P0[cc] = pya(0);        // j-3/2 .. j+1/2
+    G1[cc] = gy1v(0);
+    G30[cc] = gy3v(0); G31[cc] = gy3v(1); G32[cc] = gy3v(2);
+    Qm2[cc] = qya(-2); Qm1[cc] = qya(-1); Q0[cc] = qya(0);
+  }
+  const size_t N = (size_t)Nx * Ny;
+  const int jend = min(j0 + H, Ny);
+  int s0 = 0;   // ring slot of row j - 3
+  for (int j = j0; j < jend; ++j) {
+    store_row(s0 == 0 ? 6 : s0 - 1);   // row j + 3
+    __syncthreads();
+    if (j + 1 < jend) fetch_row(j + 4);
+    set_ring(s0);
+    double fuc[2], fvc[2];
+#pragma unroll
+    for (cc = 0; cc < 2; ++cc) {
+      // ---------------- u-momentum ----------------
+      auto Ix1 = [&](int f) { return hy * (w2 * U(f - 2, 0) + w1 * U(f - 1, 0) + w1 * U(f, 0) + w2 * U(f + 1, 0)); };
+      auto Ax1 = [&](int f) { return 0.5 * (U(f - 1, 0) + U(f, 0)); };
+      const double du2dx1 = Ix1(1) * Ax1(1) - Ix1(0) * Ax1(0);
+      auto Ix3 = [&](int r) { return 3.0 * hy * (w2 * U(r - 3, 0) + w1 * U(r - 2, 0) + w1 * U(r - 1, 0) + w2 * U(r, 0)); };
+      auto Ax3 = [&](int r) { return 0.5 * (U(r - 3, 0) + U(r, 0)); };
+      const double du2dx3 = Ix3(3) * Ax3(3) - Ix3(0) * Ax3(0);
+      const double F1n = fy1u(1), F3n = fy3u(3), Pn = pya(1);
+      const double duvdy1 = F1n - F1[cc];
+      const double duvdy3 = F3n - F30[cc];
+      const double convu = 81.0 * (du2dx1 + duvdy1) - (du2dx3 + duvdy3);
+      auto Px = [&](int k) { return (81.0 * (U(k + 1, 0) - U(k, 0)) - 3.0 * (U(k + 2, 0) - U(k - 1, 0))) * r72x; };
+      const double d2u = hy * (81.0 * (Px(0) - Px(-1)) - 3.0 * (Px(1) - Px(-2))) +
+                         hx * (81.0 * (P0[cc] - Pm1[cc]) - 3.0 * (Pn - Pm2[cc]));
+      double fu = -convu + G.invRe * d2u;
+      // ---------------- v-momentum ----------------
+      auto Uf1 = [&](int f) { return hy * (w2 * U(f, -2) + w1 * U(f, -1) + w1 * U(f, 0) + w2 * U(f, 1)); };
+      auto Bx1 = [&](int f) { return 0.5 * (V(f - 1, 0) + V(f, 0)); };
+      const double duvdx1 = Uf1(1) * Bx1(1) - Uf1(0) * Bx1(0);
+      auto Uf3 = [&](int f) { return 3.0 * hy * (w2 * U(f - 1, -2) + w1 * U(f - 1, -1) + w1 * U(f - 1, 0) + w2 * U(f - 1, 1)); };
+      auto Bx3 = [&](int f) { return 0.5 * (V(f - 3, 0) + V(f, 0)); };
+      const double duvdx3 = Uf3(3) * Bx3(3) - Uf3(0) * Bx3(0);
+      const double G1n = gy1v(1), G3n = gy3v(3), Qn = qya(1);
+      const double dv2dy1 = G1n - G1[cc];
+      const double dv2dy3 = G3n - G30[cc];
+      const double convv = 81.0 * (duvdx1 + dv2dy1) - (duvdx3 + dv2dy3);
+      auto Qx = [&](int k) { return (81.0 * (V(k + 1, 0) - V(k, 0)) - 3.0 * (V(k + 2, 0) - V(k - 1, 0))) * r72x; };
+      const double d2v = hy * (81.0 * (Qx(0) - Qx(-1)) - 3.0 * (Qx(1) - Qx(-2))) +
+                         hx * (81.0 * (Q0[cc] - Qm1[cc]) - 3.0 * (Qn - Qm2[cc]));
+      double fv = -convv + G.invRe * d2v;
+      // advance the flux queues to the next row
+      F1[cc] = F1n;
+      F30[cc] = F31[cc]; F31[cc] = F32[cc]; F32[cc] = F3n;
+      Pm2[cc] = Pm1[cc]; Pm1[cc] = P0[cc]; P0[cc] = Pn;
+      G1[cc] = G1n;
+      G30[cc] = G31[cc]; G31[cc] = G32[cc]; G32[cc] = G3n;
+      Qm2[cc] = Qm1[cc]; Qm1[cc] = Q0[cc]; Q0[cc] = Qn;
+      fuc[cc] = fu;
+      fvc[cc] = fv;
+    }
+    {
+      if (mode & RHS_WITH_P) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          double gx, gy;
+          grad_per<4>(G, p, i0 + c, j, gx, gy);
+          fuc[c] -= gx;
+          fvc[c] -= gy;
+        }
+      }
+      if (mode & RHS_SCALE_OM) {
+        fuc[0] *= G.rom; fuc[1] *= G.rom;
+        fvc[0] *= G.rom; fvc[1] *= G.rom;
+      }
+      const size_t idx = (size_t)i0 + (size_t)Nx * j;
+      if (out) {
+        *(double2*)(out + idx) = make_double2(fuc[0], fuc[1]);
+        *(double2*)(out + N + idx) = make_double2(fvc[0], fvc[1]);
+      }
+      if (vtemp) {
+        double2 a = make_double2(sc.aself * fuc[0], sc.aself * fuc[1]);
+        double2 b = make_double2(sc.aself * fvc[0], sc.aself * fvc[1]);
+        for (int q = 0; q < sc.n; ++q) {
+          const double2 ka = *(const double2*)(sc.k[q] + idx), kb = *(const double2*)(sc.k[q] + N + idx);
+          a.x += sc.a[q] * ka.x; a.y += sc.a[q] * ka.y;
+          b.x += sc.a[q] * kb.x; b.y += sc.a[q] * kb.y;
+        }
+        *(double2*)(vtemp + idx) = a;
+        *(double2*)(vtemp + N + idx) = b;
+      }
+    }
+    __syncthreads();
+    s0 = s0 == 6 ? 0 : s0 + 1;
+  }
+#undef U
+#undef V
+}
+
 // second order (fourth order: k_div_per4s)
 template <int ORDER, bool BAND>
 __global__ void __launch_bounds__(256) k_div_per(PerGrid G, const double* __restrict__ A, const double* __restrict__ B,
@@ -2123,6 +2314,18 @@ void launch_proj_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int
                      c, V + nu);
 }
 
+#ifndef KRHS4_PAIR
+#define KRHS4_PAIR 1
+#endif
+#ifndef KRHS4_H
+#define KRHS4_H 64
+#endif
+static bool k_all_aligned(const StageCombine& sc) {
+  for (int q = 0; q < sc.n; ++q)
+    if (!aligned16(sc.k[q])) return false;
+  return true;
+}
+
 void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, double invRe, const double* u,
                     const double* v, const double* p, int mode, double* out, const StageCombine& sc, double* vt) {
   const PerGrid G = per_grid(order, Nx, Ny, hx, hy, invRe);
@@ -2134,7 +2337,12 @@ void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double
     launch_timed(k_rhs_per2x<true>, grid2(Nx / 2, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
   else if (order == 2)
     launch_timed(k_rhs_per<2, true>, grid2(Nx, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
-  else
+  else if (KRHS4_PAIR && Nx % kStripW2 == 0 && aligned16(u) && aligned16(v) && aligned16(out) && aligned16(vt) &&
+           k_all_aligned(sc)) {
+    const int H2 = KRHS4_H;
+    launch_timed(k_rhs_per4s2<true>, dim3(Nx / kStripW2, (Ny + H2 - 1) / H2), dim3(64), 0, st, G, u, v, p, mode, out,
+                 sc, vt, H2);
+  } else
     launch_timed(k_rhs_per4s<true>, dim3((Nx + kStripW - 1) / kStripW, (Ny + H - 1) / H), dim3(kStripW), 0, st, G, u, v, p,
                  mode, out, sc, vt, H);
 }
