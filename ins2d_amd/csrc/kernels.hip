@@ -46,10 +46,7 @@ __global__ void __launch_bounds__(256) k_rhs_generic(const DComp* __restrict__ C
 // by kRhsRows points).  One launch: on single
 // grids a separate launch for the interior tiles measured slower at every
 // size up to 1024^2.
-#ifndef KRHS_ROWS
-#define KRHS_ROWS 4
-#endif
-constexpr int kRhsRows = KRHS_ROWS;
+constexpr int kRhsRows = 4;
 __global__ void __launch_bounds__(256) k_rhs_split(const DComp* __restrict__ Cu, const DComp* __restrict__ Cv,
                                                   const RTile* __restrict__ tiles, const double* __restrict__ u,
                                                   const double* __restrict__ v, const double* __restrict__ p, int mode,
@@ -748,6 +745,38 @@ __global__ void __launch_bounds__(256) k_proj_per(PerGrid G, const double* __res
   const double tu = Vt ? Vt[idx] : 0.0, tv = Vt ? Vt[N + idx] : 0.0;
   V[idx] = Vn[idx] + dt * (tu - c * (gx * G.rom));
   V[N + idx] = Vn[N + idx] + dt * (tv - c * (gy * G.rom));
+}
+
+// k_proj_per<4> with two cells (i, i+1) per thread (Nx even, arrays 16-byte
+// aligned): the five dp values of the pair's row and the three of each other
+// row loaded 16 bytes at a time; the same arithmetic per cell as grad_per
+// (at second order the pair form measured no faster: 1024^2, 9.9 against
+// 9.9 us)
+template <bool BAND>
+__global__ void __launch_bounds__(256) k_proj_per4x(PerGrid G, const double* __restrict__ Vn,
+                                                   const double* __restrict__ Vt, const double* __restrict__ dp,
+                                                   double dt, double c, double* __restrict__ V) {
+  const int2 bxy = block_xy<BAND>();
+  const int i = 2 * (bxy.x * blockDim.x + threadIdx.x);
+  const int j = bxy.y * blockDim.y + threadIdx.y;
+  const int Nx = G.Nx, Ny = G.Ny;
+  if (i >= Nx || j >= Ny) return;
+  const size_t N = (size_t)Nx * Ny, r0 = (size_t)Nx * j;
+  const size_t rm = (size_t)Nx * wrap(j - 1, Ny), rp = (size_t)Nx * wrap(j + 1, Ny), rmm = (size_t)Nx * wrap(j - 2, Ny);
+  auto ld2 = [](const double* a) { return *(const double2*)a; };
+  const double2 pw = ld2(dp + r0 + wrap(i - 2, Nx)), pc = ld2(dp + r0 + i), pe = ld2(dp + r0 + wrap(i + 2, Nx));
+  const double2 pm = ld2(dp + rm + i), pp = ld2(dp + rp + i), pmm = ld2(dp + rmm + i);
+  // cell i: p(i-2), p(i-1), p(i), p(i+1) = pw.x, pw.y, pc.x, pc.y; cell i+1: pw.y, pc.x, pc.y, pe.x
+  const double gx0 = 81.0 * G.hy * (pc.x - pw.y) - 3.0 * G.hy * (pc.y - pw.x);
+  const double gx1 = 81.0 * G.hy * (pc.y - pc.x) - 3.0 * G.hy * (pe.x - pw.y);
+  const double gy0 = 81.0 * G.hx * (pc.x - pm.x) - 3.0 * G.hx * (pp.x - pmm.x);
+  const double gy1 = 81.0 * G.hx * (pc.y - pm.y) - 3.0 * G.hx * (pp.y - pmm.y);
+  const size_t idx = r0 + i;
+  const double2 un = ld2(Vn + idx), vn = ld2(Vn + N + idx);
+  const double2 tu = Vt ? ld2(Vt + idx) : make_double2(0.0, 0.0), tv = Vt ? ld2(Vt + N + idx) : make_double2(0.0, 0.0);
+  *(double2*)(V + idx) = make_double2(un.x + dt * (tu.x - c * (gx0 * G.rom)), un.y + dt * (tu.y - c * (gx1 * G.rom)));
+  *(double2*)(V + N + idx) =
+      make_double2(vn.x + dt * (tv.x - c * (gy0 * G.rom)), vn.y + dt * (tv.y - c * (gy1 * G.rom)));
 }
 
 // ===========================================================================
@@ -2314,12 +2343,6 @@ void launch_proj_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int
                      c, V + nu);
 }
 
-#ifndef KRHS4_PAIR
-#define KRHS4_PAIR 1
-#endif
-#ifndef KRHS4_H
-#define KRHS4_H 64
-#endif
 static bool k_all_aligned(const StageCombine& sc) {
   for (int q = 0; q < sc.n; ++q)
     if (!aligned16(sc.k[q])) return false;
@@ -2337,9 +2360,10 @@ void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double
     launch_timed(k_rhs_per2x<true>, grid2(Nx / 2, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
   else if (order == 2)
     launch_timed(k_rhs_per<2, true>, grid2(Nx, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
-  else if (KRHS4_PAIR && Nx % kStripW2 == 0 && aligned16(u) && aligned16(v) && aligned16(out) && aligned16(vt) &&
+  else if (Nx % kStripW2 == 0 && aligned16(u) && aligned16(v) && aligned16(out) && aligned16(vt) &&
            k_all_aligned(sc)) {
-    const int H2 = KRHS4_H;
+    // 64-row strips: at 4096^2 the 2048 strips are all resident (2 per SIMD)
+    const int H2 = 64;
     launch_timed(k_rhs_per4s2<true>, dim3(Nx / kStripW2, (Ny + H2 - 1) / H2), dim3(64), 0, st, G, u, v, p, mode, out,
                  sc, vt, H2);
   } else
@@ -2365,6 +2389,8 @@ void launch_proj_per(hipStream_t st, int order, int Nx, int Ny, double hx, doubl
   dim3 b(64, 4);
   if (order == 2)
     launch_timed(k_proj_per<2, true>, grid2(Nx, Ny, b), b, 0, st, G, Vn, Vt, dp, dt, c, V);
+  else if (Nx % 2 == 0 && aligned16(Vn) && aligned16(Vt) && aligned16(dp) && aligned16(V))   // 174 -> 161.5 us at 4096^2
+    launch_timed(k_proj_per4x<false>, grid2(Nx / 2, Ny, b), b, 0, st, G, Vn, Vt, dp, dt, c, V);
   else   // banded blocks measured slower here (4096^2: 182 against 174 us)
     launch_timed(k_proj_per<4, false>, grid2(Nx, Ny, b), b, 0, st, G, Vn, Vt, dp, dt, c, V);
 }
