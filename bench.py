@@ -13,7 +13,8 @@ additional pressure solve; the timed region excludes setup and diagnostics.
 metric: Mcell-RK-stages/s = cells x RK stages x steps / seconds / 1e6.
 
 Prints one JSON line.  ``--gpus N`` with N > 1 exits: the path runs on one
-GPU (a multi-GPU layout is described in DESIGN.md).
+GPU (a sharded layout, row slabs of the grid with halo exchange and a
+distributed transpose for the FFT solve, is designed but not built).
 """
 import argparse
 import json
@@ -384,8 +385,8 @@ CASES = {
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({'error': 'this path runs on one GPU; multi-GPU sharding is described in DESIGN.md, '
-                                   'not built', 'n_gpus': a.gpus}))
+        print(json.dumps({'error': 'this path runs on one GPU; sharding over several GPUs is designed, not built',
+                          'n_gpus': a.gpus}))
         sys.exit(2)
     if a.case in ('ensemble', 'ensemble_jacobi', 'ensemble_2level'):
         return bench_ensemble(a)
