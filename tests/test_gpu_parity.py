@@ -254,6 +254,19 @@ def test_periodic_rectangular_run_matches_oracle(Nx, Ny, kernels):
     _compare_runs(case, 10, False)
 
 
+@pytest.mark.parametrize('Nx,Ny', [(128, 48), (256, 40), (96, 64)])
+def test_fourth_order_pair_kernels_match_oracle(Nx, Ny):
+    """Fourth-order periodic runs where Nx is a multiple of 128 (the RHS takes
+    two cells per thread over 128-wide strips, the strips wrapping around in
+    x and shorter than their 64 rows in y) and where it is not (96: one cell
+    per thread); the projection takes two cells per thread for every even
+    Nx."""
+    _say('fourth-order pair kernels', Nx, Ny)
+    case = load('taylor_green', Nx=Nx, Ny=Ny, Re=500.0, noise=1e-2, seed=5, dt=0.005, t_end=0.05, order4=1,
+                fft_symbol=1)
+    _compare_runs(case, 8, False)
+
+
 def test_large_periodic_invariants():
     """1024^2 Taylor-Green with noise, inviscid: after steps the field stays
     discretely divergence-free, total momentum is conserved to round-off and
