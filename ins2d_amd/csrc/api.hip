@@ -80,6 +80,9 @@ struct ins2d_handle {
   // call checks once at its end and redoes its steps from the saved state
   // on the host-checked path when any solve missed
   bool fdm_dev = false;
+  // the first stage's k_1 and Vtemp of the next step are already in kv[0]
+  // and Vt (formed with the additional solve's F of the same velocity)
+  bool k1_ready = false;
   int* d_missed = nullptr;
   double *Vsave = nullptr, *psave = nullptr;
   double *fw1 = nullptr, *fw2 = nullptr;
@@ -844,7 +847,23 @@ void additional_solve(ins2d_handle* h, const double* V, DComp* rate_cu = nullptr
 // operator records of s + 2 consecutive versions (boundary values at tn, at
 // the s stage times tn + c_i dt, and the ydM record at tn + dt); null for
 // steady boundary values
-void erk_step(ins2d_handle* h, double tn, double dt, DComp* ver = nullptr) {
+// F of V_{n+1} once at the end of a step that another step follows in the
+// same call (periodic fourth order, rhs_per_dual_ok): the additional solve's
+// (F - G p) / Omega into tV and the next step's first-stage k_1 = F / Omega
+// (into kv[0] when kept) with its stage combination into Vt, both as their
+// own launches would form them; the next erk_step then skips its first
+// right-hand side
+void additional_solve_dual(ins2d_handle* h) {
+  StageCombine sc0{};
+  sc0.aself = h->At[0];
+  sc0.n = 0;
+  launch_rhs_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, h->invRe, h->V, h->V + h->Nu, h->p, RHS_SCALE_OM,
+                 h->keep[0] ? h->kv[0] : nullptr, sc0, h->Vt, h->tV);
+  div_poisson(h, h->tV, nullptr, 1.0, 0.0, 0.0, 1.0, h->p, true);
+  h->k1_ready = true;
+}
+
+void erk_step(ins2d_handle* h, double tn, double dt, DComp* ver = nullptr, bool dual_end = false) {
   (void)tn;
   double* Vn = h->V;
   double* Vs = h->Vb;   // stage velocity, V_{n+1} at the end
@@ -869,7 +888,10 @@ void erk_step(ins2d_handle* h, double tn, double dt, DComp* ver = nullptr) {
     }
     // kV_i = Om^-1 (F_i + G p) = Om^-1 (-conv + diff - y_p)   (time_ERK.m:90-95)
     // Vtemp = sum_j At(i,j) kV_j                              (time_ERK.m:101)
-    rhs(h, Vsrc, h->p, RHS_SCALE_OM, h->keep[i] ? h->kv[i] : nullptr, sc, h->Vt);
+    if (i == 0 && h->k1_ready)
+      h->k1_ready = false;   // formed at the end of the previous step
+    else
+      rhs(h, Vsrc, h->p, RHS_SCALE_OM, h->keep[i] ? h->kv[i] : nullptr, sc, h->Vt);
     const double c = h->ct[i];
     const double* dpp;
     // unsteady BC: yM at tn + c_i dt (time_ERK.m:106-109), and a pressure
@@ -888,7 +910,10 @@ void erk_step(ins2d_handle* h, double tn, double dt, DComp* ver = nullptr) {
   std::swap(h->V, h->Vb);
   if (!ver) {
     // steady BC: always the additional solve (time_ERK.m:160-163)
-    additional_solve(h, h->V);
+    if (dual_end)
+      additional_solve_dual(h);
+    else
+      additional_solve(h, h->V);
   } else if (h->p_add) {
     // at tn + dt with ydM (time_ERK.m:147-148)
     additional_solve(h, h->V, ver + 2 * (h->s + 1), ver + 2 * (h->s + 1) + 1);
@@ -1523,7 +1548,13 @@ int ins2d_step(ins2d_handle* h, double tn, double dt, int nsteps) {
     HIPCHK(hipMemcpyAsync(h->V, h->Vsave, sizeof(double) * h->NV, hipMemcpyDeviceToDevice, h->st));
     HIPCHK(hipMemcpyAsync(h->p, h->psave, sizeof(double) * h->Np, hipMemcpyDeviceToDevice, h->st));
   }
-  for (int n = 0; n < nsteps; ++n) erk_step(h, tn + n * dt, dt);
+  h->k1_ready = false;
+  // the F of each step's end velocity formed once for the additional solve
+  // and the next step's first stage (profiling off: the profile times the
+  // launches one by one)
+  const bool dual = !h->prof && h->path == 4 && rhs_per_dual_ok(h->path, h->Nx);
+  for (int n = 0; n < nsteps; ++n) erk_step(h, tn + n * dt, dt, nullptr, dual && n + 1 < nsteps);
+  h->k1_ready = false;
   if (h->prof) prof_collect(h);
   return INS2D_OK;
   API_CATCH(h)

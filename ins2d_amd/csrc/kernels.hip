@@ -465,11 +465,11 @@ __global__ void __launch_bounds__(64) k_rhs_per4s(PerGrid G, const double* __res
 // expressions over the same ring values), the row fetch and the stores are
 // 16 bytes per field and lane
 constexpr int kStripW2 = 128;
-template <bool BAND>
+template <bool BAND, bool DUAL>
 __global__ void __launch_bounds__(64) k_rhs_per4s2(PerGrid G, const double* __restrict__ u,
                                                  const double* __restrict__ v, const double* __restrict__ p, int mode,
                                                  double* __restrict__ out, StageCombine sc,
-                                                 double* __restrict__ vtemp, int H) {
+                                                 double* __restrict__ vtemp, int H, double* __restrict__ out2) {
   __shared__ double su[7][kStripW2 + 6];
   __shared__ double sv[7][kStripW2 + 6];
   const int tx = threadIdx.x;
@@ -613,6 +613,15 @@ __global__ void __launch_bounds__(64) k_rhs_per4s2(PerGrid G, const double* __re
       fvc[cc] = fv;
     }
     {
+      if constexpr (DUAL) {
+        // also (F - G p) / Omega, as a RHS_WITH_P | RHS_SCALE_OM launch forms it
+        const size_t idx = (size_t)i0 + (size_t)Nx * j;
+        double gxc[2], gyc[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) grad_per<4>(G, p, i0 + c, j, gxc[c], gyc[c]);
+        *(double2*)(out2 + idx) = make_double2((fuc[0] - gxc[0]) * G.rom, (fuc[1] - gxc[1]) * G.rom);
+        *(double2*)(out2 + N + idx) = make_double2((fvc[0] - gyc[0]) * G.rom, (fvc[1] - gyc[1]) * G.rom);
+      }
       if (mode & RHS_WITH_P) {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
@@ -2349,9 +2358,15 @@ static bool k_all_aligned(const StageCombine& sc) {
   return true;
 }
 
+bool rhs_per_dual_ok(int order, int Nx) { return order == 4 && Nx % kStripW2 == 0; }
+
 void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, double invRe, const double* u,
-                    const double* v, const double* p, int mode, double* out, const StageCombine& sc, double* vt) {
+                    const double* v, const double* p, int mode, double* out, const StageCombine& sc, double* vt,
+                    double* out2) {
   const PerGrid G = per_grid(order, Nx, Ny, hx, hy, invRe);
+  if (out2 && !(rhs_per_dual_ok(order, Nx) && aligned16(u) && aligned16(v) && aligned16(out) && aligned16(vt) &&
+                aligned16(out2) && k_all_aligned(sc)))
+    throw std::runtime_error("right-hand side with a second output: fourth order, Nx a multiple of 128, aligned arrays");
   const int H = strip_height(Nx, Ny);
   dim3 b(64, 4);
   bool pair = order == 2 && Nx % 2 == 0 && aligned16(u) && aligned16(v) && aligned16(out) && aligned16(vt);
@@ -2364,8 +2379,12 @@ void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double
            k_all_aligned(sc)) {
     // 64-row strips: at 4096^2 the 2048 strips are all resident (2 per SIMD)
     const int H2 = 64;
-    launch_timed(k_rhs_per4s2<true>, dim3(Nx / kStripW2, (Ny + H2 - 1) / H2), dim3(64), 0, st, G, u, v, p, mode, out,
-                 sc, vt, H2);
+    if (out2)
+      launch_timed(k_rhs_per4s2<true, true>, dim3(Nx / kStripW2, (Ny + H2 - 1) / H2), dim3(64), 0, st, G, u, v, p, mode,
+                   out, sc, vt, H2, out2);
+    else
+      launch_timed(k_rhs_per4s2<true, false>, dim3(Nx / kStripW2, (Ny + H2 - 1) / H2), dim3(64), 0, st, G, u, v, p,
+                   mode, out, sc, vt, H2, out2);
   } else
     launch_timed(k_rhs_per4s<true>, dim3((Nx + kStripW - 1) / kStripW, (Ny + H - 1) / H), dim3(kStripW), 0, st, G, u, v, p,
                  mode, out, sc, vt, H);

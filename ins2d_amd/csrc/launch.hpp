@@ -68,8 +68,13 @@ void launch_div_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int 
                         const double* B, double ca, double cb, double ycoef, double oscale, int nu, double* f);
 void launch_proj_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,
                          int nu, const double* Vn, const double* Vt, const double* dp, double dt, double c, double* V);
+// out2 (optional; rhs_per_dual_ok shapes, 16-byte aligned arrays): also
+// (F - G p) / Omega from the same F, as a RHS_WITH_P | RHS_SCALE_OM launch
+// would form it
 void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, double invRe, const double* u,
-                    const double* v, const double* p, int mode, double* out, const StageCombine& sc, double* vt);
+                    const double* v, const double* p, int mode, double* out, const StageCombine& sc, double* vt,
+                    double* out2 = nullptr);
+bool rhs_per_dual_ok(int order, int Nx);
 void launch_div_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* A, const double* B,
                     double ca, double cb, double oscale, double* f);
 void launch_proj_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, const double* Vn,
