@@ -857,10 +857,27 @@ void additional_solve_dual(ins2d_handle* h) {
   StageCombine sc0{};
   sc0.aself = h->At[0];
   sc0.n = 0;
-  launch_rhs_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, h->invRe, h->V, h->V + h->Nu, h->p, RHS_SCALE_OM,
-                 h->keep[0] ? h->kv[0] : nullptr, sc0, h->Vt, h->tV);
+  if (h->path) {
+    launch_rhs_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, h->invRe, h->V, h->V + h->Nu, h->p, RHS_SCALE_OM,
+                   h->keep[0] ? h->kv[0] : nullptr, sc0, h->Vt, h->tV);
+  } else {
+    // generic operators: the first stage's right-hand side as its own launch
+    // would form it, then (F - G p) / Omega = k1 - Om^-1 G p from it
+    rhs(h, h->V, h->p, RHS_SCALE_OM, h->kv[0], sc0, h->Vt);
+    launch_sub_grad_generic(h->st, h->dCu, h->dCv, h->Nu, h->NV - h->Nu, h->kv[0], h->p, h->tV);
+  }
   div_poisson(h, h->tV, nullptr, 1.0, 0.0, 0.0, 1.0, h->p, true);
   h->k1_ready = true;
+}
+
+// the steps of an ins2d_step call whose boundary F can be shared (steady
+// boundary values): the fourth-order periodic kernels (one instance writes
+// both), the generic second-order operators when k1 is kept (the additional
+// solve's right-hand side follows from it)
+bool dual_capable(const ins2d_handle* h) {
+  if (h->prof || h->w4) return false;
+  if (h->path == 4) return rhs_per_dual_ok(h->path, h->Nx);
+  return h->path == 0 && h->plan && h->keep[0];
 }
 
 void erk_step(ins2d_handle* h, double tn, double dt, DComp* ver = nullptr, bool dual_end = false) {
@@ -1525,13 +1542,17 @@ int ins2d_step(ins2d_handle* h, double tn, double dt, int nsteps) {
     HIPCHK(hipMemcpyAsync(h->psave, h->p, sizeof(double) * h->Np, hipMemcpyDeviceToDevice, h->st));
     HIPCHK(hipMemsetAsync(h->d_missed, 0, sizeof(int), h->st));
     h->fdm_dev = true;
+    h->k1_ready = false;
+    const bool dual_fdm = dual_capable(h);
     try {
-      for (int n = 0; n < nsteps; ++n) erk_step(h, tn + n * dt, dt);
+      for (int n = 0; n < nsteps; ++n) erk_step(h, tn + n * dt, dt, nullptr, dual_fdm && n + 1 < nsteps);
     } catch (...) {
       h->fdm_dev = false;
+      h->k1_ready = false;
       throw;
     }
     h->fdm_dev = false;
+    h->k1_ready = false;
     int missed = 0;
     double rel = 0.0;
     HIPCHK(hipMemcpyAsync(&missed, h->d_missed, sizeof(int), hipMemcpyDeviceToHost, h->st));
@@ -1552,7 +1573,7 @@ int ins2d_step(ins2d_handle* h, double tn, double dt, int nsteps) {
   // the F of each step's end velocity formed once for the additional solve
   // and the next step's first stage (profiling off: the profile times the
   // launches one by one)
-  const bool dual = !h->prof && h->path == 4 && rhs_per_dual_ok(h->path, h->Nx);
+  const bool dual = dual_capable(h);
   for (int n = 0; n < nsteps; ++n) erk_step(h, tn + n * dt, dt, nullptr, dual && n + 1 < nsteps);
   h->k1_ready = false;
   if (h->prof) prof_collect(h);

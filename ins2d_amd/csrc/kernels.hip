@@ -103,6 +103,21 @@ __global__ void __launch_bounds__(256) k_proj_generic(const DComp* __restrict__ 
   V[(size_t)i + (size_t)C.nx * j] = proj_point(C, Vn, Vt, dp, dt, c, i, j);
 }
 
+// tV = k1 - Om^-1 G p over the velocity points of both components (u then
+// v, nu of them u): the additional pressure solve's right-hand side
+// (F - G p) / Omega from the first-stage k1 = F / Omega of the same velocity
+__global__ void __launch_bounds__(256) k_sub_grad_generic(const DComp* __restrict__ Cu, const DComp* __restrict__ Cv,
+                                                         int nu, int nv, const double* __restrict__ k1,
+                                                         const double* __restrict__ p, double* __restrict__ tV) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nu + nv) return;
+  const bool isv = e >= nu;
+  const DComp& C = isv ? *Cv : *Cu;
+  const int q = isv ? e - nu : e;
+  const int i = q % C.nx, j = q / C.nx;
+  tV[e] = k1[e] - kron2_at(C.G, p, i, j) / (glb(C.ox)[i] * glb(C.oy)[j]);
+}
+
 // ===========================================================================
 // periodic uniform stencils (second and fourth order)
 // ===========================================================================
@@ -2341,6 +2356,11 @@ void launch_div_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int 
   dim3 b(64, 4);
   launch_timed(k_div_generic, grid2(npx, npy, b), b, 0, st, dCu, dCv, npx, npy, A, B, ca, cb, ycoef, oscale,
                      nu, f);
+}
+
+void launch_sub_grad_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nu, int nv, const double* k1,
+                             const double* p, double* tV) {
+  launch_timed(k_sub_grad_generic, dim3((nu + nv + 255) / 256), dim3(256), 0, st, dCu, dCv, nu, nv, k1, p, tV);
 }
 
 void launch_proj_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,

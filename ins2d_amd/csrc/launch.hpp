@@ -66,6 +66,9 @@ void launch_rhs_split(hipStream_t st, const DComp* dCu, const DComp* dCv, const 
                       const StageCombine& scu, const StageCombine& scv, double* vt_u, double* vt_v);
 void launch_div_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int npx, int npy, const double* A,
                         const double* B, double ca, double cb, double ycoef, double oscale, int nu, double* f);
+// tV = k1 - Om^-1 G p at the nu u and nv v points (generic operators)
+void launch_sub_grad_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nu, int nv, const double* k1,
+                             const double* p, double* tV);
 void launch_proj_generic(hipStream_t st, const DComp* dCu, const DComp* dCv, int nux, int nuy, int nvx, int nvy,
                          int nu, const double* Vn, const double* Vt, const double* dp, double dt, double c, double* V);
 // out2 (optional; rhs_per_dual_ok shapes, 16-byte aligned arrays): also

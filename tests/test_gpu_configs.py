@@ -88,18 +88,33 @@ def test_decaying_turbulence_4096_rows_match_oracle(Nx, Ny):
     assert np.all(np.asarray(rows)[:, 4] < 1e-12)
 
 
-@pytest.mark.parametrize('Nx,Ny', [(4096, 64), (256, 128)])
-def test_steps_in_one_call_match_single_steps(Nx, Ny):
-    """Inside one ins2d_step call the fourth-order periodic path forms the F
-    of each step's end velocity once, for the additional pressure solve and
-    for the next step's first stage; a call of one step (and every profiled
-    call) forms it twice, as the reference does.  The two separate forms
-    agree bit for bit; the shared one is a second instance of the kernel
-    whose multiply-adds the compiler may fuse in another order, so it agrees
-    to rounding: 1e-13 relative (max norm) after 6 steps, far inside the
-    1e-12 of the oracle comparisons."""
-    _say('steps in one call', Nx, Ny)
-    case = load('decaying_turbulence', Nx=Nx, Ny=Ny)
+_ONE_CALL_CASES = {
+    'decaying 4096x64': lambda: load('decaying_turbulence', Nx=4096, Ny=64),
+    'decaying 256x128': lambda: load('decaying_turbulence', Nx=256, Ny=128),
+    'ldc 64 direct': lambda: load('LDC_unsteady', Nx=64, Ny=64, Re=1000.0, dt=1e-3, RK='RK44', poisson=1),
+    'ldc 64 fdm pcg': lambda: load('LDC_unsteady', Nx=64, Ny=64, Re=1000.0, dt=1e-3, RK='RK44', poisson=3,
+                                   CG_acc=1e-10, CG_maxit=200, CG_precond='fdm'),
+    'bfs 256x64 direct': lambda: load('BFS_unsteady', Nx=256, Ny=64),
+}
+
+
+@pytest.mark.parametrize('name', list(_ONE_CALL_CASES))
+def test_steps_in_one_call_match_single_steps(name):
+    """Inside one ins2d_step call the F of each step's end velocity is formed
+    once, for the additional pressure solve and for the next step's first
+    stage (fourth-order periodic: one kernel instance writes both; generic
+    operators: the additional solve's (F - G p) / Omega follows from the
+    first-stage k1 = F / Omega); a call of one step (and every profiled call)
+    forms it twice, as the reference does.  The two separate forms agree bit
+    for bit; the shared one rounds differently (another kernel instance,
+    whose multiply-adds the compiler may fuse in another order; or k1 minus
+    the scaled gradient), so it agrees to rounding: 1e-13 relative (max norm)
+    after 6 steps, and 1e-12 for the pressure of the cavity cases, whose
+    solves amplify rounding in the right-hand side by the conditioning of A
+    (velocities and pressure are compared modulo nothing: the pressure's
+    constant is the same on both paths)."""
+    _say('steps in one call', name)
+    case = _ONE_CALL_CASES[name]()
     out = []
     for mode in ('one call', 'single steps', 'profiled'):
         solver, opts, V0, p0 = host.setup(case)
@@ -117,8 +132,8 @@ def test_steps_in_one_call_match_single_steps(Nx, Ny):
     (V1, p1), (Vs, ps), (Vp, pp) = out
     assert np.array_equal(Vs, Vp) and np.array_equal(ps, pp)
     ev, ep = _rel(V1, Vs), _rel(p1, ps)
-    _say('one call against single steps', f'V {ev:.2e} p {ep:.2e}')
-    assert ev < 1e-13 and ep < 1e-13
+    _say('one call against single steps', name, f'V {ev:.2e} p {ep:.2e}')
+    assert ev < 1e-13 and ep < (1e-12 if name.startswith('ldc') else 1e-13)
 
 
 def test_decaying_turbulence_4096_properties():
