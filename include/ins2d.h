@@ -142,11 +142,12 @@ int ins2d_input_check(ins2d_handle* h, double t, int p_initial);
  * records, and the call waits on the host once; when a solve missed the
  * stopping rule, the call restores the state it started from and redoes its
  * steps with the convergence test read after every iteration (the results
- * are those of that path bit for bit).  On the fourth-order periodic path
- * (Nx a multiple of 128, profiling off) the F of each step's end velocity
- * is formed once for the additional pressure solve and, when another step
- * of the same call follows, for that step's first stage (the reference forms
- * it twice); the results agree with one-step calls to rounding. */
+ * are those of that path bit for bit).  When another step of the same call
+ * follows (profiling off; the fourth-order periodic path with Nx a multiple
+ * of 128, or the second-order generic operators), the F of a step's end
+ * velocity is formed once for the additional pressure solve and for the
+ * next step's first stage (the reference forms it twice); the results agree
+ * with one-step calls to rounding. */
 int ins2d_step(ins2d_handle* h, double tn, double dt, int nsteps);
 
 /* time derivatives of the boundary values at the time of the last
