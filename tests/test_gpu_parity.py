@@ -394,11 +394,13 @@ def test_fdm_pcg_second_iteration():
 def test_fdm_pcg_device_iteration_mode(acc, nsteps):
     """ins2d_step with the fast-diagonalization PCG runs one iteration per
     solve and checks the stopping rule once per call on the device; with
-    profiling on, every solve is checked on the host.  Both give the same V
-    and p bit for bit and the same iteration total: at 1e-10 every solve
-    meets the rule after one iteration (no restart), at 1e-14 the first
-    solve needs a second one and the call starts over from its saved state
-    on the host-checked path."""
+    profiling on, every solve is checked on the host.  Called one step at a
+    time (several steps per call also share the F between steps, which
+    rounds differently: test_steps_in_one_call_match_single_steps), both
+    give the same V and p bit for bit and the same iteration total: at 1e-10
+    every solve meets the rule after one iteration (no restart), at 1e-14
+    the first solve needs a second one and the call starts over from its
+    saved state on the host-checked path."""
     _say('fdm pcg device iteration', acc, nsteps)
     case = _wall_case('ldc', 40, 30, sx=1.03, poisson=3, CG_acc=acc, CG_maxit=30, CG_precond='fdm')
     out = []
@@ -409,8 +411,8 @@ def test_fdm_pcg_device_iteration_mode(acc, nsteps):
         solver.profile(prof)
         dt = opts.dt
         total0 = solver.solver_stats()[2]   # the initial pressure solve
-        solver.step(opts.t_start, dt, nsteps)
-        solver.step(opts.t_start + nsteps * dt, dt, 2)
+        for n in range(nsteps + 2):
+            solver.step(opts.t_start + n * dt, dt, 1)
         V, p = solver.get_state()
         it, rel, total = solver.solver_stats()
         _say('profile', prof, 'last it', it, 'relres', rel, 'total', total - total0)
