@@ -871,12 +871,12 @@ void additional_solve_dual(ins2d_handle* h) {
 }
 
 // the steps of an ins2d_step call whose boundary F can be shared (steady
-// boundary values): the fourth-order periodic kernels (one instance writes
-// both), the generic second-order operators when k1 is kept (the additional
-// solve's right-hand side follows from it)
+// boundary values): the periodic kernels with two cells per thread (one
+// instance writes both), the generic second-order operators when k1 is kept
+// (the additional solve's right-hand side follows from it)
 bool dual_capable(const ins2d_handle* h) {
   if (h->prof || h->w4) return false;
-  if (h->path == 4) return rhs_per_dual_ok(h->path, h->Nx);
+  if (h->path) return rhs_per_dual_ok(h->path, h->Nx);
   return h->path == 0 && h->plan && h->keep[0];
 }
 

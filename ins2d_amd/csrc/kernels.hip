@@ -264,17 +264,28 @@ __global__ void __launch_bounds__(256) k_rhs_per(PerGrid G, const double* __rest
 
 // second order, two cells (i, i+1) per thread (even Nx): the shared stencil
 // values loaded once, 16-byte loads and stores
-template <bool BAND>
+// DUAL: also (F - G p) / Omega into out2, as a RHS_WITH_P | RHS_SCALE_OM
+// launch forms it (the F of a step's end velocity shared by the additional
+// pressure solve and the next step's first stage)
+template <bool BAND, bool DUAL>
 __global__ void __launch_bounds__(256) k_rhs_per2x(PerGrid G, const double* __restrict__ u,
                                                   const double* __restrict__ v, const double* __restrict__ p,
                                                   int mode, double* __restrict__ out, StageCombine sc,
-                                                  double* __restrict__ vtemp) {
+                                                  double* __restrict__ vtemp, double* __restrict__ out2) {
   const int2 bxy = block_xy<BAND>();
   const int i = 2 * (bxy.x * blockDim.x + threadIdx.x);
   const int j = bxy.y * blockDim.y + threadIdx.y;
   if (i >= G.Nx || j >= G.Ny) return;
   double2 fu, fv;
   rhs_per2_pair(G, u, v, i, j, fu, fv);
+  if constexpr (DUAL) {
+    double gx0, gy0, gx1, gy1;
+    grad_per<2>(G, p, i, j, gx0, gy0);
+    grad_per<2>(G, p, i + 1, j, gx1, gy1);
+    const size_t n2 = (size_t)G.Nx * G.Ny, id2 = (size_t)i + (size_t)G.Nx * j;
+    *(double2*)(out2 + id2) = make_double2((fu.x - gx0) * G.rom, (fu.y - gx1) * G.rom);
+    *(double2*)(out2 + n2 + id2) = make_double2((fv.x - gy0) * G.rom, (fv.y - gy1) * G.rom);
+  }
   if (mode & RHS_WITH_P) {
     double gx, gy;
     grad_per<2>(G, p, i, j, gx, gy);
@@ -2378,7 +2389,7 @@ static bool k_all_aligned(const StageCombine& sc) {
   return true;
 }
 
-bool rhs_per_dual_ok(int order, int Nx) { return order == 4 && Nx % kStripW2 == 0; }
+bool rhs_per_dual_ok(int order, int Nx) { return order == 4 ? Nx % kStripW2 == 0 : Nx % 2 == 0; }
 
 void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double hy, double invRe, const double* u,
                     const double* v, const double* p, int mode, double* out, const StageCombine& sc, double* vt,
@@ -2386,13 +2397,16 @@ void launch_rhs_per(hipStream_t st, int order, int Nx, int Ny, double hx, double
   const PerGrid G = per_grid(order, Nx, Ny, hx, hy, invRe);
   if (out2 && !(rhs_per_dual_ok(order, Nx) && aligned16(u) && aligned16(v) && aligned16(out) && aligned16(vt) &&
                 aligned16(out2) && k_all_aligned(sc)))
-    throw std::runtime_error("right-hand side with a second output: fourth order, Nx a multiple of 128, aligned arrays");
+    throw std::runtime_error("right-hand side with a second output: Nx even (a multiple of 128 at fourth order), "
+                             "aligned arrays");
   const int H = strip_height(Nx, Ny);
   dim3 b(64, 4);
   bool pair = order == 2 && Nx % 2 == 0 && aligned16(u) && aligned16(v) && aligned16(out) && aligned16(vt);
   for (int q = 0; q < sc.n; ++q) pair = pair && aligned16(sc.k[q]);
-  if (pair)   // two cells per thread: 9.8-10.0 against 11.7 us at 1024^2
-    launch_timed(k_rhs_per2x<true>, grid2(Nx / 2, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
+  if (pair && out2)
+    launch_timed(k_rhs_per2x<true, true>, grid2(Nx / 2, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt, out2);
+  else if (pair)   // two cells per thread: 9.8-10.0 against 11.7 us at 1024^2
+    launch_timed(k_rhs_per2x<true, false>, grid2(Nx / 2, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt, out2);
   else if (order == 2)
     launch_timed(k_rhs_per<2, true>, grid2(Nx, Ny, b), b, 0, st, G, u, v, p, mode, out, sc, vt);
   else if (Nx % kStripW2 == 0 && aligned16(u) && aligned16(v) && aligned16(out) && aligned16(vt) &&

@@ -95,6 +95,7 @@ _ONE_CALL_CASES = {
     'ldc 64 fdm pcg': lambda: load('LDC_unsteady', Nx=64, Ny=64, Re=1000.0, dt=1e-3, RK='RK44', poisson=3,
                                    CG_acc=1e-10, CG_maxit=200, CG_precond='fdm'),
     'bfs 256x64 direct': lambda: load('BFS_unsteady', Nx=256, Ny=64),
+    'taylor-green 256x128': lambda: load('taylor_green', Nx=256, Ny=128, Re=500.0, noise=1e-2, seed=3),
 }
 
 
@@ -102,7 +103,7 @@ _ONE_CALL_CASES = {
 def test_steps_in_one_call_match_single_steps(name):
     """Inside one ins2d_step call the F of each step's end velocity is formed
     once, for the additional pressure solve and for the next step's first
-    stage (fourth-order periodic: one kernel instance writes both; generic
+    stage (periodic: one kernel instance writes both; generic
     operators: the additional solve's (F - G p) / Omega follows from the
     first-stage k1 = F / Omega); a call of one step (and every profiled call)
     forms it twice, as the reference does.  The two separate forms agree bit
