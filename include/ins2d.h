@@ -144,7 +144,7 @@ int ins2d_input_check(ins2d_handle* h, double t, int p_initial);
  * steps with the convergence test read after every iteration (the results
  * are those of that path bit for bit).  When another step of the same call
  * follows (profiling off; the periodic paths with Nx even, a multiple of 128
- * at fourth order, or the second-order generic operators), the F of a step's end
+ * at fourth order, or the generic operators), the F of a step's end
  * velocity is formed once for the additional pressure solve and for the
  * next step's first stage (the reference forms it twice); the results agree
  * with one-step calls to rounding. */

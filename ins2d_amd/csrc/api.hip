@@ -860,6 +860,11 @@ void additional_solve_dual(ins2d_handle* h) {
   if (h->path) {
     launch_rhs_per(h->st, h->path, h->Nx, h->Ny, h->hx, h->hy, h->invRe, h->V, h->V + h->Nu, h->p, RHS_SCALE_OM,
                    h->keep[0] ? h->kv[0] : nullptr, sc0, h->Vt, h->tV);
+  } else if (h->w4) {
+    // fourth order with walls: k1 - Om^-1 G p by the projection kernel
+    // (Vn = k1, no Vtemp, dt = c = 1)
+    rhs(h, h->V, h->p, RHS_SCALE_OM, h->kv[0], sc0, h->Vt);
+    launch_w4_proj(h->st, h->w4d, h->kv[0], nullptr, h->p, 1.0, 1.0, h->tV);
   } else {
     // generic operators: the first stage's right-hand side as its own launch
     // would form it, then (F - G p) / Omega = k1 - Om^-1 G p from it
@@ -872,12 +877,13 @@ void additional_solve_dual(ins2d_handle* h) {
 
 // the steps of an ins2d_step call whose boundary F can be shared (steady
 // boundary values): the periodic kernels with two cells per thread (one
-// instance writes both), the generic second-order operators when k1 is kept
-// (the additional solve's right-hand side follows from it)
+// instance writes both), the generic operators (second order, and fourth
+// order with walls) when k1 is kept (the additional solve's right-hand side
+// follows from it)
 bool dual_capable(const ins2d_handle* h) {
-  if (h->prof || h->w4) return false;
+  if (h->prof) return false;
   if (h->path) return rhs_per_dual_ok(h->path, h->Nx);
-  return h->path == 0 && h->plan && h->keep[0];
+  return (h->w4 || h->plan) && h->keep[0];
 }
 
 void erk_step(ins2d_handle* h, double tn, double dt, DComp* ver = nullptr, bool dual_end = false) {

@@ -96,6 +96,7 @@ _ONE_CALL_CASES = {
                                    CG_acc=1e-10, CG_maxit=200, CG_precond='fdm'),
     'bfs 256x64 direct': lambda: load('BFS_unsteady', Nx=256, Ny=64),
     'taylor-green 256x128': lambda: load('taylor_green', Nx=256, Ny=128, Re=500.0, noise=1e-2, seed=3),
+    'ldc inviscid 32': lambda: load('LDC_inviscid', Nx=32, Ny=32, dt=0.005),
 }
 
 
